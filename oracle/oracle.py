@@ -1,0 +1,211 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg as the checker.  The product never imports this module.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _lib.orc_trace.restype = C.c_int64
+        _lib.orc_tau.restype = C.c_int64
+        _lib.orc_phase_hg.restype = C.c_double
+        _lib.orc_phase_hg.argtypes = [C.c_double, C.c_double]
+        _lib.orc_frame_size.restype = C.c_int64
+    return _lib
+
+
+P = C.POINTER
+f64p = P(C.c_double)
+i32p = P(C.c_int32)
+i64p = P(C.c_int64)
+
+
+class OrcGrid(C.Structure):
+    _fields_ = [("type", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("xb", f64p), ("yb", f64p), ("zb", f64p),
+                ("nnodes", C.c_int64), ("bounds", f64p), ("child_first", i32p), ("cell_index", i32p),
+                ("nb_off", i64p), ("nb_id", i32p), ("box", C.c_double * 6)]
+
+
+class OrcInstrument(C.Structure):
+    _fields_ = [("inclination", C.c_double), ("azimuth", C.c_double), ("roll", C.c_double),
+                ("nx", C.c_int32), ("ny", C.c_int32),
+                ("fovx", C.c_double), ("fovy", C.c_double), ("cx", C.c_double), ("cy", C.c_double)]
+
+
+class OrcSim(C.Structure):
+    _fields_ = [("grid", P(OrcGrid)), ("density", f64p), ("nlambda", C.c_int32),
+                ("lam", f64p), ("sext", f64p), ("ssca", f64p), ("g", f64p), ("lpack", f64p),
+                ("src", C.c_double * 3), ("xi", C.c_double), ("min_weight_reduction", C.c_double),
+                ("min_scatt_events", C.c_int32), ("ninstr", C.c_int32), ("instr", P(OrcInstrument)),
+                ("seed", C.c_uint64)]
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+class Grid:
+    """Keeps the numpy arrays alive behind an OrcGrid."""
+
+    def __init__(self, kind, **kw):
+        self.keep = []
+        g = OrcGrid()
+        if kind == "cartesian":
+            g.type = 0
+            xb, yb, zb = (np.ascontiguousarray(kw[k], dtype=np.float64) for k in ("xb", "yb", "zb"))
+            self.keep += [xb, yb, zb]
+            g.nx, g.ny, g.nz = len(xb) - 1, len(yb) - 1, len(zb) - 1
+            g.xb, g.yb, g.zb = _p(xb, f64p), _p(yb, f64p), _p(zb, f64p)
+            box = (xb[0], yb[0], zb[0], xb[-1], yb[-1], zb[-1])
+        elif kind == "octree":
+            g.type = 1
+            bounds = np.ascontiguousarray(kw["bounds"], dtype=np.float64).reshape(-1, 6)
+            cf = np.ascontiguousarray(kw["child_first"], dtype=np.int32)
+            ci = np.ascontiguousarray(kw["cell_index"], dtype=np.int32)
+            nbo = np.ascontiguousarray(kw["nb_off"], dtype=np.int64)
+            nbi = np.ascontiguousarray(kw["nb_id"], dtype=np.int32)
+            self.keep += [bounds, cf, ci, nbo, nbi]
+            g.nnodes = len(bounds)
+            g.bounds, g.child_first, g.cell_index = _p(bounds, f64p), _p(cf, i32p), _p(ci, i32p)
+            g.nb_off, g.nb_id = _p(nbo, i64p), _p(nbi, i32p)
+            box = tuple(bounds[0])
+        else:
+            raise ValueError(kind)
+        g.box = (C.c_double * 6)(*box)
+        self.g = g
+
+
+def trace(grid: Grid, rays, cap=None):
+    rays = np.ascontiguousarray(rays, dtype=np.float64)
+    n = len(rays)
+    cap = cap or max(1024, n * 4096)
+    off = np.zeros(n + 1, np.int64)
+    m = np.zeros(cap, np.int32)
+    ds = np.zeros(cap, np.float64)
+    t = lib().orc_trace(C.byref(grid.g), _p(rays, f64p), C.c_int64(n), C.c_int64(cap), _p(off, i64p),
+                        _p(m, i32p), _p(ds, f64p))
+    if t < 0:
+        raise RuntimeError("oracle trace capacity exceeded")
+    return off, m[:t].copy(), ds[:t].copy()
+
+
+def tau(grid: Grid, density, section, rays, L=1.0, cap=None):
+    rays = np.ascontiguousarray(rays, dtype=np.float64)
+    density = np.ascontiguousarray(density, dtype=np.float64)
+    n = len(rays)
+    cap = cap or max(1024, n * 4096)
+    toff = np.zeros(n + 1, np.int64)
+    tv = np.zeros(cap)
+    sv = np.zeros(cap)
+    mv = np.zeros(cap, np.int32)
+    tobs = np.zeros(n)
+    t = lib().orc_tau(C.byref(grid.g), _p(density, f64p), C.c_double(section), _p(rays, f64p), C.c_int64(n),
+                      C.c_double(L), C.c_int64(cap), _p(toff, i64p), _p(tv, f64p), _p(sv, f64p), _p(mv, i32p),
+                      _p(tobs, f64p))
+    if t < 0:
+        raise RuntimeError("oracle tau capacity exceeded")
+    return toff, tv[:t].copy(), sv[:t].copy(), mv[:t].copy(), tobs
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox(_p(c, P(C.c_uint32)), _p(k, P(C.c_uint32)), _p(out, P(C.c_uint32)))
+    return out
+
+
+def uniforms(seed, history, n, tag=0):
+    out = np.zeros(n)
+    lib().orc_uniforms(C.c_uint64(seed), C.c_uint64(history), C.c_uint32(tag), C.c_int64(n), _p(out, f64p))
+    return out
+
+
+def phase_hg(g, c):
+    return lib().orc_phase_hg(float(g), float(c))
+
+
+def scatter_hg(seed, history, g, k, n):
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    out = np.zeros((n, 3))
+    lib().orc_scatter_hg(C.c_uint64(seed), C.c_uint64(history), C.c_double(g), _p(k, f64p), C.c_int64(n),
+                         _p(out, f64p))
+    return out
+
+
+def expon_cutoff(seed, history, xmax, n):
+    out = np.zeros(n)
+    lib().orc_expon_cutoff(C.c_uint64(seed), C.c_uint64(history), C.c_double(xmax), C.c_int64(n), _p(out, f64p))
+    return out
+
+
+def simulate(grid: Grid, density, lam, sext, ssca, g, lpack, instruments, first, count, threads=8, seed=1,
+             xi=0.5, min_weight_reduction=1e4, min_scatt_events=0, src=(0.0, 0.0, 0.0)):
+    """Run histories [first, first+count); returns (frames list, counters).
+
+    instruments: list of dicts with inclination, azimuth, roll (rad), nx, ny, fovx, fovy, cx, cy (m).
+    Each frame has shape (nlambda, ny, nx)."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (density, lam, sext, ssca, g, lpack)]
+    ins = (OrcInstrument * len(instruments))()
+    for i, d in enumerate(instruments):
+        ins[i] = OrcInstrument(d["inclination"], d["azimuth"], d.get("roll", 0.0), d["nx"], d["ny"],
+                               d["fovx"], d["fovy"], d.get("cx", 0.0), d.get("cy", 0.0))
+    s = OrcSim()
+    s.grid = C.pointer(grid.g)
+    s.density = _p(arrs[0], f64p)
+    s.nlambda = len(arrs[1])
+    s.lam, s.sext, s.ssca, s.g, s.lpack = (_p(a, f64p) for a in arrs[1:])
+    s.src = (C.c_double * 3)(*src)
+    s.xi = xi
+    s.min_weight_reduction = min_weight_reduction
+    s.min_scatt_events = min_scatt_events
+    s.ninstr = len(instruments)
+    s.instr = ins
+    s.seed = seed
+    n = lib().orc_frame_size(C.byref(s))
+    frame = np.zeros(n)
+    cnt = np.zeros(4, np.int64)
+    lib().orc_simulate(C.byref(s), C.c_uint64(first), C.c_uint64(count), C.c_int(threads), _p(frame, f64p),
+                       _p(cnt, i64p))
+    frames = []
+    o = 0
+    nl = len(arrs[1])
+    for d in instruments:
+        sz = d["nx"] * d["ny"] * nl
+        frames.append(frame[o:o + sz].reshape(nl, d["ny"], d["nx"]))
+        o += sz
+    return frames, cnt
+
+
+def child_first_from(child_begin, child_count, child_index):
+    """First child id per node (-1 for leaves) from the flattened reference children lists."""
+    cb = np.asarray(child_begin)
+    cc = np.asarray(child_count)
+    ci = np.asarray(child_index)
+    out = np.full(len(cb), -1, np.int32)
+    has = cc > 0
+    out[has] = ci[cb[has]]
+    return out

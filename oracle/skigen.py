@@ -1,0 +1,146 @@
+"""Render SKIRT ski files for the workloads used by the tests and the CPU baseline.
+
+Test infrastructure: the ski files drive the reference build in oracle/_ref (fixture
+generation and the "reference" CPU timing).  The item names and attributes are the
+reference's configuration vocabulary (ExpDiskGeometry, SpiralStructureGeometryDecorator,
+MeanListDustMix, CartesianSpatialGrid, PolicyTreeSpatialGrid/DensityTreePolicy,
+FrameInstrument, PhotonPacketOptions); a working oligochromatic point-source setup needs a
+BlackBodySED with a SpecificLuminosityNormalization (SingleWavelengthSED is rejected in
+oligochromatic mode).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Sequence, Tuple
+
+
+@dataclass
+class SkiSpec:
+    num_packets: float = 1e5
+    wavelengths_um: Sequence[float] = (0.55,)
+    # dust table: (lambda_um, kappa_ext m2/kg, albedo, g); log-log resampled by the reference
+    dust: Sequence[Tuple[float, float, float, float]] = ((0.55, 1.0, 0.6, 0.6),)
+    spiral: bool = False
+    tau_v: float = 1.0
+    grid: Tuple = ("cartesian", 64)  # or ("octree", min_level, max_level, max_dust_fraction)
+    half_size_pc: float = 10.0
+    # instruments: (name, inclination deg, azimuth deg)
+    instruments: Sequence[Tuple[str, float, float]] = (("i60", 60.0, 0.0),)
+    num_pixels: int = 200
+    fov_pc: float = 20.0
+    seed: int = 0
+    force_scattering: bool = True
+    min_weight_reduction: float = 1e4
+    min_scatt_events: int = 0
+    path_length_bias: float = 0.5
+    num_density_samples: int = 100
+
+
+def _fmt_list(vals, unit):
+    return ", ".join(f"{v:.9g} {unit}" for v in vals)
+
+
+def render(spec: SkiSpec) -> str:
+    wl = _fmt_list(spec.wavelengths_um, "micron")
+    dl = _fmt_list([d[0] for d in spec.dust], "micron")
+    kappa = ", ".join(f"{d[1]:.9g} m2/kg" for d in spec.dust)
+    alb = ", ".join(f"{d[2]:.9g}" for d in spec.dust)
+    gg = ", ".join(f"{d[3]:.9g}" for d in spec.dust)
+    disk = ('<ExpDiskGeometry scaleLength="3 pc" scaleHeight="0.3 pc" minRadius="0 pc" '
+            'maxRadius="0 pc" maxZ="0 pc"/>')
+    if spec.spiral:
+        geom = ('<SpiralStructureGeometryDecorator numArms="2" pitchAngle="15 deg" radiusZeroPoint="3 pc" '
+                'phaseZeroPoint="0 deg" perturbationWeight="0.5" index="1"><geometry type="AxGeometry">'
+                + disk + '</geometry></SpiralStructureGeometryDecorator>')
+    else:
+        geom = disk
+    h = spec.half_size_pc
+    box = f'minX="-{h} pc" maxX="{h} pc" minY="-{h} pc" maxY="{h} pc" minZ="-{h} pc" maxZ="{h} pc"'
+    if spec.grid[0] == "cartesian":
+        n = spec.grid[1]
+        grid = (f'<CartesianSpatialGrid {box}>'
+                f'<meshX type="Mesh"><LinMesh numBins="{n}"/></meshX>'
+                f'<meshY type="Mesh"><LinMesh numBins="{n}"/></meshY>'
+                f'<meshZ type="Mesh"><LinMesh numBins="{n}"/></meshZ></CartesianSpatialGrid>')
+    elif spec.grid[0] == "octree":
+        _, lmin, lmax, frac = spec.grid
+        grid = (f'<PolicyTreeSpatialGrid {box} treeType="OctTree"><policy type="TreePolicy">'
+                f'<DensityTreePolicy minLevel="{lmin}" maxLevel="{lmax}" maxDustFraction="{frac:.9g}" '
+                'maxDustOpticalDepth="0" wavelength="0.55 micron" maxDustDensityDispersion="0" '
+                'maxElectronFraction="1e-6" maxGasFraction="1e-6"/></policy></PolicyTreeSpatialGrid>')
+    else:
+        raise ValueError(spec.grid)
+    instr = "".join(
+        f'<FrameInstrument instrumentName="{name}" distance="10 Mpc" inclination="{inc:.9g} deg" '
+        f'azimuth="{az:.9g} deg" roll="0 deg" fieldOfViewX="{spec.fov_pc:.9g} pc" numPixelsX="{spec.num_pixels}" '
+        f'centerX="0 pc" fieldOfViewY="{spec.fov_pc:.9g} pc" numPixelsY="{spec.num_pixels}" centerY="0 pc" '
+        'recordComponents="false" numScatteringLevels="0" recordPolarization="false" recordStatistics="false"/>'
+        for (name, inc, az) in spec.instruments)
+    force = "true" if spec.force_scattering else "false"
+    return f"""<?xml version="1.0" encoding="UTF-8"?>
+<skirt-simulation-hierarchy type="MonteCarloSimulation" format="9" producer="skigen" time="2026-01-01T00:00:00">
+    <MonteCarloSimulation userLevel="Expert" simulationMode="OligoExtinctionOnly" iteratePrimaryEmission="false" numPackets="{spec.num_packets:.9g}">
+        <random type="Random"><Random seed="{spec.seed}"/></random>
+        <units type="Units"><ExtragalacticUnits fluxOutputStyle="Frequency"/></units>
+        <cosmology type="Cosmology"><LocalUniverseCosmology/></cosmology>
+        <sourceSystem type="SourceSystem">
+            <SourceSystem wavelengths="{wl}" sourceBias="0.5">
+                <sources type="Source">
+                    <PointSource positionX="0 pc" positionY="0 pc" positionZ="0 pc" velocityX="0 km/s" velocityY="0 km/s" velocityZ="0 km/s" sourceWeight="1" wavelengthBias="0.5">
+                        <sed type="SED"><BlackBodySED temperature="5800 K"/></sed>
+                        <normalization type="LuminosityNormalization">
+                            <SpecificLuminosityNormalization wavelength="0.55 micron" unitStyle="wavelengthmonluminosity" specificLuminosity="1 Lsun/micron"/>
+                        </normalization>
+                    </PointSource>
+                </sources>
+            </SourceSystem>
+        </sourceSystem>
+        <mediumSystem type="MediumSystem">
+            <MediumSystem>
+                <photonPacketOptions type="PhotonPacketOptions">
+                    <PhotonPacketOptions explicitAbsorption="false" forceScattering="{force}" minWeightReduction="{spec.min_weight_reduction:.9g}" minScattEvents="{spec.min_scatt_events}" pathLengthBias="{spec.path_length_bias:.9g}"/>
+                </photonPacketOptions>
+                <media type="Medium">
+                    <GeometricMedium velocityMagnitude="0 km/s" magneticFieldStrength="0 uG">
+                        <geometry type="Geometry">{geom}</geometry>
+                        <materialMix type="MaterialMix">
+                            <MeanListDustMix wavelengths="{dl}" extinctionCoefficients="{kappa}" albedos="{alb}" asymmetryParameters="{gg}"/>
+                        </materialMix>
+                        <normalization type="MaterialNormalization">
+                            <OpticalDepthMaterialNormalization axis="Z" wavelength="0.55 micron" opticalDepth="{spec.tau_v:.9g}"/>
+                        </normalization>
+                    </GeometricMedium>
+                </media>
+                <samplingOptions type="SamplingOptions">
+                    <SamplingOptions numDensitySamples="{spec.num_density_samples}" numPropertySamples="1" aggregateVelocity="Average"/>
+                </samplingOptions>
+                <grid type="SpatialGrid">{grid}</grid>
+            </MediumSystem>
+        </mediumSystem>
+        <instrumentSystem type="InstrumentSystem">
+            <InstrumentSystem><instruments type="Instrument">{instr}</instruments></InstrumentSystem>
+        </instrumentSystem>
+        <probeSystem type="ProbeSystem"><ProbeSystem/></probeSystem>
+    </MonteCarloSimulation>
+</skirt-simulation-hierarchy>
+"""
+
+
+def log_wavelengths(lo: float, hi: float, n: int) -> List[float]:
+    import math
+    return [lo * (hi / lo) ** (i / (n - 1)) for i in range(n)] if n > 1 else [lo]
+
+
+def config0(num_packets=1e6, seed=0) -> SkiSpec:
+    """Cartesian 64^3 exponential disk, tau_V=1, 0.55 um, albedo 0.6, g 0.6, frame at 60 deg."""
+    return SkiSpec(num_packets=num_packets, seed=seed)
+
+
+def config1(num_packets=1e8, seed=0, max_level=10, min_level=3, max_frac=1e-6, tau_v=1.0, nlambda=40,
+            lam_range=(0.1, 10.0)) -> SkiSpec:
+    """Spiral disk on a density-refined octree with a power-law opacity (kappa ~ lambda^-1.5)."""
+    return SkiSpec(num_packets=num_packets, seed=seed, spiral=True, tau_v=tau_v,
+                   wavelengths_um=log_wavelengths(lam_range[0], lam_range[1], nlambda),
+                   dust=((0.1, 1000.0, 0.5, 0.6), (10.0, 1.0, 0.1, 0.6)),
+                   grid=("octree", min_level, max_level, max_frac))

@@ -1,0 +1,158 @@
+/* skirt_mi355x.h -- C-ABI of the MI355X photon-packet engine for SKIRT.
+ *
+ * The engine replaces SKIRT's CPU photon-packet loop behind the seams the reference
+ * offloads through `class GpuAcceleration` (SKIRT/core/GpuAcceleration.hpp:30-607):
+ *   - spatial grid upload + path segments   (SpatialGrid::createPathSegmentGenerator,
+ *                                            SKIRT/core/SpatialGrid.hpp:65-69)
+ *   - medium state / dust tables            (MediumSystem setup, MediumSystem.cpp:234-514)
+ *   - optical depths along paths            (MediumSystem::setExtinctionOpticalDepths :1785,
+ *                                            MediumSystem::getExtinctionOpticalDepth :2328)
+ *   - the forced-scattering life cycle      (MonteCarloSimulation::performLifeCycle :614-694)
+ *   - frame instruments                     (FrameInstrument::detect, FrameInstrument.cpp:37-41;
+ *                                            FluxRecorder::detect, FluxRecorder.cpp:372-536)
+ *
+ * Conventions (kept from the reference boundary, GpuAcceleration.hpp:27-29):
+ *   - every entry point returns SKG_OK (0) or a negative status; on any non-OK status the
+ *     caller runs its CPU code path, and skg_last_error() says why;
+ *   - the caller owns all host buffers; outputs are written only on SKG_OK;
+ *   - one engine object drives one GPU; calls on one engine are serialized internally.
+ * Plain C types only: pointers, sizes, doubles.  Units are SI (metres, radians).
+ */
+#ifndef SKIRT_MI355X_H
+#define SKIRT_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SKG_OK 0
+#define SKG_ERR_ARGUMENT -1   /* invalid argument or inconsistent sizes */
+#define SKG_ERR_DEVICE -2     /* HIP runtime error (allocation, launch, copy) */
+#define SKG_ERR_CAPACITY -3   /* an output buffer is too small */
+#define SKG_ERR_STATE -4      /* a required upload (grid, medium, ...) is missing */
+
+#define SKG_GRID_CARTESIAN 0
+#define SKG_GRID_OCTREE 1
+
+#define SKG_FP64 0            /* optical depth and path length sums in FP64 */
+#define SKG_FP32_KAHAN 1      /* the same sums in FP32 with compensated summation */
+
+typedef struct skg_engine skg_engine;
+
+/* Creates an engine on HIP device `device`.  Replaces GpuAcceleration's runtime set-up
+ * (GpuAcceleration.cpp:10716-10831).  *out is set only on success. */
+int skg_create(int device, skg_engine** out);
+void skg_destroy(skg_engine* e);
+/* Text of the last error on this engine (or of the last failed skg_create when e is NULL);
+ * the counterpart of GpuAcceleration::status() (GpuAcceleration.hpp:40-41). */
+const char* skg_last_error(const skg_engine* e);
+
+/* ---- spatial grids ------------------------------------------------------------------ */
+
+/* Cartesian grid from its cell borders (nx+1, ny+1, nz+1 increasing values), cell index
+ * k + nz*j + nz*ny*i.  Replaces CartesianSpatialGrid::setupSelfAfter (CartesianSpatialGrid.cpp:14-25)
+ * for the path; traversal follows CartesianSpatialGrid.cpp:77-153. */
+int skg_grid_cartesian(skg_engine* e, const double* xb, int32_t nx, const double* yb, int32_t ny,
+                       const double* zb, int32_t nz);
+
+/* Octree from the flattened node table of TreeSpatialGrid (TreeSpatialGrid.cpp:37-70):
+ * bounds (6 per node: xmin,ymin,zmin,xmax,ymax,zmax; node 0 is the root = domain),
+ * child_first (id of the first of 8 consecutive children, -1 for a leaf, OctTreeNode.cpp:22-33),
+ * cell_index (-1 for non-leaves), and per node per wall (BACK,FRONT,LEFT,RIGHT,BOTTOM,TOP)
+ * the neighbour lists of TreeNode (TreeNode.cpp:103-112, sorted as TreeNode::sortNeighbors)
+ * as CSR: nb_off[6*nnodes+1], nb_id[nb_off[6*nnodes]].  Traversal follows TreeSpatialGrid.cpp:143-228. */
+int skg_grid_octree(skg_engine* e, int64_t nnodes, const double* bounds, const int32_t* child_first,
+                    const int32_t* cell_index, const int64_t* nb_off, const int32_t* nb_id);
+
+/* Number of cells of the current grid (SpatialGrid::numCells). */
+int64_t skg_num_cells(const skg_engine* e);
+
+/* ---- medium ------------------------------------------------------------------------- */
+
+/* Number density per cell for the single dust medium (MediumState number density,
+ * MediumState.hpp:222-266; sampled as MediumSystem.cpp:401-504). */
+int skg_medium(skg_engine* e, int64_t ncells, const double* number_density);
+
+/* Dust properties at the simulation's wavelength bins: wavelength (m), extinction and
+ * scattering cross sections (m2) and HG asymmetry parameter, as DustMix::sectionExt/
+ * sectionSca/asymmpar return them at those wavelengths (DustMix.cpp:319-343). */
+int skg_dust(skg_engine* e, int32_t nlambda, const double* lambda, const double* sext, const double* ssca,
+             const double* g);
+
+/* ---- path service (the SpatialGrid seam) ---------------------------------------------- */
+
+/* For n rays {rx,ry,rz,kx,ky,kz} (host memory), returns the raw path segments produced by
+ * SpatialGrid::createPathSegmentGenerator()->next() (including empty segments, m = -1):
+ * off[n+1] offsets, m[total], ds[total].  cap = capacity of m and ds; *total receives the
+ * number of segments (also when it exceeds cap, with SKG_ERR_CAPACITY). */
+int skg_trace_paths(skg_engine* e, const double* rays, int64_t n, int64_t cap, int64_t* off, int32_t* m,
+                    double* ds, int64_t* total);
+
+/* Optical depths at wavelength bin `ell` for n rays (host memory):
+ *   tau_path[n]  total extinction optical depth of the complete path (the last cumulative
+ *                value of MediumSystem::setExtinctionOpticalDepths, MediumSystem.cpp:1785-1810)
+ *   tau_obs[n]   MediumSystem::getExtinctionOpticalDepth(pp, inf) for a packet of luminosity L
+ *                (early-out +inf at log(L)+745, MediumSystem.cpp:2328-2376)
+ * and, when toff is not NULL, the cumulative optical depth after every stored (ds>0) segment
+ * (tau_cum at toff[i]..toff[i+1]; toff must be the stored-segment offsets of these rays).
+ * fp_mode selects SKG_FP64 or SKG_FP32_KAHAN. */
+int skg_trace_tau(skg_engine* e, const double* rays, int64_t n, int32_t ell, double L, int32_t fp_mode,
+                  double* tau_path, double* tau_obs, const int64_t* toff, double* tau_cum);
+
+/* ---- instruments -------------------------------------------------------------------- */
+
+/* Adds a distant frame instrument (FrameInstrument.cpp:12-33, DistantInstrument.cpp:13-63);
+ * returns its index (>= 0) or a negative status.  At most 8 instruments. */
+int skg_frame_add(skg_engine* e, double inclination, double azimuth, double roll, int32_t nx, int32_t ny,
+                  double fovx, double fovy, double cx, double cy);
+int skg_frame_remove_all(skg_engine* e);
+/* Zeroes all frame accumulators. */
+int skg_frame_clear(skg_engine* e);
+/* Copies the recorded luminosities of instrument `index` (nlambda*ny*nx doubles, index
+ * i + nx*j + nx*ny*ell as FluxRecorder.cpp:501) to host memory; the counterpart of
+ * FluxRecorder::flush (FluxRecorder.cpp:856-872) before calibration. */
+int skg_frame_read(skg_engine* e, int32_t index, double* out, int64_t count);
+
+/* ---- sources and the life cycle (the MonteCarloSimulation seam) ------------------------ */
+
+/* Isotropic point source at (x,y,z) (PointSource.cpp:32-42).  lpack[nlambda] is the launch
+ * luminosity of one packet at each wavelength bin (SourceSystem::launch + NormalizedSource::launch
+ * with the oligochromatic bias weight, SourceSystem.cpp:101-115, NormalizedSource.cpp:73-110);
+ * the bin of each packet is drawn uniformly (OligoWavelengthDistribution.cpp:34-38). */
+int skg_source_point(skg_engine* e, double x, double y, double z, const double* lpack);
+
+/* PhotonPacketOptions (pathLengthBias, minWeightReduction, minScattEvents,
+ * PhotonPacketOptions.hpp:71-86), the precision mode and the random seed. */
+int skg_options(skg_engine* e, double path_length_bias, double min_weight_reduction, int32_t min_scatt_events,
+                int32_t fp_mode, uint64_t seed);
+
+/* Runs the forced-scattering life cycle of histories [first, first+count) with peel-off to
+ * every instrument, accumulating into the frames on the device (MonteCarloSimulation.cpp:614-694,
+ * :1492-1544, :2034-2092).  Synchronous. */
+int skg_run_packets(skg_engine* e, uint64_t first_history, uint64_t count);
+
+/* Same, asynchronous on the engine's stream; skg_sync waits for completion. */
+int skg_run_packets_async(skg_engine* e, uint64_t first_history, uint64_t count);
+int skg_sync(skg_engine* e);
+
+/* Device counters accumulated since the last skg_frame_clear: [0] histories, [1] peel-off
+ * traversals, [2] forced-path traversals (first pass), [3] scattering events,
+ * [4] segment crossings, [5] detector atomics, [6] second-pass traversals, [7] reserved. */
+int skg_counters(skg_engine* e, int64_t* out8);
+
+/* Launch shape of the persistent life-cycle kernel: workgroups (0 = automatic), threads per
+ * workgroup (64..1024, multiple of 64) and the number of lanes of a wavefront that must be
+ * waiting before a wave-wide transition round runs (1..64).  Chosen by the autotuner. */
+int skg_tune(skg_engine* e, int32_t workgroups, int32_t threads, int32_t transition_threshold);
+
+/* Duration in milliseconds of the last life-cycle kernel, measured with HIP events on the
+ * engine's stream. */
+double skg_last_kernel_ms(const skg_engine* e);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SKIRT_MI355X_H */

@@ -1,0 +1,222 @@
+"""Host-side binding of the MI355X photon-packet engine (libskirt_mi355x.so).
+
+The engine is C++/HIP behind the C-ABI in include/skirt_mi355x.h; this module is the
+thin ctypes layer the tests and the benchmark use.  Class and method names follow the
+reference's seams: a SpatialGrid with path segments, a MediumSystem with cell densities
+and dust, FrameInstrument detection and the MonteCarloSimulation life cycle.
+
+The product path never falls back to a CPU implementation: if the shared library is
+missing, or no GPU is present, the calls raise.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libskirt_mi355x.so")
+
+SKG_OK = 0
+SKG_GRID_CARTESIAN = 0
+SKG_GRID_OCTREE = 1
+SKG_FP64 = 0
+SKG_FP32_KAHAN = 1
+
+_lib = None
+
+P = C.POINTER
+f64p = P(C.c_double)
+i32p = P(C.c_int32)
+i64p = P(C.c_int64)
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def build(arch: str = "gfx950") -> None:
+    """Compile the HIP library in-tree (hipcc --offload-arch=gfx950)."""
+    subprocess.run(["make", "-C", HERE, "-s", f"ARCH={arch}", "-B"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EngineError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.skg_last_error.restype = C.c_char_p
+        L.skg_last_error.argtypes = [C.c_void_p]
+        L.skg_num_cells.restype = C.c_int64
+        L.skg_num_cells.argtypes = [C.c_void_p]
+        L.skg_last_kernel_ms.restype = C.c_double
+        L.skg_last_kernel_ms.argtypes = [C.c_void_p]
+        for name in ("skg_create",):
+            getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+EXPORTED = [
+    "skg_create", "skg_destroy", "skg_last_error", "skg_grid_cartesian", "skg_grid_octree", "skg_num_cells",
+    "skg_medium", "skg_dust", "skg_trace_paths", "skg_trace_tau", "skg_frame_add", "skg_frame_remove_all",
+    "skg_frame_clear", "skg_frame_read", "skg_source_point", "skg_options", "skg_run_packets",
+    "skg_run_packets_async", "skg_sync", "skg_counters", "skg_tune", "skg_last_kernel_ms",
+]
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Engine:
+    """One engine drives one GPU (GpuAcceleration's per-device runtime)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        st = lib().skg_create(C.c_int(device), C.byref(self._h))
+        if st != SKG_OK:
+            raise EngineError(lib().skg_last_error(None).decode())
+        self.nlambda = 0
+        self.frames = []
+
+    def close(self):
+        if self._h:
+            lib().skg_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, st):
+        if st < 0:
+            raise EngineError(f"status {st}: {lib().skg_last_error(self._h).decode()}")
+        return st
+
+    # ---- grids
+    def grid_cartesian(self, xb, yb, zb):
+        xb, yb, zb = _f64(xb), _f64(yb), _f64(zb)
+        self._chk(lib().skg_grid_cartesian(self._h, _p(xb, f64p), C.c_int32(len(xb) - 1), _p(yb, f64p),
+                                           C.c_int32(len(yb) - 1), _p(zb, f64p), C.c_int32(len(zb) - 1)))
+
+    def grid_octree(self, bounds, child_first, cell_index, nb_off, nb_id):
+        bounds = _f64(bounds).reshape(-1, 6)
+        cf = np.ascontiguousarray(child_first, dtype=np.int32)
+        ci = np.ascontiguousarray(cell_index, dtype=np.int32)
+        nbo = np.ascontiguousarray(nb_off, dtype=np.int64)
+        nbi = np.ascontiguousarray(nb_id, dtype=np.int32)
+        self._chk(lib().skg_grid_octree(self._h, C.c_int64(len(bounds)), _p(bounds, f64p), _p(cf, i32p),
+                                        _p(ci, i32p), _p(nbo, i64p), _p(nbi, i32p)))
+
+    def num_cells(self):
+        return int(lib().skg_num_cells(self._h))
+
+    # ---- medium
+    def medium(self, density):
+        d = _f64(density)
+        self._chk(lib().skg_medium(self._h, C.c_int64(len(d)), _p(d, f64p)))
+
+    def dust(self, lam, sext, ssca, g):
+        a = [_f64(x) for x in (lam, sext, ssca, g)]
+        self.nlambda = len(a[0])
+        self._chk(lib().skg_dust(self._h, C.c_int32(self.nlambda), *[_p(x, f64p) for x in a]))
+
+    # ---- path service
+    def trace_paths(self, rays, cap=None):
+        rays = _f64(rays).reshape(-1, 6)
+        n = len(rays)
+        off = np.zeros(n + 1, np.int64)
+        total = C.c_int64(0)
+        cap = cap if cap is not None else n * 64
+        m = np.zeros(max(cap, 1), np.int32)
+        ds = np.zeros(max(cap, 1), np.float64)
+        st = lib().skg_trace_paths(self._h, _p(rays, f64p), C.c_int64(n), C.c_int64(cap), _p(off, i64p),
+                                   _p(m, i32p), _p(ds, f64p), C.byref(total))
+        if st == -3:  # capacity: retry with the exact size
+            cap = total.value
+            m = np.zeros(max(cap, 1), np.int32)
+            ds = np.zeros(max(cap, 1), np.float64)
+            st = lib().skg_trace_paths(self._h, _p(rays, f64p), C.c_int64(n), C.c_int64(cap), _p(off, i64p),
+                                       _p(m, i32p), _p(ds, f64p), C.byref(total))
+        self._chk(st)
+        t = total.value
+        return off, m[:t].copy(), ds[:t].copy()
+
+    def trace_tau(self, rays, ell=0, L=1.0, fp_mode=SKG_FP64, toff=None):
+        rays = _f64(rays).reshape(-1, 6)
+        n = len(rays)
+        tp = np.zeros(n)
+        to = np.zeros(n)
+        if toff is not None:
+            toff = np.ascontiguousarray(toff, dtype=np.int64)
+            cum = np.zeros(max(int(toff[-1]), 1))
+            st = lib().skg_trace_tau(self._h, _p(rays, f64p), C.c_int64(n), C.c_int32(ell), C.c_double(L),
+                                     C.c_int32(fp_mode), _p(tp, f64p), _p(to, f64p), _p(toff, i64p), _p(cum, f64p))
+            self._chk(st)
+            return tp, to, cum[:int(toff[-1])]
+        st = lib().skg_trace_tau(self._h, _p(rays, f64p), C.c_int64(n), C.c_int32(ell), C.c_double(L),
+                                 C.c_int32(fp_mode), _p(tp, f64p), _p(to, f64p), None, None)
+        self._chk(st)
+        return tp, to
+
+    # ---- instruments
+    def frame_add(self, inclination, azimuth, roll, nx, ny, fovx, fovy, cx=0.0, cy=0.0):
+        idx = self._chk(lib().skg_frame_add(self._h, C.c_double(inclination), C.c_double(azimuth), C.c_double(roll),
+                                            C.c_int32(nx), C.c_int32(ny), C.c_double(fovx), C.c_double(fovy),
+                                            C.c_double(cx), C.c_double(cy)))
+        self.frames.append((nx, ny))
+        return idx
+
+    def frame_remove_all(self):
+        self._chk(lib().skg_frame_remove_all(self._h))
+        self.frames = []
+
+    def frame_clear(self):
+        self._chk(lib().skg_frame_clear(self._h))
+
+    def frame_read(self, index):
+        nx, ny = self.frames[index]
+        out = np.zeros(self.nlambda * nx * ny)
+        self._chk(lib().skg_frame_read(self._h, C.c_int32(index), _p(out, f64p), C.c_int64(len(out))))
+        return out.reshape(self.nlambda, ny, nx)
+
+    # ---- source, options, life cycle
+    def source_point(self, pos, lpack):
+        lp = _f64(lpack)
+        self._chk(lib().skg_source_point(self._h, C.c_double(pos[0]), C.c_double(pos[1]), C.c_double(pos[2]),
+                                         _p(lp, f64p)))
+
+    def options(self, path_length_bias=0.5, min_weight_reduction=1e4, min_scatt_events=0, fp_mode=SKG_FP64, seed=1):
+        self._chk(lib().skg_options(self._h, C.c_double(path_length_bias), C.c_double(min_weight_reduction),
+                                    C.c_int32(min_scatt_events), C.c_int32(fp_mode), C.c_uint64(seed)))
+
+    def tune(self, workgroups=0, threads=256, threshold=16):
+        self._chk(lib().skg_tune(self._h, C.c_int32(workgroups), C.c_int32(threads), C.c_int32(threshold)))
+
+    def run_packets(self, first, count):
+        self._chk(lib().skg_run_packets(self._h, C.c_uint64(first), C.c_uint64(count)))
+
+    def run_packets_async(self, first, count):
+        self._chk(lib().skg_run_packets_async(self._h, C.c_uint64(first), C.c_uint64(count)))
+
+    def sync(self):
+        self._chk(lib().skg_sync(self._h))
+
+    def counters(self):
+        out = np.zeros(8, np.int64)
+        self._chk(lib().skg_counters(self._h, _p(out, i64p)))
+        return out
+
+    def last_kernel_ms(self):
+        return float(lib().skg_last_kernel_ms(self._h))

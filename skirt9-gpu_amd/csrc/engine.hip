@@ -1,0 +1,1185 @@
+// MI355X photon-packet engine: HIP kernels and the C-ABI declared in include/skirt_mi355x.h.
+//
+// Kernels
+//   trace_count / trace_write   path service: raw (m, ds) segments per ray (SpatialGrid seam)
+//   trace_tau                   total / observer / cumulative optical depths per ray
+//   lifecycle                   persistent forced-scattering life cycle with peel-off
+//
+// The life-cycle kernel is persistent: each lane owns one photon packet at a time and runs
+// it as a small state machine (peel-off traversal, forced path first pass, second pass to the
+// interaction point, scattering).  Every loop iteration advances each busy lane by one cell
+// crossing; lanes whose traversal ended wait until enough lanes of the wavefront are waiting
+// (the transition threshold), then the wave runs one transition round together, so the
+// divergent event code is amortised over many lanes.  Lanes whose packet terminated refill
+// from a global history counter with one atomic per wavefront (ballot + mbcnt prefix), which
+// stream-compacts packets of very uneven path length without a separate compaction pass.
+// Peel-off contributions go to the frame with native FP64 atomics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/skirt_mi355x.h"
+#include "lifecycle.hpp"
+#include "traverse.hpp"
+
+namespace skg
+{
+    // ------------------------------------------------------------------ path service kernels
+    template<int GT> __global__ void trace_count(GridView g, const double* __restrict__ rays, int64_t n, int64_t* counts)
+    {
+        int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+        if (t >= n) return;
+        const double* r = rays + 6 * t;
+        Gen<GT> gen;
+        gen.start(r[0], r[1], r[2], r[3], r[4], r[5]);
+        int m;
+        double ds;
+        int64_t c = 0, cs = 0;
+        while (gen.next(g, m, ds))
+        {
+            ++c;
+            if (ds > 0.) ++cs;
+        }
+        counts[2 * t] = c;
+        counts[2 * t + 1] = cs;
+    }
+
+    template<int GT>
+    __global__ void trace_write(GridView g, const double* __restrict__ rays, int64_t n, const int64_t* off, int32_t* mo,
+                                double* dso)
+    {
+        int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+        if (t >= n) return;
+        const double* r = rays + 6 * t;
+        Gen<GT> gen;
+        gen.start(r[0], r[1], r[2], r[3], r[4], r[5]);
+        int m;
+        double ds;
+        int64_t o = off[t];
+        while (gen.next(g, m, ds))
+        {
+            mo[o] = m;
+            dso[o] = ds;
+            ++o;
+        }
+    }
+
+    // compensated (Kahan) FP32 accumulator used in SKG_FP32_KAHAN mode
+    struct Kahan32
+    {
+        float sum, comp;
+        __device__ __forceinline__ void reset()
+        {
+            sum = 0.f;
+            comp = 0.f;
+        }
+        __device__ __forceinline__ void add(float x)
+        {
+            float y = x - comp;
+            float t = sum + y;
+            comp = (t - sum) - y;
+            sum = t;
+        }
+    };
+
+    template<int GT, int FP>
+    __global__ void trace_tau(GridView g, const double* __restrict__ dens, const float* __restrict__ dens32,
+                              double section, double taumax, const double* __restrict__ rays, int64_t n,
+                              double* tau_path, double* tau_obs, const int64_t* toff, double* tau_cum)
+    {
+        int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+        if (t >= n) return;
+        const double* r = rays + 6 * t;
+        Gen<GT> gen;
+        int m;
+        double ds;
+        // complete path, cumulative tau over stored (ds > 0) segments
+        gen.start(r[0], r[1], r[2], r[3], r[4], r[5]);
+        double tau = 0.;
+        Kahan32 k32;
+        k32.reset();
+        const float s32 = static_cast<float>(section);
+        int64_t o = toff ? toff[t] : 0;
+        const int64_t oend = toff ? toff[t + 1] : 0;
+        while (gen.next(g, m, ds))
+        {
+            if (ds > 0.)
+            {
+                if (m >= 0)
+                {
+                    if (FP == SKG_FP64)
+                        tau += section * dens[m] * ds;
+                    else
+                        k32.add(s32 * dens32[m] * static_cast<float>(ds));
+                }
+                if (tau_cum && o < oend) tau_cum[o++] = (FP == SKG_FP64) ? tau : static_cast<double>(k32.sum);
+            }
+        }
+        tau_path[t] = (FP == SKG_FP64) ? tau : static_cast<double>(k32.sum);
+        // observer optical depth with early-out (every segment from next() counts)
+        gen.start(r[0], r[1], r[2], r[3], r[4], r[5]);
+        tau = 0.;
+        k32.reset();
+        double res = 0.;
+        bool inf = false;
+        while (gen.next(g, m, ds))
+        {
+            if (m >= 0)
+            {
+                if (FP == SKG_FP64)
+                {
+                    tau += section * dens[m] * ds;
+                    if (tau >= taumax)
+                    {
+                        inf = true;
+                        break;
+                    }
+                }
+                else
+                {
+                    k32.add(s32 * dens32[m] * static_cast<float>(ds));
+                    if (static_cast<double>(k32.sum) >= taumax)
+                    {
+                        inf = true;
+                        break;
+                    }
+                }
+            }
+        }
+        res = (FP == SKG_FP64) ? tau : static_cast<double>(k32.sum);
+        tau_obs[t] = inf ? INFINITY : res;
+    }
+
+    // ------------------------------------------------------------------ the life cycle
+    struct SimParams
+    {
+        GridView grid;
+        const double* dens;
+        const float* dens32;
+        int nlambda;
+        int nframes;
+        const double* lambda;
+        const double* sext;
+        const double* ssca;
+        const double* gpar;
+        const double* lpack;
+        double src[3];
+        double xi, min_weight_reduction;
+        int min_scatt;
+        int threshold;
+        uint64_t seed;
+        uint64_t first, count;
+        unsigned long long* queue;     // next history offset (work queue head)
+        unsigned long long* counters;  // 8 event counters
+        double* frame;
+        const FrameGeom* frames;  // device copy of the instrument geometry
+    };
+
+    enum Phase : int
+    {
+        kFetch = 0,
+        kPeel = 1,
+        kFwd1 = 2,
+        kFwd2 = 3,
+        kDone = 4
+    };
+
+    constexpr int kMaxScatterings = 1 << 20;  // guard against non-terminating histories (NaN weights)
+    constexpr int kMaxSteps = 1 << 26;        // guard against non-terminating traversals
+
+    template<int GT, int FP> struct Lane
+    {
+        // packet
+        Philox rng;
+        double rx, ry, rz, kx, ky, kz;
+        double W, lam, section, Lthr;
+        float section32;
+        int ell, nscatt;
+        // traversal
+        Gen<GT> gen;
+        int phase;
+        bool ended;
+        int steps;
+        double tau, s, tprev, sprev;  // FP64 sums
+        Kahan32 tau32, s32;           // FP32 sums
+        int lastm;
+        // forced propagation
+        double taupath, tint;
+        int mint;
+        double sint;
+        // peel-off
+        int f;
+        bool scat;
+        bool haveTau;
+        double tauCached, Wp, taumax;
+        int pix;
+        // event counters of this lane (flushed to the workgroup's LDS counters)
+        unsigned int* lds;
+
+        __device__ __forceinline__ double tau_value() const
+        {
+            return FP == SKG_FP64 ? tau : static_cast<double>(tau32.sum);
+        }
+        __device__ __forceinline__ double s_value() const
+        {
+            return FP == SKG_FP64 ? s : static_cast<double>(s32.sum);
+        }
+
+        __device__ __forceinline__ void start_traversal(double dx, double dy, double dz, int ph)
+        {
+            gen.start(rx, ry, rz, dx, dy, dz);
+            phase = ph;
+            ended = false;
+            steps = 0;
+            tau = 0.;
+            s = 0.;
+            tprev = 0.;
+            sprev = 0.;
+            tau32.reset();
+            s32.reset();
+            lastm = -1;
+        }
+
+        // MonteCarloSimulation::performLifeCycle launch + SourceSystem::launch
+        __device__ __forceinline__ void launch(const SimParams& p, uint64_t h)
+        {
+            rng.init(p.seed, h);
+            ell = static_cast<int>(rng.uniform() * p.nlambda);
+            if (ell >= p.nlambda) ell = p.nlambda - 1;
+            lam = p.lambda[ell];
+            section = p.sext[ell];
+            section32 = static_cast<float>(section);
+            W = p.lpack[ell] * lam;
+            rx = p.src[0];
+            ry = p.src[1];
+            rz = p.src[2];
+            isotropic(rng, kx, ky, kz);
+            nscatt = 0;
+            if (!(W / lam > 0))
+            {
+                finish();
+                return;
+            }
+            Lthr = (W / lam) / p.min_weight_reduction;
+            begin_peel(p, false);
+        }
+
+        __device__ __forceinline__ void finish()
+        {
+            atomicAdd(lds + 0, 1u);
+            phase = kFetch;
+            ended = false;
+        }
+
+        __device__ __forceinline__ void deposit(const SimParams& p, const FrameGeom& fg, double L, double t)
+        {
+            double v = L * exp(-t);
+            unsafeAtomicAdd(p.frame + fg.offset + pix + static_cast<int64_t>(ell) * fg.nx * fg.ny, v);
+            atomicAdd(lds + 5, 1u);
+        }
+
+        __device__ __forceinline__ void begin_peel(const SimParams& p, bool scattering)
+        {
+            scat = scattering;
+            f = -1;
+            next_peel(p);
+        }
+
+        // peel-off toward each instrument in turn (MonteCarloSimulation.cpp:698-715, :2034-2092;
+        // FluxRecorder::detect skips the optical depth for packets outside the frame, :375)
+        __device__ __forceinline__ void next_peel(const SimParams& p)
+        {
+            while (++f < p.nframes)
+            {
+                const FrameGeom& fg = p.frames[f];
+                if (!fg.same_as_prev)
+                {
+                    haveTau = false;
+                    Wp = W;
+                    if (scat)
+                    {
+                        double c = kx * fg.kobs[0] + ky * fg.kobs[1] + kz * fg.kobs[2];
+                        Wp = W * peel_phase(p.gpar[ell], c);
+                    }
+                }
+                pix = fg.pixel(rx, ry, rz);
+                if (pix < 0) continue;
+                double L = Wp / lam;
+                if (haveTau)
+                {
+                    deposit(p, fg, L, tauCached);
+                    continue;
+                }
+                if (L <= 0)
+                {
+                    haveTau = true;
+                    tauCached = INFINITY;
+                    continue;
+                }
+                taumax = log(L) + 745;
+                atomicAdd(lds + 1, 1u);
+                start_traversal(fg.kobs[0], fg.kobs[1], fg.kobs[2], kPeel);
+                return;
+            }
+            // all instruments done: scatter (after a scattering peel-off) and start a forced path
+            if (scat)
+            {
+                scatter_hg(rng, p.gpar[ell], kx, ky, kz);
+                nscatt++;
+                atomicAdd(lds + 3, 1u);
+            }
+            atomicAdd(lds + 2, 1u);
+            start_traversal(kx, ky, kz, kFwd1);
+        }
+
+        // one cell crossing of the current traversal
+        __device__ __forceinline__ void step(const SimParams& p)
+        {
+            int m;
+            double ds;
+            if (!gen.next(p.grid, m, ds) || ++steps > kMaxSteps)
+            {
+                ended = true;
+                atomicAdd(lds + 4, static_cast<unsigned int>(steps));
+                return;
+            }
+            if (phase == kPeel)
+            {
+                if (m >= 0)
+                {
+                    if (FP == SKG_FP64)
+                    {
+                        tau += section * p.dens[m] * ds;
+                        if (tau >= taumax)
+                        {
+                            tau = INFINITY;
+                            ended = true;
+                            atomicAdd(lds + 4, static_cast<unsigned int>(steps));
+                        }
+                    }
+                    else
+                    {
+                        tau32.add(section32 * p.dens32[m] * static_cast<float>(ds));
+                        if (static_cast<double>(tau32.sum) >= taumax)
+                        {
+                            tau32.sum = INFINITY;
+                            ended = true;
+                            atomicAdd(lds + 4, static_cast<unsigned int>(steps));
+                        }
+                    }
+                }
+            }
+            else if (ds > 0.)
+            {
+                if (FP == SKG_FP64)
+                {
+                    tprev = tau;
+                    sprev = s;
+                    s += ds;
+                    if (m >= 0) tau += section * p.dens[m] * ds;
+                }
+                else
+                {
+                    tprev = static_cast<double>(tau32.sum);
+                    sprev = static_cast<double>(s32.sum);
+                    s32.add(static_cast<float>(ds));
+                    if (m >= 0) tau32.add(section32 * p.dens32[m] * static_cast<float>(ds));
+                }
+                lastm = m;
+                // second pass: stop at the first segment whose exit optical depth exceeds tau_int
+                if (phase == kFwd2 && tint < tau_value())
+                {
+                    mint = m;
+                    double tv = tau_value(), sv = s_value();
+                    sint = sprev + ((tint - tprev) / (tv - tprev)) * (sv - sprev);
+                    ended = true;
+                    atomicAdd(lds + 4, static_cast<unsigned int>(steps));
+                }
+            }
+        }
+
+        // event handling once a traversal ended
+        __device__ __forceinline__ void transition(const SimParams& p)
+        {
+            if (phase == kPeel)
+            {
+                haveTau = true;
+                tauCached = tau_value();
+                const FrameGeom& fg = p.frames[f];
+                deposit(p, fg, Wp / lam, tauCached);
+                next_peel(p);
+            }
+            else if (phase == kFwd1)
+            {
+                // MonteCarloSimulation::simulateForcedPropagation (:1492-1544)
+                taupath = tau_value();
+                if (taupath <= 0.)
+                {
+                    W *= 0.;
+                    finish();
+                    return;
+                }
+                double xi = p.xi;
+                if (xi == 0.)
+                    tint = expon_cutoff(rng, taupath);
+                else
+                {
+                    tint = rng.uniform() < xi ? rng.uniform() * taupath : expon_cutoff(rng, taupath);
+                    double pp = -exp(-tint) / expm1(-taupath);
+                    double q = (1.0 - xi) * pp + xi / taupath;
+                    W *= pp / q;
+                }
+                mint = -1;
+                atomicAdd(lds + 6, 1u);
+                start_traversal(kx, ky, kz, kFwd2);
+            }
+            else if (phase == kFwd2)
+            {
+                // SpatialGridPath::findInteractionPoint (SpatialGridPath.cpp:161-203): beyond the
+                // last segment, use the last segment's cell and exit distance
+                if (mint < 0 && !(tint < tau_value()))
+                {
+                    mint = lastm;
+                    sint = s_value();
+                }
+                int m = mint;
+                double n = m >= 0 ? p.dens[m] : 0.;
+                double ksca = n > 0. ? n * p.ssca[ell] : 0.;
+                double kext = n > 0. ? n * p.sext[ell] : 0.;
+                double albedo = kext > 0. ? ksca / kext : 0.;
+                W *= -expm1(-taupath) * albedo;
+                rx += sint * kx;
+                ry += sint * ky;
+                rz += sint * kz;
+                double L = W / lam;
+                if (L <= 0 || (L <= Lthr && nscatt >= p.min_scatt) || nscatt >= kMaxScatterings)
+                {
+                    finish();
+                    return;
+                }
+                begin_peel(p, true);
+            }
+        }
+    };
+
+    __device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+    template<int GT, int FP> __global__ void __launch_bounds__(256, 4) lifecycle(SimParams p)
+    {
+        __shared__ unsigned int wgcnt[8];
+        if (threadIdx.x < 8) wgcnt[threadIdx.x] = 0;
+        __syncthreads();
+        Lane<GT, FP> L;
+        L.phase = kFetch;
+        L.ended = false;
+        L.lds = wgcnt;
+        const int lane = lane_id();
+        while (true)
+        {
+            // advance every busy lane by one cell crossing
+            bool busy = (L.phase == kPeel || L.phase == kFwd1 || L.phase == kFwd2) && !L.ended;
+            if (busy) L.step(p);
+
+            // wave-wide transition round once enough lanes wait
+            unsigned long long waiting = __ballot(L.ended || L.phase == kFetch);
+            unsigned long long active = __ballot(L.phase != kDone);
+            if (active == 0) break;
+            if (__popcll(waiting) >= p.threshold || waiting == active)
+            {
+                if (L.ended) L.transition(p);
+                unsigned long long need = __ballot(L.phase == kFetch);
+                if (need)
+                {
+                    int cntNeed = __popcll(need);
+                    int leader = __ffsll(static_cast<long long>(need)) - 1;
+                    unsigned long long base = 0;
+                    if (lane == leader) base = atomicAdd(p.queue, static_cast<unsigned long long>(cntNeed));
+                    base = __shfl(base, leader);
+                    if (L.phase == kFetch)
+                    {
+                        unsigned long long below = need & ((1ull << lane) - 1ull);
+                        unsigned long long mine = base + __popcll(below);
+                        if (mine < p.count)
+                            L.launch(p, p.first + mine);
+                        else
+                            L.phase = kDone;
+                    }
+                }
+            }
+        }
+        // workgroup event counters to the global counters
+        __syncthreads();
+        if (threadIdx.x < 8 && wgcnt[threadIdx.x])
+            atomicAdd(p.counters + threadIdx.x, static_cast<unsigned long long>(wgcnt[threadIdx.x]));
+    }
+}
+
+// ==================================================================== host side
+using namespace skg;
+
+struct skg_engine
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    std::mutex mu;
+    int cus = 0;
+    // grid
+    int gridType = -1;
+    GridView grid{};
+    int64_t ncells = 0;
+    std::vector<void*> gridBufs;
+    // medium
+    double* dens = nullptr;
+    float* dens32 = nullptr;
+    int64_t densCells = 0;
+    // dust
+    int nlambda = 0;
+    double *lambda = nullptr, *sext = nullptr, *ssca = nullptr, *gpar = nullptr, *lpack = nullptr;
+    std::vector<double> hLambda, hSext, hSsca, hG, hLpack;
+    int lpackCount = 0;
+    // source / options
+    double src[3] = {0, 0, 0};
+    bool haveSource = false;
+    double xi = 0.5, mwr = 1e4;
+    int minScatt = 0;
+    int fpMode = SKG_FP64;
+    uint64_t seed = 0;
+    // instruments
+    struct FrameDesc
+    {
+        double inc, az, roll;
+        int nx, ny;
+        double fovx, fovy, cx, cy;
+    };
+    std::vector<FrameDesc> frames;
+    double* frameBuf = nullptr;
+    FrameGeom* frameGeom = nullptr;
+    int64_t frameSize = 0;
+    bool frameDirty = true;
+    unsigned long long* queue = nullptr;
+    unsigned long long* counters = nullptr;
+    // launch shape
+    int workgroups = 0, threads = 256, threshold = 16;
+    float lastMs = 0.f;
+};
+
+namespace
+{
+    thread_local std::string t_createError;
+
+    int fail(skg_engine* e, int code, const std::string& msg)
+    {
+        if (e) e->err = msg;
+        return code;
+    }
+
+#define HIPCHK(e, call)                                                                                   \
+    do                                                                                                    \
+    {                                                                                                     \
+        hipError_t _st = (call);                                                                          \
+        if (_st != hipSuccess)                                                                            \
+            return fail((e), SKG_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(_st));         \
+    } while (0)
+
+    template<typename T> int upload(skg_engine* e, T** dst, const T* src, size_t n)
+    {
+        if (*dst)
+        {
+            hipFree(*dst);
+            *dst = nullptr;
+        }
+        if (n == 0) return SKG_OK;
+        HIPCHK(e, hipMalloc(reinterpret_cast<void**>(dst), n * sizeof(T)));
+        HIPCHK(e, hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+        return SKG_OK;
+    }
+
+    void freeGrid(skg_engine* e)
+    {
+        for (void* p : e->gridBufs) hipFree(p);
+        e->gridBufs.clear();
+        e->gridType = -1;
+        e->ncells = 0;
+    }
+
+    template<typename T> int gridUpload(skg_engine* e, const T* src, size_t n, const T** out)
+    {
+        T* d = nullptr;
+        int st = upload(e, &d, src, n);
+        if (st) return st;
+        e->gridBufs.push_back(d);
+        *out = d;
+        return SKG_OK;
+    }
+
+    void fillFrameGeom(const skg_engine* e, std::vector<FrameGeom>& out)
+    {
+        int64_t off = 0;
+        out.assign(e->frames.size(), FrameGeom{});
+        for (size_t f = 0; f < e->frames.size(); ++f)
+        {
+            const auto& d = e->frames[f];
+            FrameGeom& g = out[f];
+            g.costheta = std::cos(d.inc);
+            g.sintheta = std::sin(d.inc);
+            g.cosphi = std::cos(d.az);
+            g.sinphi = std::sin(d.az);
+            g.cosomega = std::cos(d.roll);
+            g.sinomega = std::sin(d.roll);
+            g.nx = d.nx;
+            g.ny = d.ny;
+            g.xpmin = d.cx - 0.5 * d.fovx;
+            g.xpsiz = d.fovx / d.nx;
+            g.ypmin = d.cy - 0.5 * d.fovy;
+            g.ypsiz = d.fovy / d.ny;
+            // Direction(theta, phi), Direction.cpp:11-36
+            const double eps = 1e-8;
+            if (d.inc <= eps)
+            {
+                g.kobs[0] = 0;
+                g.kobs[1] = 0;
+                g.kobs[2] = 1;
+            }
+            else if (d.inc >= M_PI - eps)
+            {
+                g.kobs[0] = 0;
+                g.kobs[1] = 0;
+                g.kobs[2] = -1;
+            }
+            else
+            {
+                double st = std::sin(d.inc);
+                g.kobs[0] = st * std::cos(d.az);
+                g.kobs[1] = st * std::sin(d.az);
+                g.kobs[2] = std::cos(d.inc);
+            }
+            g.same_as_prev = f > 0 && d.inc == e->frames[f - 1].inc && d.az == e->frames[f - 1].az
+                             && d.roll == e->frames[f - 1].roll;
+            g.offset = off;
+            off += static_cast<int64_t>(d.nx) * d.ny * e->nlambda;
+        }
+    }
+
+    int ensureFrames(skg_engine* e)
+    {
+        if (!e->frameDirty) return SKG_OK;
+        int64_t total = 0;
+        for (auto& f : e->frames) total += static_cast<int64_t>(f.nx) * f.ny * std::max(1, e->nlambda);
+        if (e->frameBuf) hipFree(e->frameBuf);
+        e->frameBuf = nullptr;
+        e->frameSize = total;
+        if (total) HIPCHK(e, hipMalloc(&e->frameBuf, total * sizeof(double)));
+        if (total) HIPCHK(e, hipMemset(e->frameBuf, 0, total * sizeof(double)));
+        HIPCHK(e, hipMemset(e->counters, 0, 8 * sizeof(unsigned long long)));
+        std::vector<FrameGeom> geo;
+        fillFrameGeom(e, geo);
+        if (!geo.empty()) HIPCHK(e, hipMemcpy(e->frameGeom, geo.data(), geo.size() * sizeof(FrameGeom), hipMemcpyHostToDevice));
+        e->frameDirty = false;
+        return SKG_OK;
+    }
+
+    template<int GT, int FP> hipError_t launchLifecycle(const SimParams& p, int blocks, int threads, hipStream_t s)
+    {
+        hipLaunchKernelGGL((lifecycle<GT, FP>), dim3(blocks), dim3(threads), 0, s, p);
+        return hipGetLastError();
+    }
+}
+
+namespace
+{
+    int checkTraceState(skg_engine* e)
+    {
+        if (e->gridType < 0) return fail(e, SKG_ERR_STATE, "no grid uploaded");
+        return SKG_OK;
+    }
+
+    template<typename F> int dispatchGrid(skg_engine* e, F&& f)
+    {
+        return e->gridType == SKG_GRID_CARTESIAN ? f(std::integral_constant<int, kCartesian>()) : f(std::integral_constant<int, kOctree>());
+    }
+}
+
+extern "C" {
+
+int skg_create(int device, skg_engine** out)
+{
+    if (!out)
+    {
+        t_createError = "skg_create: null output pointer";
+        return SKG_ERR_ARGUMENT;
+    }
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n)
+    {
+        t_createError = "skg_create: no HIP device " + std::to_string(device);
+        return SKG_ERR_DEVICE;
+    }
+    auto* e = new skg_engine();
+    e->device = device;
+    hipError_t st = hipSetDevice(device);
+    if (st == hipSuccess) st = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (st == hipSuccess) st = hipEventCreate(&e->ev0);
+    if (st == hipSuccess) st = hipEventCreate(&e->ev1);
+    if (st == hipSuccess) st = hipMalloc(&e->queue, sizeof(unsigned long long));
+    if (st == hipSuccess) st = hipMalloc(&e->counters, 8 * sizeof(unsigned long long));
+    if (st == hipSuccess) st = hipMemset(e->counters, 0, 8 * sizeof(unsigned long long));
+    if (st == hipSuccess) st = hipMalloc(&e->frameGeom, kMaxFrames * sizeof(FrameGeom));
+    hipDeviceProp_t prop;
+    if (st == hipSuccess) st = hipGetDeviceProperties(&prop, device);
+    if (st != hipSuccess)
+    {
+        t_createError = std::string("skg_create: ") + hipGetErrorString(st);
+        delete e;
+        return SKG_ERR_DEVICE;
+    }
+    e->cus = prop.multiProcessorCount;
+    *out = e;
+    return SKG_OK;
+}
+
+void skg_destroy(skg_engine* e)
+{
+    if (!e) return;
+    hipSetDevice(e->device);
+    hipStreamSynchronize(e->stream);
+    freeGrid(e);
+    for (void* p : {static_cast<void*>(e->dens), static_cast<void*>(e->dens32), static_cast<void*>(e->lambda),
+                    static_cast<void*>(e->sext), static_cast<void*>(e->ssca), static_cast<void*>(e->gpar),
+                    static_cast<void*>(e->lpack), static_cast<void*>(e->frameBuf), static_cast<void*>(e->frameGeom), static_cast<void*>(e->queue),
+                    static_cast<void*>(e->counters)})
+        if (p) hipFree(p);
+    hipEventDestroy(e->ev0);
+    hipEventDestroy(e->ev1);
+    hipStreamDestroy(e->stream);
+    delete e;
+}
+
+const char* skg_last_error(const skg_engine* e) { return e ? e->err.c_str() : t_createError.c_str(); }
+
+int skg_grid_cartesian(skg_engine* e, const double* xb, int32_t nx, const double* yb, int32_t ny, const double* zb,
+                       int32_t nz)
+{
+    if (!e || !xb || !yb || !zb || nx < 1 || ny < 1 || nz < 1) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_cartesian: bad arguments");
+    for (auto [b, n] : {std::pair<const double*, int>{xb, nx}, {yb, ny}, {zb, nz}})
+        for (int i = 0; i < n; ++i)
+            if (!(b[i] < b[i + 1])) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_cartesian: borders must increase");
+    if (static_cast<int64_t>(nx) * ny * nz > INT32_MAX) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_cartesian: too many cells");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    freeGrid(e);
+    GridView g{};
+    int st;
+    if ((st = gridUpload(e, xb, nx + 1, &g.xb)) || (st = gridUpload(e, yb, ny + 1, &g.yb))
+        || (st = gridUpload(e, zb, nz + 1, &g.zb)))
+        return st;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.box[0] = xb[0];
+    g.box[1] = yb[0];
+    g.box[2] = zb[0];
+    g.box[3] = xb[nx];
+    g.box[4] = yb[ny];
+    g.box[5] = zb[nz];
+    // Box::diagonal (Box.hpp:125-129); eps as CartesianSpatialGrid.cpp:92
+    double dx = g.box[3] - g.box[0], dy = g.box[4] - g.box[1], dz = g.box[5] - g.box[2];
+    g.eps = 1e-12 * std::sqrt(dx * dx + dy * dy + dz * dz);
+    e->grid = g;
+    e->gridType = SKG_GRID_CARTESIAN;
+    e->ncells = static_cast<int64_t>(nx) * ny * nz;
+    return SKG_OK;
+}
+
+int skg_grid_octree(skg_engine* e, int64_t nnodes, const double* bounds, const int32_t* child_first,
+                    const int32_t* cell_index, const int64_t* nb_off, const int32_t* nb_id)
+{
+    if (!e || nnodes < 1 || !bounds || !child_first || !cell_index || !nb_off || (!nb_id && nb_off[6 * nnodes] > 0))
+        return fail(e, SKG_ERR_ARGUMENT, "skg_grid_octree: bad arguments");
+    if (nnodes > INT32_MAX / 6 || nb_off[6 * nnodes] > INT32_MAX)
+        return fail(e, SKG_ERR_ARGUMENT, "skg_grid_octree: table too large for 32-bit offsets");
+    int64_t ncells = 0;
+    for (int64_t n = 0; n < nnodes; ++n)
+    {
+        int32_t c = child_first[n];
+        if (c >= 0 && (c + 8 > nnodes || c <= n)) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_octree: bad child index");
+        if (c < 0)
+        {
+            if (cell_index[n] < 0) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_octree: leaf without cell index");
+            ncells = std::max<int64_t>(ncells, cell_index[n] + 1);
+        }
+    }
+    const int64_t nnb = nb_off[6 * nnodes];
+    for (int64_t i = 0; i < nnb; ++i)
+        if (nb_id[i] < 0 || nb_id[i] >= nnodes) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_octree: bad neighbour id");
+    std::vector<int32_t> off32(6 * nnodes + 1);
+    for (int64_t i = 0; i <= 6 * nnodes; ++i)
+    {
+        if (i && nb_off[i] < nb_off[i - 1]) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_octree: offsets must not decrease");
+        off32[i] = static_cast<int32_t>(nb_off[i]);
+    }
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    freeGrid(e);
+    GridView g{};
+    int st;
+    if ((st = gridUpload(e, bounds, 6 * nnodes, &g.bounds)) || (st = gridUpload(e, child_first, nnodes, &g.child_first))
+        || (st = gridUpload(e, cell_index, nnodes, &g.cell_of))
+        || (st = gridUpload(e, off32.data(), off32.size(), &g.nb_off)))
+        return st;
+    if (nnb)
+    {
+        if ((st = gridUpload(e, nb_id, nnb, &g.nb_id))) return st;
+    }
+    for (int q = 0; q < 6; ++q) g.box[q] = bounds[q];
+    double dx = g.box[3] - g.box[0], dy = g.box[4] - g.box[1], dz = g.box[5] - g.box[2];
+    g.eps = 1e-12 * std::sqrt(dx * dx + dy * dy + dz * dz);  // TreeSpatialGrid.cpp:28
+    e->grid = g;
+    e->gridType = SKG_GRID_OCTREE;
+    e->ncells = ncells;
+    return SKG_OK;
+}
+
+int64_t skg_num_cells(const skg_engine* e) { return e ? e->ncells : 0; }
+
+int skg_medium(skg_engine* e, int64_t ncells, const double* number_density)
+{
+    if (!e || ncells < 1 || !number_density) return fail(e, SKG_ERR_ARGUMENT, "skg_medium: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    std::vector<float> d32(number_density, number_density + ncells);
+    int st;
+    if ((st = upload(e, &e->dens, number_density, ncells)) || (st = upload(e, &e->dens32, d32.data(), ncells))) return st;
+    e->densCells = ncells;
+    return SKG_OK;
+}
+
+int skg_dust(skg_engine* e, int32_t nlambda, const double* lambda, const double* sext, const double* ssca,
+             const double* g)
+{
+    if (!e || nlambda < 1 || !lambda || !sext || !ssca || !g) return fail(e, SKG_ERR_ARGUMENT, "skg_dust: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    e->hLambda.assign(lambda, lambda + nlambda);
+    e->hSext.assign(sext, sext + nlambda);
+    e->hSsca.assign(ssca, ssca + nlambda);
+    e->hG.assign(g, g + nlambda);
+    int st;
+    if ((st = upload(e, &e->lambda, lambda, nlambda)) || (st = upload(e, &e->sext, sext, nlambda))
+        || (st = upload(e, &e->ssca, ssca, nlambda)) || (st = upload(e, &e->gpar, g, nlambda)))
+        return st;
+    if (e->nlambda != nlambda) e->frameDirty = true;
+    e->nlambda = nlambda;
+    return SKG_OK;
+}
+
+
+int skg_trace_paths(skg_engine* e, const double* rays, int64_t n, int64_t cap, int64_t* off, int32_t* m, double* ds,
+                    int64_t* total)
+{
+    if (!e || n < 0 || !off || (n && !rays)) return fail(e, SKG_ERR_ARGUMENT, "skg_trace_paths: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    int st = checkTraceState(e);
+    if (st) return st;
+    off[0] = 0;
+    if (total) *total = 0;
+    if (n == 0) return SKG_OK;
+    double* dRays = nullptr;
+    int64_t* dCnt = nullptr;
+    HIPCHK(e, hipMalloc(&dRays, n * 6 * sizeof(double)));
+    HIPCHK(e, hipMalloc(&dCnt, n * 2 * sizeof(int64_t)));
+    HIPCHK(e, hipMemcpy(dRays, rays, n * 6 * sizeof(double), hipMemcpyHostToDevice));
+    int blocks = static_cast<int>((n + 255) / 256);
+    GridView g = e->grid;
+    dispatchGrid(e, [&](auto gt) {
+        hipLaunchKernelGGL((trace_count<decltype(gt)::value>), dim3(blocks), dim3(256), 0, e->stream, g, dRays, n, dCnt);
+        return 0;
+    });
+    std::vector<int64_t> cnt(2 * n);
+    HIPCHK(e, hipMemcpyAsync(cnt.data(), dCnt, n * 2 * sizeof(int64_t), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    std::vector<int64_t> hoff(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i) hoff[i + 1] = hoff[i] + cnt[2 * i];
+    int64_t t = hoff[n];
+    if (total) *total = t;
+    if (t > cap || (t && (!m || !ds)))
+    {
+        hipFree(dRays);
+        hipFree(dCnt);
+        return fail(e, SKG_ERR_CAPACITY, "skg_trace_paths: capacity " + std::to_string(cap) + " < " + std::to_string(t));
+    }
+    int32_t* dM = nullptr;
+    double* dDs = nullptr;
+    int64_t* dOff = nullptr;
+    if (t)
+    {
+        HIPCHK(e, hipMalloc(&dM, t * sizeof(int32_t)));
+        HIPCHK(e, hipMalloc(&dDs, t * sizeof(double)));
+    }
+    HIPCHK(e, hipMalloc(&dOff, (n + 1) * sizeof(int64_t)));
+    HIPCHK(e, hipMemcpy(dOff, hoff.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    if (t)
+    {
+        dispatchGrid(e, [&](auto gt) {
+            hipLaunchKernelGGL((trace_write<decltype(gt)::value>), dim3(blocks), dim3(256), 0, e->stream, g, dRays, n, dOff,
+                               dM, dDs);
+            return 0;
+        });
+        HIPCHK(e, hipGetLastError());
+        HIPCHK(e, hipMemcpyAsync(m, dM, t * sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(e, hipMemcpyAsync(ds, dDs, t * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+    }
+    std::memcpy(off, hoff.data(), (n + 1) * sizeof(int64_t));
+    hipFree(dRays);
+    hipFree(dCnt);
+    hipFree(dOff);
+    if (dM) hipFree(dM);
+    if (dDs) hipFree(dDs);
+    return SKG_OK;
+}
+
+int skg_trace_tau(skg_engine* e, const double* rays, int64_t n, int32_t ell, double L, int32_t fp_mode,
+                  double* tau_path, double* tau_obs, const int64_t* toff, double* tau_cum)
+{
+    if (!e || n < 0 || (n && (!rays || !tau_path || !tau_obs)) || (toff && !tau_cum) || !(fp_mode == SKG_FP64 || fp_mode == SKG_FP32_KAHAN))
+        return fail(e, SKG_ERR_ARGUMENT, "skg_trace_tau: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    int st = checkTraceState(e);
+    if (st) return st;
+    if (!e->dens || e->densCells < e->ncells) return fail(e, SKG_ERR_STATE, "skg_trace_tau: no medium for this grid");
+    if (ell < 0 || ell >= e->nlambda) return fail(e, SKG_ERR_ARGUMENT, "skg_trace_tau: wavelength bin out of range");
+    if (n == 0) return SKG_OK;
+    int64_t ncum = toff ? toff[n] : 0;
+    double *dRays = nullptr, *dTp = nullptr, *dTo = nullptr, *dCum = nullptr;
+    int64_t* dToff = nullptr;
+    HIPCHK(e, hipMalloc(&dRays, n * 6 * sizeof(double)));
+    HIPCHK(e, hipMalloc(&dTp, n * sizeof(double)));
+    HIPCHK(e, hipMalloc(&dTo, n * sizeof(double)));
+    HIPCHK(e, hipMemcpy(dRays, rays, n * 6 * sizeof(double), hipMemcpyHostToDevice));
+    if (toff)
+    {
+        HIPCHK(e, hipMalloc(&dToff, (n + 1) * sizeof(int64_t)));
+        HIPCHK(e, hipMemcpy(dToff, toff, (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+        if (ncum) HIPCHK(e, hipMalloc(&dCum, ncum * sizeof(double)));
+    }
+    double section = e->hSext[ell];
+    double taumax = L > 0 ? std::log(L) + 745 : -INFINITY;
+    int blocks = static_cast<int>((n + 255) / 256);
+    GridView g = e->grid;
+    dispatchGrid(e, [&](auto gt) {
+        if (fp_mode == SKG_FP64)
+            hipLaunchKernelGGL((trace_tau<decltype(gt)::value, SKG_FP64>), dim3(blocks), dim3(256), 0, e->stream, g, e->dens,
+                               e->dens32, section, taumax, dRays, n, dTp, dTo, dToff, dCum);
+        else
+            hipLaunchKernelGGL((trace_tau<decltype(gt)::value, SKG_FP32_KAHAN>), dim3(blocks), dim3(256), 0, e->stream, g,
+                               e->dens, e->dens32, section, taumax, dRays, n, dTp, dTo, dToff, dCum);
+        return 0;
+    });
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipMemcpyAsync(tau_path, dTp, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipMemcpyAsync(tau_obs, dTo, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    if (ncum) HIPCHK(e, hipMemcpyAsync(tau_cum, dCum, ncum * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    if (L <= 0)
+        for (int64_t i = 0; i < n; ++i) tau_obs[i] = INFINITY;
+    hipFree(dRays);
+    hipFree(dTp);
+    hipFree(dTo);
+    if (dToff) hipFree(dToff);
+    if (dCum) hipFree(dCum);
+    return SKG_OK;
+}
+
+int skg_frame_add(skg_engine* e, double inclination, double azimuth, double roll, int32_t nx, int32_t ny, double fovx,
+                  double fovy, double cx, double cy)
+{
+    if (!e || nx < 1 || ny < 1 || !(fovx > 0) || !(fovy > 0)) return fail(e, SKG_ERR_ARGUMENT, "skg_frame_add: bad arguments");
+    if (inclination < -1e-8 || inclination > M_PI + 1e-8) return fail(e, SKG_ERR_ARGUMENT, "skg_frame_add: inclination outside [0, pi]");
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (e->frames.size() >= static_cast<size_t>(kMaxFrames)) return fail(e, SKG_ERR_CAPACITY, "skg_frame_add: at most 8 instruments");
+    e->frames.push_back({inclination, azimuth, roll, nx, ny, fovx, fovy, cx, cy});
+    e->frameDirty = true;
+    return static_cast<int>(e->frames.size()) - 1;
+}
+
+int skg_frame_remove_all(skg_engine* e)
+{
+    if (!e) return SKG_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->frames.clear();
+    e->frameDirty = true;
+    return SKG_OK;
+}
+
+int skg_frame_clear(skg_engine* e)
+{
+    if (!e) return SKG_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    int st = ensureFrames(e);
+    if (st) return st;
+    if (e->frameSize) HIPCHK(e, hipMemsetAsync(e->frameBuf, 0, e->frameSize * sizeof(double), e->stream));
+    HIPCHK(e, hipMemsetAsync(e->counters, 0, 8 * sizeof(unsigned long long), e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return SKG_OK;
+}
+
+int skg_frame_read(skg_engine* e, int32_t index, double* out, int64_t count)
+{
+    if (!e || !out || index < 0) return fail(e, SKG_ERR_ARGUMENT, "skg_frame_read: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    if (index >= static_cast<int>(e->frames.size())) return fail(e, SKG_ERR_ARGUMENT, "skg_frame_read: no such instrument");
+    int st = ensureFrames(e);
+    if (st) return st;
+    int64_t off = 0;
+    for (int f = 0; f < index; ++f) off += static_cast<int64_t>(e->frames[f].nx) * e->frames[f].ny * e->nlambda;
+    int64_t sz = static_cast<int64_t>(e->frames[index].nx) * e->frames[index].ny * e->nlambda;
+    if (count < sz) return fail(e, SKG_ERR_CAPACITY, "skg_frame_read: output too small");
+    HIPCHK(e, hipMemcpyAsync(out, e->frameBuf + off, sz * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return SKG_OK;
+}
+
+int skg_source_point(skg_engine* e, double x, double y, double z, const double* lpack)
+{
+    if (!e || !lpack) return fail(e, SKG_ERR_ARGUMENT, "skg_source_point: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    if (e->nlambda < 1) return fail(e, SKG_ERR_STATE, "skg_source_point: upload the dust wavelengths first");
+    e->src[0] = x;
+    e->src[1] = y;
+    e->src[2] = z;
+    e->hLpack.assign(lpack, lpack + e->nlambda);
+    int st = upload(e, &e->lpack, lpack, e->nlambda);
+    if (st) return st;
+    e->lpackCount = e->nlambda;
+    e->haveSource = true;
+    return SKG_OK;
+}
+
+int skg_options(skg_engine* e, double path_length_bias, double min_weight_reduction, int32_t min_scatt_events,
+                int32_t fp_mode, uint64_t seed)
+{
+    if (!e || path_length_bias < 0 || path_length_bias > 1 || !(min_weight_reduction > 0) || min_scatt_events < 0
+        || !(fp_mode == SKG_FP64 || fp_mode == SKG_FP32_KAHAN))
+        return fail(e, SKG_ERR_ARGUMENT, "skg_options: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->xi = path_length_bias;
+    e->mwr = min_weight_reduction;
+    e->minScatt = min_scatt_events;
+    e->fpMode = fp_mode;
+    e->seed = seed;
+    return SKG_OK;
+}
+
+int skg_tune(skg_engine* e, int32_t workgroups, int32_t threads, int32_t transition_threshold)
+{
+    if (!e || workgroups < 0 || threads < 64 || threads > 256 || threads % 64 || transition_threshold < 1
+        || transition_threshold > 64)
+        return fail(e, SKG_ERR_ARGUMENT, "skg_tune: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->workgroups = workgroups;
+    e->threads = threads;
+    e->threshold = transition_threshold;
+    return SKG_OK;
+}
+
+int skg_run_packets_async(skg_engine* e, uint64_t first_history, uint64_t count)
+{
+    if (!e) return SKG_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    if (e->gridType < 0) return fail(e, SKG_ERR_STATE, "skg_run_packets: no grid");
+    if (!e->dens || e->densCells < e->ncells) return fail(e, SKG_ERR_STATE, "skg_run_packets: no medium for this grid");
+    if (e->nlambda < 1 || !e->haveSource) return fail(e, SKG_ERR_STATE, "skg_run_packets: no dust/source");
+    if (e->lpackCount != e->nlambda)
+        return fail(e, SKG_ERR_STATE, "skg_run_packets: source luminosities do not match the dust wavelength bins");
+    int st = ensureFrames(e);
+    if (st) return st;
+    if (count == 0) return SKG_OK;
+    SimParams p{};
+    p.grid = e->grid;
+    p.dens = e->dens;
+    p.dens32 = e->dens32;
+    p.nlambda = e->nlambda;
+    p.lambda = e->lambda;
+    p.sext = e->sext;
+    p.ssca = e->ssca;
+    p.gpar = e->gpar;
+    p.lpack = e->lpack;
+    for (int q = 0; q < 3; ++q) p.src[q] = e->src[q];
+    p.xi = e->xi;
+    p.min_weight_reduction = e->mwr;
+    p.min_scatt = e->minScatt;
+    p.threshold = e->threshold;
+    p.seed = e->seed;
+    p.first = first_history;
+    p.count = count;
+    p.queue = e->queue;
+    p.counters = e->counters;
+    p.frame = e->frameBuf;
+    p.frames = e->frameGeom;
+    p.nframes = static_cast<int>(e->frames.size());
+    HIPCHK(e, hipMemsetAsync(e->queue, 0, sizeof(unsigned long long), e->stream));
+    int threads = e->threads;
+    int blocks = e->workgroups > 0 ? e->workgroups : e->cus * 8;
+    // never more lanes than packets (rounded up to whole workgroups)
+    int64_t needBlocks = static_cast<int64_t>((count + threads - 1) / threads);
+    blocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(blocks, needBlocks)));
+    HIPCHK(e, hipEventRecord(e->ev0, e->stream));
+    hipError_t err;
+    if (e->gridType == SKG_GRID_CARTESIAN)
+        err = e->fpMode == SKG_FP64 ? launchLifecycle<kCartesian, SKG_FP64>(p, blocks, threads, e->stream)
+                                    : launchLifecycle<kCartesian, SKG_FP32_KAHAN>(p, blocks, threads, e->stream);
+    else
+        err = e->fpMode == SKG_FP64 ? launchLifecycle<kOctree, SKG_FP64>(p, blocks, threads, e->stream)
+                                    : launchLifecycle<kOctree, SKG_FP32_KAHAN>(p, blocks, threads, e->stream);
+    if (err != hipSuccess) return fail(e, SKG_ERR_DEVICE, std::string("lifecycle launch: ") + hipGetErrorString(err));
+    HIPCHK(e, hipEventRecord(e->ev1, e->stream));
+    return SKG_OK;
+}
+
+int skg_sync(skg_engine* e)
+{
+    if (!e) return SKG_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e->ev0, e->ev1) == hipSuccess) e->lastMs = ms;
+    return SKG_OK;
+}
+
+int skg_run_packets(skg_engine* e, uint64_t first_history, uint64_t count)
+{
+    int st = skg_run_packets_async(e, first_history, count);
+    if (st) return st;
+    return skg_sync(e);
+}
+
+int skg_counters(skg_engine* e, int64_t* out8)
+{
+    if (!e || !out8) return SKG_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    unsigned long long c[8];
+    HIPCHK(e, hipMemcpyAsync(c, e->counters, sizeof(c), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    for (int i = 0; i < 8; ++i) out8[i] = static_cast<int64_t>(c[i]);
+    return SKG_OK;
+}
+
+double skg_last_kernel_ms(const skg_engine* e) { return e ? e->lastMs : 0.0; }
+
+}  // extern "C"

@@ -1,0 +1,210 @@
+// Device pieces of the forced-scattering photon-packet life cycle.
+//
+// Random numbers: Philox4x32-10 (Salmon et al., SC'11), one counter stream per history
+// (counter = history lo, history hi, block, tag; key = seed), two 52-bit uniforms per block.
+// Distributions follow SKIRT's Random (Random.cpp:98-164) and DustMix HG code
+// (DustMix.cpp:395-512); peel-off detection follows FrameInstrument.cpp:117-136.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace skg
+{
+    struct Philox
+    {
+        uint32_t k0, k1, h0, h1;
+        uint32_t draw;
+        uint32_t block;
+        uint32_t c[4];
+
+        __device__ __forceinline__ void init(uint64_t seed, uint64_t history)
+        {
+            k0 = static_cast<uint32_t>(seed);
+            k1 = static_cast<uint32_t>(seed >> 32);
+            h0 = static_cast<uint32_t>(history);
+            h1 = static_cast<uint32_t>(history >> 32);
+            draw = 0;
+            block = 0xFFFFFFFFu;
+        }
+
+        __device__ __forceinline__ void gen(uint32_t b)
+        {
+            uint32_t x0 = h0, x1 = h1, x2 = b, x3 = 0u;
+            uint32_t a0 = k0, a1 = k1;
+#pragma unroll
+            for (int r = 0; r < 10; ++r)
+            {
+                if (r)
+                {
+                    a0 += 0x9E3779B9u;
+                    a1 += 0xBB67AE85u;
+                }
+                uint32_t lo0 = 0xD2511F53u * x0;
+                uint32_t hi0 = __umulhi(0xD2511F53u, x0);
+                uint32_t lo1 = 0xCD9E8D57u * x2;
+                uint32_t hi1 = __umulhi(0xCD9E8D57u, x2);
+                uint32_t n0 = hi1 ^ x1 ^ a0;
+                uint32_t n2 = hi0 ^ x3 ^ a1;
+                x0 = n0;
+                x1 = lo1;
+                x2 = n2;
+                x3 = lo0;
+            }
+            c[0] = x0;
+            c[1] = x1;
+            c[2] = x2;
+            c[3] = x3;
+            block = b;
+        }
+
+        __device__ __forceinline__ double uniform()
+        {
+            uint32_t b = draw >> 1;
+            if (b != block) gen(b);
+            uint32_t hi = (draw & 1) ? c[2] : c[0];
+            uint32_t lo = (draw & 1) ? c[3] : c[1];
+            ++draw;
+            uint64_t a = (static_cast<uint64_t>(hi) << 20) ^ (lo >> 12);
+            return (static_cast<double>(a) + 0.5) * (1.0 / 4503599627370496.0);
+        }
+    };
+
+    // Direction(theta, phi), Direction.cpp:11-36
+    __device__ __forceinline__ void direction_from_angles(double theta, double phi, double& kx, double& ky, double& kz)
+    {
+        const double eps = 1e-8;
+        if (theta <= eps)
+        {
+            kx = 0;
+            ky = 0;
+            kz = 1;
+        }
+        else if (theta >= M_PI - eps)
+        {
+            kx = 0;
+            ky = 0;
+            kz = -1;
+        }
+        else
+        {
+            double st = sin(theta);
+            kx = st * cos(phi);
+            ky = st * sin(phi);
+            kz = cos(theta);
+        }
+    }
+
+    // Random::direction(), Random.cpp:121-126
+    __device__ __forceinline__ void isotropic(Philox& rng, double& kx, double& ky, double& kz)
+    {
+        double theta = acos(2.0 * rng.uniform() - 1.0);
+        double phi = 2.0 * M_PI * rng.uniform();
+        direction_from_angles(theta, phi, kx, ky, kz);
+    }
+
+    // Random::direction(bfk, costheta), Random.cpp:130-164
+    __device__ __forceinline__ void rotate(Philox& rng, double& kx, double& ky, double& kz, double costheta)
+    {
+        double phi = 2.0 * M_PI * rng.uniform();
+        double cosphi = cos(phi);
+        double sinphi = sin(phi);
+        double sintheta = sqrt(fabs((1.0 - costheta) * (1.0 + costheta)));
+        double x = kx, y = ky, z = kz;
+        if (z > 0.99999)
+        {
+            kx = cosphi * sintheta;
+            ky = sinphi * sintheta;
+            kz = costheta;
+        }
+        else if (z < -0.99999)
+        {
+            kx = cosphi * sintheta;
+            ky = sinphi * sintheta;
+            kz = -costheta;
+        }
+        else
+        {
+            double root = sqrt((1.0 - z) * (1.0 + z));
+            kx = sintheta / root * (-x * z * cosphi + y * sinphi) + x * costheta;
+            ky = -sintheta / root * (y * z * cosphi + x * sinphi) + y * costheta;
+            kz = root * sintheta * cosphi + z * costheta;
+        }
+    }
+
+    // DustMix::performScattering, HG branch (DustMix.cpp:496-512)
+    __device__ __forceinline__ void scatter_hg(Philox& rng, double g, double& kx, double& ky, double& kz)
+    {
+        if (fabs(g) < 1e-6)
+            isotropic(rng, kx, ky, kz);
+        else
+        {
+            double f = ((1.0 - g) * (1.0 + g)) / (1.0 - g + 2.0 * g * rng.uniform());
+            double costheta = (1.0 + g * g - f * f) / (2.0 * g);
+            rotate(rng, kx, ky, kz, costheta);
+        }
+    }
+
+    // Random::exponCutoff (Random.cpp:105-117)
+    __device__ __forceinline__ double expon_cutoff(Philox& rng, double xmax)
+    {
+        if (xmax == 0.0) return 0.0;
+        if (xmax < 1e-10) return rng.uniform() * xmax;
+        double x = -log(1.0 - rng.uniform() * (1.0 - exp(-xmax)));
+        while (x > xmax) x = -log(1.0 - rng.uniform() * (1.0 - exp(-xmax)));
+        return x;
+    }
+
+    // HG phase function value and its 4-degree average for |g| > 0.95 (DustMix.cpp:395-445)
+    __device__ __forceinline__ double value_hg(double g, double c)
+    {
+        double t = 1. + g * g - 2. * g * c;
+        return (1. - g) * (1. + g) / sqrt(t * t * t);
+    }
+    __device__ __forceinline__ double integral_hg(double g, double ca, double cb)
+    {
+        double ta = sqrt(1. + g * g - 2. * g * ca);
+        double tb = sqrt(1. + g * g - 2. * g * cb);
+        double f1 = (1. - g) * (1. + g) / g;
+        double f2 = (tb - ta) / (tb * ta);
+        return f1 * f2;
+    }
+    __device__ __forceinline__ double peel_phase(double g, double c)
+    {
+        if (!(fabs(g) > 0.95)) return value_hg(g, c);
+        const double delta = 4. * M_PI / 180.;
+        double theta = acos(c);
+        double ca = cos(theta - delta);
+        double cb = cos(theta + delta);
+        if (theta < delta) return (integral_hg(g, 1., ca) + integral_hg(g, 1., cb)) / (2. - ca - cb);
+        if (theta > M_PI - delta) return (integral_hg(g, ca, -1.) + integral_hg(g, cb, -1.)) / (2. + ca + cb);
+        return integral_hg(g, ca, cb) / (ca - cb);
+    }
+
+    // distant frame instrument geometry (FrameInstrument.cpp:12-33, DistantInstrument.cpp:40-48)
+    struct FrameGeom
+    {
+        double costheta, sintheta, cosphi, sinphi, cosomega, sinomega;
+        double xpmin, xpsiz, ypmin, ypsiz;
+        double kobs[3];
+        int nx, ny;
+        int same_as_prev;
+        int pad;
+        int64_t offset;  // first element of this instrument in the frame buffer
+
+        // FrameInstrument::pixelOnDetector (FrameInstrument.cpp:117-136)
+        __device__ __forceinline__ int pixel(double x, double y, double z) const
+        {
+            double xpp = -sinphi * x + cosphi * y;
+            double ypp = -cosphi * costheta * x - sinphi * costheta * y + sintheta * z;
+            double xp = cosomega * xpp - sinomega * ypp;
+            double yp = sinomega * xpp + cosomega * ypp;
+            int i = static_cast<int>(floor((xp - xpmin) / xpsiz));
+            int j = static_cast<int>(floor((yp - ypmin) / ypsiz));
+            if (i < 0 || i >= nx || j < 0 || j >= ny) return -1;
+            return i + nx * j;
+        }
+    };
+
+    constexpr int kMaxFrames = 8;
+}

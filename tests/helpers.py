@@ -1,0 +1,65 @@
+"""Shared helpers for the tests: package loading, golden fixtures and small workloads."""
+
+from __future__ import annotations
+
+import importlib.util
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+PC = 3.08567758e16  # parsec (m)
+
+
+def load_pkg():
+    name = "skirt9_gpu_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "skirt9-gpu_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def golden(name):
+    return np.load(os.path.join(GOLDEN, name))
+
+
+def cart_borders(n, lo, hi):
+    """LinMesh borders scaled as CartesianSpatialGrid::setupSelfAfter (i/N)*(max-min)+min."""
+    t = np.array([0.0 + i * ((1.0 - 0.0) / n) for i in range(n + 1)])
+    return t * (hi - lo) + lo
+
+
+def child_first(child_begin, child_count, child_index):
+    cb, cc, ci = map(np.asarray, (child_begin, child_count, child_index))
+    out = np.full(len(cb), -1, np.int32)
+    has = cc > 0
+    out[has] = ci[cb[has]]
+    return out
+
+
+def disk_density_cartesian(n=64, half=10 * PC, tau_v=1.0, sext=1.5e-29, samples=100, seed=5):
+    """Cell-averaged number density of the exponential disk (scale length 3 pc, height 0.3 pc)
+    normalised to face-on tau_V (ExpDiskGeometry.cpp, OpticalDepthMaterialNormalization.cpp),
+    sampled at `samples` random points per cell like MediumSystem's PropertySampler."""
+    b = cart_borders(n, -half, half)
+    hR, hz = 3 * PC, 0.3 * PC
+    rho0 = 1 / (4 * np.pi * hR * hR * hz)
+    N = tau_v / sext / (2 * rho0 * hz)
+    rng = np.random.default_rng(seed)
+    ii, jj, kk = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    acc = np.zeros((n, n, n))
+    for _ in range(samples):
+        x = b[ii] + rng.random(ii.shape) * (b[ii + 1] - b[ii])
+        y = b[jj] + rng.random(ii.shape) * (b[jj + 1] - b[jj])
+        z = b[kk] + rng.random(ii.shape) * (b[kk + 1] - b[kk])
+        acc += rho0 * np.exp(-np.sqrt(x * x + y * y) / hR) * np.exp(-np.abs(z) / hz)
+    return b, (N * acc / samples).reshape(-1)
+
+
+def frame60(npix=200, fov=20 * PC):
+    return dict(inclination=np.radians(60.0), azimuth=0.0, roll=0.0, nx=npix, ny=npix, fovx=fov, fovy=fov)

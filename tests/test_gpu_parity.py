@@ -1,0 +1,153 @@
+"""Parity of the HIP engine (through the C-ABI) with the reference golden vectors and the
+FP64 CPU oracle.
+
+Bars: bit-exact cell indices and FP64 segment lengths for identical input rays (the engine
+is compiled without FMA contraction); FP64 optical depths equal to the reference's; FP32
+Kahan optical depths within 1e-5 relative; images of the same histories (same Philox
+streams) agree with the oracle to 1e-9 relative in total and per-pixel median.
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import PC, cart_borders, child_first, disk_density_cartesian, frame60, golden, load_pkg
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    pkg = load_pkg()
+    e = pkg.Engine(0)
+    yield e
+    e.close()
+
+
+def _cart(eng, key):
+    f = golden("cart_paths.npz")
+    h = float(f[key + "_half"])
+    n = f[key + "_n"]
+    eng.grid_cartesian(cart_borders(n[0], -h, h), cart_borders(n[1], -h, h), cart_borders(n[2], -h, h))
+    return f
+
+
+@pytest.mark.parametrize("key", ["c64", "c8"])
+def test_cartesian_paths_bit_exact(eng, key):
+    f = _cart(eng, key)
+    print("case cartesian paths", key, flush=True)
+    off, m, ds = eng.trace_paths(f[key + "_rays"])
+    np.testing.assert_array_equal(off, f[key + "_off"])
+    np.testing.assert_array_equal(m, f[key + "_m"])
+    np.testing.assert_array_equal(ds, f[key + "_ds"])
+
+
+def test_cartesian_tau(eng):
+    f = _cart(eng, "c8")
+    eng.medium(f["c8_density"])
+    s = float(f["c8_sext"][0])
+    eng.dust([0.55e-6], [s], [0.6 * s], [0.6])
+    print("case cartesian tau", flush=True)
+    tp, to, cum = eng.trace_tau(f["c8_rays"], toff=f["c8_toff"])
+    np.testing.assert_array_equal(cum, f["c8_tau"])
+    np.testing.assert_array_equal(to, f["c8_tauobs"])
+    last = np.array([f["c8_tau"][f["c8_toff"][i + 1] - 1] if f["c8_toff"][i + 1] > f["c8_toff"][i] else 0.0
+                     for i in range(len(tp))])
+    np.testing.assert_array_equal(tp, last)
+    tp32, to32 = eng.trace_tau(f["c8_rays"], fp_mode=1)
+    np.testing.assert_allclose(tp32, tp, rtol=1e-5, atol=1e-12)
+    np.testing.assert_allclose(to32, to, rtol=1e-5, atol=1e-12)
+
+
+def _tree(eng):
+    t = golden("tree_paths.npz")
+    eng.grid_octree(t["bounds"], child_first(t["child_begin"], t["child_count"], t["child_index"]), t["cell_index"],
+                    t["nb_off"], t["nb_id"])
+    return t
+
+
+def test_octree_paths_bit_exact(eng):
+    t = _tree(eng)
+    print("case octree paths", flush=True)
+    off, m, ds = eng.trace_paths(t["rays"])
+    np.testing.assert_array_equal(off, t["off"])
+    np.testing.assert_array_equal(m, t["m"])
+    np.testing.assert_array_equal(ds, t["ds"])
+
+
+def test_octree_tau(eng):
+    t = _tree(eng)
+    eng.medium(t["density"])
+    s = float(t["sext"][0])
+    eng.dust([0.55e-6], [s], [0.6 * s], [0.6])
+    print("case octree tau", flush=True)
+    tp, to, cum = eng.trace_tau(t["rays"], toff=t["toff"])
+    np.testing.assert_array_equal(cum, t["tau"])
+    np.testing.assert_array_equal(to, t["tauobs"])
+    tp32, to32 = eng.trace_tau(t["rays"], fp_mode=1)
+    np.testing.assert_allclose(tp32, tp, rtol=1e-5, atol=1e-12)
+    np.testing.assert_allclose(to32, to, rtol=1e-5, atol=1e-12)
+
+
+def test_empty_and_outside_rays(eng):
+    _cart(eng, "c8")
+    print("case empty rays", flush=True)
+    off, m, ds = eng.trace_paths(np.zeros((0, 6)))
+    assert len(off) == 1 and len(m) == 0
+    far = np.array([[1e20, 1e20, 1e20, 1.0, 0.0, 0.0]])
+    off, m, ds = eng.trace_paths(far)
+    assert off[-1] == 0
+
+
+def _lifecycle_case(eng, npk, seed, fp_mode=0, threshold=16, with_oracle=True):
+    b, dens = disk_density_cartesian(samples=10, seed=9)
+    eng.grid_cartesian(b, b, b)
+    eng.medium(dens)
+    s = 1.5e-29
+    eng.dust([0.55e-6], [s], [0.6 * s], [0.6])
+    eng.frame_remove_all()
+    fr = frame60()
+    eng.frame_add(fr["inclination"], fr["azimuth"], fr["roll"], fr["nx"], fr["ny"], fr["fovx"], fr["fovy"])
+    eng.source_point((0.0, 0.0, 0.0), [1.0 / npk])
+    eng.options(0.5, 1e4, 0, fp_mode, seed)
+    eng.tune(0, 256, threshold)
+    eng.frame_clear()
+    print("case lifecycle", npk, seed, fp_mode, threshold, flush=True)
+    eng.run_packets(0, npk)
+    img = eng.frame_read(0)[0]
+    cnt = eng.counters()
+    if not with_oracle:
+        return img, cnt, None, None
+    g = O.Grid("cartesian", xb=b, yb=b, zb=b)
+    ofr, ocnt = O.simulate(g, dens, [0.55e-6], [s], [0.6 * s], [0.6], [1.0 / npk], [fr], 0, npk, threads=8, seed=seed)
+    return img, cnt, ofr[0][0], ocnt
+
+
+def test_lifecycle_matches_oracle_fp64(eng):
+    """Same histories, same random streams: the frame equals the oracle's up to rare histories
+    whose discrete decisions flip on last-bit differences of the device math library."""
+    img, cnt, oimg, ocnt = _lifecycle_case(eng, 20000, 3)
+    assert cnt[0] == ocnt[0] == 20000
+    assert abs(int(cnt[2]) - int(ocnt[2])) <= 0.001 * ocnt[2]  # forced paths
+    assert abs(img.sum() / oimg.sum() - 1) < 1e-6
+    nz = oimg > 0
+    rel = np.abs(img[nz] - oimg[nz]) / oimg[nz]
+    assert np.median(rel) < 1e-9
+    assert np.mean(rel < 1e-6) > 0.99
+
+
+def test_lifecycle_fp32_kahan_close_to_fp64(eng):
+    img64, _, _, _ = _lifecycle_case(eng, 20000, 5, fp_mode=0, with_oracle=False)
+    img32, _, _, _ = _lifecycle_case(eng, 20000, 5, fp_mode=1, with_oracle=False)
+    assert abs(img32.sum() / img64.sum() - 1) < 1e-3
+    nz = img64 > 0
+    rel = np.abs(img32[nz] - img64[nz]) / img64[nz]
+    assert np.median(rel) < 0.01
+
+
+def test_lifecycle_independent_of_launch_shape(eng):
+    """Results do not depend on the transition threshold (scheduling only)."""
+    a, ca, _, _ = _lifecycle_case(eng, 5000, 7, threshold=1, with_oracle=False)
+    b, cb, _, _ = _lifecycle_case(eng, 5000, 7, threshold=64, with_oracle=False)
+    np.testing.assert_array_equal(ca[[0, 1, 2, 3]], cb[[0, 1, 2, 3]])
+    np.testing.assert_allclose(a, b, rtol=1e-12, atol=0)

@@ -151,6 +151,61 @@ int skg_tune(skg_engine* e, int32_t workgroups, int32_t threads, int32_t transit
  * engine's stream. */
 double skg_last_kernel_ms(const skg_engine* e);
 
+/* ---- medium set-up on the host (mirrors the reference set-up before the photon loop) --- */
+
+/* Exponential disk (ExpDiskGeometry, no truncation) optionally decorated with a logarithmic
+ * spiral perturbation (SpiralStructureGeometryDecorator), normalised to a face-on optical
+ * depth tau_v along Z for a dust of extinction cross section sext_v at the normalisation
+ * wavelength (OpticalDepthMaterialNormalization).  Lengths in metres, angles in radians. */
+typedef struct skg_disk_model
+{
+    double scale_length, scale_height;
+    int32_t spiral_arms;   /* 0: no spiral perturbation */
+    int32_t spiral_index;  /* N of the sin^(2N) arm profile */
+    double pitch_angle, radius_zero, phase_zero, perturbation_weight;
+    double tau_v, sext_v;
+} skg_disk_model;
+
+typedef struct skg_octree skg_octree;
+
+/* Number density of the model at a point (GeometricMedium::numberDensity). */
+double skg_disk_density(const skg_disk_model* model, double x, double y, double z);
+
+/* Density-refined octree over box[6] (PolicyTreeSpatialGrid + DensityTreePolicy::constructTree,
+ * DensityTreePolicy.cpp:180-241, maxDustFraction criterion of needsSubdivide :82-170), with the
+ * neighbour lists maintained as OctTreeNode::addNeighbors (OctTreeNode.cpp:46-136) and sorted as
+ * TreeNode::sortNeighbors (TreeNode.cpp:200-207).  num_samples random points per node; threads
+ * 0 = all cores.  Results do not depend on the thread count. */
+int skg_octree_build(const double* box, const skg_disk_model* model, int32_t min_level, int32_t max_level,
+                     double max_dust_fraction, int32_t num_samples, uint64_t seed, int32_t threads,
+                     skg_octree** out);
+/* The octree of a given topology: node l (in creation order) is subdivided when divide[l] != 0,
+ * the order DensityTreePolicy::constructTree subdivides in, so ids, bounds and neighbour lists
+ * reproduce the grid the flags describe (e.g. flags from TreeSpatialGrid's child counts). */
+int skg_octree_from_flags(const double* box, int64_t nflags, const uint8_t* divide, skg_octree** out);
+int skg_octree_info(const skg_octree* t, int64_t* nnodes, int64_t* ncells, int64_t* nneighbors);
+/* Flattened tables in the layout skg_grid_octree takes (non-leaf nodes have empty lists). */
+int skg_octree_export(const skg_octree* t, double* bounds, int32_t* child_first, int32_t* cell_index,
+                      int64_t* nb_off, int32_t* nb_id);
+/* skg_grid_octree with the tables of t. */
+int skg_octree_upload(skg_engine* e, const skg_octree* t);
+void skg_octree_free(skg_octree* t);
+
+/* Cell-averaged number densities from num_samples random points per cell
+ * (MediumSystem.cpp:401-504, PropertySampler :129-155). */
+int skg_sample_density_cartesian(const double* xb, int32_t nx, const double* yb, int32_t ny, const double* zb,
+                                 int32_t nz, const skg_disk_model* model, int32_t num_samples, uint64_t seed,
+                                 int32_t threads, double* out);
+int skg_sample_density_octree(const skg_octree* t, const skg_disk_model* model, int32_t num_samples,
+                              uint64_t seed, int32_t threads, double* out);
+
+/* Dust cross sections and asymmetry parameters at the simulation wavelengths from a
+ * MeanListDustMix table (TabulatedDustMix::getOpticalProperties, TabulatedDustMix.cpp:12-43;
+ * NR::clampedResample log-log for cross sections, log-lin for g; g clamped to +-0.999999). */
+int skg_dust_meanlist(int32_t nin, const double* lambda_in, const double* kappa_in, const double* albedo_in,
+                      const double* g_in, double mass_per_hydrogen, int32_t nout, const double* lambda_out,
+                      double* sext, double* ssca, double* g);
+
 #ifdef __cplusplus
 }
 #endif

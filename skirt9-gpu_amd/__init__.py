@@ -66,6 +66,9 @@ EXPORTED = [
     "skg_medium", "skg_dust", "skg_trace_paths", "skg_trace_tau", "skg_frame_add", "skg_frame_remove_all",
     "skg_frame_clear", "skg_frame_read", "skg_source_point", "skg_options", "skg_run_packets",
     "skg_run_packets_async", "skg_sync", "skg_counters", "skg_tune", "skg_last_kernel_ms",
+    "skg_disk_density", "skg_octree_build", "skg_octree_from_flags", "skg_octree_info", "skg_octree_export",
+    "skg_octree_upload", "skg_octree_free", "skg_sample_density_cartesian", "skg_sample_density_octree",
+    "skg_dust_meanlist",
 ]
 
 
@@ -220,3 +223,115 @@ class Engine:
 
     def last_kernel_ms(self):
         return float(lib().skg_last_kernel_ms(self._h))
+
+
+# ---------------------------------------------------------------------------- host set-up
+class DiskModel(C.Structure):
+    """Exponential disk with optional spiral perturbation, normalised to face-on tau_v."""
+    _fields_ = [("scale_length", C.c_double), ("scale_height", C.c_double), ("spiral_arms", C.c_int32),
+                ("spiral_index", C.c_int32), ("pitch_angle", C.c_double), ("radius_zero", C.c_double),
+                ("phase_zero", C.c_double), ("perturbation_weight", C.c_double), ("tau_v", C.c_double),
+                ("sext_v", C.c_double)]
+
+
+PC = 3.08567758e16  # parsec (m)
+
+
+def disk_model(tau_v=1.0, sext_v=1.5e-29, spiral=False, hR=3 * PC, hz=0.3 * PC):
+    """The workloads' disk: h_R = 3 pc, h_z = 0.3 pc; spiral: 2 arms, pitch 15 deg, R0 = 3 pc, weight 0.5."""
+    return DiskModel(hR, hz, 2 if spiral else 0, 1, np.radians(15.0), 3 * PC, 0.0, 0.5, tau_v, sext_v)
+
+
+def disk_density(model, x, y, z):
+    lib().skg_disk_density.restype = C.c_double
+    return lib().skg_disk_density(C.byref(model), C.c_double(x), C.c_double(y), C.c_double(z))
+
+
+class Octree:
+    """Host-built octree (DensityTreePolicy semantics) in the engine's table layout."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def build(cls, box, model, min_level, max_level, max_dust_fraction, num_samples=100, seed=1, threads=0):
+        b = _f64(box)
+        h = C.c_void_p()
+        st = lib().skg_octree_build(_p(b, f64p), C.byref(model), C.c_int32(min_level), C.c_int32(max_level),
+                                    C.c_double(max_dust_fraction), C.c_int32(num_samples), C.c_uint64(seed),
+                                    C.c_int32(threads), C.byref(h))
+        if st != SKG_OK:
+            raise EngineError(f"skg_octree_build failed with status {st}")
+        return cls(h)
+
+    @classmethod
+    def from_flags(cls, box, divide):
+        b = _f64(box)
+        d = np.ascontiguousarray(divide, dtype=np.uint8)
+        h = C.c_void_p()
+        st = lib().skg_octree_from_flags(_p(b, f64p), C.c_int64(len(d)), _p(d, P(C.c_uint8)), C.byref(h))
+        if st != SKG_OK:
+            raise EngineError(f"skg_octree_from_flags failed with status {st}")
+        return cls(h)
+
+    def info(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        lib().skg_octree_info(self._h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+    def export(self):
+        nn, nc, nnb = self.info()
+        out = dict(bounds=np.zeros((nn, 6)), child_first=np.zeros(nn, np.int32), cell_index=np.zeros(nn, np.int32),
+                   nb_off=np.zeros(6 * nn + 1, np.int64), nb_id=np.zeros(max(nnb, 1), np.int32))
+        lib().skg_octree_export(self._h, _p(out["bounds"], f64p), _p(out["child_first"], i32p),
+                                _p(out["cell_index"], i32p), _p(out["nb_off"], i64p), _p(out["nb_id"], i32p))
+        out["nb_id"] = out["nb_id"][:nnb]
+        out["ncells"] = nc
+        return out
+
+    def upload(self, engine):
+        engine._chk(lib().skg_octree_upload(engine._h, self._h))
+
+    def sample_density(self, model, num_samples=100, seed=1, threads=0):
+        nn, nc, _ = self.info()
+        out = np.zeros(nc)
+        st = lib().skg_sample_density_octree(self._h, C.byref(model), C.c_int32(num_samples), C.c_uint64(seed),
+                                             C.c_int32(threads), _p(out, f64p))
+        if st != SKG_OK:
+            raise EngineError(f"skg_sample_density_octree failed with status {st}")
+        return out
+
+    def close(self):
+        if self._h:
+            lib().skg_octree_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sample_density_cartesian(xb, yb, zb, model, num_samples=100, seed=1, threads=0):
+    xb, yb, zb = _f64(xb), _f64(yb), _f64(zb)
+    out = np.zeros((len(xb) - 1) * (len(yb) - 1) * (len(zb) - 1))
+    st = lib().skg_sample_density_cartesian(_p(xb, f64p), C.c_int32(len(xb) - 1), _p(yb, f64p), C.c_int32(len(yb) - 1),
+                                            _p(zb, f64p), C.c_int32(len(zb) - 1), C.byref(model),
+                                            C.c_int32(num_samples), C.c_uint64(seed), C.c_int32(threads),
+                                            _p(out, f64p))
+    if st != SKG_OK:
+        raise EngineError(f"skg_sample_density_cartesian failed with status {st}")
+    return out
+
+
+def dust_meanlist(lam_in, kappa_in, albedo_in, g_in, lam_out, mu=1.5e-29):
+    """Dust cross sections and g at lam_out from a MeanListDustMix table (mu: kg per H, as the reference)."""
+    a = [_f64(x) for x in (lam_in, kappa_in, albedo_in, g_in)]
+    lo = _f64(lam_out)
+    sext, ssca, g = np.zeros(len(lo)), np.zeros(len(lo)), np.zeros(len(lo))
+    st = lib().skg_dust_meanlist(C.c_int32(len(a[0])), *[_p(x, f64p) for x in a], C.c_double(mu), C.c_int32(len(lo)),
+                                 _p(lo, f64p), _p(sext, f64p), _p(ssca, f64p), _p(g, f64p))
+    if st != SKG_OK:
+        raise EngineError(f"skg_dust_meanlist failed with status {st}")
+    return sext, ssca, g

@@ -1,0 +1,533 @@
+// Host-side set-up of the medium the life cycle reads: the density model, the
+// density-refined octree with its per-wall neighbour lists, cell-averaged number densities
+// and the dust tables at the simulation wavelengths.  These mirror the reference set-up
+// that precedes the photon loop (TreeSpatialGrid::setupSelfAfter, DensityTreePolicy,
+// MediumSystem::setupSelfAfter, TabulatedDustMix) so that the engine can be driven end to
+// end without the reference.
+//
+// Random sampling (tree refinement, cell densities) uses per-node / per-cell Philox streams,
+// so results do not depend on the number of threads.
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "../../include/skirt_mi355x.h"
+#include "philox.hpp"
+
+namespace
+{
+    constexpr uint32_t kTagTree = 0x54524545u;     // stream tag for tree refinement samples
+    constexpr uint32_t kTagDensity = 0x44454e53u;  // stream tag for cell density samples
+
+    struct Model
+    {
+        double hR, hz, rho0;
+        bool spiral;
+        double m, tanp, R0, phi0, w, cn;
+        int N;
+        double nnorm;  // number density per unit normalised geometric density
+
+        explicit Model(const skg_disk_model& d)
+        {
+            hR = d.scale_length;
+            hz = d.scale_height;
+            // ExpDiskGeometry::setupSelfBefore (no truncation): rho0 = 1 / (2 pi hR^2 * 2 hz)
+            double intphi = 2.0 * M_PI;
+            double intz = 2.0 * hz;
+            double intR = hR * hR * (1.0 - 0.0);
+            rho0 = 1.0 / (intR * intphi * intz);
+            spiral = d.spiral_arms > 0;
+            m = d.spiral_arms;
+            N = d.spiral_index;
+            tanp = std::tan(d.pitch_angle);
+            R0 = d.radius_zero;
+            phi0 = d.phase_zero;
+            w = d.perturbation_weight;
+            // SpiralStructureGeometryDecorator::setupSelfBefore: cn = sqrt(pi) Gamma(N+1) / Gamma(N+1/2)
+            cn = std::sqrt(M_PI) * std::tgamma(N + 1.0) / std::tgamma(N + 0.5);
+            // OpticalDepthMaterialNormalization along Z: number = tau / (sigma * SigmaZ), SigmaZ = 2 rho0 hz
+            double sigmaZ = 2.0 * rho0 * hz;
+            nnorm = d.tau_v / d.sext_v / sigmaZ;
+        }
+
+        // normalised geometric density (integrates to one)
+        double geom(double x, double y, double z) const
+        {
+            double R = std::sqrt(x * x + y * y);
+            double v = rho0 * std::exp(-R / hR) * std::exp(-std::fabs(z) / hz);
+            if (spiral)
+            {
+                double phi = std::atan2(y, x);
+                double gamma = std::log(R / R0) / tanp + phi0 + 0.5 * M_PI / m;
+                v *= (1.0 - w) + w * cn * std::pow(std::sin(0.5 * m * (gamma - phi)), 2 * N);
+            }
+            return v;
+        }
+    };
+
+    template<typename F> void parallelFor(int64_t n, int threads, F&& f)
+    {
+        if (threads <= 1 || n < 4096)
+        {
+            f(0, n);
+            return;
+        }
+        std::atomic<int64_t> next{0};
+        const int64_t chunk = std::max<int64_t>(1024, n / (threads * 16));
+        auto work = [&]() {
+            while (true)
+            {
+                int64_t b = next.fetch_add(chunk);
+                if (b >= n) break;
+                f(b, std::min(n, b + chunk));
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+        work();
+        for (auto& th : pool) th.join();
+    }
+
+    int defaultThreads(int threads)
+    {
+        if (threads > 0) return threads;
+        unsigned hc = std::thread::hardware_concurrency();
+        return hc ? static_cast<int>(std::min(hc, 32u)) : 4;
+    }
+
+    // mean normalised density over n random points in the box (Random::position(box): x, y, z
+    // drawn in that order, x = xmin + u (xmax - xmin))
+    double sampleMean(const Model& md, const double* b, int n, uint64_t seed, uint64_t id, uint32_t tag)
+    {
+        skg::Philox rng;
+        rng.init(seed, id, tag);
+        double sum = 0.;
+        for (int i = 0; i < n; ++i)
+        {
+            double x = b[0] + rng.uniform() * (b[3] - b[0]);
+            double y = b[1] + rng.uniform() * (b[4] - b[1]);
+            double z = b[2] + rng.uniform() * (b[5] - b[2]);
+            sum += md.geom(x, y, z);
+        }
+        return sum / n;
+    }
+}
+
+// ------------------------------------------------------------------------------- octree
+struct skg_octree
+{
+    struct Node
+    {
+        double b[6];
+        int32_t first = -1;  // first child id
+        int32_t level = 0;
+        std::vector<int32_t> nb[6];
+    };
+    std::vector<Node> nodes;
+    std::vector<int32_t> cellOf;
+    int64_t ncells = 0;
+};
+
+namespace
+{
+    using Node = skg_octree::Node;
+    enum Wall
+    {
+        BACK = 0,
+        FRONT,
+        LEFT,
+        RIGHT,
+        BOTTOM,
+        TOP
+    };
+
+    // TreeNode::makeNeighbors / deleteNeighbor (TreeNode.cpp:116-141)
+    inline void makeNeighbors(std::vector<Node>& v, int wall1, int32_t n1, int32_t n2)
+    {
+        static const int complement[] = {FRONT, BACK, RIGHT, LEFT, TOP, BOTTOM};
+        v[n1].nb[wall1].push_back(n2);
+        v[n2].nb[complement[wall1]].push_back(n1);
+    }
+    inline void deleteNeighbor(std::vector<Node>& v, int32_t node, int wall, int32_t other)
+    {
+        auto& l = v[node].nb[wall];
+        auto it = std::find(l.begin(), l.end(), other);
+        if (it != l.end()) l.erase(it);
+    }
+
+    // OctTreeNode::createChildren + TreeNode::subdivide + OctTreeNode::addNeighbors
+    // (OctTreeNode.cpp:22-136): the eight children get consecutive ids, sibling links are made
+    // first, then each wall's neighbours are relinked to the children they touch.
+    void subdivide(std::vector<Node>& v, int32_t p)
+    {
+        const int32_t c = static_cast<int32_t>(v.size());
+        double b[6];
+        std::memcpy(b, v[p].b, sizeof(b));
+        double xc = 0.5 * (b[0] + b[3]), yc = 0.5 * (b[1] + b[4]), zc = 0.5 * (b[2] + b[5]);
+        const double boxes[8][6] = {{b[0], b[1], b[2], xc, yc, zc}, {xc, b[1], b[2], b[3], yc, zc},
+                                    {b[0], yc, b[2], xc, b[4], zc}, {xc, yc, b[2], b[3], b[4], zc},
+                                    {b[0], b[1], zc, xc, yc, b[5]}, {xc, b[1], zc, b[3], yc, b[5]},
+                                    {b[0], yc, zc, xc, b[4], b[5]}, {xc, yc, zc, b[3], b[4], b[5]}};
+        const int32_t lvl = v[p].level + 1;
+        v.resize(v.size() + 8);
+        for (int l = 0; l < 8; ++l)
+        {
+            std::memcpy(v[c + l].b, boxes[l], sizeof(boxes[l]));
+            v[c + l].level = lvl;
+        }
+        v[p].first = c;
+        makeNeighbors(v, FRONT, c + 0, c + 1);
+        makeNeighbors(v, RIGHT, c + 0, c + 2);
+        makeNeighbors(v, TOP, c + 0, c + 4);
+        makeNeighbors(v, RIGHT, c + 1, c + 3);
+        makeNeighbors(v, TOP, c + 1, c + 5);
+        makeNeighbors(v, FRONT, c + 2, c + 3);
+        makeNeighbors(v, TOP, c + 2, c + 6);
+        makeNeighbors(v, TOP, c + 3, c + 7);
+        makeNeighbors(v, FRONT, c + 4, c + 5);
+        makeNeighbors(v, RIGHT, c + 4, c + 6);
+        makeNeighbors(v, RIGHT, c + 5, c + 7);
+        makeNeighbors(v, FRONT, c + 6, c + 7);
+        // the split point is the upper corner of child 0
+        const double X = v[c].b[3], Y = v[c].b[4], Z = v[c].b[5];
+        auto relink = [&](int wall, int opposite, int a0, int a1, int a2, int a3, int u, int w) {
+            // u/w: which two coordinates (0=x,1=y,2=z) span the wall
+            const double U = (u == 0 ? X : u == 1 ? Y : Z), Wc = (w == 0 ? X : w == 1 ? Y : Z);
+            std::vector<int32_t> list = v[p].nb[wall];
+            for (int32_t nbr : list)
+            {
+                deleteNeighbor(v, nbr, opposite, p);
+                const double* nb = v[nbr].b;
+                double umin = nb[u], umax = nb[u + 3], wmin = nb[w], wmax = nb[w + 3];
+                if (umin <= U && wmin <= Wc) makeNeighbors(v, opposite, nbr, c + a0);
+                if (umax >= U && wmin <= Wc) makeNeighbors(v, opposite, nbr, c + a1);
+                if (umin <= U && wmax >= Wc) makeNeighbors(v, opposite, nbr, c + a2);
+                if (umax >= U && wmax >= Wc) makeNeighbors(v, opposite, nbr, c + a3);
+            }
+        };
+        relink(BACK, FRONT, 0, 2, 4, 6, 1, 2);
+        relink(FRONT, BACK, 1, 3, 5, 7, 1, 2);
+        relink(LEFT, RIGHT, 0, 1, 4, 5, 0, 2);
+        relink(RIGHT, LEFT, 2, 3, 6, 7, 0, 2);
+        relink(BOTTOM, TOP, 0, 1, 2, 3, 0, 1);
+        relink(TOP, BOTTOM, 4, 5, 6, 7, 0, 1);
+        // the subdivided node's own lists are never read again
+        for (auto& l : v[p].nb) std::vector<int32_t>().swap(l);
+    }
+
+    // TreeNode::sortNeighbors with the LargerOverlap order (TreeNode.cpp:145-207)
+    double overlapArea(const double* base, const double* nb, int wall)
+    {
+        int u, w;
+        if (wall == BACK || wall == FRONT)
+        {
+            u = 1;
+            w = 2;
+        }
+        else if (wall == LEFT || wall == RIGHT)
+        {
+            u = 0;
+            w = 2;
+        }
+        else
+        {
+            u = 0;
+            w = 1;
+        }
+        return std::max(std::min(base[u + 3], nb[u + 3]) - std::max(base[u], nb[u]), 0.)
+               * std::max(std::min(base[w + 3], nb[w + 3]) - std::max(base[w], nb[w]), 0.);
+    }
+}
+
+extern "C" {
+
+double skg_disk_density(const skg_disk_model* model, double x, double y, double z)
+{
+    if (!model) return 0.;
+    Model md(*model);
+    return md.nnorm * md.geom(x, y, z);
+}
+
+int skg_octree_build(const double* box, const skg_disk_model* model, int32_t min_level, int32_t max_level,
+                     double max_dust_fraction, int32_t num_samples, uint64_t seed, int32_t threads, skg_octree** out)
+{
+    if (!box || !model || !out || min_level < 0 || max_level < min_level || max_level > 20 || num_samples < 1
+        || !(box[3] > box[0] && box[4] > box[1] && box[5] > box[2]))
+        return SKG_ERR_ARGUMENT;
+    try
+    {
+        Model md(*model);
+        threads = defaultThreads(threads);
+        auto t = std::make_unique<skg_octree>();
+        auto& v = t->nodes;
+        v.reserve(1 << 20);
+        v.emplace_back();
+        std::memcpy(v[0].b, box, 6 * sizeof(double));
+        // DensityTreePolicy::constructTree (DensityTreePolicy.cpp:180-241): evaluate all nodes of a
+        // level, then subdivide the flagged ones in node order
+        size_t lbeg = 0, lend = 1;
+        while (lend != lbeg)
+        {
+            const size_t n = lend - lbeg;
+            std::vector<uint8_t> divide(n, 0);
+            parallelFor(static_cast<int64_t>(n), threads, [&](int64_t b, int64_t e) {
+                for (int64_t i = b; i < e; ++i)
+                {
+                    const Node& nd = v[lbeg + i];
+                    // DensityTreePolicy::needsSubdivide (:82-170), maxDustFraction criterion
+                    if (nd.level < min_level)
+                        divide[i] = 1;
+                    else if (nd.level >= max_level)
+                        divide[i] = 0;
+                    else
+                    {
+                        double rho = sampleMean(md, nd.b, num_samples, seed, lbeg + i, kTagTree);
+                        double vol = (nd.b[3] - nd.b[0]) * (nd.b[4] - nd.b[1]) * (nd.b[5] - nd.b[2]);
+                        divide[i] = rho * vol > max_dust_fraction ? 1 : 0;
+                    }
+                }
+            });
+            for (size_t i = 0; i < n; ++i)
+                if (divide[i]) subdivide(v, static_cast<int32_t>(lbeg + i));
+            lbeg = lend;
+            lend = v.size();
+            if (v.size() > static_cast<size_t>(INT32_MAX / 8)) return SKG_ERR_CAPACITY;
+        }
+        // cell indices in node order (TreeSpatialGrid.cpp:37-49)
+        t->cellOf.assign(v.size(), -1);
+        int32_t m = 0;
+        for (size_t l = 0; l < v.size(); ++l)
+            if (v[l].first < 0) t->cellOf[l] = m++;
+        t->ncells = m;
+        // sort the leaves' neighbour lists by decreasing overlap area
+        parallelFor(static_cast<int64_t>(v.size()), threads, [&](int64_t b, int64_t e) {
+            for (int64_t l = b; l < e; ++l)
+            {
+                if (v[l].first >= 0) continue;
+                for (int w = 0; w < 6; ++w)
+                {
+                    const double* base = v[l].b;
+                    std::sort(v[l].nb[w].begin(), v[l].nb[w].end(), [&](int32_t a, int32_t c) {
+                        return overlapArea(base, v[a].b, w) > overlapArea(base, v[c].b, w);
+                    });
+                }
+            }
+        });
+        *out = t.release();
+        return SKG_OK;
+    }
+    catch (const std::bad_alloc&)
+    {
+        return SKG_ERR_CAPACITY;
+    }
+}
+
+int skg_octree_info(const skg_octree* t, int64_t* nnodes, int64_t* ncells, int64_t* nneighbors)
+{
+    if (!t) return SKG_ERR_ARGUMENT;
+    if (nnodes) *nnodes = static_cast<int64_t>(t->nodes.size());
+    if (ncells) *ncells = t->ncells;
+    if (nneighbors)
+    {
+        int64_t s = 0;
+        for (const auto& n : t->nodes)
+            for (const auto& l : n.nb) s += static_cast<int64_t>(l.size());
+        *nneighbors = s;
+    }
+    return SKG_OK;
+}
+
+int skg_octree_export(const skg_octree* t, double* bounds, int32_t* child_first, int32_t* cell_index, int64_t* nb_off,
+                      int32_t* nb_id)
+{
+    if (!t || !bounds || !child_first || !cell_index || !nb_off || !nb_id) return SKG_ERR_ARGUMENT;
+    const auto& v = t->nodes;
+    int64_t o = 0;
+    for (size_t l = 0; l < v.size(); ++l)
+    {
+        std::memcpy(bounds + 6 * l, v[l].b, 6 * sizeof(double));
+        child_first[l] = v[l].first;
+        cell_index[l] = t->cellOf[l];
+        for (int w = 0; w < 6; ++w)
+        {
+            nb_off[6 * l + w] = o;
+            for (int32_t id : v[l].nb[w]) nb_id[o++] = id;
+        }
+    }
+    nb_off[6 * v.size()] = o;
+    return SKG_OK;
+}
+
+void skg_octree_free(skg_octree* t) { delete t; }
+
+// Replays a given topology: nodes are visited in creation order and node l is subdivided when
+// divide[l] != 0 -- the order DensityTreePolicy::constructTree subdivides in (level by level,
+// node order within a level), so the node ids, bounds and neighbour lists are those of the
+// grid the flags were taken from.
+int skg_octree_from_flags(const double* box, int64_t nflags, const uint8_t* divide, skg_octree** out)
+{
+    if (!box || !out || nflags < 1 || !divide) return SKG_ERR_ARGUMENT;
+    try
+    {
+        auto t = std::make_unique<skg_octree>();
+        auto& v = t->nodes;
+        v.emplace_back();
+        std::memcpy(v[0].b, box, 6 * sizeof(double));
+        for (int64_t l = 0; l < static_cast<int64_t>(v.size()); ++l)
+        {
+            if (l >= nflags) return SKG_ERR_ARGUMENT;
+            if (divide[l]) subdivide(v, static_cast<int32_t>(l));
+        }
+        if (static_cast<int64_t>(v.size()) != nflags) return SKG_ERR_ARGUMENT;
+        t->cellOf.assign(v.size(), -1);
+        int32_t m = 0;
+        for (size_t l = 0; l < v.size(); ++l)
+            if (v[l].first < 0) t->cellOf[l] = m++;
+        t->ncells = m;
+        for (size_t l = 0; l < v.size(); ++l)
+        {
+            if (v[l].first >= 0) continue;
+            for (int w = 0; w < 6; ++w)
+            {
+                const double* base = v[l].b;
+                std::sort(v[l].nb[w].begin(), v[l].nb[w].end(), [&](int32_t a, int32_t c) {
+                    return overlapArea(base, v[a].b, w) > overlapArea(base, v[c].b, w);
+                });
+            }
+        }
+        *out = t.release();
+        return SKG_OK;
+    }
+    catch (const std::bad_alloc&)
+    {
+        return SKG_ERR_CAPACITY;
+    }
+}
+
+int skg_sample_density_cartesian(const double* xb, int32_t nx, const double* yb, int32_t ny, const double* zb,
+                                 int32_t nz, const skg_disk_model* model, int32_t num_samples, uint64_t seed,
+                                 int32_t threads, double* out)
+{
+    if (!xb || !yb || !zb || nx < 1 || ny < 1 || nz < 1 || !model || num_samples < 1 || !out) return SKG_ERR_ARGUMENT;
+    Model md(*model);
+    const int64_t n = static_cast<int64_t>(nx) * ny * nz;
+    parallelFor(n, defaultThreads(threads), [&](int64_t b, int64_t e) {
+        for (int64_t m = b; m < e; ++m)
+        {
+            // cell index k + nz*j + nz*ny*i (CartesianSpatialGrid.cpp:200-203)
+            int64_t i = m / (static_cast<int64_t>(nz) * ny), j = (m / nz) % ny, k = m % nz;
+            double bx[6] = {xb[i], yb[j], zb[k], xb[i + 1], yb[j + 1], zb[k + 1]};
+            out[m] = md.nnorm * sampleMean(md, bx, num_samples, seed, static_cast<uint64_t>(m), kTagDensity);
+        }
+    });
+    return SKG_OK;
+}
+
+int skg_sample_density_octree(const skg_octree* t, const skg_disk_model* model, int32_t num_samples, uint64_t seed,
+                              int32_t threads, double* out)
+{
+    if (!t || !model || num_samples < 1 || !out) return SKG_ERR_ARGUMENT;
+    Model md(*model);
+    const auto& v = t->nodes;
+    parallelFor(static_cast<int64_t>(v.size()), defaultThreads(threads), [&](int64_t b, int64_t e) {
+        for (int64_t l = b; l < e; ++l)
+        {
+            int32_t m = t->cellOf[l];
+            if (m < 0) continue;
+            out[m] = md.nnorm * sampleMean(md, v[l].b, num_samples, seed, static_cast<uint64_t>(m), kTagDensity);
+        }
+    });
+    return SKG_OK;
+}
+
+// TabulatedDustMix::getOpticalProperties + NR::clampedResample (TabulatedDustMix.cpp:12-43,
+// NR.hpp:338-434): sigma_abs = mu kappa (1-a) and sigma_sca = mu kappa a resampled log-log,
+// g resampled log-lin, values clamped beyond the table; sigma_ext = sigma_abs + sigma_sca.
+int skg_dust_meanlist(int32_t nin, const double* lam_in, const double* kappa_in, const double* albedo_in,
+                      const double* g_in, double mu, int32_t nout, const double* lam_out, double* sext, double* ssca,
+                      double* g)
+{
+    if (nin < 1 || nout < 1 || !lam_in || !kappa_in || !albedo_in || !g_in || !lam_out || !sext || !ssca || !g)
+        return SKG_ERR_ARGUMENT;
+    std::vector<double> L(lam_in, lam_in + nin), K(kappa_in, kappa_in + nin), A(albedo_in, albedo_in + nin),
+        G(g_in, g_in + nin);
+    if (nin > 1 && L[0] > L[nin - 1])
+    {
+        std::reverse(L.begin(), L.end());
+        std::reverse(K.begin(), K.end());
+        std::reverse(A.begin(), A.end());
+        std::reverse(G.begin(), G.end());
+    }
+    std::vector<double> sabs(nin), ssc(nin);
+    for (int i = 0; i < nin; ++i)
+    {
+        sabs[i] = mu * K[i] * (1. - A[i]);
+        ssc[i] = mu * K[i] * A[i];
+    }
+    auto locate = [&](double x) {
+        // NR::locate
+        if (x == L[nin - 1]) return nin - 2;
+        int jl = -1, ju = nin;
+        while (ju - jl > 1)
+        {
+            int jm = (ju + jl) >> 1;
+            if (x < L[jm])
+                ju = jm;
+            else
+                jl = jm;
+        }
+        return jl;
+    };
+    auto loglog = [](double x, double x1, double x2, double f1, double f2) {
+        if (f1 <= 0 || f2 <= 0)
+        {
+            if (x == x1) return f1;
+            if (x == x2) return f2;
+            return 0.;
+        }
+        return f1 * std::exp(std::log(x / x1) / std::log(x2 / x1) * (std::log(f2 / f1)));
+    };
+    auto loglin = [](double x, double x1, double x2, double f1, double f2) {
+        if (x1 <= 0 || x2 <= 0) return 0.;
+        return f1 + std::log(x / x1) / std::log(x2 / x1) * (f2 - f1);
+    };
+    auto clamped = [&](double x, const std::vector<double>& y, auto interp) {
+        if (nin == 1) return y[0];
+        int i = locate(x);
+        if (i < 0) return y[0];
+        if (i >= nin - 1) return y[nin - 1];
+        return interp(x, L[i], L[i + 1], y[i], y[i + 1]);
+    };
+    for (int l = 0; l < nout; ++l)
+    {
+        double a = clamped(lam_out[l], sabs, loglog);
+        double s = clamped(lam_out[l], ssc, loglog);
+        double gg = clamped(lam_out[l], G, loglin);
+        if (std::fabs(gg) > 0.999999) gg = std::copysign(0.999999, gg);  // DustMix.cpp:131-146
+        ssca[l] = s;
+        sext[l] = a + s;
+        g[l] = gg;
+    }
+    return SKG_OK;
+}
+
+int skg_octree_upload(skg_engine* e, const skg_octree* t)
+{
+    if (!e || !t) return SKG_ERR_ARGUMENT;
+    int64_t nn = 0, nc = 0, nnb = 0;
+    skg_octree_info(t, &nn, &nc, &nnb);
+    std::vector<double> bounds(6 * nn);
+    std::vector<int32_t> cf(nn), ci(nn), nbi(std::max<int64_t>(nnb, 1));
+    std::vector<int64_t> nbo(6 * nn + 1);
+    skg_octree_export(t, bounds.data(), cf.data(), ci.data(), nbo.data(), nbi.data());
+    return skg_grid_octree(e, nn, bounds.data(), cf.data(), ci.data(), nbo.data(), nbi.data());
+}
+
+}  // extern "C"

@@ -1,7 +1,6 @@
 // Device pieces of the forced-scattering photon-packet life cycle.
 //
-// Random numbers: Philox4x32-10 (Salmon et al., SC'11), one counter stream per history
-// (counter = history lo, history hi, block, tag; key = seed), two 52-bit uniforms per block.
+// Random numbers come from one Philox4x32-10 stream per history (philox.hpp).
 // Distributions follow SKIRT's Random (Random.cpp:98-164) and DustMix HG code
 // (DustMix.cpp:395-512); peel-off detection follows FrameInstrument.cpp:117-136.
 #pragma once
@@ -9,67 +8,10 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "philox.hpp"
+
 namespace skg
 {
-    struct Philox
-    {
-        uint32_t k0, k1, h0, h1;
-        uint32_t draw;
-        uint32_t block;
-        uint32_t c[4];
-
-        __device__ __forceinline__ void init(uint64_t seed, uint64_t history)
-        {
-            k0 = static_cast<uint32_t>(seed);
-            k1 = static_cast<uint32_t>(seed >> 32);
-            h0 = static_cast<uint32_t>(history);
-            h1 = static_cast<uint32_t>(history >> 32);
-            draw = 0;
-            block = 0xFFFFFFFFu;
-        }
-
-        __device__ __forceinline__ void gen(uint32_t b)
-        {
-            uint32_t x0 = h0, x1 = h1, x2 = b, x3 = 0u;
-            uint32_t a0 = k0, a1 = k1;
-#pragma unroll
-            for (int r = 0; r < 10; ++r)
-            {
-                if (r)
-                {
-                    a0 += 0x9E3779B9u;
-                    a1 += 0xBB67AE85u;
-                }
-                uint32_t lo0 = 0xD2511F53u * x0;
-                uint32_t hi0 = __umulhi(0xD2511F53u, x0);
-                uint32_t lo1 = 0xCD9E8D57u * x2;
-                uint32_t hi1 = __umulhi(0xCD9E8D57u, x2);
-                uint32_t n0 = hi1 ^ x1 ^ a0;
-                uint32_t n2 = hi0 ^ x3 ^ a1;
-                x0 = n0;
-                x1 = lo1;
-                x2 = n2;
-                x3 = lo0;
-            }
-            c[0] = x0;
-            c[1] = x1;
-            c[2] = x2;
-            c[3] = x3;
-            block = b;
-        }
-
-        __device__ __forceinline__ double uniform()
-        {
-            uint32_t b = draw >> 1;
-            if (b != block) gen(b);
-            uint32_t hi = (draw & 1) ? c[2] : c[0];
-            uint32_t lo = (draw & 1) ? c[3] : c[1];
-            ++draw;
-            uint64_t a = (static_cast<uint64_t>(hi) << 20) ^ (lo >> 12);
-            return (static_cast<double>(a) + 0.5) * (1.0 / 4503599627370496.0);
-        }
-    };
-
     // Direction(theta, phi), Direction.cpp:11-36
     __device__ __forceinline__ void direction_from_angles(double theta, double phi, double& kx, double& ky, double& kz)
     {

@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Photon-packet throughput of the MI355X engine on the octree disk at face-on tau_V = 50.
+
+One step = one launch of the forced-scattering life cycle over a batch of photon packets
+(launch, emission peel-off, forced paths, HG scattering, scattering peel-off into a 200x200
+frame at 60 degrees) on the spiral-disk octree (levels 3..10, max dust fraction 1e-6,
+~2.9e6 leaves), 5 wavelengths 0.3..2 um.  The grid, densities and dust tables are resident
+in HBM before the timed region; `value` is completed histories per second.
+
+Printed JSON line (rank 0, N = 1):
+  roofline      dominant kernel (the life cycle): algorithmic bytes per launch
+                (64 B per octree segment crossing + 8 B per detector atomic) divided by its
+                average duration from HIP events on the engine's stream, against 8 TB/s HBM
+  cpu_baseline  the reference SKIRT CPU path (oracle/_ref) on a bounded sample of the same
+                workload on this host's cores; falls back to the FP64 oracle ("port")
+"""
+
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_SEGMENT = 64.0  # octree: node bounds 48 B + leaf/cell 4 B + neighbour entry 4 B + density 8 B
+BYTES_PER_ATOMIC = 8.0
+
+
+def load_pkg():
+    name = "skirt9_gpu_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "skirt9-gpu_amd", "__init__.py"),
+                                                  submodule_search_locations=[os.path.join(ROOT, "skirt9-gpu_amd")])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def host_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except Exception:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline_reference(tau_v, nlambda, lam_range, packets, threads):
+    """Times the reference SKIRT build on the same workload (its own tree and densities);
+    packets/s over the primary emission phase, as logged by the reference."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "bin", "skirt", "ref", "skirt_ref")
+    if not os.path.exists(exe):
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import skigen
+    spec = skigen.config1(num_packets=packets, tau_v=tau_v, nlambda=nlambda, lam_range=lam_range)
+    with tempfile.TemporaryDirectory() as tmp:
+        ski = os.path.join(tmp, "bench.ski")
+        with open(ski, "w") as f:
+            f.write(skigen.render(spec))
+        res = subprocess.run([exe, "run", ski, tmp, str(threads)], capture_output=True, text=True, timeout=900)
+        if res.returncode != 0:
+            return None
+        log = open(os.path.join(tmp, "bench_log.txt")).read()
+    m = re.search(r"Finished primary emission in ([0-9.]+) s", log)
+    if not m:
+        return None
+    t = float(m.group(1))
+    return dict(value=packets / t, unit="packets/s", cores=threads, kind="reference",
+                sample=f"{packets:.0f} packets of the same workload (reference-built tree), primary emission "
+                       f"{t:.2f} s")
+
+
+def cpu_baseline_port(wl, packets, threads, seed):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    ex = wl.octree.export()
+    g = O.Grid("octree", bounds=ex["bounds"], child_first=ex["child_first"], cell_index=ex["cell_index"],
+               nb_off=ex["nb_off"], nb_id=ex["nb_id"])
+    t0 = time.perf_counter()
+    O.simulate(g, wl.density, wl.lam, wl.sext, wl.ssca, wl.g, np.ones(len(wl.lam)), wl.frames, 0, packets,
+               threads=threads, seed=seed)
+    t = time.perf_counter() - t0
+    return dict(value=packets / t, unit="packets/s", cores=threads, kind="port",
+                sample=f"{packets} packets of the same workload and grid through the FP64 oracle, {t:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--packets", type=int, default=1 << 20, help="photon packets per step")
+    ap.add_argument("--tau", type=float, default=50.0)
+    ap.add_argument("--fp", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--cpu-packets", type=float, default=4e5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--threshold", type=int, default=16)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported on this build "
+              "(packets shard across GPUs with one detector reduction at the end).", file=sys.stderr)
+        sys.exit(2)
+
+    pkg = load_pkg()
+    from skirt9_gpu_amd import workloads
+    threads = host_threads()
+    nlambda, lam_range = 5, (0.3, 2.0)
+    wl = workloads.octree_spiral(tau_v=a.tau, nlambda=nlambda, lam_range_um=lam_range, threads=threads)
+    print(f"setup: {wl.octree.info()[1]} cells, {wl.setup_s:.1f} s", file=sys.stderr, flush=True)
+
+    eng = pkg.Engine(0)
+    wl.apply(eng, npk_norm=1.0)
+    fp = pkg.SKG_FP64 if a.fp == "fp64" else pkg.SKG_FP32_KAHAN
+    eng.options(0.5, 1e4, 0, fp, 12345)
+    eng.tune(0, 256, a.threshold)
+    eng.frame_clear()
+    first = 0
+    for _ in range(a.warmup):
+        eng.run_packets(first, a.packets)
+        first += a.packets
+    c0 = eng.counters()
+    kernel_ms = []
+    eng.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        eng.run_packets(first, a.packets)
+        kernel_ms.append(eng.last_kernel_ms())
+        first += a.packets
+    eng.sync()
+    t1 = time.perf_counter()
+    c1 = eng.counters()
+    dt = t1 - t0
+    d = c1 - c0
+    histories = int(d[0])
+    value = histories / dt
+    avg_ms = float(np.mean(kernel_ms))
+    bytes_per_launch = (BYTES_PER_SEGMENT * d[4] + BYTES_PER_ATOMIC * d[5]) / a.steps
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    print(f"timed: {histories} histories in {dt:.3f} s; per launch {avg_ms:.1f} ms, "
+          f"{d[4] / a.steps:.3e} segments, {d[5] / a.steps:.3e} atomics", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline_reference(a.tau, nlambda, lam_range, a.cpu_packets, threads)
+        except Exception as ex:  # the reference build may be absent or fail on this host
+            print(f"reference CPU baseline unavailable: {ex}", file=sys.stderr)
+        if cpu is None:
+            cpu = cpu_baseline_port(wl, int(a.cpu_packets // 10), threads, 12345)
+
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "lifecycle_octree_tau50_traffic.json")
+    if os.path.exists(prof):
+        try:
+            traffic = json.load(open(prof)).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    line = {
+        "metric": "photon packets/s + 1/(var·time) guided vs plain, 1 MI355X, octree τV=50",
+        "value": value,
+        "unit": "packets/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if fp == pkg.SKG_FP64 else "f32+kahan/f64",
+        "data": "synthetic",
+        "config": {"workload": "octree spiral disk, face-on tauV=%g, 5 wavelengths 0.3-2 um, plain HG (no guide)" % a.tau,
+                   "cells": int(wl.octree.info()[1]), "packets_per_step": a.packets, "frame": "200x200 at 60 deg",
+                   "segments_per_packet": float(d[4] / max(histories, 1))},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "lifecycle", "kernel_ms": avg_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,98 @@
+"""The benchmark and test workloads, set up the way the corresponding ski files configure the
+reference (an isotropic point source at the origin, an exponential dust disk, one HG dust
+mix, forced scattering with path-length bias 0.5, distant frame instruments).
+
+  cartesian_disk   64^3 Cartesian grid over +-10 pc, disk normalised to face-on tau_V = 1,
+                   0.55 um only, albedo 0.6, g 0.6, 200x200 frame at 60 deg
+  octree_spiral    spiral-perturbed disk on a density-refined octree (levels 3..10, max dust
+                   fraction 1e-6, ~2.9e6 leaves), power-law dust kappa ~ lambda^-1.5 with
+                   albedo 0.5 -> 0.1 over 0.1..10 um, at a chosen face-on tau_V and wavelengths
+"""
+
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PC = 3.08567758e16
+MU_DUST = 1.5e-29  # MeanListDustMix mass per hydrogen atom (kg)
+
+
+def _pkg():
+    import sys
+    return sys.modules[__package__] if __package__ else __import__("skirt9_gpu_amd")
+
+
+@dataclass
+class Workload:
+    name: str
+    grid: str
+    lam: np.ndarray
+    sext: np.ndarray
+    ssca: np.ndarray
+    g: np.ndarray
+    density: np.ndarray
+    tables: dict = field(default_factory=dict)  # grid tables (borders or octree export)
+    octree: object = None
+    frames: list = field(default_factory=list)
+    tau_v: float = 1.0
+    setup_s: float = 0.0
+
+    def apply(self, eng, npk_norm=1.0):
+        if self.grid == "cartesian":
+            eng.grid_cartesian(self.tables["xb"], self.tables["yb"], self.tables["zb"])
+        else:
+            self.octree.upload(eng)
+        eng.medium(self.density)
+        eng.dust(self.lam, self.sext, self.ssca, self.g)
+        eng.frame_remove_all()
+        for f in self.frames:
+            eng.frame_add(f["inclination"], f["azimuth"], f["roll"], f["nx"], f["ny"], f["fovx"], f["fovy"])
+        eng.source_point((0.0, 0.0, 0.0), np.full(len(self.lam), 1.0 / npk_norm))
+
+
+def frame(inc_deg, npix=200, fov=20 * PC, az_deg=0.0):
+    return dict(inclination=np.radians(inc_deg), azimuth=np.radians(az_deg), roll=0.0, nx=npix, ny=npix, fovx=fov,
+                fovy=fov)
+
+
+def log_wavelengths(lo_um, hi_um, n):
+    """Log-spaced wavelengths (m), as the ski files list them (9 significant digits)."""
+    v = [lo_um * (hi_um / lo_um) ** (i / (n - 1)) for i in range(n)] if n > 1 else [lo_um]
+    return np.array([float(f"{x:.9g}") for x in v]) * 1e-6
+
+
+def cartesian_disk(n=64, tau_v=1.0, samples=100, seed=1, threads=0):
+    import time
+    pkg = _pkg()
+    t0 = time.time()
+    h = 10 * PC
+    t = np.array([0.0 + i * ((1.0 - 0.0) / n) for i in range(n + 1)])
+    b = t * (h - (-h)) + (-h)
+    sext = MU_DUST * 1.0
+    model = pkg.disk_model(tau_v=tau_v, sext_v=sext)
+    dens = pkg.sample_density_cartesian(b, b, b, model, samples, seed, threads)
+    return Workload("cartesian_disk", "cartesian", np.array([0.55e-6]), np.array([sext]), np.array([0.6 * sext]),
+                    np.array([0.6]), dens, dict(xb=b, yb=b, zb=b), frames=[frame(60.0)], tau_v=tau_v,
+                    setup_s=time.time() - t0)
+
+
+def octree_spiral(tau_v=1.0, nlambda=40, lam_range_um=(0.1, 10.0), min_level=3, max_level=10, max_frac=1e-6,
+                  samples=100, seed=1, threads=0, frames=None):
+    import time
+    pkg = _pkg()
+    t0 = time.time()
+    h = 10 * PC
+    box = [-h, -h, -h, h, h, h]
+    lam = log_wavelengths(lam_range_um[0], lam_range_um[1], nlambda)
+    # power-law dust: kappa 1000 -> 1 m2/kg and albedo 0.5 -> 0.1 between 0.1 and 10 um, g = 0.6
+    tab = np.array([[0.1e-6, 1000.0, 0.5, 0.6], [10e-6, 1.0, 0.1, 0.6]])
+    sext, ssca, g = pkg.dust_meanlist(tab[:, 0], tab[:, 1], tab[:, 2], tab[:, 3], lam, MU_DUST)
+    sv, _, _ = pkg.dust_meanlist(tab[:, 0], tab[:, 1], tab[:, 2], tab[:, 3], [0.55e-6], MU_DUST)
+    model = pkg.disk_model(tau_v=tau_v, sext_v=float(sv[0]), spiral=True)
+    tree = pkg.Octree.build(box, model, min_level, max_level, max_frac, samples, seed, threads)
+    dens = tree.sample_density(model, samples, seed + 1, threads)
+    return Workload("octree_spiral", "octree", lam, sext, ssca, g, dens, octree=tree,
+                    frames=frames if frames is not None else [frame(60.0)], tau_v=tau_v, setup_s=time.time() - t0)

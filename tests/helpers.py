@@ -17,7 +17,8 @@ def load_pkg():
     name = "skirt9_gpu_amd"
     if name in sys.modules:
         return sys.modules[name]
-    spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "skirt9-gpu_amd", "__init__.py"))
+    spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "skirt9-gpu_amd", "__init__.py"),
+                                                  submodule_search_locations=[os.path.join(ROOT, "skirt9-gpu_amd")])
     mod = importlib.util.module_from_spec(spec)
     sys.modules[name] = mod
     spec.loader.exec_module(mod)

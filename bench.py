@@ -8,9 +8,10 @@ frame at 60 degrees) on the spiral-disk octree (levels 3..10, max dust fraction 
 in HBM before the timed region; `value` is completed histories per second.
 
 Printed JSON line (rank 0, N = 1):
-  roofline      dominant kernel (the life cycle): algorithmic bytes per launch
-                (64 B per octree segment crossing + 8 B per detector atomic) divided by its
-                average duration from HIP events on the engine's stream, against 8 TB/s HBM
+  roofline      dominant kernel (pipe_trace, the grid walk of the wavefront pipeline):
+                algorithmic bytes per launch (64 B per octree segment crossing + 96 B of
+                request and packet state per traversal) divided by its average duration from
+                HIP events on the engine's stream, against 8 TB/s HBM
   cpu_baseline  the reference SKIRT CPU path (oracle/_ref) on a bounded sample of the same
                 workload on this host's cores; falls back to the FP64 oracle ("port")
 """
@@ -33,6 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_SEGMENT = 64.0  # octree: node bounds 48 B + leaf/cell 4 B + neighbour entry 4 B + density 8 B
 BYTES_PER_ATOMIC = 8.0
+BYTES_PER_REQUEST = 96.0  # per traversal: request 32 B + packet position/direction/bin 52 B + result 12 B
 
 
 def load_pkg():
@@ -106,6 +108,10 @@ def main():
     ap.add_argument("--cpu-packets", type=float, default=4e5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--threshold", type=int, default=16)
+    ap.add_argument("--mode", choices=["pipeline", "persistent"], default="pipeline")
+    ap.add_argument("--pool", type=int, default=1 << 21, help="packet slots in flight (pipeline)")
+    ap.add_argument("--trace-wgs", type=int, default=0)
+    ap.add_argument("--graph-iters", type=int, default=8)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported on this build "
@@ -124,6 +130,8 @@ def main():
     fp = pkg.SKG_FP64 if a.fp == "fp64" else pkg.SKG_FP32_KAHAN
     eng.options(0.5, 1e4, 0, fp, 12345)
     eng.tune(0, 256, a.threshold)
+    mode = pkg.SKG_MODE_PIPELINE if a.mode == "pipeline" else pkg.SKG_MODE_PERSISTENT
+    eng.tune_pipeline(mode, a.pool, a.trace_wgs, a.graph_iters)
     eng.frame_clear()
     first = 0
     for _ in range(a.warmup):
@@ -131,11 +139,17 @@ def main():
         first += a.packets
     c0 = eng.counters()
     kernel_ms = []
+    trace_ms, trace_launches, iterations = 0.0, 0, 0
     eng.sync()
     t0 = time.perf_counter()
     for _ in range(a.steps):
         eng.run_packets(first, a.packets)
         kernel_ms.append(eng.last_kernel_ms())
+        if mode == pkg.SKG_MODE_PIPELINE:
+            tm, tl, it = eng.last_run_stats()
+            trace_ms += tm
+            trace_launches += tl
+            iterations += it
         first += a.packets
     eng.sync()
     t1 = time.perf_counter()
@@ -144,11 +158,23 @@ def main():
     d = c1 - c0
     histories = int(d[0])
     value = histories / dt
-    avg_ms = float(np.mean(kernel_ms))
-    bytes_per_launch = (BYTES_PER_SEGMENT * d[4] + BYTES_PER_ATOMIC * d[5]) / a.steps
+    requests = int(d[1] + d[2] + d[6])
+    if mode == pkg.SKG_MODE_PIPELINE:
+        # dominant kernel: the trace kernel; its launches and summed duration from HIP events
+        kname = "pipe_trace"
+        launches = max(trace_launches, 1)
+        avg_ms = trace_ms / launches
+        total_bytes = BYTES_PER_SEGMENT * d[4] + BYTES_PER_REQUEST * requests
+    else:
+        kname = "lifecycle"
+        launches = a.steps
+        avg_ms = float(np.mean(kernel_ms))
+        total_bytes = BYTES_PER_SEGMENT * d[4] + BYTES_PER_ATOMIC * d[5]
+    bytes_per_launch = total_bytes / launches
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    print(f"timed: {histories} histories in {dt:.3f} s; per launch {avg_ms:.1f} ms, "
-          f"{d[4] / a.steps:.3e} segments, {d[5] / a.steps:.3e} atomics", file=sys.stderr, flush=True)
+    print(f"timed: {histories} histories in {dt:.3f} s; {kname}: {launches} launches, avg {avg_ms:.3f} ms, "
+          f"{d[4] / a.steps:.3e} segments and {requests / a.steps:.3e} traversals per step, "
+          f"{iterations / a.steps:.0f} pipeline iterations per step", file=sys.stderr, flush=True)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -185,7 +211,7 @@ def main():
                    "segments_per_packet": float(d[4] / max(histories, 1))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "lifecycle", "kernel_ms": avg_ms},
+                     "kernel": kname, "kernel_ms": avg_ms, "launches_per_step": launches / a.steps},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

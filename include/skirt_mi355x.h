@@ -36,6 +36,9 @@ extern "C" {
 #define SKG_GRID_CARTESIAN 0
 #define SKG_GRID_OCTREE 1
 
+#define SKG_MODE_PIPELINE 0   /* wavefront pipeline: event kernel + persistent trace kernel, hipGraph */
+#define SKG_MODE_PERSISTENT 1 /* one persistent kernel running whole packets per lane */
+
 #define SKG_FP64 0            /* optical depth and path length sums in FP64 */
 #define SKG_FP32_KAHAN 1      /* the same sums in FP32 with compensated summation */
 
@@ -147,9 +150,19 @@ int skg_counters(skg_engine* e, int64_t* out8);
  * waiting before a wave-wide transition round runs (1..64).  Chosen by the autotuner. */
 int skg_tune(skg_engine* e, int32_t workgroups, int32_t threads, int32_t transition_threshold);
 
-/* Duration in milliseconds of the last life-cycle kernel, measured with HIP events on the
- * engine's stream. */
+/* Duration in milliseconds of the last life-cycle run (all its kernels), measured with HIP
+ * events on the engine's stream. */
 double skg_last_kernel_ms(const skg_engine* e);
+
+/* Engine mode and pipeline shape: SKG_MODE_PIPELINE (default) or SKG_MODE_PERSISTENT; the
+ * number of packet slots in flight, workgroups of the trace kernel (0 = 8 per compute unit)
+ * and pipeline iterations per captured hipGraph (even, 2..256).  Chosen by the autotuner. */
+int skg_tune_pipeline(skg_engine* e, int32_t mode, int64_t pool_slots, int32_t trace_workgroups,
+                      int32_t graph_iterations);
+
+/* Of the last pipeline run: summed duration of the trace kernels (HIP events on the engine's
+ * stream), the number of trace launches and of pipeline iterations. */
+int skg_last_run_stats(const skg_engine* e, double* trace_ms, int64_t* trace_launches, int64_t* iterations);
 
 /* ---- medium set-up on the host (mirrors the reference set-up before the photon loop) --- */
 

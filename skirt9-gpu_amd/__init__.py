@@ -68,8 +68,11 @@ EXPORTED = [
     "skg_run_packets_async", "skg_sync", "skg_counters", "skg_tune", "skg_last_kernel_ms",
     "skg_disk_density", "skg_octree_build", "skg_octree_from_flags", "skg_octree_info", "skg_octree_export",
     "skg_octree_upload", "skg_octree_free", "skg_sample_density_cartesian", "skg_sample_density_octree",
-    "skg_dust_meanlist",
+    "skg_dust_meanlist", "skg_tune_pipeline", "skg_last_run_stats",
 ]
+
+SKG_MODE_PIPELINE = 0
+SKG_MODE_PERSISTENT = 1
 
 
 def _p(a, t):
@@ -206,6 +209,15 @@ class Engine:
 
     def tune(self, workgroups=0, threads=256, threshold=16):
         self._chk(lib().skg_tune(self._h, C.c_int32(workgroups), C.c_int32(threads), C.c_int32(threshold)))
+
+    def tune_pipeline(self, mode=0, pool_slots=1 << 21, trace_workgroups=0, graph_iterations=8):
+        self._chk(lib().skg_tune_pipeline(self._h, C.c_int32(mode), C.c_int64(pool_slots), C.c_int32(trace_workgroups),
+                                          C.c_int32(graph_iterations)))
+
+    def last_run_stats(self):
+        ms, launches, iters = C.c_double(), C.c_int64(), C.c_int64()
+        self._chk(lib().skg_last_run_stats(self._h, C.byref(ms), C.byref(launches), C.byref(iters)))
+        return ms.value, launches.value, iters.value
 
     def run_packets(self, first, count):
         self._chk(lib().skg_run_packets(self._h, C.c_uint64(first), C.c_uint64(count)))

@@ -26,6 +26,7 @@
 
 #include "../../include/skirt_mi355x.h"
 #include "lifecycle.hpp"
+#include "pipeline.hpp"
 #include "traverse.hpp"
 
 namespace skg
@@ -569,6 +570,22 @@ struct skg_engine
     // launch shape
     int workgroups = 0, threads = 256, threshold = 16;
     float lastMs = 0.f;
+    // wavefront pipeline
+    int mode = SKG_MODE_PIPELINE;
+    unsigned long long* ctl = nullptr;  // 8 control words followed by the 8 event counters
+    unsigned long long* hostCtl = nullptr;
+    Pool pool{};
+    void* poolMem = nullptr;
+    Request* q[2] = {nullptr, nullptr};
+    double* qtau[2] = {nullptr, nullptr};
+    int64_t qcap = 0;
+    int64_t poolSlots = 1 << 21;
+    int traceBlocks = 0;
+    int graphIters = 8;
+    std::vector<hipEvent_t> traceEv;
+    double lastTraceMs = 0.;
+    int64_t lastTraceLaunches = 0;
+    int64_t lastIterations = 0;
 };
 
 namespace
@@ -729,8 +746,10 @@ int skg_create(int device, skg_engine** out)
     if (st == hipSuccess) st = hipEventCreate(&e->ev0);
     if (st == hipSuccess) st = hipEventCreate(&e->ev1);
     if (st == hipSuccess) st = hipMalloc(&e->queue, sizeof(unsigned long long));
-    if (st == hipSuccess) st = hipMalloc(&e->counters, 8 * sizeof(unsigned long long));
-    if (st == hipSuccess) st = hipMemset(e->counters, 0, 8 * sizeof(unsigned long long));
+    if (st == hipSuccess) st = hipMalloc(&e->ctl, 16 * sizeof(unsigned long long));
+    if (st == hipSuccess) st = hipMemset(e->ctl, 0, 16 * sizeof(unsigned long long));
+    if (st == hipSuccess) e->counters = e->ctl + kCtlCnt;
+    if (st == hipSuccess) st = hipHostMalloc(&e->hostCtl, 16 * sizeof(unsigned long long), hipHostMallocDefault);
     if (st == hipSuccess) st = hipMalloc(&e->frameGeom, kMaxFrames * sizeof(FrameGeom));
     hipDeviceProp_t prop;
     if (st == hipSuccess) st = hipGetDeviceProperties(&prop, device);
@@ -754,8 +773,11 @@ void skg_destroy(skg_engine* e)
     for (void* p : {static_cast<void*>(e->dens), static_cast<void*>(e->dens32), static_cast<void*>(e->lambda),
                     static_cast<void*>(e->sext), static_cast<void*>(e->ssca), static_cast<void*>(e->gpar),
                     static_cast<void*>(e->lpack), static_cast<void*>(e->frameBuf), static_cast<void*>(e->frameGeom), static_cast<void*>(e->queue),
-                    static_cast<void*>(e->counters)})
+                    static_cast<void*>(e->ctl), e->poolMem, static_cast<void*>(e->q[0]), static_cast<void*>(e->q[1]),
+                    static_cast<void*>(e->qtau[0]), static_cast<void*>(e->qtau[1])})
         if (p) hipFree(p);
+    if (e->hostCtl) hipHostFree(e->hostCtl);
+    for (auto ev : e->traceEv) hipEventDestroy(ev);
     hipEventDestroy(e->ev0);
     hipEventDestroy(e->ev1);
     hipStreamDestroy(e->stream);
@@ -1095,19 +1117,10 @@ int skg_tune(skg_engine* e, int32_t workgroups, int32_t threads, int32_t transit
     return SKG_OK;
 }
 
-int skg_run_packets_async(skg_engine* e, uint64_t first_history, uint64_t count)
+} // extern "C"
+
+static int runPersistent(skg_engine* e, uint64_t first_history, uint64_t count)
 {
-    if (!e) return SKG_ERR_ARGUMENT;
-    std::lock_guard<std::mutex> lk(e->mu);
-    hipSetDevice(e->device);
-    if (e->gridType < 0) return fail(e, SKG_ERR_STATE, "skg_run_packets: no grid");
-    if (!e->dens || e->densCells < e->ncells) return fail(e, SKG_ERR_STATE, "skg_run_packets: no medium for this grid");
-    if (e->nlambda < 1 || !e->haveSource) return fail(e, SKG_ERR_STATE, "skg_run_packets: no dust/source");
-    if (e->lpackCount != e->nlambda)
-        return fail(e, SKG_ERR_STATE, "skg_run_packets: source luminosities do not match the dust wavelength bins");
-    int st = ensureFrames(e);
-    if (st) return st;
-    if (count == 0) return SKG_OK;
     SimParams p{};
     p.grid = e->grid;
     p.dens = e->dens;
@@ -1149,6 +1162,221 @@ int skg_run_packets_async(skg_engine* e, uint64_t first_history, uint64_t count)
     HIPCHK(e, hipEventRecord(e->ev1, e->stream));
     return SKG_OK;
 }
+
+
+namespace
+{
+    int ensurePool(skg_engine* e, int64_t slots, int64_t qcap)
+    {
+        if (slots <= e->pool.capacity && qcap <= e->qcap) return SKG_OK;
+        if (e->poolMem) hipFree(e->poolMem);
+        if (e->q[0]) hipFree(e->q[0]);
+        if (e->q[1]) hipFree(e->q[1]);
+        if (e->qtau[0]) hipFree(e->qtau[0]);
+        if (e->qtau[1]) hipFree(e->qtau[1]);
+        e->poolMem = nullptr;
+        e->q[0] = e->q[1] = nullptr;
+        e->qtau[0] = e->qtau[1] = nullptr;
+        e->pool = Pool{};
+        e->qcap = 0;
+        // 13 eight-byte and 5 four-byte arrays per slot
+        size_t bytes = static_cast<size_t>(slots) * (13 * 8 + 5 * 4) + 18 * 256;
+        HIPCHK(e, hipMalloc(&e->poolMem, bytes));
+        char* b = static_cast<char*>(e->poolMem);
+        auto take = [&](size_t sz) {
+            char* r = b;
+            b += (static_cast<size_t>(slots) * sz + 255) / 256 * 256;
+            return r;
+        };
+        Pool& P = e->pool;
+        P.rx = reinterpret_cast<double*>(take(8));
+        P.ry = reinterpret_cast<double*>(take(8));
+        P.rz = reinterpret_cast<double*>(take(8));
+        P.kx = reinterpret_cast<double*>(take(8));
+        P.ky = reinterpret_cast<double*>(take(8));
+        P.kz = reinterpret_cast<double*>(take(8));
+        P.W = reinterpret_cast<double*>(take(8));
+        P.lthr = reinterpret_cast<double*>(take(8));
+        P.taupath = reinterpret_cast<double*>(take(8));
+        P.tint = reinterpret_cast<double*>(take(8));
+        P.res_tau = reinterpret_cast<double*>(take(8));
+        P.res_s = reinterpret_cast<double*>(take(8));
+        P.hist = reinterpret_cast<uint64_t*>(take(8));
+        P.draw = reinterpret_cast<uint32_t*>(take(4));
+        P.res_m = reinterpret_cast<int32_t*>(take(4));
+        P.ell = reinterpret_cast<int32_t*>(take(4));
+        P.nscatt = reinterpret_cast<int32_t*>(take(4));
+        P.state = reinterpret_cast<int32_t*>(take(4));
+        P.capacity = slots;
+        HIPCHK(e, hipMalloc(&e->q[0], qcap * sizeof(Request)));
+        HIPCHK(e, hipMalloc(&e->q[1], qcap * sizeof(Request)));
+        HIPCHK(e, hipMalloc(&e->qtau[0], qcap * sizeof(double)));
+        HIPCHK(e, hipMalloc(&e->qtau[1], qcap * sizeof(double)));
+        e->qcap = qcap;
+        return SKG_OK;
+    }
+
+    template<int GT, int FP> int capturePipeline(skg_engine* e, const PipeParams& p, int eventBlocks, int traceBlocks,
+                                                 int iters, hipGraphExec_t* exec)
+    {
+        const int detectBlocks = e->cus * 4;
+        hipGraph_t graph = nullptr;
+        HIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < iters; ++i)
+        {
+            int par = i & 1;
+            hipLaunchKernelGGL((pipe_event<GT>), dim3(eventBlocks), dim3(256), 0, e->stream, p, par);
+            hipEventRecord(e->traceEv[2 * i], e->stream);
+            hipLaunchKernelGGL((pipe_trace<GT, FP>), dim3(traceBlocks), dim3(256), 0, e->stream, p, par);
+            hipEventRecord(e->traceEv[2 * i + 1], e->stream);
+            hipLaunchKernelGGL(pipe_detect, dim3(detectBlocks), dim3(256), 0, e->stream, p, par);
+        }
+        hipError_t st = hipStreamEndCapture(e->stream, &graph);
+        if (st != hipSuccess) return fail(e, SKG_ERR_DEVICE, std::string("pipeline capture: ") + hipGetErrorString(st));
+        st = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (st != hipSuccess) return fail(e, SKG_ERR_DEVICE, std::string("pipeline instantiate: ") + hipGetErrorString(st));
+        return SKG_OK;
+    }
+
+    int runPipeline(skg_engine* e, uint64_t first_history, uint64_t count)
+    {
+        int ngroups = 0;
+        for (size_t f = 0; f < e->frames.size(); ++f)
+            if (!(f > 0 && e->frames[f].inc == e->frames[f - 1].inc && e->frames[f].az == e->frames[f - 1].az
+                  && e->frames[f].roll == e->frames[f - 1].roll))
+                ++ngroups;
+        int64_t slots = std::max<int64_t>(256, std::min<int64_t>(e->poolSlots, static_cast<int64_t>(count)));
+        slots = (slots + 255) / 256 * 256;
+        int st = ensurePool(e, slots, slots * (1 + ngroups));
+        if (st) return st;
+        PipeParams p{};
+        p.grid = e->grid;
+        p.pool = e->pool;
+        p.pool.capacity = slots;
+        p.dens = e->dens;
+        p.dens32 = e->dens32;
+        p.nlambda = e->nlambda;
+        p.nframes = static_cast<int>(e->frames.size());
+        p.lambda = e->lambda;
+        p.sext = e->sext;
+        p.ssca = e->ssca;
+        p.gpar = e->gpar;
+        p.lpack = e->lpack;
+        for (int q = 0; q < 3; ++q) p.src[q] = e->src[q];
+        p.xi = e->xi;
+        p.min_weight_reduction = e->mwr;
+        p.min_scatt = e->minScatt;
+        p.seed = e->seed;
+        p.first = first_history;
+        p.count = count;
+        p.frame = e->frameBuf;
+        p.frames = e->frameGeom;
+        p.q[0] = e->q[0];
+        p.q[1] = e->q[1];
+        p.qtau[0] = e->qtau[0];
+        p.qtau[1] = e->qtau[1];
+        p.qcap = slots * (1 + ngroups);
+        p.ctl = e->ctl;
+        const int iters = std::max(2, e->graphIters & ~1);
+        while (static_cast<int>(e->traceEv.size()) < 2 * iters)
+        {
+            hipEvent_t ev;
+            HIPCHK(e, hipEventCreate(&ev));
+            e->traceEv.push_back(ev);
+        }
+        const int eventBlocks = static_cast<int>((slots + 255) / 256);
+        const int traceBlocks = e->traceBlocks > 0 ? e->traceBlocks : e->cus * 8;
+        hipGraphExec_t exec = nullptr;
+        if (e->gridType == SKG_GRID_CARTESIAN)
+            st = e->fpMode == SKG_FP64 ? capturePipeline<kCartesian, SKG_FP64>(e, p, eventBlocks, traceBlocks, iters, &exec)
+                                       : capturePipeline<kCartesian, SKG_FP32_KAHAN>(e, p, eventBlocks, traceBlocks, iters, &exec);
+        else
+            st = e->fpMode == SKG_FP64 ? capturePipeline<kOctree, SKG_FP64>(e, p, eventBlocks, traceBlocks, iters, &exec)
+                                       : capturePipeline<kOctree, SKG_FP32_KAHAN>(e, p, eventBlocks, traceBlocks, iters, &exec);
+        if (st) return st;
+        HIPCHK(e, hipEventRecord(e->ev0, e->stream));
+        hipLaunchKernelGGL(pipe_reset, dim3(eventBlocks), dim3(256), 0, e->stream, p.pool, e->ctl);
+        double traceMs = 0.;
+        int64_t launches = 0;
+        int64_t rounds = 0;
+        // every packet needs at most ~2 (1 + scatterings) iterations; the bound only stops a
+        // runaway loop (e.g. non-terminating weights), never a regular run
+        const int64_t maxRounds = 1 + (static_cast<int64_t>(count) / slots + 1) * ((1 << 21) / iters + 64);
+        hipError_t hst = hipSuccess;
+        while (true)
+        {
+            hst = hipGraphLaunch(exec, e->stream);
+            if (hst != hipSuccess) break;
+            ++rounds;
+            hst = hipMemcpyAsync(e->hostCtl, e->ctl, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, e->stream);
+            if (hst != hipSuccess) break;
+            hst = hipStreamSynchronize(e->stream);
+            if (hst != hipSuccess) break;
+            for (int i = 0; i < iters; ++i)
+            {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, e->traceEv[2 * i], e->traceEv[2 * i + 1]) == hipSuccess) traceMs += ms;
+            }
+            launches += iters;
+            unsigned long long active = e->hostCtl[kCtlActive + ((iters - 1) & 1)];
+            if (active == 0) break;
+            if (rounds > maxRounds)
+            {
+                hipGraphExecDestroy(exec);
+                return fail(e, SKG_ERR_DEVICE, "pipeline did not drain");
+            }
+        }
+        hipGraphExecDestroy(exec);
+        if (hst != hipSuccess) return fail(e, SKG_ERR_DEVICE, std::string("pipeline: ") + hipGetErrorString(hst));
+        HIPCHK(e, hipEventRecord(e->ev1, e->stream));
+        e->lastTraceMs = traceMs;
+        e->lastTraceLaunches = launches;
+        e->lastIterations = rounds * iters;
+        return SKG_OK;
+    }
+}
+
+int skg_run_packets_async(skg_engine* e, uint64_t first_history, uint64_t count)
+{
+    if (!e) return SKG_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    if (e->gridType < 0) return fail(e, SKG_ERR_STATE, "skg_run_packets: no grid");
+    if (!e->dens || e->densCells < e->ncells) return fail(e, SKG_ERR_STATE, "skg_run_packets: no medium for this grid");
+    if (e->nlambda < 1 || !e->haveSource) return fail(e, SKG_ERR_STATE, "skg_run_packets: no dust/source");
+    if (e->lpackCount != e->nlambda)
+        return fail(e, SKG_ERR_STATE, "skg_run_packets: source luminosities do not match the dust wavelength bins");
+    int st = ensureFrames(e);
+    if (st) return st;
+    if (count == 0) return SKG_OK;
+    if (e->mode == SKG_MODE_PIPELINE) return runPipeline(e, first_history, count);
+    return runPersistent(e, first_history, count);
+}
+
+int skg_tune_pipeline(skg_engine* e, int32_t mode, int64_t pool_slots, int32_t trace_workgroups, int32_t graph_iterations)
+{
+    if (!e || !(mode == SKG_MODE_PIPELINE || mode == SKG_MODE_PERSISTENT) || pool_slots < 256 || pool_slots > (1LL << 30)
+        || trace_workgroups < 0 || graph_iterations < 2 || graph_iterations > 256)
+        return fail(e, SKG_ERR_ARGUMENT, "skg_tune_pipeline: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->mode = mode;
+    e->poolSlots = pool_slots;
+    e->traceBlocks = trace_workgroups;
+    e->graphIters = graph_iterations;
+    return SKG_OK;
+}
+
+int skg_last_run_stats(const skg_engine* e, double* trace_ms, int64_t* trace_launches, int64_t* iterations)
+{
+    if (!e) return SKG_ERR_ARGUMENT;
+    if (trace_ms) *trace_ms = e->lastTraceMs;
+    if (trace_launches) *trace_launches = e->lastTraceLaunches;
+    if (iterations) *iterations = e->lastIterations;
+    return SKG_OK;
+}
+
+extern "C" {
 
 int skg_sync(skg_engine* e)
 {

@@ -1,0 +1,595 @@
+// Wavefront-style life-cycle pipeline.
+//
+// The photon packets in flight live in a pool of slots (structure-of-arrays in HBM).  One
+// pipeline iteration is two kernels:
+//
+//   event   one thread per slot: consumes the traversal results of the previous trace launch
+//           and advances the packet's life cycle (launch, forced-propagation sampling, the
+//           interaction point, weight updates, HG scattering), then appends the traversals
+//           the packet needs next to a request queue -- one per distinct observer for
+//           peel-off, plus the packet's own forced path (first pass: total optical depth;
+//           second pass: locate the interaction point).  Appends are stream-compacted with
+//           one atomic per wavefront (ballot + mbcnt prefix).  Terminated slots refill from
+//           the global history counter in the same way.
+//   trace   persistent kernel: each lane takes one request at a time from the queue
+//           (wave-batched work stealing) and walks the grid cell by cell; peel-off requests
+//           deposit L exp(-tau) into the frames at the end, forced-path requests write their
+//           result back to the slot.  Its register footprint is small, so many wavefronts
+//           hide the latency of the dependent grid reads.
+//
+// Iterations are captured in a hipGraph; the host replays it until no slot is active.
+// The order of operations within each packet is the reference's
+// (MonteCarloSimulation.cpp:614-694, :1492-1544, :2034-2092); only the interleaving of
+// independent packets differs, which changes nothing but the order of detector additions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "lifecycle.hpp"
+#include "traverse.hpp"
+
+namespace skg
+{
+    enum SlotState : int
+    {
+        kSlotEmpty = 0,    // needs a new history
+        kSlotWaitFwd1 = 1, // waiting for the first pass (total optical depth)
+        kSlotWaitFwd2 = 2, // waiting for the second pass (interaction point)
+        kSlotDone = 3      // no histories left
+    };
+
+    enum RequestKind : int
+    {
+        kReqPeel = 0,
+        kReqFwd1 = 1,
+        kReqFwd2 = 2
+    };
+
+    struct Request
+    {
+        int32_t slot;
+        int32_t kind;
+        int32_t frame;  // peel-off: first instrument of the observer group
+        int32_t pad;
+        double L;       // peel-off: luminosity to deposit
+        double limit;   // peel-off: optical depth beyond which the contribution vanishes;
+                        // second pass: the interaction optical depth
+    };
+
+    struct Pool
+    {
+        double *rx, *ry, *rz, *kx, *ky, *kz;
+        double *W, *lthr, *taupath, *tint;
+        double *res_tau, *res_s;
+        int32_t* res_m;
+        uint64_t* hist;
+        uint32_t* draw;
+        int32_t *ell, *nscatt, *state;
+        int64_t capacity;
+    };
+
+    struct PipeParams
+    {
+        GridView grid;
+        Pool pool;
+        const double* dens;
+        const float* dens32;
+        int nlambda;
+        int nframes;
+        const double* lambda;
+        const double* sext;
+        const double* ssca;
+        const double* gpar;
+        const double* lpack;
+        double src[3];
+        double xi, min_weight_reduction;
+        int min_scatt;
+        int pad;
+        uint64_t seed;
+        uint64_t first, count;
+        double* frame;
+        const FrameGeom* frames;
+        // queues: two request buffers used alternately; ctl holds the control words
+        Request* q[2];
+        double* qtau[2];  // peel-off results (optical depth) per request
+        int64_t qcap;
+        unsigned long long* ctl;  // [0] history counter, [1..2] queue counts, [3..4] queue heads,
+                                  // [5..6] active slots per parity, [8..15] event counters
+    };
+
+    constexpr int kCtlHist = 0;
+    constexpr int kCtlCount = 1;
+    constexpr int kCtlHead = 3;
+    constexpr int kCtlActive = 5;
+    constexpr int kCtlCnt = 8;
+
+    // wave-aggregated append of `want` entries per lane; returns the lane's first index
+    __device__ __forceinline__ unsigned long long wave_reserve(unsigned long long* counter, unsigned want)
+    {
+        // prefix sum of `want` across the wavefront with one atomic by the first active lane
+        unsigned long long mask = __ballot(want > 0);
+        if (!mask) return 0;
+        const int lane = __lane_id();
+        unsigned incl = want;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            unsigned v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        unsigned total = __shfl(incl, 63);
+        unsigned long long base = 0;
+        int leader = __ffsll(static_cast<long long>(mask)) - 1;
+        if (lane == leader) base = atomicAdd(counter, static_cast<unsigned long long>(total));
+        base = __shfl(base, leader);
+        return base + (incl - want);
+    }
+
+    // ------------------------------------------------------------------ event kernel
+    template<int GT> __global__ void __launch_bounds__(256) pipe_event(PipeParams p, int out)
+    {
+        const int lane = __lane_id();
+        const int64_t slot = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+        const bool valid = slot < p.pool.capacity;
+        // reset the queue the following trace launch does not read (the one this kernel's
+        // predecessor trace consumed) and this parity's active counter
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+        {
+            p.ctl[kCtlCount + (out ^ 1)] = 0;
+            p.ctl[kCtlHead + (out ^ 1)] = 0;
+            p.ctl[kCtlActive + (out ^ 1)] = 0;
+        }
+        Pool& P = p.pool;
+        int st = valid ? P.state[slot] : kSlotDone;
+        Philox rng;
+        double rx = 0, ry = 0, rz = 0, kx = 0, ky = 0, kz = 0, W = 0;
+        int ell = 0, nscatt = 0;
+        bool emitFwd = false;
+        int emitFwdKind = kReqFwd1;
+        // peel-off requests: one bit per observer group leader; the luminosity is recomputed when
+        // the request is written (from the direction before scattering, kept in kxo..kzo)
+        unsigned peelMask = 0;
+        bool peelScat = false;
+        double kxo = 0, kyo = 0, kzo = 0;
+        bool terminated = false;
+        unsigned cntHist = 0, cntScat = 0, cntFwd2 = 0;
+
+        if (st == kSlotWaitFwd1 || st == kSlotWaitFwd2)
+        {
+            rng.init(p.seed, P.hist[slot]);
+            rng.draw = P.draw[slot];
+            ell = P.ell[slot];
+            W = P.W[slot];
+            nscatt = P.nscatt[slot];
+            rx = P.rx[slot];
+            ry = P.ry[slot];
+            rz = P.rz[slot];
+            kx = P.kx[slot];
+            ky = P.ky[slot];
+            kz = P.kz[slot];
+        }
+        if (st == kSlotWaitFwd1)
+        {
+            // MonteCarloSimulation::simulateForcedPropagation (:1492-1544), sampling part
+            double taupath = P.res_tau[slot];
+            if (taupath <= 0.)
+            {
+                terminated = true;
+            }
+            else
+            {
+                double xi = p.xi, tint;
+                if (xi == 0.)
+                    tint = expon_cutoff(rng, taupath);
+                else
+                {
+                    tint = rng.uniform() < xi ? rng.uniform() * taupath : expon_cutoff(rng, taupath);
+                    double pp = -exp(-tint) / expm1(-taupath);
+                    double q = (1.0 - xi) * pp + xi / taupath;
+                    W *= pp / q;
+                }
+                P.taupath[slot] = taupath;
+                P.tint[slot] = tint;
+                emitFwd = true;
+                emitFwdKind = kReqFwd2;
+                cntFwd2++;
+            }
+        }
+        else if (st == kSlotWaitFwd2)
+        {
+            // interaction point, albedo weight, propagation, termination rule, peel-off, scattering
+            double taupath = P.taupath[slot];
+            int m = P.res_m[slot];
+            double sint = P.res_s[slot];
+            double n = m >= 0 ? p.dens[m] : 0.;
+            double ksca = n > 0. ? n * p.ssca[ell] : 0.;
+            double kext = n > 0. ? n * p.sext[ell] : 0.;
+            double albedo = kext > 0. ? ksca / kext : 0.;
+            W *= -expm1(-taupath) * albedo;
+            rx += sint * kx;
+            ry += sint * ky;
+            rz += sint * kz;
+            double lam = p.lambda[ell];
+            double L = W / lam;
+            if (L <= 0 || (L <= P.lthr[slot] && nscatt >= p.min_scatt) || nscatt >= (1 << 20))
+            {
+                terminated = true;
+            }
+            else
+            {
+                // scattering peel-off toward each distinct observer (FluxRecorder::detect skips the
+                // optical depth when no instrument of the group sees the packet)
+                for (int f = 0; f < p.nframes; ++f)
+                {
+                    const FrameGeom& fg = p.frames[f];
+                    if (fg.same_as_prev) continue;
+                    bool seen = false;
+                    for (int f2 = f; f2 < p.nframes && (f2 == f || p.frames[f2].same_as_prev); ++f2)
+                        seen |= p.frames[f2].pixel(rx, ry, rz) >= 0;
+                    if (!seen) continue;
+                    peelMask |= 1u << f;
+                }
+                peelScat = true;
+                kxo = kx;
+                kyo = ky;
+                kzo = kz;
+                scatter_hg(rng, p.gpar[ell], kx, ky, kz);
+                nscatt++;
+                cntScat++;
+                emitFwd = true;
+                emitFwdKind = kReqFwd1;
+            }
+        }
+        if (terminated)
+        {
+            cntHist++;
+            st = kSlotEmpty;
+        }
+
+        // refill empty slots from the history counter (one atomic per wavefront)
+        bool need = valid && st == kSlotEmpty;
+        unsigned long long h = wave_reserve(p.ctl + kCtlHist, need ? 1u : 0u);
+        if (need)
+        {
+            if (h < p.count)
+            {
+                // launch: SourceSystem::launch, PhotonPacket::launch, isotropic point source
+                rng.init(p.seed, p.first + h);
+                ell = static_cast<int>(rng.uniform() * p.nlambda);
+                if (ell >= p.nlambda) ell = p.nlambda - 1;
+                double lam = p.lambda[ell];
+                W = p.lpack[ell] * lam;
+                rx = p.src[0];
+                ry = p.src[1];
+                rz = p.src[2];
+                isotropic(rng, kx, ky, kz);
+                nscatt = 0;
+                if (!(W / lam > 0))
+                {
+                    // zero-luminosity packet: done immediately
+                    cntHist++;
+                    st = kSlotEmpty;
+                }
+                else
+                {
+                    P.hist[slot] = p.first + h;
+                    P.lthr[slot] = (W / lam) / p.min_weight_reduction;
+                    for (int f = 0; f < p.nframes; ++f)
+                    {
+                        const FrameGeom& fg = p.frames[f];
+                        if (fg.same_as_prev) continue;
+                        bool seen = false;
+                        for (int f2 = f; f2 < p.nframes && (f2 == f || p.frames[f2].same_as_prev); ++f2)
+                            seen |= p.frames[f2].pixel(rx, ry, rz) >= 0;
+                        if (!seen) continue;
+                        peelMask |= 1u << f;
+                    }
+                    emitFwd = true;
+                    emitFwdKind = kReqFwd1;
+                }
+            }
+            else
+            {
+                st = kSlotDone;
+            }
+        }
+
+        // write the packet state back and append its traversal requests
+        if (emitFwd)
+        {
+            P.state[slot] = emitFwdKind == kReqFwd1 ? kSlotWaitFwd1 : kSlotWaitFwd2;
+            P.draw[slot] = rng.draw;
+            P.ell[slot] = ell;
+            P.W[slot] = W;
+            P.nscatt[slot] = nscatt;
+            P.rx[slot] = rx;
+            P.ry[slot] = ry;
+            P.rz[slot] = rz;
+            P.kx[slot] = kx;
+            P.ky[slot] = ky;
+            P.kz[slot] = kz;
+        }
+        else if (valid)
+        {
+            P.state[slot] = st;
+        }
+        unsigned want = (emitFwd ? 1u : 0u) + __popc(peelMask);
+        unsigned long long base = wave_reserve(p.ctl + kCtlCount + out, want);
+        if (want)
+        {
+            Request* q = p.q[out];
+            unsigned long long o = base;
+            const double lam = p.lambda[ell];
+            for (int f = 0; f < p.nframes; ++f)
+            {
+                if (!(peelMask & (1u << f))) continue;
+                double Wp = W;
+                if (peelScat)
+                {
+                    const FrameGeom& fg = p.frames[f];
+                    double c = kxo * fg.kobs[0] + kyo * fg.kobs[1] + kzo * fg.kobs[2];
+                    Wp = W * peel_phase(p.gpar[ell], c);
+                }
+                if (o < static_cast<unsigned long long>(p.qcap))
+                {
+                    Request r;
+                    r.slot = static_cast<int32_t>(slot);
+                    r.kind = kReqPeel;
+                    r.frame = f;
+                    r.pad = 0;
+                    r.L = Wp / lam;
+                    r.limit = log(r.L) + 745;  // MediumSystem.cpp:2335
+                    q[o] = r;
+                }
+                ++o;
+            }
+            if (emitFwd && o < static_cast<unsigned long long>(p.qcap))
+            {
+                Request r;
+                r.slot = static_cast<int32_t>(slot);
+                r.kind = emitFwdKind;
+                r.frame = -1;
+                r.pad = 0;
+                r.L = 0.;
+                r.limit = emitFwdKind == kReqFwd2 ? P.tint[slot] : 0.;
+                q[o] = r;
+            }
+        }
+        // active slots and event counters, aggregated per wavefront
+        unsigned long long act = __ballot(emitFwd);
+        unsigned long long nh = 0, ns = 0, nf = 0, npeel = 0;
+        {
+            unsigned a = cntHist, b = cntScat, c = cntFwd2, d = __popc(peelMask);
+            for (int o = 32; o > 0; o >>= 1)
+            {
+                a += __shfl_xor(a, o);
+                b += __shfl_xor(b, o);
+                c += __shfl_xor(c, o);
+                d += __shfl_xor(d, o);
+            }
+            nh = a;
+            ns = b;
+            nf = c;
+            npeel = d;
+        }
+        if (lane == 0)
+        {
+            if (act) atomicAdd(p.ctl + kCtlActive + out, static_cast<unsigned long long>(__popcll(act)));
+            if (nh) atomicAdd(p.ctl + kCtlCnt + 0, nh);
+            if (npeel) atomicAdd(p.ctl + kCtlCnt + 1, npeel);
+            if (ns) atomicAdd(p.ctl + kCtlCnt + 3, ns);
+            if (nf) atomicAdd(p.ctl + kCtlCnt + 6, nf);
+            unsigned long long f1 = __popcll(__ballot(emitFwd && emitFwdKind == kReqFwd1));
+            if (f1) atomicAdd(p.ctl + kCtlCnt + 2, f1);
+        }
+        (void)lane;
+    }
+
+    // ------------------------------------------------------------------ trace kernel
+    template<int GT, int FP> __global__ void __launch_bounds__(256) pipe_trace(PipeParams p, int in)
+    {
+        const int lane = __lane_id();
+        const unsigned long long count = p.ctl[kCtlCount + in];
+        const Request* q = p.q[in];
+        const Pool& P = p.pool;
+        Gen<GT> gen;
+        bool busy = false;
+        bool done = false;
+        int kind = 0, slot = 0, steps = 0, lastm = -1;
+        unsigned long long req = 0;
+        double tau = 0., s = 0., tprev = 0., sprev = 0., limit = 0.;
+        float tau32 = 0.f, ctau = 0.f, s32 = 0.f, cs = 0.f;
+        double section = 0.;
+        float section32 = 0.f;
+        unsigned long long segs = 0;
+        while (true)
+        {
+            // idle lanes take the next requests, one atomic per wavefront
+            bool idle = !busy && !done;
+            unsigned long long want = __ballot(idle);
+            if (want)
+            {
+                unsigned long long base = wave_reserve(p.ctl + kCtlHead + in, idle ? 1u : 0u);
+                if (idle)
+                {
+                    if (base < count && base < static_cast<unsigned long long>(p.qcap))
+                    {
+                        const Request& r = q[base];
+                        req = base;
+                        slot = r.slot;
+                        kind = r.kind;
+                        limit = r.limit;
+                        int ell = P.ell[slot];
+                        section = p.sext[ell];
+                        section32 = static_cast<float>(section);
+                        double dx, dy, dz;
+                        if (kind == kReqPeel)
+                        {
+                            const FrameGeom& fg = p.frames[r.frame];
+                            dx = fg.kobs[0];
+                            dy = fg.kobs[1];
+                            dz = fg.kobs[2];
+                        }
+                        else
+                        {
+                            dx = P.kx[slot];
+                            dy = P.ky[slot];
+                            dz = P.kz[slot];
+                        }
+                        gen.start(P.rx[slot], P.ry[slot], P.rz[slot], dx, dy, dz);
+                        tau = s = tprev = sprev = 0.;
+                        tau32 = ctau = s32 = cs = 0.f;
+                        lastm = -1;
+                        steps = 0;
+                        busy = true;
+                    }
+                    else
+                        done = true;
+                }
+            }
+            if (__ballot(busy) == 0) break;
+            if (!busy) continue;
+
+            // one cell crossing
+            int m;
+            double ds;
+            bool more = gen.next(p.grid, m, ds) && ++steps <= (1 << 26);
+            bool finished = !more;
+            if (more)
+            {
+                ++segs;
+                if (kind == kReqPeel)
+                {
+                    if (m >= 0)
+                    {
+                        if (FP == SKG_FP64)
+                        {
+                            tau += section * p.dens[m] * ds;
+                            if (tau >= limit)
+                            {
+                                tau = INFINITY;
+                                finished = true;
+                            }
+                        }
+                        else
+                        {
+                            float y = section32 * p.dens32[m] * static_cast<float>(ds) - ctau;
+                            float t = tau32 + y;
+                            ctau = (t - tau32) - y;
+                            tau32 = t;
+                            if (static_cast<double>(tau32) >= limit)
+                            {
+                                tau32 = INFINITY;
+                                finished = true;
+                            }
+                        }
+                    }
+                }
+                else if (ds > 0.)
+                {
+                    if (FP == SKG_FP64)
+                    {
+                        tprev = tau;
+                        sprev = s;
+                        s += ds;
+                        if (m >= 0) tau += section * p.dens[m] * ds;
+                    }
+                    else
+                    {
+                        tprev = static_cast<double>(tau32);
+                        sprev = static_cast<double>(s32);
+                        float y = static_cast<float>(ds) - cs;
+                        float t = s32 + y;
+                        cs = (t - s32) - y;
+                        s32 = t;
+                        if (m >= 0)
+                        {
+                            float y2 = section32 * p.dens32[m] * static_cast<float>(ds) - ctau;
+                            float t2 = tau32 + y2;
+                            ctau = (t2 - tau32) - y2;
+                            tau32 = t2;
+                        }
+                    }
+                    lastm = m;
+                    if (kind == kReqFwd2)
+                    {
+                        double tv = FP == SKG_FP64 ? tau : static_cast<double>(tau32);
+                        if (limit < tv)
+                        {
+                            // SpatialGridPath::findInteractionPoint: first segment whose exit optical
+                            // depth exceeds tau_int, linear interpolation of the distance
+                            double sv = FP == SKG_FP64 ? s : static_cast<double>(s32);
+                            P.res_m[slot] = m;
+                            P.res_s[slot] = sprev + ((limit - tprev) / (tv - tprev)) * (sv - sprev);
+                            busy = false;
+                            continue;
+                        }
+                    }
+                }
+            }
+            if (finished)
+            {
+                double tv = FP == SKG_FP64 ? tau : static_cast<double>(tau32);
+                if (kind == kReqPeel)
+                {
+                    p.qtau[in][req] = tv;
+                }
+                else if (kind == kReqFwd1)
+                {
+                    P.res_tau[slot] = tv;
+                }
+                else
+                {
+                    // beyond the last segment: the last segment's cell and exit distance
+                    P.res_m[slot] = lastm;
+                    P.res_s[slot] = FP == SKG_FP64 ? s : static_cast<double>(s32);
+                }
+                busy = false;
+            }
+        }
+        // segment counter, one atomic per wavefront
+        for (int o = 32; o > 0; o >>= 1) segs += __shfl_xor(segs, o);
+        if (lane == 0 && segs) atomicAdd(p.ctl + kCtlCnt + 4, segs);
+    }
+
+    // ------------------------------------------------------------------ detect kernel
+    // FluxRecorder::detect (FluxRecorder.cpp:372-418) for the peel-off requests of one trace
+    // launch: L exp(-tau) into every instrument of the observer group that sees the packet.
+    __global__ void __launch_bounds__(256) pipe_detect(PipeParams p, int in)
+    {
+        const unsigned long long count = p.ctl[kCtlCount + in];
+        const Pool& P = p.pool;
+        unsigned long long atomics = 0;
+        const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
+        const unsigned long long n = count < static_cast<unsigned long long>(p.qcap) ? count : p.qcap;
+        for (unsigned long long i = blockIdx.x * static_cast<unsigned long long>(blockDim.x) + threadIdx.x; i < n;
+             i += stride)
+        {
+            const Request& r = p.q[in][i];
+            if (r.kind != kReqPeel) continue;
+            double v = r.L * exp(-p.qtau[in][i]);
+            int slot = r.slot;
+            double x = P.rx[slot], y = P.ry[slot], z = P.rz[slot];
+            int ell = P.ell[slot];
+            for (int f = r.frame; f < p.nframes && (f == r.frame || p.frames[f].same_as_prev); ++f)
+            {
+                const FrameGeom& fg = p.frames[f];
+                int pix = fg.pixel(x, y, z);
+                if (pix < 0) continue;
+                unsafeAtomicAdd(p.frame + fg.offset + pix + static_cast<int64_t>(ell) * fg.nx * fg.ny, v);
+                ++atomics;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) atomics += __shfl_xor(atomics, o);
+        if (__lane_id() == 0 && atomics) atomicAdd(p.ctl + kCtlCnt + 5, atomics);
+    }
+
+    // resets all control words and marks every slot empty
+    __global__ void pipe_reset(Pool pool, unsigned long long* ctl)
+    {
+        int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+        if (i < kCtlCnt) ctl[i] = 0;
+        if (i < pool.capacity) pool.state[i] = kSlotEmpty;
+    }
+}

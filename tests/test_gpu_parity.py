@@ -99,7 +99,7 @@ def test_empty_and_outside_rays(eng):
     assert off[-1] == 0
 
 
-def _lifecycle_case(eng, npk, seed, fp_mode=0, threshold=16, with_oracle=True):
+def _lifecycle_case(eng, npk, seed, fp_mode=0, threshold=16, with_oracle=True, mode=0, pool=1 << 21, wgs=0, iters=8):
     b, dens = disk_density_cartesian(samples=10, seed=9)
     eng.grid_cartesian(b, b, b)
     eng.medium(dens)
@@ -111,8 +111,9 @@ def _lifecycle_case(eng, npk, seed, fp_mode=0, threshold=16, with_oracle=True):
     eng.source_point((0.0, 0.0, 0.0), [1.0 / npk])
     eng.options(0.5, 1e4, 0, fp_mode, seed)
     eng.tune(0, 256, threshold)
+    eng.tune_pipeline(mode, pool, wgs, iters)
     eng.frame_clear()
-    print("case lifecycle", npk, seed, fp_mode, threshold, flush=True)
+    print("case lifecycle", npk, seed, fp_mode, threshold, mode, pool, wgs, iters, flush=True)
     eng.run_packets(0, npk)
     img = eng.frame_read(0)[0]
     cnt = eng.counters()
@@ -123,10 +124,11 @@ def _lifecycle_case(eng, npk, seed, fp_mode=0, threshold=16, with_oracle=True):
     return img, cnt, ofr[0][0], ocnt
 
 
-def test_lifecycle_matches_oracle_fp64(eng):
+@pytest.mark.parametrize("mode", [0, 1], ids=["pipeline", "persistent"])
+def test_lifecycle_matches_oracle_fp64(eng, mode):
     """Same histories, same random streams: the frame equals the oracle's up to rare histories
     whose discrete decisions flip on last-bit differences of the device math library."""
-    img, cnt, oimg, ocnt = _lifecycle_case(eng, 20000, 3)
+    img, cnt, oimg, ocnt = _lifecycle_case(eng, 20000, 3, mode=mode)
     assert cnt[0] == ocnt[0] == 20000
     assert abs(int(cnt[2]) - int(ocnt[2])) <= 0.001 * ocnt[2]  # forced paths
     assert abs(img.sum() / oimg.sum() - 1) < 1e-6
@@ -136,9 +138,10 @@ def test_lifecycle_matches_oracle_fp64(eng):
     assert np.mean(rel < 1e-6) > 0.99
 
 
-def test_lifecycle_fp32_kahan_close_to_fp64(eng):
-    img64, _, _, _ = _lifecycle_case(eng, 20000, 5, fp_mode=0, with_oracle=False)
-    img32, _, _, _ = _lifecycle_case(eng, 20000, 5, fp_mode=1, with_oracle=False)
+@pytest.mark.parametrize("mode", [0, 1], ids=["pipeline", "persistent"])
+def test_lifecycle_fp32_kahan_close_to_fp64(eng, mode):
+    img64, _, _, _ = _lifecycle_case(eng, 20000, 5, fp_mode=0, with_oracle=False, mode=mode)
+    img32, _, _, _ = _lifecycle_case(eng, 20000, 5, fp_mode=1, with_oracle=False, mode=mode)
     assert abs(img32.sum() / img64.sum() - 1) < 1e-3
     nz = img64 > 0
     rel = np.abs(img32[nz] - img64[nz]) / img64[nz]
@@ -146,8 +149,38 @@ def test_lifecycle_fp32_kahan_close_to_fp64(eng):
 
 
 def test_lifecycle_independent_of_launch_shape(eng):
-    """Results do not depend on the transition threshold (scheduling only)."""
-    a, ca, _, _ = _lifecycle_case(eng, 5000, 7, threshold=1, with_oracle=False)
-    b, cb, _, _ = _lifecycle_case(eng, 5000, 7, threshold=64, with_oracle=False)
-    np.testing.assert_array_equal(ca[[0, 1, 2, 3]], cb[[0, 1, 2, 3]])
-    np.testing.assert_allclose(a, b, rtol=1e-12, atol=0)
+    """Results do not depend on scheduling: transition threshold, engine mode, pool size,
+    trace workgroups and graph length change only the order of detector additions."""
+    a, ca, _, _ = _lifecycle_case(eng, 5000, 7, threshold=1, with_oracle=False, mode=1)
+    b, cb, _, _ = _lifecycle_case(eng, 5000, 7, threshold=64, with_oracle=False, mode=1)
+    c, cc, _, _ = _lifecycle_case(eng, 5000, 7, with_oracle=False, mode=0, pool=512, wgs=3, iters=2)
+    d, cd, _, _ = _lifecycle_case(eng, 5000, 7, with_oracle=False, mode=0, pool=1 << 16, wgs=0, iters=16)
+    for x, cx in ((b, cb), (c, cc), (d, cd)):
+        np.testing.assert_array_equal(ca[[0, 1, 2, 3]], cx[[0, 1, 2, 3]])
+        np.testing.assert_allclose(a, x, rtol=1e-12, atol=0)
+
+
+def test_pipeline_multiple_observers(eng):
+    """Three instruments, two of them the same observer (shared optical depth): each frame
+    matches the oracle's for the same histories."""
+    b, dens = disk_density_cartesian(samples=10, seed=9)
+    eng.grid_cartesian(b, b, b)
+    eng.medium(dens)
+    s = 1.5e-29
+    eng.dust([0.55e-6], [s], [0.6 * s], [0.6])
+    eng.frame_remove_all()
+    frs = [frame60(), frame60(npix=100), dict(frame60(), inclination=np.radians(20.0), azimuth=np.radians(30.0))]
+    for fr in frs:
+        eng.frame_add(fr["inclination"], fr["azimuth"], fr["roll"], fr["nx"], fr["ny"], fr["fovx"], fr["fovy"])
+    npk = 8000
+    eng.source_point((0.0, 0.0, 0.0), [1.0 / npk])
+    eng.options(0.5, 1e4, 0, 0, 21)
+    eng.tune_pipeline(0, 1 << 21, 0, 8)
+    eng.frame_clear()
+    print("case multiple observers", flush=True)
+    eng.run_packets(0, npk)
+    g = O.Grid("cartesian", xb=b, yb=b, zb=b)
+    ofr, _ = O.simulate(g, dens, [0.55e-6], [s], [0.6 * s], [0.6], [1.0 / npk], frs, 0, npk, threads=8, seed=21)
+    for i in range(3):
+        img = eng.frame_read(i)[0]
+        assert abs(img.sum() / ofr[i][0].sum() - 1) < 1e-6, i

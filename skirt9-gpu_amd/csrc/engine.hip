@@ -1145,6 +1145,7 @@ static int runPersistent(skg_engine* e, uint64_t first_history, uint64_t count)
     p.frames = e->frameGeom;
     p.nframes = static_cast<int>(e->frames.size());
     HIPCHK(e, hipMemsetAsync(e->queue, 0, sizeof(unsigned long long), e->stream));
+    (void)hipGetLastError();
     int threads = e->threads;
     int blocks = e->workgroups > 0 ? e->workgroups : e->cus * 8;
     // never more lanes than packets (rounded up to whole workgroups)
@@ -1226,9 +1227,18 @@ namespace
         {
             int par = i & 1;
             hipLaunchKernelGGL((pipe_event<GT>), dim3(eventBlocks), dim3(256), 0, e->stream, p, par);
-            hipEventRecord(e->traceEv[2 * i], e->stream);
+            // external event nodes: recorded on every replay, so each trace launch is timed
+            hipError_t r1 = hipEventRecordWithFlags(e->traceEv[2 * i], e->stream, hipEventRecordExternal);
             hipLaunchKernelGGL((pipe_trace<GT, FP>), dim3(traceBlocks), dim3(256), 0, e->stream, p, par);
-            hipEventRecord(e->traceEv[2 * i + 1], e->stream);
+            hipError_t r2 = hipEventRecordWithFlags(e->traceEv[2 * i + 1], e->stream, hipEventRecordExternal);
+            if (r1 != hipSuccess || r2 != hipSuccess)
+            {
+                hipGraph_t g = nullptr;
+                hipStreamEndCapture(e->stream, &g);
+                if (g) hipGraphDestroy(g);
+                (void)hipGetLastError();
+                return fail(e, SKG_ERR_DEVICE, "pipeline capture: event record node");
+            }
             hipLaunchKernelGGL(pipe_detect, dim3(detectBlocks), dim3(256), 0, e->stream, p, par);
         }
         hipError_t st = hipStreamEndCapture(e->stream, &graph);
@@ -1318,6 +1328,7 @@ namespace
                 float ms = 0.f;
                 if (hipEventElapsedTime(&ms, e->traceEv[2 * i], e->traceEv[2 * i + 1]) == hipSuccess) traceMs += ms;
             }
+            (void)hipGetLastError();  // a failed timing query must not leave a sticky error
             launches += iters;
             unsigned long long active = e->hostCtl[kCtlActive + ((iters - 1) & 1)];
             if (active == 0) break;

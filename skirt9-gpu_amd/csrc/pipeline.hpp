@@ -358,6 +358,7 @@ namespace skg
         }
         // active slots and event counters, aggregated per wavefront
         unsigned long long act = __ballot(emitFwd);
+        unsigned long long f1 = __popcll(__ballot(emitFwd && emitFwdKind == kReqFwd1));
         unsigned long long nh = 0, ns = 0, nf = 0, npeel = 0;
         {
             unsigned a = cntHist, b = cntScat, c = cntFwd2, d = __popc(peelMask);
@@ -380,7 +381,6 @@ namespace skg
             if (npeel) atomicAdd(p.ctl + kCtlCnt + 1, npeel);
             if (ns) atomicAdd(p.ctl + kCtlCnt + 3, ns);
             if (nf) atomicAdd(p.ctl + kCtlCnt + 6, nf);
-            unsigned long long f1 = __popcll(__ballot(emitFwd && emitFwdKind == kReqFwd1));
             if (f1) atomicAdd(p.ctl + kCtlCnt + 2, f1);
         }
         (void)lane;

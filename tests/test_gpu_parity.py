@@ -179,6 +179,9 @@ def test_pipeline_multiple_observers(eng):
     eng.frame_clear()
     print("case multiple observers", flush=True)
     eng.run_packets(0, npk)
+    ms, launches, iters = eng.last_run_stats()
+    assert ms > 0 and launches >= 2 and iters >= launches
+    assert eng.last_kernel_ms() >= ms
     g = O.Grid("cartesian", xb=b, yb=b, zb=b)
     ofr, _ = O.simulate(g, dens, [0.55e-6], [s], [0.6 * s], [0.6], [1.0 / npk], frs, 0, npk, threads=8, seed=21)
     for i in range(3):

@@ -219,6 +219,38 @@ int skg_dust_meanlist(int32_t nin, const double* lambda_in, const double* kappa_
                       const double* g_in, double mass_per_hydrogen, int32_t nout, const double* lambda_out,
                       double* sext, double* ssca, double* g);
 
+/* ---- learned path guide (no counterpart in the reference) --------------------------------
+ * A 9 -> 64 -> 64 -> 64 -> 40 MLP (BF16 MFMA, FP32 accumulation) mapping a scattering event
+ * (position / half_size, incoming direction, normalised log wavelength, |r|, |z|) to an 8-lobe
+ * von Mises-Fisher mixture of outgoing directions.  In guided mode each scattering samples the
+ * defensive mixture (1 - mix) HG + mix guide and multiplies the packet weight by
+ * Phi_HG / p_mix, which keeps every estimator unbiased; peel-off is unchanged (it uses Phi_HG).
+ * The guide replaces the direction sampling of DustMix::performScattering (DustMix.cpp:496-512). */
+
+#define SKG_GUIDE_PARAMS 11560
+
+/* Parameters in the order W0 (64x9, row-major), b0, W1 (64x64), b1, W2 (64x64), b2,
+ * W3 (40x64), b3.  Setting them resets the optimiser state. */
+int skg_guide_params(skg_engine* e, const float* theta, int32_t n);
+int skg_guide_read_params(skg_engine* e, float* theta, int32_t n);
+/* Raw network outputs (n x 40) for n feature rows (n x 9), host memory. */
+int skg_guide_infer(skg_engine* e, const float* features, int64_t n, float* out);
+/* Gradient of sum_i -w_i ln q(k_i | x_i) with respect to the parameters for n records of 16
+ * floats (9 features, direction k (3), weight w, 3 unused), and the loss. */
+int skg_guide_gradient(skg_engine* e, const float* records, int64_t n, float* grad, double* loss);
+/* mode 0: plain HG; 1: guided (defensive mixture with probability mix of sampling the guide);
+ * 2: plain HG that records training data at every scattering.  half_size, lambda_center (m)
+ * and lambda_scale set the feature normalisation. */
+int skg_guide_mode(skg_engine* e, int32_t mode, double mix, double half_size, double lambda_center,
+                   double lambda_scale);
+/* Allocates (or resets) the training-record buffer for mode-2 runs. */
+int skg_guide_collect(skg_engine* e, int64_t capacity);
+/* Number of records of the last mode-2 run; copies up to cap of them (16 floats each; the
+ * weight is the detector luminosity the packet deposited after the event) when out != NULL. */
+int skg_guide_records(skg_engine* e, int64_t* n, float* out, int64_t cap);
+/* Adam on the collected records: steps minibatches of `batch` records (multiple of 64). */
+int skg_guide_train(skg_engine* e, int32_t steps, int32_t batch, float lr, double* loss_first, double* loss_last);
+
 #ifdef __cplusplus
 }
 #endif

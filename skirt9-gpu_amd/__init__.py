@@ -69,7 +69,11 @@ EXPORTED = [
     "skg_disk_density", "skg_octree_build", "skg_octree_from_flags", "skg_octree_info", "skg_octree_export",
     "skg_octree_upload", "skg_octree_free", "skg_sample_density_cartesian", "skg_sample_density_octree",
     "skg_dust_meanlist", "skg_tune_pipeline", "skg_last_run_stats",
+    "skg_guide_params", "skg_guide_read_params", "skg_guide_infer", "skg_guide_gradient", "skg_guide_mode",
+    "skg_guide_collect", "skg_guide_records", "skg_guide_train",
 ]
+
+GUIDE_PARAMS = 11560
 
 SKG_MODE_PIPELINE = 0
 SKG_MODE_PERSISTENT = 1
@@ -218,6 +222,52 @@ class Engine:
         ms, launches, iters = C.c_double(), C.c_int64(), C.c_int64()
         self._chk(lib().skg_last_run_stats(self._h, C.byref(ms), C.byref(launches), C.byref(iters)))
         return ms.value, launches.value, iters.value
+
+    # ---- path guide
+    def guide_params(self, theta):
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        self._chk(lib().skg_guide_params(self._h, _p(t, P(C.c_float)), C.c_int32(len(t))))
+
+    def guide_read_params(self):
+        t = np.zeros(GUIDE_PARAMS, np.float32)
+        self._chk(lib().skg_guide_read_params(self._h, _p(t, P(C.c_float)), C.c_int32(len(t))))
+        return t
+
+    def guide_infer(self, feats):
+        f = np.ascontiguousarray(feats, dtype=np.float32).reshape(-1, 9)
+        out = np.zeros((len(f), 40), np.float32)
+        self._chk(lib().skg_guide_infer(self._h, _p(f, P(C.c_float)), C.c_int64(len(f)), _p(out, P(C.c_float))))
+        return out
+
+    def guide_gradient(self, records):
+        r = np.ascontiguousarray(records, dtype=np.float32).reshape(-1, 16)
+        g = np.zeros(GUIDE_PARAMS, np.float32)
+        loss = C.c_double()
+        self._chk(lib().skg_guide_gradient(self._h, _p(r, P(C.c_float)), C.c_int64(len(r)), _p(g, P(C.c_float)),
+                                           C.byref(loss)))
+        return g, loss.value
+
+    def guide_mode(self, mode, mix=0.5, half_size=10 * 3.08567758e16, lambda_center=0.775e-6, lambda_scale=1.0):
+        self._chk(lib().skg_guide_mode(self._h, C.c_int32(mode), C.c_double(mix), C.c_double(half_size),
+                                       C.c_double(lambda_center), C.c_double(lambda_scale)))
+
+    def guide_collect(self, capacity):
+        self._chk(lib().skg_guide_collect(self._h, C.c_int64(capacity)))
+
+    def guide_records(self, copy=0):
+        n = C.c_int64()
+        if copy:
+            out = np.zeros((copy, 16), np.float32)
+            self._chk(lib().skg_guide_records(self._h, C.byref(n), _p(out, P(C.c_float)), C.c_int64(copy)))
+            return n.value, out[:min(copy, n.value)]
+        self._chk(lib().skg_guide_records(self._h, C.byref(n), None, C.c_int64(0)))
+        return n.value
+
+    def guide_train(self, steps, batch=4096, lr=1e-3):
+        a, b = C.c_double(), C.c_double()
+        self._chk(lib().skg_guide_train(self._h, C.c_int32(steps), C.c_int32(batch), C.c_float(lr), C.byref(a),
+                                        C.byref(b)))
+        return a.value, b.value
 
     def run_packets(self, first, count):
         self._chk(lib().skg_run_packets(self._h, C.c_uint64(first), C.c_uint64(count)))

@@ -40,9 +40,9 @@ namespace skg
         Gen<GT> gen;
         gen.start(r[0], r[1], r[2], r[3], r[4], r[5]);
         int m;
-        double ds;
+        double ds, nd;
         int64_t c = 0, cs = 0;
-        while (gen.next(g, m, ds))
+        while (gen.next(g, m, ds, nd))
         {
             ++c;
             if (ds > 0.) ++cs;
@@ -61,14 +61,21 @@ namespace skg
         Gen<GT> gen;
         gen.start(r[0], r[1], r[2], r[3], r[4], r[5]);
         int m;
-        double ds;
+        double ds, nd;
         int64_t o = off[t];
-        while (gen.next(g, m, ds))
+        while (gen.next(g, m, ds, nd))
         {
             mo[o] = m;
             dso[o] = ds;
             ++o;
         }
+    }
+
+    // copies the cell densities into the octree leaf records
+    __global__ void fill_leaf_density(OctLeaf* leaf, const double* dens, int64_t n)
+    {
+        int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+        if (i < n) leaf[i].dens = dens[i];
     }
 
     // compensated (Kahan) FP32 accumulator used in SKG_FP32_KAHAN mode
@@ -99,7 +106,7 @@ namespace skg
         const double* r = rays + 6 * t;
         Gen<GT> gen;
         int m;
-        double ds;
+        double ds, nd;
         // complete path, cumulative tau over stored (ds > 0) segments
         gen.start(r[0], r[1], r[2], r[3], r[4], r[5]);
         double tau = 0.;
@@ -108,16 +115,16 @@ namespace skg
         const float s32 = static_cast<float>(section);
         int64_t o = toff ? toff[t] : 0;
         const int64_t oend = toff ? toff[t + 1] : 0;
-        while (gen.next(g, m, ds))
+        while (gen.next(g, m, ds, nd))
         {
             if (ds > 0.)
             {
                 if (m >= 0)
                 {
                     if (FP == SKG_FP64)
-                        tau += section * dens[m] * ds;
+                        tau += section * nd * ds;
                     else
-                        k32.add(s32 * dens32[m] * static_cast<float>(ds));
+                        k32.add(s32 * static_cast<float>(nd) * static_cast<float>(ds));
                 }
                 if (tau_cum && o < oend) tau_cum[o++] = (FP == SKG_FP64) ? tau : static_cast<double>(k32.sum);
             }
@@ -129,13 +136,13 @@ namespace skg
         k32.reset();
         double res = 0.;
         bool inf = false;
-        while (gen.next(g, m, ds))
+        while (gen.next(g, m, ds, nd))
         {
             if (m >= 0)
             {
                 if (FP == SKG_FP64)
                 {
-                    tau += section * dens[m] * ds;
+                    tau += section * nd * ds;
                     if (tau >= taumax)
                     {
                         inf = true;
@@ -144,7 +151,7 @@ namespace skg
                 }
                 else
                 {
-                    k32.add(s32 * dens32[m] * static_cast<float>(ds));
+                    k32.add(s32 * static_cast<float>(nd) * static_cast<float>(ds));
                     if (static_cast<double>(k32.sum) >= taumax)
                     {
                         inf = true;
@@ -343,8 +350,8 @@ namespace skg
         __device__ __forceinline__ void step(const SimParams& p)
         {
             int m;
-            double ds;
-            if (!gen.next(p.grid, m, ds) || ++steps > kMaxSteps)
+            double ds, nd;
+            if (!gen.next(p.grid, m, ds, nd) || ++steps > kMaxSteps)
             {
                 ended = true;
                 atomicAdd(lds + 4, static_cast<unsigned int>(steps));
@@ -356,7 +363,7 @@ namespace skg
                 {
                     if (FP == SKG_FP64)
                     {
-                        tau += section * p.dens[m] * ds;
+                        tau += section * nd * ds;
                         if (tau >= taumax)
                         {
                             tau = INFINITY;
@@ -366,7 +373,7 @@ namespace skg
                     }
                     else
                     {
-                        tau32.add(section32 * p.dens32[m] * static_cast<float>(ds));
+                        tau32.add(section32 * static_cast<float>(nd) * static_cast<float>(ds));
                         if (static_cast<double>(tau32.sum) >= taumax)
                         {
                             tau32.sum = INFINITY;
@@ -383,14 +390,14 @@ namespace skg
                     tprev = tau;
                     sprev = s;
                     s += ds;
-                    if (m >= 0) tau += section * p.dens[m] * ds;
+                    if (m >= 0) tau += section * nd * ds;
                 }
                 else
                 {
                     tprev = static_cast<double>(tau32.sum);
                     sprev = static_cast<double>(s32.sum);
                     s32.add(static_cast<float>(ds));
-                    if (m >= 0) tau32.add(section32 * p.dens32[m] * static_cast<float>(ds));
+                    if (m >= 0) tau32.add(section32 * static_cast<float>(nd) * static_cast<float>(ds));
                 }
                 lastm = m;
                 // second pass: stop at the first segment whose exit optical depth exceeds tau_int
@@ -586,6 +593,27 @@ struct skg_engine
     double lastTraceMs = 0.;
     int64_t lastTraceLaunches = 0;
     int64_t lastIterations = 0;
+    // path guide
+    int guideMode = 0;
+    double guideMix = 0.5;
+    GuideFrame gframe{1.0, 0.0, 1.0};
+    float* gTheta = nullptr;  // FP32 parameters, Adam moments and gradient (4 x kGuideParams)
+    uint16_t* gBf = nullptr;  // packed BF16 tiles (W and W^T per layer)
+    float* gBias = nullptr;   // padded FP32 biases
+    GuideNet net{};
+    int adamStep = 0;
+    float* rec = nullptr;
+    uint32_t* recHist = nullptr;
+    double* recA = nullptr;
+    unsigned long long* recCount = nullptr;
+    double* recSum = nullptr;
+    int64_t recCap = 0;
+    int64_t recN = 0;
+    double* contrib = nullptr;
+    int64_t* recLast = nullptr;
+    int64_t slotCapGuide = 0;
+    double* finalLum = nullptr;
+    int64_t finalCap = 0;
 };
 
 namespace
@@ -625,6 +653,17 @@ namespace
         e->gridBufs.clear();
         e->gridType = -1;
         e->ncells = 0;
+    }
+
+    int refreshLeafDensity(skg_engine* e)
+    {
+        if (e->gridType != SKG_GRID_OCTREE || !e->dens || e->densCells < e->ncells) return SKG_OK;
+        int64_t n = e->ncells;
+        hipLaunchKernelGGL(fill_leaf_density, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, e->stream,
+                           const_cast<OctLeaf*>(e->grid.leaf), e->dens, n);
+        HIPCHK(e, hipGetLastError());
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+        return SKG_OK;
     }
 
     template<typename T> int gridUpload(skg_engine* e, const T* src, size_t n, const T** out)
@@ -774,7 +813,11 @@ void skg_destroy(skg_engine* e)
                     static_cast<void*>(e->sext), static_cast<void*>(e->ssca), static_cast<void*>(e->gpar),
                     static_cast<void*>(e->lpack), static_cast<void*>(e->frameBuf), static_cast<void*>(e->frameGeom), static_cast<void*>(e->queue),
                     static_cast<void*>(e->ctl), e->poolMem, static_cast<void*>(e->q[0]), static_cast<void*>(e->q[1]),
-                    static_cast<void*>(e->qtau[0]), static_cast<void*>(e->qtau[1])})
+                    static_cast<void*>(e->qtau[0]), static_cast<void*>(e->qtau[1]), static_cast<void*>(e->gTheta),
+                    static_cast<void*>(e->gBf), static_cast<void*>(e->gBias), static_cast<void*>(e->rec),
+                    static_cast<void*>(e->recHist), static_cast<void*>(e->recA), static_cast<void*>(e->recCount),
+                    static_cast<void*>(e->recSum), static_cast<void*>(e->contrib), static_cast<void*>(e->recLast),
+                    static_cast<void*>(e->finalLum)})
         if (p) hipFree(p);
     if (e->hostCtl) hipHostFree(e->hostCtl);
     for (auto ev : e->traceEv) hipEventDestroy(ev);
@@ -837,28 +880,59 @@ int skg_grid_octree(skg_engine* e, int64_t nnodes, const double* bounds, const i
             if (cell_index[n] < 0) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_octree: leaf without cell index");
             ncells = std::max<int64_t>(ncells, cell_index[n] + 1);
         }
+        else if (cell_index[n] >= 0)
+            return fail(e, SKG_ERR_ARGUMENT, "skg_grid_octree: non-leaf node with a cell index");
     }
-    const int64_t nnb = nb_off[6 * nnodes];
-    for (int64_t i = 0; i < nnb; ++i)
-        if (nb_id[i] < 0 || nb_id[i] >= nnodes) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_octree: bad neighbour id");
-    std::vector<int32_t> off32(6 * nnodes + 1);
-    for (int64_t i = 0; i <= 6 * nnodes; ++i)
+    if (ncells > INT32_MAX) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_octree: too many cells");
+    // cell-indexed leaf records with the neighbour lists translated to cell ids
+    std::vector<OctLeaf> leaves(ncells);
+    std::vector<int32_t> nbc;
+    nbc.reserve(static_cast<size_t>(nb_off[6 * nnodes]));
+    std::vector<char> seen(ncells, 0);
+    for (int64_t n = 0; n < nnodes; ++n)
     {
-        if (i && nb_off[i] < nb_off[i - 1]) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_octree: offsets must not decrease");
-        off32[i] = static_cast<int32_t>(nb_off[i]);
+        if (child_first[n] >= 0) continue;
+        int32_t m = cell_index[n];
+        if (seen[m]) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_octree: duplicate cell index");
+        seen[m] = 1;
+        OctLeaf& L = leaves[m];
+        std::memset(&L, 0, sizeof(L));
+        for (int q = 0; q < 3; ++q)
+        {
+            L.lo[q] = bounds[6 * n + q];
+            L.hi[q] = bounds[6 * n + 3 + q];
+        }
+        L.node = static_cast<int32_t>(n);
+        for (int w = 0; w < 6; ++w)
+        {
+            int64_t b0 = nb_off[6 * n + w], b1 = nb_off[6 * n + w + 1];
+            if (b1 < b0) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_octree: offsets must not decrease");
+            L.off[w] = static_cast<int32_t>(nbc.size());
+            L.first[w] = -1;
+            for (int64_t q = b0; q < b1; ++q)
+            {
+                int32_t id = nb_id[q];
+                if (id < 0 || id >= nnodes || cell_index[id] < 0)
+                    return fail(e, SKG_ERR_ARGUMENT, "skg_grid_octree: a leaf's neighbour must be a leaf");
+                if (q == b0) L.first[w] = cell_index[id];
+                nbc.push_back(cell_index[id]);
+            }
+        }
+        L.off_end = static_cast<int32_t>(nbc.size());
     }
+    for (int64_t m = 0; m < ncells; ++m)
+        if (!seen[m]) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_octree: cell indices must be 0..ncells-1");
     std::lock_guard<std::mutex> lk(e->mu);
     hipSetDevice(e->device);
     freeGrid(e);
     GridView g{};
     int st;
     if ((st = gridUpload(e, bounds, 6 * nnodes, &g.bounds)) || (st = gridUpload(e, child_first, nnodes, &g.child_first))
-        || (st = gridUpload(e, cell_index, nnodes, &g.cell_of))
-        || (st = gridUpload(e, off32.data(), off32.size(), &g.nb_off)))
+        || (st = gridUpload(e, cell_index, nnodes, &g.cell_of)) || (st = gridUpload(e, leaves.data(), leaves.size(), &g.leaf)))
         return st;
-    if (nnb)
+    if (!nbc.empty())
     {
-        if ((st = gridUpload(e, nb_id, nnb, &g.nb_id))) return st;
+        if ((st = gridUpload(e, nbc.data(), nbc.size(), &g.nbc))) return st;
     }
     for (int q = 0; q < 6; ++q) g.box[q] = bounds[q];
     double dx = g.box[3] - g.box[0], dy = g.box[4] - g.box[1], dz = g.box[5] - g.box[2];
@@ -866,7 +940,7 @@ int skg_grid_octree(skg_engine* e, int64_t nnodes, const double* bounds, const i
     e->grid = g;
     e->gridType = SKG_GRID_OCTREE;
     e->ncells = ncells;
-    return SKG_OK;
+    return refreshLeafDensity(e);
 }
 
 int64_t skg_num_cells(const skg_engine* e) { return e ? e->ncells : 0; }
@@ -880,7 +954,7 @@ int skg_medium(skg_engine* e, int64_t ncells, const double* number_density)
     int st;
     if ((st = upload(e, &e->dens, number_density, ncells)) || (st = upload(e, &e->dens32, d32.data(), ncells))) return st;
     e->densCells = ncells;
-    return SKG_OK;
+    return refreshLeafDensity(e);
 }
 
 int skg_dust(skg_engine* e, int32_t nlambda, const double* lambda, const double* sext, const double* ssca,
@@ -921,6 +995,7 @@ int skg_trace_paths(skg_engine* e, const double* rays, int64_t n, int64_t cap, i
     HIPCHK(e, hipMemcpy(dRays, rays, n * 6 * sizeof(double), hipMemcpyHostToDevice));
     int blocks = static_cast<int>((n + 255) / 256);
     GridView g = e->grid;
+    g.dens = e->dens;
     dispatchGrid(e, [&](auto gt) {
         hipLaunchKernelGGL((trace_count<decltype(gt)::value>), dim3(blocks), dim3(256), 0, e->stream, g, dRays, n, dCnt);
         return 0;
@@ -998,6 +1073,7 @@ int skg_trace_tau(skg_engine* e, const double* rays, int64_t n, int32_t ell, dou
     double taumax = L > 0 ? std::log(L) + 745 : -INFINITY;
     int blocks = static_cast<int>((n + 255) / 256);
     GridView g = e->grid;
+    g.dens = e->dens;
     dispatchGrid(e, [&](auto gt) {
         if (fp_mode == SKG_FP64)
             hipLaunchKernelGGL((trace_tau<decltype(gt)::value, SKG_FP64>), dim3(blocks), dim3(256), 0, e->stream, g, e->dens,
@@ -1123,6 +1199,7 @@ static int runPersistent(skg_engine* e, uint64_t first_history, uint64_t count)
 {
     SimParams p{};
     p.grid = e->grid;
+    p.grid.dens = e->dens;
     p.dens = e->dens;
     p.dens32 = e->dens32;
     p.nlambda = e->nlambda;
@@ -1226,7 +1303,12 @@ namespace
         for (int i = 0; i < iters; ++i)
         {
             int par = i & 1;
-            hipLaunchKernelGGL((pipe_event<GT>), dim3(eventBlocks), dim3(256), 0, e->stream, p, par);
+            if (p.guideMode == 1)
+                hipLaunchKernelGGL((pipe_event<GT, 1>), dim3(eventBlocks), dim3(256), 0, e->stream, p, par);
+            else if (p.guideMode == 2)
+                hipLaunchKernelGGL((pipe_event<GT, 2>), dim3(eventBlocks), dim3(256), 0, e->stream, p, par);
+            else
+                hipLaunchKernelGGL((pipe_event<GT, 0>), dim3(eventBlocks), dim3(256), 0, e->stream, p, par);
             // external event nodes: recorded on every replay, so each trace launch is timed
             hipError_t r1 = hipEventRecordWithFlags(e->traceEv[2 * i], e->stream, hipEventRecordExternal);
             hipLaunchKernelGGL((pipe_trace<GT, FP>), dim3(traceBlocks), dim3(256), 0, e->stream, p, par);
@@ -1262,6 +1344,8 @@ namespace
         if (st) return st;
         PipeParams p{};
         p.grid = e->grid;
+        p.grid.dens = e->dens;
+    p.grid.dens = e->dens;
         p.pool = e->pool;
         p.pool.capacity = slots;
         p.dens = e->dens;
@@ -1288,6 +1372,41 @@ namespace
         p.qtau[1] = e->qtau[1];
         p.qcap = slots * (1 + ngroups);
         p.ctl = e->ctl;
+        p.guideMode = e->guideMode;
+        p.guideMix = e->guideMix;
+        p.net = e->net;
+        p.gframe = e->gframe;
+        if (e->guideMode == 1 && !e->gTheta) return fail(e, SKG_ERR_STATE, "guided run without guide parameters");
+        if (e->guideMode == 2)
+        {
+            if (!e->rec) return fail(e, SKG_ERR_STATE, "collect run without a record buffer (skg_guide_collect)");
+            if (e->slotCapGuide < slots)
+            {
+                if (e->contrib) hipFree(e->contrib);
+                if (e->recLast) hipFree(e->recLast);
+                e->contrib = nullptr;
+                e->recLast = nullptr;
+                HIPCHK(e, hipMalloc(&e->contrib, slots * sizeof(double)));
+                HIPCHK(e, hipMalloc(&e->recLast, slots * sizeof(int64_t)));
+                e->slotCapGuide = slots;
+            }
+            if (e->finalCap < static_cast<int64_t>(count))
+            {
+                if (e->finalLum) hipFree(e->finalLum);
+                e->finalLum = nullptr;
+                HIPCHK(e, hipMalloc(&e->finalLum, count * sizeof(double)));
+                e->finalCap = static_cast<int64_t>(count);
+            }
+            HIPCHK(e, hipMemsetAsync(e->recCount, 0, sizeof(unsigned long long), e->stream));
+            p.rec = e->rec;
+            p.recHist = e->recHist;
+            p.recA = e->recA;
+            p.recCount = e->recCount;
+            p.recCap = e->recCap;
+            p.contrib = e->contrib;
+            p.recLast = e->recLast;
+            p.finalLum = e->finalLum;
+        }
         const int iters = std::max(2, e->graphIters & ~1);
         while (static_cast<int>(e->traceEv.size()) < 2 * iters)
         {
@@ -1341,6 +1460,17 @@ namespace
         hipGraphExecDestroy(exec);
         if (hst != hipSuccess) return fail(e, SKG_ERR_DEVICE, std::string("pipeline: ") + hipGetErrorString(hst));
         HIPCHK(e, hipEventRecord(e->ev1, e->stream));
+        if (e->guideMode == 2)
+        {
+            unsigned long long nrec = 0;
+            HIPCHK(e, hipMemcpy(&nrec, e->recCount, sizeof(nrec), hipMemcpyDeviceToHost));
+            int64_t n = std::min<int64_t>(static_cast<int64_t>(nrec), e->recCap);
+            if (n > 0)
+                hipLaunchKernelGGL(guide_finalize_records, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                                   e->stream, e->rec, e->recHist, e->recA, e->finalLum, n);
+            HIPCHK(e, hipStreamSynchronize(e->stream));
+            e->recN = n;
+        }
         e->lastTraceMs = traceMs;
         e->lastTraceLaunches = launches;
         e->lastIterations = rounds * iters;
@@ -1388,6 +1518,231 @@ int skg_last_run_stats(const skg_engine* e, double* trace_ms, int64_t* trace_lau
 }
 
 extern "C" {
+
+
+namespace
+{
+    int ensureGuide(skg_engine* e)
+    {
+        if (e->gTheta) return SKG_OK;
+        HIPCHK(e, hipMalloc(&e->gTheta, 4 * kGuideParams * sizeof(float)));
+        HIPCHK(e, hipMemset(e->gTheta, 0, 4 * kGuideParams * sizeof(float)));
+        const size_t tiles = 64 * 16 + 64 * 64 + 64 * 64 + 48 * 64;
+        HIPCHK(e, hipMalloc(&e->gBf, 2 * tiles * sizeof(uint16_t)));
+        HIPCHK(e, hipMalloc(&e->gBias, (64 * 3 + 48) * sizeof(float)));
+        uint16_t* w = e->gBf;
+        uint16_t* t = e->gBf + tiles;
+        const size_t sz[4] = {64 * 16, 64 * 64, 64 * 64, 48 * 64};
+        for (int l = 0; l < 4; ++l)
+        {
+            e->net.W[l] = w;
+            e->net.WT[l] = t;
+            w += sz[l];
+            t += sz[l];
+        }
+        e->net.b[0] = e->gBias;
+        e->net.b[1] = e->gBias + 64;
+        e->net.b[2] = e->gBias + 128;
+        e->net.b[3] = e->gBias + 192;
+        return SKG_OK;
+    }
+
+    int packGuide(skg_engine* e)
+    {
+        const int total = 64 * 16 + 64 * 64 + 64 * 64 + 48 * 64;
+        hipLaunchKernelGGL(guide_pack_kernel, dim3((total + 255) / 256), dim3(256), 0, e->stream, e->gTheta,
+                           const_cast<uint16_t*>(e->net.W[0]), const_cast<uint16_t*>(e->net.W[1]),
+                           const_cast<uint16_t*>(e->net.W[2]), const_cast<uint16_t*>(e->net.W[3]),
+                           const_cast<uint16_t*>(e->net.WT[0]), const_cast<uint16_t*>(e->net.WT[1]),
+                           const_cast<uint16_t*>(e->net.WT[2]), const_cast<uint16_t*>(e->net.WT[3]),
+                           const_cast<float*>(e->net.b[0]), const_cast<float*>(e->net.b[1]),
+                           const_cast<float*>(e->net.b[2]), const_cast<float*>(e->net.b[3]));
+        HIPCHK(e, hipGetLastError());
+        return SKG_OK;
+    }
+}
+
+int skg_guide_params(skg_engine* e, const float* theta, int32_t n)
+{
+    if (!e || !theta || n != kGuideParams) return fail(e, SKG_ERR_ARGUMENT, "skg_guide_params: expected 11560 values");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    int st = ensureGuide(e);
+    if (st) return st;
+    HIPCHK(e, hipMemcpy(e->gTheta, theta, n * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemset(e->gTheta + kGuideParams, 0, 3 * kGuideParams * sizeof(float)));
+    e->adamStep = 0;
+    st = packGuide(e);
+    if (st) return st;
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return SKG_OK;
+}
+
+int skg_guide_read_params(skg_engine* e, float* theta, int32_t n)
+{
+    if (!e || !theta || n != kGuideParams) return fail(e, SKG_ERR_ARGUMENT, "skg_guide_read_params: expected 11560 values");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    if (!e->gTheta) return fail(e, SKG_ERR_STATE, "skg_guide_read_params: no guide");
+    HIPCHK(e, hipMemcpy(theta, e->gTheta, n * sizeof(float), hipMemcpyDeviceToHost));
+    return SKG_OK;
+}
+
+int skg_guide_infer(skg_engine* e, const float* features, int64_t n, float* out)
+{
+    if (!e || n < 0 || (n && (!features || !out))) return fail(e, SKG_ERR_ARGUMENT, "skg_guide_infer: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    if (!e->gTheta) return fail(e, SKG_ERR_STATE, "skg_guide_infer: no guide parameters");
+    if (n == 0) return SKG_OK;
+    float *dF = nullptr, *dO = nullptr;
+    HIPCHK(e, hipMalloc(&dF, n * kGuideIn * sizeof(float)));
+    HIPCHK(e, hipMalloc(&dO, n * kGuideOut * sizeof(float)));
+    HIPCHK(e, hipMemcpy(dF, features, n * kGuideIn * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(guide_infer_kernel, dim3(static_cast<unsigned>((n + 63) / 64)), dim3(64), 0, e->stream, e->net, dF,
+                       n, dO);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipMemcpyAsync(out, dO, n * kGuideOut * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    hipFree(dF);
+    hipFree(dO);
+    return SKG_OK;
+}
+
+int skg_guide_gradient(skg_engine* e, const float* records, int64_t n, float* grad, double* loss)
+{
+    if (!e || n < 1 || !records || !grad) return fail(e, SKG_ERR_ARGUMENT, "skg_guide_gradient: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    if (!e->gTheta) return fail(e, SKG_ERR_STATE, "skg_guide_gradient: no guide parameters");
+    float* dR = nullptr;
+    double* dL = nullptr;
+    HIPCHK(e, hipMalloc(&dR, n * 16 * sizeof(float)));
+    HIPCHK(e, hipMalloc(&dL, sizeof(double)));
+    HIPCHK(e, hipMemcpy(dR, records, n * 16 * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(e, hipMemset(dL, 0, sizeof(double)));
+    float* g = e->gTheta + 3 * kGuideParams;
+    HIPCHK(e, hipMemset(g, 0, kGuideParams * sizeof(float)));
+    hipLaunchKernelGGL(guide_grad_kernel, dim3(static_cast<unsigned>((n + 63) / 64)), dim3(64), 0, e->stream, e->net, dR, 16,
+                       n, 1.0f, g, dL);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipMemcpyAsync(grad, g, kGuideParams * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    double l = 0.;
+    HIPCHK(e, hipMemcpyAsync(&l, dL, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    HIPCHK(e, hipMemset(g, 0, kGuideParams * sizeof(float)));
+    if (loss) *loss = l;
+    hipFree(dR);
+    hipFree(dL);
+    return SKG_OK;
+}
+
+int skg_guide_mode(skg_engine* e, int32_t mode, double mix, double half_size, double lambda_center, double lambda_scale)
+{
+    if (!e || mode < 0 || mode > 2 || !(mix >= 0 && mix < 1) || !(half_size > 0) || !(lambda_scale > 0))
+        return fail(e, SKG_ERR_ARGUMENT, "skg_guide_mode: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->guideMode = mode;
+    e->guideMix = mix;
+    e->gframe.inv_half = 1.0 / half_size;
+    e->gframe.lam_c = std::log(lambda_center);
+    e->gframe.lam_inv_s = 1.0 / lambda_scale;
+    return SKG_OK;
+}
+
+int skg_guide_collect(skg_engine* e, int64_t capacity)
+{
+    if (!e || capacity < 0) return fail(e, SKG_ERR_ARGUMENT, "skg_guide_collect: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    if (capacity > e->recCap)
+    {
+        for (void* q : {static_cast<void*>(e->rec), static_cast<void*>(e->recHist), static_cast<void*>(e->recA)})
+            if (q) hipFree(q);
+        e->rec = nullptr;
+        e->recHist = nullptr;
+        e->recA = nullptr;
+        HIPCHK(e, hipMalloc(&e->rec, capacity * 16 * sizeof(float)));
+        HIPCHK(e, hipMalloc(&e->recHist, capacity * sizeof(uint32_t)));
+        HIPCHK(e, hipMalloc(&e->recA, capacity * sizeof(double)));
+        e->recCap = capacity;
+    }
+    if (!e->recCount) HIPCHK(e, hipMalloc(&e->recCount, sizeof(unsigned long long)));
+    if (!e->recSum) HIPCHK(e, hipMalloc(&e->recSum, sizeof(double)));
+    HIPCHK(e, hipMemset(e->recCount, 0, sizeof(unsigned long long)));
+    e->recN = 0;
+    return SKG_OK;
+}
+
+int skg_guide_records(skg_engine* e, int64_t* n, float* out, int64_t cap)
+{
+    if (!e || !n) return fail(e, SKG_ERR_ARGUMENT, "skg_guide_records: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    *n = e->recN;
+    if (out && cap > 0 && e->recN > 0)
+    {
+        int64_t m = std::min(cap, e->recN);
+        HIPCHK(e, hipMemcpy(out, e->rec, m * 16 * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return SKG_OK;
+}
+
+int skg_guide_train(skg_engine* e, int32_t steps, int32_t batch, float lr, double* loss_first, double* loss_last)
+{
+    if (!e || steps < 1 || batch < 64 || batch % 64 || !(lr > 0)) return fail(e, SKG_ERR_ARGUMENT, "skg_guide_train: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    if (!e->gTheta) return fail(e, SKG_ERR_STATE, "skg_guide_train: no guide parameters");
+    if (e->recN < batch) return fail(e, SKG_ERR_STATE, "skg_guide_train: fewer records than one batch");
+    // mean training weight over all records, so that a batch's weights sum to about one
+    std::vector<float> w(e->recN);
+    HIPCHK(e, hipMemcpy2D(w.data(), sizeof(float), e->rec + 12, 16 * sizeof(float), sizeof(float), e->recN,
+                          hipMemcpyDeviceToHost));
+    double sum = 0.;
+    for (float x : w) sum += x;
+    if (!(sum > 0)) return fail(e, SKG_ERR_STATE, "skg_guide_train: the records carry no detector luminosity");
+    const float cscale = static_cast<float>(static_cast<double>(e->recN) / (static_cast<double>(batch) * sum));
+    double* dL = nullptr;
+    HIPCHK(e, hipMalloc(&dL, sizeof(double)));
+    float* th = e->gTheta;
+    float* m1 = e->gTheta + kGuideParams;
+    float* m2 = e->gTheta + 2 * kGuideParams;
+    float* g = e->gTheta + 3 * kGuideParams;
+    HIPCHK(e, hipMemset(g, 0, kGuideParams * sizeof(float)));
+    const int64_t nb = e->recN / batch;
+    for (int s = 0; s < steps; ++s)
+    {
+        const int64_t off = (s % nb) * batch;
+        bool track = s == 0 || s == steps - 1;
+        if (track) HIPCHK(e, hipMemsetAsync(dL, 0, sizeof(double), e->stream));
+        hipLaunchKernelGGL(guide_grad_kernel, dim3(batch / 64), dim3(64), 0, e->stream, e->net, e->rec + off * 16, 16,
+                           static_cast<int64_t>(batch), cscale, g, track ? dL : nullptr);
+        ++e->adamStep;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.f - std::pow(b1, static_cast<float>(e->adamStep));
+        const float c2 = 1.f - std::pow(b2, static_cast<float>(e->adamStep));
+        hipLaunchKernelGGL(guide_adam_kernel, dim3((kGuideParams + 255) / 256), dim3(256), 0, e->stream, th, m1, m2, g,
+                           kGuideParams, lr, b1, b2, 1e-8f, c1, c2);
+        int st = packGuide(e);
+        if (st)
+        {
+            hipFree(dL);
+            return st;
+        }
+        if (track)
+        {
+            double l = 0.;
+            HIPCHK(e, hipMemcpyAsync(&l, dL, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+            HIPCHK(e, hipStreamSynchronize(e->stream));
+            if (s == 0 && loss_first) *loss_first = l;
+            if (s == steps - 1 && loss_last) *loss_last = l;
+        }
+    }
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    hipFree(dL);
+    return SKG_OK;
+}
 
 int skg_sync(skg_engine* e)
 {

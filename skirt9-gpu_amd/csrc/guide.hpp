@@ -1,0 +1,503 @@
+// Path-guide network on the matrix cores: BF16 MFMA (v_mfma_f32_16x16x16_bf16) with FP32
+// accumulation, for inference inside the life cycle and for training.
+//
+// Network: 9 features -> 64 -> 64 -> 64 -> 40 (ReLU on the hidden layers); the 40 outputs
+// are 8 von Mises-Fisher lobes x (mu_x, mu_y, mu_z, kappa_raw, weight_logit).
+//
+// Layout.  A wavefront evaluates 16 samples per MFMA column tile.  Layers are computed in
+// transposed form, H_l^T = W_l H_{l-1}^T: the weights are the A operand (row = output unit,
+// 4 consecutive inputs per lane group) and the activations the B operand (column = sample,
+// 4 consecutive units per lane group).  The MFMA result layout (lane holds 4 consecutive
+// output rows of one sample column) is then exactly the B-operand layout of the next layer,
+// so activations stay in registers between layers.  The backward pass uses the same trick
+// with transposed weight copies; weight gradients contract over samples and go through LDS.
+//
+// Parameter vector (FP32, the order of the CPU statement): W0 (64x9), b0, W1 (64x64), b1,
+// W2 (64x64), b2, W3 (40x64), b3 -- 11,560 values.  BF16 copies, padded to whole tiles
+// (inputs 9 -> 16, outputs 40 -> 48), are rebuilt after every optimiser step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace skg
+{
+    constexpr int kGuideIn = 9;
+    constexpr int kGuideHidden = 64;
+    constexpr int kGuideOut = 40;
+    constexpr int kGuideLobes = 8;
+    constexpr int kGuideParams = 64 * 9 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 40 * 64 + 40;  // 11560
+    constexpr double kGuideKMin = 0.2;
+    constexpr double kGuideKMax = 400.0;
+
+    // offsets of the FP32 parameter blocks
+    constexpr int kOffW0 = 0, kOffB0 = kOffW0 + 64 * 9, kOffW1 = kOffB0 + 64, kOffB1 = kOffW1 + 64 * 64,
+                  kOffW2 = kOffB1 + 64, kOffB2 = kOffW2 + 64 * 64, kOffW3 = kOffB2 + 64, kOffB3 = kOffW3 + 40 * 64;
+
+    typedef short bf16x4 __attribute__((ext_vector_type(4)));
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+    __host__ __device__ __forceinline__ uint16_t f2bf(float f)
+    {
+        uint32_t u = __builtin_bit_cast(uint32_t, f);
+        u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+        return static_cast<uint16_t>(u >> 16);
+    }
+    __host__ __device__ __forceinline__ float bf2f(uint16_t b)
+    {
+        return __builtin_bit_cast(float, static_cast<uint32_t>(b) << 16);
+    }
+
+    // BF16 weight tiles: W_l row-major (out_pad x in_pad) and its transpose (in_pad x out_pad)
+    struct GuideNet
+    {
+        const uint16_t* W[4];
+        const uint16_t* WT[4];
+        const float* b[4];
+    };
+    // padded sizes per layer
+    __host__ __device__ constexpr int guide_in_pad(int l) { return l == 0 ? 16 : 64; }
+    __host__ __device__ constexpr int guide_out_pad(int l) { return l == 3 ? 48 : 64; }
+
+    __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    }
+
+    __device__ __forceinline__ bf16x4 load_bf4(const uint16_t* p)
+    {
+        // 8-byte aligned load of 4 consecutive bf16
+        uint2 v = *reinterpret_cast<const uint2*>(p);
+        bf16x4 r;
+        r[0] = static_cast<short>(v.x & 0xFFFFu);
+        r[1] = static_cast<short>(v.x >> 16);
+        r[2] = static_cast<short>(v.y & 0xFFFFu);
+        r[3] = static_cast<short>(v.y >> 16);
+        return r;
+    }
+    __device__ __forceinline__ bf16x4 pack_bf4(f32x4 v)
+    {
+        bf16x4 r;
+        r[0] = static_cast<short>(f2bf(v[0]));
+        r[1] = static_cast<short>(f2bf(v[1]));
+        r[2] = static_cast<short>(f2bf(v[2]));
+        r[3] = static_cast<short>(f2bf(v[3]));
+        return r;
+    }
+
+    // Forward pass of one 16-sample column tile.  x: B-operand fragment of the input
+    // (features 4g..4g+3 of sample l%16, g = lane/16).  Returns the 3 output tiles (48 rows,
+    // 40 used) in result layout; h1..h3 (post-ReLU, BF16, result layout) are optional outputs
+    // for the backward pass.
+    __device__ __forceinline__ void guide_tile_forward(const GuideNet& net, bf16x4 x, f32x4 out[3], bf16x4* h1,
+                                                       bf16x4* h2, bf16x4* h3)
+    {
+        const int lane = __lane_id();
+        const int row = lane & 15, grp = lane >> 4;
+        bf16x4 cur[4];
+        // layer 0: K = 16 (one k tile), 4 output tiles
+#pragma unroll
+        for (int o = 0; o < 4; ++o)
+        {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma16(load_bf4(net.W[0] + (o * 16 + row) * 16 + grp * 4), x, acc);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = fmaxf(acc[t] + net.b[0][o * 16 + grp * 4 + t], 0.f);
+            cur[o] = pack_bf4(acc);
+            if (h1) h1[o] = cur[o];
+        }
+        // layers 1, 2: K = 64 (4 k tiles = previous output tiles), 4 output tiles
+#pragma unroll
+        for (int l = 1; l <= 2; ++l)
+        {
+            bf16x4 nxt[4];
+#pragma unroll
+            for (int o = 0; o < 4; ++o)
+            {
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int k = 0; k < 4; ++k) acc = mfma16(load_bf4(net.W[l] + (o * 16 + row) * 64 + k * 16 + grp * 4), cur[k], acc);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) acc[t] = fmaxf(acc[t] + net.b[l][o * 16 + grp * 4 + t], 0.f);
+                nxt[o] = pack_bf4(acc);
+            }
+#pragma unroll
+            for (int o = 0; o < 4; ++o)
+            {
+                cur[o] = nxt[o];
+                if (l == 1 && h2) h2[o] = cur[o];
+                if (l == 2 && h3) h3[o] = cur[o];
+            }
+        }
+        // output layer: 3 output tiles (48 rows, 40 used)
+#pragma unroll
+        for (int o = 0; o < 3; ++o)
+        {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) acc = mfma16(load_bf4(net.W[3] + (o * 16 + row) * 64 + k * 16 + grp * 4), cur[k], acc);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] += net.b[3][o * 16 + grp * 4 + t];
+            out[o] = acc;
+        }
+    }
+
+    // ---------------------------------------------------------------- mixture on one sample
+    struct Lobes
+    {
+        double mu[kGuideLobes][3];
+        double kap[kGuideLobes];
+        double w[kGuideLobes];
+    };
+
+    __device__ __forceinline__ double softplus_d(double x) { return x > 20 ? x : log1p(exp(x)); }
+
+    __device__ __forceinline__ void guide_lobes(const float* o, Lobes& L)
+    {
+        double mx = -1e300;
+        for (int j = 0; j < kGuideLobes; ++j) mx = fmax(mx, static_cast<double>(o[5 * j + 4]));
+        double s = 0.;
+        for (int j = 0; j < kGuideLobes; ++j)
+        {
+            double a = o[5 * j], b = o[5 * j + 1], c = o[5 * j + 2];
+            double n = sqrt(a * a + b * b + c * c);
+            n = n > 1e-12 ? n : 1e-12;
+            L.mu[j][0] = a / n;
+            L.mu[j][1] = b / n;
+            L.mu[j][2] = c / n;
+            L.kap[j] = fmin(kGuideKMin + softplus_d(o[5 * j + 3]), kGuideKMax);
+            L.w[j] = exp(static_cast<double>(o[5 * j + 4]) - mx);
+            s += L.w[j];
+        }
+        for (int j = 0; j < kGuideLobes; ++j) L.w[j] /= s;
+    }
+
+    // mixture density per steradian at unit vector k
+    __device__ __forceinline__ double guide_pdf(const Lobes& L, double kx, double ky, double kz)
+    {
+        double q = 0.;
+        for (int j = 0; j < kGuideLobes; ++j)
+        {
+            double c = L.mu[j][0] * kx + L.mu[j][1] * ky + L.mu[j][2] * kz;
+            double kap = L.kap[j];
+            q += L.w[j] * kap / (2 * M_PI * (-expm1(-2 * kap))) * exp(kap * (c - 1));
+        }
+        return q;
+    }
+
+    // samples the mixture: lobe by weight, then the 3-D vMF by w = 1 + ln(u + (1-u) e^{-2 kappa}) / kappa
+    template<typename Rng> __device__ __forceinline__ void guide_sample(const Lobes& L, Rng& rng, double& kx, double& ky, double& kz)
+    {
+        double u = rng.uniform();
+        int j = kGuideLobes - 1;
+        double cum = 0.;
+        for (int q = 0; q < kGuideLobes; ++q)
+        {
+            cum += L.w[q];
+            if (u < cum)
+            {
+                j = q;
+                break;
+            }
+        }
+        double kap = L.kap[j];
+        double w = 1. + log(rng.uniform() * (-expm1(-2 * kap)) + exp(-2 * kap)) / kap;
+        w = fmin(1., fmax(-1., w));
+        double phi = 2 * M_PI * rng.uniform();
+        double s = sqrt(fmax(0., 1. - w * w));
+        double mx = L.mu[j][0], my = L.mu[j][1], mz = L.mu[j][2];
+        // orthonormal frame around mu
+        double ax = fabs(mz) < 0.9 ? 0. : 1., az = fabs(mz) < 0.9 ? 1. : 0.;
+        double e1x = my * az, e1y = mz * ax - mx * az, e1z = -my * ax;
+        double n1 = sqrt(e1x * e1x + e1y * e1y + e1z * e1z);
+        e1x /= n1;
+        e1y /= n1;
+        e1z /= n1;
+        double e2x = my * e1z - mz * e1y, e2y = mz * e1x - mx * e1z, e2z = mx * e1y - my * e1x;
+        double cp = cos(phi), sp = sin(phi);
+        kx = w * mx + s * (cp * e1x + sp * e2x);
+        ky = w * my + s * (cp * e1y + sp * e2y);
+        kz = w * mz + s * (cp * e1z + sp * e2z);
+        double n = sqrt(kx * kx + ky * ky + kz * kz);
+        kx /= n;
+        ky /= n;
+        kz /= n;
+    }
+
+    // guide features of a scattering event (9 values): position / half-size, incoming
+    // direction, normalised log wavelength, |r| / half-size, |z| / half-size
+    struct GuideFrame
+    {
+        double inv_half;
+        double lam_c, lam_inv_s;
+    };
+    __device__ __forceinline__ void guide_features(const GuideFrame& gf, double x, double y, double z, double kx, double ky,
+                                                   double kz, double lam, float* f)
+    {
+        f[0] = static_cast<float>(x * gf.inv_half);
+        f[1] = static_cast<float>(y * gf.inv_half);
+        f[2] = static_cast<float>(z * gf.inv_half);
+        f[3] = static_cast<float>(kx);
+        f[4] = static_cast<float>(ky);
+        f[5] = static_cast<float>(kz);
+        f[6] = static_cast<float>((log(lam) - gf.lam_c) * gf.lam_inv_s);
+        f[7] = static_cast<float>(sqrt(x * x + y * y + z * z) * gf.inv_half);
+        f[8] = static_cast<float>(fabs(z) * gf.inv_half);
+    }
+
+    // Wave-level inference: every lane supplies its 9 features (lanes without a sample supply
+    // anything) and receives its 40 raw outputs.  Every thread of the workgroup must call this
+    // together (it synchronises the workgroup around the LDS staging); each wavefront uses its
+    // own staging buffers.
+    // lds: 64 x 16 bf16 staging for the inputs plus 64 x 48 floats for the outputs.
+    __device__ __forceinline__ void guide_wave_infer(const GuideNet& net, const float* f, float* out40, uint16_t* lds_in,
+                                                     float* lds_out)
+    {
+        const int lane = __lane_id();
+        const int row = lane & 15, grp = lane >> 4;
+        for (int q = 0; q < 16; ++q) lds_in[lane * 16 + q] = q < kGuideIn ? f2bf(f[q]) : 0;
+        __syncthreads();
+        for (int t = 0; t < 4; ++t)
+        {
+            const int s = t * 16 + row;  // sample of this lane's column
+            bf16x4 x = load_bf4(lds_in + s * 16 + grp * 4);
+            f32x4 o[3];
+            guide_tile_forward(net, x, o, nullptr, nullptr, nullptr);
+            for (int ot = 0; ot < 3; ++ot)
+                for (int q = 0; q < 4; ++q) lds_out[s * 48 + ot * 16 + grp * 4 + q] = o[ot][q];
+        }
+        __syncthreads();
+        for (int q = 0; q < kGuideOut; ++q) out40[q] = lds_out[lane * 48 + q];
+    }
+
+    // ---------------------------------------------------------------- kernels
+    // batch inference: features n x 9 (row-major floats) -> outputs n x 40
+    __global__ void __launch_bounds__(64) guide_infer_kernel(GuideNet net, const float* feat, int64_t n, float* out)
+    {
+        __shared__ uint16_t lin[64 * 16];
+        __shared__ float lout[64 * 48];
+        const int lane = threadIdx.x;
+        const int64_t s = blockIdx.x * 64LL + lane;
+        float f[kGuideIn];
+        for (int q = 0; q < kGuideIn; ++q) f[q] = s < n ? feat[s * kGuideIn + q] : 0.f;
+        float o[kGuideOut];
+        guide_wave_infer(net, f, o, lin, lout);
+        if (s < n)
+            for (int q = 0; q < kGuideOut; ++q) out[s * kGuideOut + q] = o[q];
+    }
+
+    // dL/d(raw outputs) of the weighted negative log-likelihood for one sample (FP64 math),
+    // scaled by cw = c_i / sum(c); the expression of the CPU statement
+    __device__ __forceinline__ void guide_dout(const float* o, double kx, double ky, double kz, double cw, float* d)
+    {
+        Lobes L;
+        guide_lobes(o, L);
+        double f[kGuideLobes], q = 0.;
+        double cdot[kGuideLobes];
+        for (int j = 0; j < kGuideLobes; ++j)
+        {
+            cdot[j] = L.mu[j][0] * kx + L.mu[j][1] * ky + L.mu[j][2] * kz;
+            double kap = L.kap[j];
+            f[j] = kap / (2 * M_PI * (-expm1(-2 * kap))) * exp(kap * (cdot[j] - 1));
+            q += L.w[j] * f[j];
+        }
+        const double g = -cw;
+        for (int j = 0; j < kGuideLobes; ++j)
+        {
+            double r = L.w[j] * f[j] / q;
+            double kap = L.kap[j];
+            double sp = kGuideKMin + softplus_d(o[5 * j + 3]);
+            double dlnf = 1 / kap - 2 * exp(-2 * kap) / (-expm1(-2 * kap)) + (cdot[j] - 1);
+            double dk = (sp < kGuideKMax) ? g * r * dlnf : 0.;
+            double sig = 1 / (1 + exp(-static_cast<double>(o[5 * j + 3])));
+            double a = o[5 * j], b = o[5 * j + 1], c = o[5 * j + 2];
+            double nrm = sqrt(a * a + b * b + c * c);
+            nrm = nrm > 1e-12 ? nrm : 1e-12;
+            double s = g * r * kap;
+            double dmx = s * kx, dmy = s * ky, dmz = s * kz;
+            double proj = dmx * L.mu[j][0] + dmy * L.mu[j][1] + dmz * L.mu[j][2];
+            d[5 * j + 0] = static_cast<float>((dmx - proj * L.mu[j][0]) / nrm);
+            d[5 * j + 1] = static_cast<float>((dmy - proj * L.mu[j][1]) / nrm);
+            d[5 * j + 2] = static_cast<float>((dmz - proj * L.mu[j][2]) / nrm);
+            d[5 * j + 3] = static_cast<float>(dk * sig);
+            d[5 * j + 4] = static_cast<float>(g * (r - L.w[j]));
+        }
+    }
+
+    // One training pass over records [0, n): records hold 9 features, the sampled direction
+    // (3) and the contribution weight, multiplied by cscale (so a batch's weights sum to ~1).  Each
+    // wavefront handles 64 samples; weight gradients are contracted over samples with MFMA
+    // through LDS and added to grad (FP32, parameter order).  lossOut accumulates the loss.
+    __global__ void __launch_bounds__(64) guide_grad_kernel(GuideNet net, const float* rec, int recStride, int64_t n,
+                                                            float cscale, float* grad, double* lossOut)
+    {
+        // per-wave staging: inputs, activations h1..h3 and deltas as [sample][unit] bf16
+        __shared__ uint16_t sX[64 * 16];
+        __shared__ uint16_t sH[3][64 * 64];
+        __shared__ uint16_t sD[2][64 * 64];
+        __shared__ float sO[64 * 48];
+        const int lane = threadIdx.x;
+        const int row = lane & 15, grp = lane >> 4;
+        const int64_t s0 = blockIdx.x * 64LL;
+        const int64_t s = s0 + lane;
+        const bool valid = s < n;
+        // inputs
+        for (int q = 0; q < 16; ++q) sX[lane * 16 + q] = (valid && q < kGuideIn) ? f2bf(rec[s * recStride + q]) : 0;
+        __syncthreads();
+        // forward, keeping the activations in LDS
+        for (int t = 0; t < 4; ++t)
+        {
+            const int smp = t * 16 + row;
+            bf16x4 x = load_bf4(sX + smp * 16 + grp * 4);
+            f32x4 o[3];
+            bf16x4 h1[4], h2[4], h3[4];
+            guide_tile_forward(net, x, o, h1, h2, h3);
+            for (int u = 0; u < 4; ++u)
+                for (int q = 0; q < 4; ++q)
+                {
+                    const int unit = u * 16 + grp * 4 + q;
+                    sH[0][smp * 64 + unit] = static_cast<uint16_t>(h1[u][q]);
+                    sH[1][smp * 64 + unit] = static_cast<uint16_t>(h2[u][q]);
+                    sH[2][smp * 64 + unit] = static_cast<uint16_t>(h3[u][q]);
+                }
+            for (int ot = 0; ot < 3; ++ot)
+                for (int q = 0; q < 4; ++q) sO[smp * 48 + ot * 16 + grp * 4 + q] = o[ot][q];
+        }
+        __syncthreads();
+        // loss gradient per sample (this lane's sample)
+        float d[kGuideOut];
+        double lossv = 0.;
+        if (valid)
+        {
+            const float* r = rec + s * recStride;
+            double kx = r[9], ky = r[10], kz = r[11], cw = static_cast<double>(r[12]) * cscale;
+            guide_dout(sO + lane * 48, kx, ky, kz, cw, d);
+            Lobes L;
+            guide_lobes(sO + lane * 48, L);
+            lossv = -cw * log(guide_pdf(L, kx, ky, kz));
+        }
+        else
+            for (int q = 0; q < kGuideOut; ++q) d[q] = 0.f;
+        __syncthreads();
+        for (int q = 0; q < 48; ++q) sD[0][lane * 64 + q] = q < kGuideOut ? f2bf(d[q]) : 0;
+        __syncthreads();
+        // backward through the layers: delta of layer l output in sD[cur] ([sample][unit]),
+        // input activations of layer l in sH[l-1] (or sX for l = 0)
+        int cur = 0;
+        for (int l = 3; l >= 0; --l)
+        {
+            const int outp = guide_out_pad(l), inp = guide_in_pad(l);
+            const int ot_n = outp / 16, it_n = inp / 16;
+            const int wOff = l == 0 ? kOffW0 : l == 1 ? kOffW1 : l == 2 ? kOffW2 : kOffW3;
+            const int bOff = l == 0 ? kOffB0 : l == 1 ? kOffB1 : l == 2 ? kOffB2 : kOffB3;
+            const int nout = l == 3 ? kGuideOut : kGuideHidden, nin = l == 0 ? kGuideIn : kGuideHidden;
+            const uint16_t* Hin = l == 0 ? sX : sH[l - 1];
+            const int hstride = l == 0 ? 16 : 64;
+            // dW_l (out x in) = sum over samples of delta[s][o] * Hin[s][i]: A = delta^T
+            // (row o, k = sample), B = Hin (column i, k = sample)
+            for (int ot = 0; ot < ot_n; ++ot)
+                for (int it = 0; it < it_n; ++it)
+                {
+                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                    for (int kb = 0; kb < 4; ++kb)
+                    {
+                        bf16x4 a, b;
+                        for (int q = 0; q < 4; ++q)
+                        {
+                            const int smp = kb * 16 + grp * 4 + q;
+                            a[q] = static_cast<short>(sD[cur][smp * 64 + ot * 16 + row]);
+                            b[q] = static_cast<short>(Hin[smp * hstride + it * 16 + row]);
+                        }
+                        acc = mfma16(a, b, acc);
+                    }
+                    // result: rows o = ot*16 + grp*4 + q, column i = it*16 + row
+                    for (int q = 0; q < 4; ++q)
+                    {
+                        const int o = ot * 16 + grp * 4 + q, i = it * 16 + row;
+                        if (o < nout && i < nin && acc[q] != 0.f) atomicAdd(grad + wOff + o * nin + i, acc[q]);
+                    }
+                }
+            // bias gradient: sum over samples
+            if (lane < nout)
+            {
+                float sb = 0.f;
+                for (int smp = 0; smp < 64; ++smp) sb += bf2f(sD[cur][smp * 64 + lane]);
+                if (sb != 0.f) atomicAdd(grad + bOff + lane, sb);
+            }
+            if (l == 0) break;
+            // delta of the previous layer: (W_l^T delta^T) * relu'(h_{l-1}); A = W_l^T tiles
+            // (row = input unit, k = output unit), B = delta^T (column = sample, k = output unit)
+            const int nxt = cur ^ 1;
+            for (int t = 0; t < 4; ++t)
+            {
+                const int smp = t * 16 + row;
+                for (int it = 0; it < 4; ++it)
+                {
+                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                    for (int kt = 0; kt < ot_n; ++kt)
+                    {
+                        bf16x4 a = load_bf4(net.WT[l] + (it * 16 + row) * outp + kt * 16 + grp * 4);
+                        bf16x4 b = load_bf4(sD[cur] + smp * 64 + kt * 16 + grp * 4);
+                        acc = mfma16(a, b, acc);
+                    }
+                    for (int q = 0; q < 4; ++q)
+                    {
+                        const int unit = it * 16 + grp * 4 + q;
+                        float hv = bf2f(sH[l - 1][smp * 64 + unit]);
+                        sD[nxt][smp * 64 + unit] = hv > 0.f ? f2bf(acc[q]) : 0;
+                    }
+                }
+            }
+            __syncthreads();
+            cur = nxt;
+        }
+        // loss, one atomic per wavefront
+        for (int o = 32; o > 0; o >>= 1) lossv += __shfl_xor(lossv, o);
+        if (lane == 0 && lossOut) atomicAdd(lossOut, lossv);
+    }
+
+    // Adam step on the FP32 parameters, then zero the gradient
+    __global__ void guide_adam_kernel(float* theta, float* m, float* v, float* grad, int n, float lr, float b1, float b2,
+                                      float eps, float c1, float c2)
+    {
+        int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= n) return;
+        float g = grad[i];
+        float mi = b1 * m[i] + (1.f - b1) * g;
+        float vi = b2 * v[i] + (1.f - b2) * g * g;
+        m[i] = mi;
+        v[i] = vi;
+        theta[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        grad[i] = 0.f;
+    }
+
+    // rebuilds the padded BF16 weight tiles (and transposes) and FP32 biases from theta
+    __global__ void guide_pack_kernel(const float* theta, uint16_t* W0, uint16_t* W1, uint16_t* W2, uint16_t* W3,
+                                      uint16_t* T0, uint16_t* T1, uint16_t* T2, uint16_t* T3, float* B0, float* B1,
+                                      float* B2, float* B3)
+    {
+        int i = blockIdx.x * blockDim.x + threadIdx.x;
+        // layer l: out_pad x in_pad entries
+        const int sizes[4] = {64 * 16, 64 * 64, 64 * 64, 48 * 64};
+        int l = 0, base = 0;
+        while (l < 4 && i >= base + sizes[l])
+        {
+            base += sizes[l];
+            ++l;
+        }
+        if (l >= 4) return;
+        const int j = i - base;
+        const int inp = guide_in_pad(l), outp = guide_out_pad(l);
+        const int o = j / inp, c = j % inp;
+        const int nin = l == 0 ? kGuideIn : kGuideHidden, nout = l == 3 ? kGuideOut : kGuideHidden;
+        const int wOff = l == 0 ? kOffW0 : l == 1 ? kOffW1 : l == 2 ? kOffW2 : kOffW3;
+        const int bOff = l == 0 ? kOffB0 : l == 1 ? kOffB1 : l == 2 ? kOffB2 : kOffB3;
+        float w = (o < nout && c < nin) ? theta[wOff + o * nin + c] : 0.f;
+        uint16_t* W = l == 0 ? W0 : l == 1 ? W1 : l == 2 ? W2 : W3;
+        uint16_t* T = l == 0 ? T0 : l == 1 ? T1 : l == 2 ? T2 : T3;
+        float* B = l == 0 ? B0 : l == 1 ? B1 : l == 2 ? B2 : B3;
+        W[o * inp + c] = f2bf(w);
+        T[c * outp + o] = f2bf(w);
+        if (c == 0) B[o] = o < nout ? theta[bOff + o] : 0.f;
+    }
+}

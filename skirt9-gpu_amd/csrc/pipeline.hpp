@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "guide.hpp"
 #include "lifecycle.hpp"
 #include "traverse.hpp"
 
@@ -96,6 +97,21 @@ namespace skg
         int64_t qcap;
         unsigned long long* ctl;  // [0] history counter, [1..2] queue counts, [3..4] queue heads,
                                   // [5..6] active slots per parity, [8..15] event counters
+        // path guide: 0 plain HG, 1 guided (mixture with HG), 2 plain HG collecting training records
+        int guideMode;
+        double guideMix;  // probability of sampling the guide instead of HG
+        GuideNet net;
+        GuideFrame gframe;
+        // training records (collect mode): 16 floats each (9 features, direction out, weight),
+        // the history offset and the packet's accumulated detector luminosity at the record
+        float* rec;
+        uint32_t* recHist;
+        double* recA;
+        unsigned long long* recCount;
+        int64_t recCap;
+        double* contrib;     // per slot: luminosity its packet has deposited so far
+        int64_t* recLast;    // per slot: record awaiting its luminosity mark, or -1
+        double* finalLum;    // per history offset: total luminosity deposited by the packet
     };
 
     constexpr int kCtlHist = 0;
@@ -127,7 +143,7 @@ namespace skg
     }
 
     // ------------------------------------------------------------------ event kernel
-    template<int GT> __global__ void __launch_bounds__(256) pipe_event(PipeParams p, int out)
+    template<int GT, int GM> __global__ void __launch_bounds__(256) pipe_event(PipeParams p, int out)
     {
         const int lane = __lane_id();
         const int64_t slot = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
@@ -171,6 +187,15 @@ namespace skg
         }
         if (st == kSlotWaitFwd1)
         {
+            if (GM == 2)
+            {
+                int64_t rl = p.recLast[slot];
+                if (rl >= 0)
+                {
+                    p.recA[rl] = p.contrib[slot];
+                    p.recLast[slot] = -1;
+                }
+            }
             // MonteCarloSimulation::simulateForcedPropagation (:1492-1544), sampling part
             double taupath = P.res_tau[slot];
             if (taupath <= 0.)
@@ -234,16 +259,73 @@ namespace skg
                 kxo = kx;
                 kyo = ky;
                 kzo = kz;
-                scatter_hg(rng, p.gpar[ell], kx, ky, kz);
-                nscatt++;
-                cntScat++;
                 emitFwd = true;
                 emitFwdKind = kReqFwd1;
+            }
+        }
+
+        // scattering: HG (DustMix::performScattering) or, in guided mode, the defensive mixture
+        // (1 - mix) HG + mix guide with the weight Phi_HG / p_mix that keeps the estimator unbiased
+        const bool doScatter = peelScat;
+        if constexpr (GM == 1)
+        {
+            bool useGuide = false;
+            if (doScatter) useGuide = rng.uniform() < p.guideMix;
+            __shared__ uint16_t gIn[4][64 * 16];
+            __shared__ float gOut[4][64 * 48];
+            const int wv = threadIdx.x >> 6;
+            float feat[kGuideIn];
+            guide_features(p.gframe, rx, ry, rz, kx, ky, kz, p.lambda[ell], feat);
+            float o[kGuideOut];
+            // every thread of the block takes part (uniform control flow around the MFMA)
+            guide_wave_infer(p.net, feat, o, gIn[wv], gOut[wv]);
+            if (doScatter)
+            {
+                Lobes Lb;
+                guide_lobes(o, Lb);
+                const double g = p.gpar[ell];
+                if (useGuide)
+                    guide_sample(Lb, rng, kx, ky, kz);
+                else
+                    scatter_hg(rng, g, kx, ky, kz);
+                double c = kxo * kx + kyo * ky + kzo * kz;
+                double phi = (fabs(g) < 1e-6 ? 1.0 : value_hg(g, c)) / (4 * M_PI);
+                double q = guide_pdf(Lb, kx, ky, kz);
+                W *= phi / ((1.0 - p.guideMix) * phi + p.guideMix * q);
+            }
+        }
+        else if (doScatter)
+        {
+            scatter_hg(rng, p.gpar[ell], kx, ky, kz);
+        }
+        if (doScatter)
+        {
+            nscatt++;
+            cntScat++;
+            if (GM == 2)
+            {
+                // training record: features of the event, the sampled direction, and the mark of
+                // the luminosity deposited so far (set at this slot's next event)
+                unsigned long long ri = atomicAdd(p.recCount, 1ull);
+                if (ri < static_cast<unsigned long long>(p.recCap))
+                {
+                    float feat[kGuideIn];
+                    guide_features(p.gframe, rx, ry, rz, kxo, kyo, kzo, p.lambda[ell], feat);
+                    float* r = p.rec + ri * 16;
+                    for (int q = 0; q < kGuideIn; ++q) r[q] = feat[q];
+                    r[9] = static_cast<float>(kx);
+                    r[10] = static_cast<float>(ky);
+                    r[11] = static_cast<float>(kz);
+                    r[12] = 0.f;
+                    p.recHist[ri] = static_cast<uint32_t>(P.hist[slot] - p.first);
+                    p.recLast[slot] = static_cast<int64_t>(ri);
+                }
             }
         }
         if (terminated)
         {
             cntHist++;
+            if (GM == 2) p.finalLum[P.hist[slot] - p.first] = p.contrib[slot];
             st = kSlotEmpty;
         }
 
@@ -269,12 +351,18 @@ namespace skg
                 {
                     // zero-luminosity packet: done immediately
                     cntHist++;
+                    if (GM == 2) p.finalLum[h] = 0.;
                     st = kSlotEmpty;
                 }
                 else
                 {
                     P.hist[slot] = p.first + h;
                     P.lthr[slot] = (W / lam) / p.min_weight_reduction;
+                    if (GM == 2)
+                    {
+                        p.contrib[slot] = 0.;
+                        p.recLast[slot] = -1;
+                    }
                     for (int f = 0; f < p.nframes; ++f)
                     {
                         const FrameGeom& fg = p.frames[f];
@@ -453,8 +541,8 @@ namespace skg
 
             // one cell crossing
             int m;
-            double ds;
-            bool more = gen.next(p.grid, m, ds) && ++steps <= (1 << 26);
+            double ds, nd;
+            bool more = gen.next(p.grid, m, ds, nd) && ++steps <= (1 << 26);
             bool finished = !more;
             if (more)
             {
@@ -465,7 +553,7 @@ namespace skg
                     {
                         if (FP == SKG_FP64)
                         {
-                            tau += section * p.dens[m] * ds;
+                            tau += section * nd * ds;
                             if (tau >= limit)
                             {
                                 tau = INFINITY;
@@ -474,7 +562,7 @@ namespace skg
                         }
                         else
                         {
-                            float y = section32 * p.dens32[m] * static_cast<float>(ds) - ctau;
+                            float y = section32 * static_cast<float>(nd) * static_cast<float>(ds) - ctau;
                             float t = tau32 + y;
                             ctau = (t - tau32) - y;
                             tau32 = t;
@@ -493,7 +581,7 @@ namespace skg
                         tprev = tau;
                         sprev = s;
                         s += ds;
-                        if (m >= 0) tau += section * p.dens[m] * ds;
+                        if (m >= 0) tau += section * nd * ds;
                     }
                     else
                     {
@@ -505,7 +593,7 @@ namespace skg
                         s32 = t;
                         if (m >= 0)
                         {
-                            float y2 = section32 * p.dens32[m] * static_cast<float>(ds) - ctau;
+                            float y2 = section32 * static_cast<float>(nd) * static_cast<float>(ds) - ctau;
                             float t2 = tau32 + y2;
                             ctau = (t2 - tau32) - y2;
                             tau32 = t2;
@@ -572,17 +660,31 @@ namespace skg
             int slot = r.slot;
             double x = P.rx[slot], y = P.ry[slot], z = P.rz[slot];
             int ell = P.ell[slot];
+            double dep = 0.;
             for (int f = r.frame; f < p.nframes && (f == r.frame || p.frames[f].same_as_prev); ++f)
             {
                 const FrameGeom& fg = p.frames[f];
                 int pix = fg.pixel(x, y, z);
                 if (pix < 0) continue;
                 unsafeAtomicAdd(p.frame + fg.offset + pix + static_cast<int64_t>(ell) * fg.nx * fg.ny, v);
+                dep += v;
                 ++atomics;
             }
+            if (p.guideMode == 2 && dep > 0.) atomicAdd(p.contrib + slot, dep);
         }
         for (int o = 32; o > 0; o >>= 1) atomics += __shfl_xor(atomics, o);
         if (__lane_id() == 0 && atomics) atomicAdd(p.ctl + kCtlCnt + 5, atomics);
+    }
+
+    // training weights of the collected records: the luminosity the packet deposited after the
+    // record's scattering event (total at termination minus the mark at the event)
+    __global__ void guide_finalize_records(float* rec, const uint32_t* recHist, const double* recA, const double* finalLum,
+                                           int64_t n)
+    {
+        int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+        if (i >= n) return;
+        double c = finalLum[recHist[i]] - recA[i];
+        rec[i * 16 + 12] = static_cast<float>(c > 0. ? c : 0.);
     }
 
     // resets all control words and marks every slot empty

@@ -20,9 +20,27 @@ namespace skg
         kOctree = 1
     };
 
-    // Read-only grid tables resident in HBM.  The octree tables are node-indexed:
-    // bounds (48 B per node), first child (children are 8 consecutive ids), cell index,
-    // and per-node per-wall neighbour lists in CSR form (int32 offsets).
+    // One 128-byte record per octree leaf (indexed by cell): everything a cell crossing
+    // reads -- the leaf's FP64 bounds, its number density, the first (largest-overlap)
+    // neighbour on each wall and the start of each wall's full neighbour list -- sits in one
+    // cache line, so the common crossing (the first neighbour contains the new position)
+    // touches a single line.
+    struct alignas(128) OctLeaf
+    {
+        double lo[3], hi[3];  // bounds
+        double dens;          // number density of the cell
+        int32_t node;         // node id in the full tree
+        int32_t off_end;      // end of this leaf's neighbour lists in nbc
+        int32_t first[6];     // first neighbour cell per wall (BACK..TOP), -1 if none
+        int32_t off[6];       // start of each wall's neighbour list in nbc
+        int32_t pad[4];
+    };
+    static_assert(sizeof(OctLeaf) == 128, "leaf record must be one 128-byte line");
+
+    // Read-only grid tables resident in HBM.  Octree: node-indexed bounds, first child
+    // (children are 8 consecutive ids) and cell index for the top-down search, plus the
+    // cell-indexed leaf records and their neighbour lists (cell ids, sorted by decreasing
+    // overlap as TreeNode::sortNeighbors).
     struct GridView
     {
         double box[6];  // xmin, ymin, zmin, xmax, ymax, zmax of the domain
@@ -31,11 +49,12 @@ namespace skg
         const double* yb;
         const double* zb;
         int nx, ny, nz;
+        const double* dens;  // Cartesian: number density per cell
         const double* bounds;
         const int32_t* child_first;
         const int32_t* cell_of;
-        const int32_t* nb_off;
-        const int32_t* nb_id;
+        const OctLeaf* leaf;
+        const int32_t* nbc;
     };
 
     enum GenState : int
@@ -85,7 +104,26 @@ namespace skg
     {
         double rx, ry, rz, kx, ky, kz;
         int state;
-        int i, j, k;  // Cartesian cell indices; octree: i = current node
+        int i, j, k;  // Cartesian cell indices; octree: i = current cell
+        double b0, b1, b2, b3, b4, b5, bn;  // octree: bounds and density of the current cell
+
+        __device__ __forceinline__ void load_leaf(const GridView& g, int c)
+        {
+            const OctLeaf& L = g.leaf[c];
+            b0 = L.lo[0];
+            b1 = L.lo[1];
+            b2 = L.lo[2];
+            b3 = L.hi[0];
+            b4 = L.hi[1];
+            b5 = L.hi[2];
+            bn = L.dens;
+        }
+
+        __device__ __forceinline__ int leaf_cell(const GridView& g, double x, double y, double z) const
+        {
+            int node = leaf_child(g, x, y, z);
+            return node >= 0 ? g.cell_of[node] : -1;
+        }
 
         __device__ __forceinline__ void start(double r0, double r1, double r2, double k0, double k1, double k2)
         {
@@ -164,8 +202,9 @@ namespace skg
             return true;
         }
 
-        // Returns false when the path has no more segments; otherwise sets (m, ds).
-        __device__ __forceinline__ bool next(const GridView& g, int& m, double& ds)
+        // Returns false when the path has no more segments; otherwise sets the cell m, the
+        // segment length ds and the cell's number density n (0 for m = -1).
+        __device__ __forceinline__ bool next(const GridView& g, int& m, double& ds, double& n)
         {
             if (state == kOutside) return false;
             if (state == kUnknown)
@@ -180,12 +219,19 @@ namespace skg
                 }
                 else
                 {
-                    i = leaf_child(g, rx, ry, rz);
+                    i = leaf_cell(g, rx, ry, rz);
+                    if (i < 0)
+                    {
+                        state = kOutside;
+                        return false;
+                    }
+                    load_leaf(g, i);
                 }
                 if (cumds > 0.)
                 {
                     m = -1;
                     ds = cumds;
+                    n = 0.;
                     return true;
                 }
             }
@@ -199,6 +245,7 @@ namespace skg
                 double dsy = (fabs(ky) > 1e-15) ? (yE - ry) / ky : DBL_MAX;
                 double dsz = (fabs(kz) > 1e-15) ? (zE - rz) / kz : DBL_MAX;
                 m = mm;
+                n = g.dens ? g.dens[mm] : 0.;
                 if (dsx <= dsy && dsx <= dsz)
                 {
                     ds = dsx;
@@ -230,11 +277,10 @@ namespace skg
             }
             else
             {
-                const int node = i;
-                const double* b = g.bounds + 6 * static_cast<int64_t>(node);
-                double xnext = (kx < 0.0) ? b[0] : b[3];
-                double ynext = (ky < 0.0) ? b[1] : b[4];
-                double znext = (kz < 0.0) ? b[2] : b[5];
+                const int cur = i;
+                double xnext = (kx < 0.0) ? b0 : b3;
+                double ynext = (ky < 0.0) ? b1 : b4;
+                double znext = (kz < 0.0) ? b2 : b5;
                 double dsx = (fabs(kx) > 1e-15) ? (xnext - rx) / kx : DBL_MAX;
                 double dsy = (fabs(ky) > 1e-15) ? (ynext - ry) / ky : DBL_MAX;
                 double dsz = (fabs(kz) > 1e-15) ? (znext - rz) / kz : DBL_MAX;
@@ -259,29 +305,47 @@ namespace skg
                 rx += kx * step;
                 ry += ky * step;
                 rz += kz * step;
-                m = g.cell_of[node];
+                m = cur;
                 ds = d;
+                n = bn;
                 // first neighbour on the crossed wall whose closed box contains r
+                const OctLeaf& R = g.leaf[cur];
                 int nxt = -1;
-                int e0 = g.nb_off[6 * node + wall], e1 = g.nb_off[6 * node + wall + 1];
-                for (int q = e0; q < e1; ++q)
+                int f = R.first[wall];
+                if (f >= 0)
                 {
-                    int c = g.nb_id[q];
-                    if (node_contains(g.bounds + 6 * static_cast<int64_t>(c), rx, ry, rz))
+                    const OctLeaf& F = g.leaf[f];
+                    if (rx >= F.lo[0] && rx <= F.hi[0] && ry >= F.lo[1] && ry <= F.hi[1] && rz >= F.lo[2] && rz <= F.hi[2])
+                        nxt = f;
+                    else
                     {
-                        nxt = c;
-                        break;
+                        int e0 = R.off[wall] + 1;
+                        int e1 = wall < 5 ? R.off[wall + 1] : R.off_end;
+                        for (int q = e0; q < e1; ++q)
+                        {
+                            int c = g.nbc[q];
+                            const OctLeaf& C = g.leaf[c];
+                            if (rx >= C.lo[0] && rx <= C.hi[0] && ry >= C.lo[1] && ry <= C.hi[1] && rz >= C.lo[2]
+                                && rz <= C.hi[2])
+                            {
+                                nxt = c;
+                                break;
+                            }
+                        }
                     }
                 }
-                if (nxt < 0) nxt = leaf_child(g, rx, ry, rz);
-                if (nxt == node)
+                if (nxt < 0) nxt = leaf_cell(g, rx, ry, rz);
+                if (nxt == cur)
                 {
                     rx = nextafter(rx, (kx < 0.) ? -DBL_MAX : DBL_MAX);
                     ry = nextafter(ry, (ky < 0.) ? -DBL_MAX : DBL_MAX);
                     rz = nextafter(rz, (kz < 0.) ? -DBL_MAX : DBL_MAX);
-                    nxt = leaf_child(g, rx, ry, rz);
+                    nxt = leaf_cell(g, rx, ry, rz);
                 }
-                if (nxt < 0 || nxt == node) state = kOutside;
+                if (nxt < 0 || nxt == cur)
+                    state = kOutside;
+                else
+                    load_leaf(g, nxt);
                 i = nxt;
                 return true;
             }

@@ -1,0 +1,138 @@
+"""The path guide on the GPU: BF16 MFMA inference and gradients against the FP64 CPU
+statement (oracle/guide_oracle.py), training, and unbiasedness of guided sampling.
+
+Tolerances: inference is compared with the CPU statement evaluated with the device's BF16
+rounding of weights, inputs and hidden activations (1e-2 of the output scale, from FP32
+accumulation order and rounding-boundary flips); gradients with the exact FP64 gradient
+(cosine similarity > 0.995, relative L2 error < 0.1, from BF16 operands)."""
+
+import numpy as np
+import pytest
+
+import guide_oracle as G
+from helpers import PC, disk_density_cartesian, frame60, load_pkg
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    pkg = load_pkg()
+    e = pkg.Engine(0)
+    yield e
+    e.close()
+
+
+def _params(seed):
+    p = G.init_params(seed)
+    rng = np.random.default_rng(seed + 100)
+    for k in p:
+        p[k] = p[k] + rng.normal(0, 0.05, p[k].shape)
+    p["W3"] *= 5.0
+    return p
+
+
+def test_guide_inference_matches_cpu(eng):
+    p = _params(1)
+    eng.guide_params(G.flatten(p).astype(np.float32))
+    rng = np.random.default_rng(2)
+    X = rng.normal(size=(1000, 9))
+    print("case guide inference", flush=True)
+    out = eng.guide_infer(X)
+    ref = G.forward_bf16(p, X.astype(np.float32))
+    scale = np.abs(ref).max()
+    assert np.max(np.abs(out - ref)) < 1e-2 * scale
+    ex, _ = G.forward(p, X)
+    assert np.max(np.abs(out - ex)) < 5e-2 * np.abs(ex).max()
+    back = eng.guide_read_params()
+    np.testing.assert_array_equal(back, G.flatten(p).astype(np.float32))
+
+
+def test_guide_gradient_matches_cpu(eng):
+    p = _params(3)
+    eng.guide_params(G.flatten(p).astype(np.float32))
+    rng = np.random.default_rng(4)
+    n = 640
+    X = rng.normal(size=(n, 9))
+    K = rng.normal(size=(n, 3))
+    K /= np.linalg.norm(K, axis=1, keepdims=True)
+    Cw = rng.random(n)
+    rec = np.zeros((n, 16), np.float32)
+    rec[:, :9] = X
+    rec[:, 9:12] = K
+    rec[:, 12] = Cw / Cw.sum()
+    print("case guide gradient", flush=True)
+    g, loss = eng.guide_gradient(rec)
+    gr, _ = G.grad(p, X.astype(np.float32).astype(np.float64), K.astype(np.float32).astype(np.float64), Cw)
+    ref = G.flatten(gr)
+    cos = g @ ref / (np.linalg.norm(g) * np.linalg.norm(ref))
+    assert cos > 0.995, cos
+    assert np.linalg.norm(g - ref) / np.linalg.norm(ref) < 0.1
+    lref = G.loss(p, X, K, Cw)
+    assert abs(loss - lref) < 2e-2 * abs(lref) + 1e-3
+
+
+def test_guide_trains_on_collected_records(eng):
+    """Plain run that records training data, then Adam on the records: the weighted negative
+    log-likelihood of the recorded directions decreases."""
+    b, dens = disk_density_cartesian(n=32, samples=10, seed=9, tau_v=5.0)
+    eng.grid_cartesian(b, b, b)
+    eng.medium(dens)
+    s = 1.5e-29
+    eng.dust([0.55e-6], [s], [0.6 * s], [0.6])
+    eng.frame_remove_all()
+    fr = frame60()
+    eng.frame_add(fr["inclination"], fr["azimuth"], fr["roll"], fr["nx"], fr["ny"], fr["fovx"], fr["fovy"])
+    npk = 200000
+    eng.source_point((0.0, 0.0, 0.0), [1.0 / npk])
+    eng.options(0.5, 1e4, 0, 0, 5)
+    eng.tune_pipeline(0, 1 << 18, 0, 8)
+    eng.guide_params(G.flatten(G.init_params(7)).astype(np.float32))
+    eng.guide_collect(4_000_000)
+    eng.guide_mode(2)
+    eng.frame_clear()
+    print("case guide collect", flush=True)
+    eng.run_packets(0, npk)
+    nrec, recs = eng.guide_records(copy=1000)
+    cnt = eng.counters()
+    assert nrec == cnt[3]  # one record per scattering
+    assert np.all(recs[:, 12] >= 0) and recs[:, 12].max() > 0
+    np.testing.assert_allclose(np.linalg.norm(recs[:, 9:12], axis=1), 1, atol=1e-5)
+    print("case guide train", flush=True)
+    l0, l1 = eng.guide_train(300, batch=4096, lr=2e-3)
+    eng.guide_mode(0)
+    assert np.isfinite(l0) and np.isfinite(l1)
+    assert l1 < l0 - 0.05, (l0, l1)
+
+
+def test_guided_estimator_is_unbiased(eng):
+    """Guided sampling (defensive mixture, weight Phi_HG / p_mix) reproduces the plain total
+    flux within Monte Carlo error, with an untrained (arbitrary) guide."""
+    b, dens = disk_density_cartesian(n=32, samples=10, seed=9, tau_v=5.0)
+    eng.grid_cartesian(b, b, b)
+    eng.medium(dens)
+    s = 1.5e-29
+    eng.dust([0.55e-6], [s], [0.6 * s], [0.6])
+    eng.frame_remove_all()
+    fr = frame60()
+    eng.frame_add(fr["inclination"], fr["azimuth"], fr["roll"], 20, 20, fr["fovx"], fr["fovy"])
+    nb, per = 16, 40000
+    eng.source_point((0.0, 0.0, 0.0), [1.0 / per])
+    eng.options(0.5, 1e4, 0, 0, 9)
+    eng.tune_pipeline(0, 1 << 18, 0, 8)
+    eng.guide_params(G.flatten(_params(11)).astype(np.float32))
+    totals = {}
+    for mode in (0, 1):
+        eng.guide_mode(mode, mix=0.5)
+        t = []
+        for bi in range(nb):
+            eng.frame_clear()
+            print("case guided unbiased", mode, bi, flush=True)
+            eng.run_packets(bi * per, per)
+            t.append(eng.frame_read(0).sum())
+        totals[mode] = np.array(t)
+    eng.guide_mode(0)
+    m0, m1 = totals[0].mean(), totals[1].mean()
+    e0 = totals[0].std(ddof=1) / np.sqrt(nb)
+    e1 = totals[1].std(ddof=1) / np.sqrt(nb)
+    assert abs(m1 - m0) < 4 * np.hypot(e0, e1), (m0, m1, e0, e1)

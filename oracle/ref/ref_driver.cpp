@@ -14,7 +14,9 @@
 //                                                     with MediumSystem (MediumSystem.cpp:1785, :2328)
 //   skirt_ref grid  <ski> <out.bin>                   dump per-cell number densities and, for tree
 //                                                     grids, node bounds, children and the per-wall
-//                                                     neighbour lists (replayed with TreeNode/OctTreeNode)
+//                                                     neighbour lists (replayed with TreeNode/OctTreeNode);
+//                                                     for Voronoi and tetrahedral grids their traversal tables
+//   skirt_ref cells <ski> <points.bin> <out.bin>      SpatialGrid::cellIndex for each point
 //   skirt_ref dust  <ski> <out.bin> <ncos>            dump sectionExt/Sca, asymmpar and peel-off phase
 //                                                     values (DustMix.cpp:395-445) at the simulation
 //                                                     wavelengths for ncos cosines in [-1,1]
@@ -40,6 +42,8 @@
 #include "StringUtils.hpp"
 #include "System.hpp"
 #include "TreeSpatialGrid.hpp"
+#include "TetraMeshSpatialGrid.hpp"
+#include "VoronoiMeshSpatialGrid.hpp"
 #include "Configuration.hpp"
 #include "XmlHierarchyCreator.hpp"
 #include <chrono>
@@ -145,11 +149,19 @@ namespace
     }
 
     // out.bin: "SKGD" int32(1) int64 ncells; float64 density[ncells]
-    //          int32 isTree; if tree:
+    //          int32 kind (0 other, 1 tree, 2 Voronoi, 3 tetra); if tree:
     //            int64 nnodes; float64 bounds[6*nnodes]; int32 childBegin[nnodes]; int32 childCount[nnodes];
     //            int64 nchild; int32 childIndex[nchild]; int32 cellIndex[nnodes];
     //            int64 nboff; int64 nbOff[6*nnodes+1]; int32 nbId[nbOff[end]]   (replayed, sorted neighbours)
     //            int32 boundsMatch (1 if the replayed node bounds equal the grid's, bit for bit)
+    //          if Voronoi (VoronoiMeshSnapshot.hpp:338-349):
+    //            float64 box[6]; float64 eps; int64 nsites; float64 sites[3*nsites];
+    //            int32 nbBegin[nsites]; int32 nbCount[nsites]; int64 nnb; int32 nbIndex[nnb];
+    //            int32 blockN; int32 blockBegin[N^3]; int32 blockCount[N^3]; int64 nbl; int32 blockIndex[nbl]
+    //          if tetra (TetraMeshSpatialGrid.hpp:293-303):
+    //            float64 box[6]; float64 eps; int64 nverts; float64 verts[3*nverts]; int64 ntets;
+    //            int32 tetVerts[4*ntets]; int32 faceAnchor[4*ntets]; int32 faceNeighbor[4*ntets];
+    //            float64 faceNormal[12*ntets]; float64 centroid[3*ntets]
     int doGrid(const string& ski, const string& outPath)
     {
         auto top = loadSki(ski, "/tmp", 1);
@@ -168,7 +180,44 @@ namespace
         put<int64_t>(out, ncells);
         putv(out, dens);
         auto tree = dynamic_cast<TreeSpatialGrid*>(grid);
-        put<int32_t>(out, tree ? 1 : 0);
+        auto vor = dynamic_cast<VoronoiMeshSpatialGrid*>(grid);
+        auto tet = dynamic_cast<TetraMeshSpatialGrid*>(grid);
+        put<int32_t>(out, tree ? 1 : vor ? 2 : tet ? 3 : 0);
+        if (vor || tet)
+        {
+            auto box = dynamic_cast<BoxSpatialGrid*>(grid)->extent();
+            std::vector<double> bv{box.xmin(), box.ymin(), box.zmin(), box.xmax(), box.ymax(), box.zmax()};
+            putv(out, bv);
+        }
+        if (vor)
+        {
+            put<double>(out, vor->traversalEpsilon());
+            const auto& sites = vor->traversalSiteCoordinates();
+            put<int64_t>(out, static_cast<int64_t>(sites.size() / 3));
+            putv(out, sites);
+            putv(out, vor->traversalNeighborBegin());
+            putv(out, vor->traversalNeighborCount());
+            put<int64_t>(out, static_cast<int64_t>(vor->traversalNeighborIndex().size()));
+            putv(out, vor->traversalNeighborIndex());
+            put<int32_t>(out, vor->traversalBlockGridN());
+            putv(out, vor->traversalBlockBegin());
+            putv(out, vor->traversalBlockCount());
+            put<int64_t>(out, static_cast<int64_t>(vor->traversalBlockIndex().size()));
+            putv(out, vor->traversalBlockIndex());
+        }
+        if (tet)
+        {
+            put<double>(out, tet->traversalEpsilon());
+            const auto& verts = tet->traversalVertexCoordinates();
+            put<int64_t>(out, static_cast<int64_t>(verts.size() / 3));
+            putv(out, verts);
+            put<int64_t>(out, static_cast<int64_t>(tet->traversalTetraVertexIndices().size() / 4));
+            putv(out, tet->traversalTetraVertexIndices());
+            putv(out, tet->traversalFaceAnchorIndices());
+            putv(out, tet->traversalFaceNeighborIndices());
+            putv(out, tet->traversalFaceNormals());
+            putv(out, tet->traversalCentroids());
+        }
         if (tree)
         {
             const auto& bounds = tree->traversalNodeBounds();
@@ -213,6 +262,30 @@ namespace
             put<int32_t>(out, match);
             for (auto node : nodev) delete node;
         }
+        return 0;
+    }
+
+    // points.bin: int64 n; n x {x, y, z} float64
+    // out.bin   : "SKCI" int32(1) int64 n; int32 cell[n]
+    int doCells(const string& ski, const string& pointsPath, const string& outPath)
+    {
+        auto top = loadSki(ski, "/tmp", 1);
+        auto simulation = dynamic_cast<MonteCarloSimulation*>(top.get());
+        simulation->log()->setLowestLevel(Log::Level::Warning);
+        simulation->setup();
+        SpatialGrid* grid = simulation->find<MediumSystem>()->grid();
+        std::ifstream in(pointsPath, std::ios::binary);
+        int64_t n = 0;
+        in.read(reinterpret_cast<char*>(&n), 8);
+        std::vector<double> pts(3 * n);
+        in.read(reinterpret_cast<char*>(pts.data()), 8 * pts.size());
+        std::vector<int32_t> cell(n);
+        for (int64_t i = 0; i != n; ++i) cell[i] = grid->cellIndex(Position(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+        std::ofstream out(outPath, std::ios::binary);
+        out.write("SKCI", 4);
+        put<int32_t>(out, 1);
+        put<int64_t>(out, n);
+        putv(out, cell);
         return 0;
     }
 
@@ -274,7 +347,7 @@ int main(int argc, char** argv)
     SimulationItemRegistry registry("v9.0-ref", "9");
     if (argc < 2)
     {
-        std::fprintf(stderr, "usage: skirt_ref run|paths|grid|dust ...\n");
+        std::fprintf(stderr, "usage: skirt_ref run|paths|grid|cells|dust ...\n");
         return 2;
     }
     string mode = argv[1];
@@ -283,6 +356,7 @@ int main(int argc, char** argv)
         if (mode == "run" && argc == 5) return doRun(argv[2], argv[3], std::atoi(argv[4]));
         if (mode == "paths" && argc == 6) return doPaths(argv[2], argv[3], argv[4], std::atof(argv[5]));
         if (mode == "grid" && argc == 4) return doGrid(argv[2], argv[3]);
+        if (mode == "cells" && argc == 5) return doCells(argv[2], argv[3], argv[4]);
         if (mode == "dust" && argc == 5) return doDust(argv[2], argv[3], std::atoi(argv[4]));
     }
     catch (FatalError& error)

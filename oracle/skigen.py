@@ -69,6 +69,14 @@ def render(spec: SkiSpec) -> str:
                 f'<DensityTreePolicy minLevel="{lmin}" maxLevel="{lmax}" maxDustFraction="{frac:.9g}" '
                 'maxDustOpticalDepth="0" wavelength="0.55 micron" maxDustDensityDispersion="0" '
                 'maxElectronFraction="1e-6" maxGasFraction="1e-6"/></policy></PolicyTreeSpatialGrid>')
+    elif spec.grid[0] == "voronoi":
+        # ("voronoi", num_sites, policy): sites drawn by the grid's policy (VoronoiMeshSpatialGrid.cpp:51-140)
+        _, nsites, policy = spec.grid
+        grid = (f'<VoronoiMeshSpatialGrid {box} policy="{policy}" numSites="{nsites}" relaxSites="false"/>')
+    elif spec.grid[0] == "tetra":
+        # ("tetra", num_sites, policy): Delaunay tetrahedralisation of the sampled vertices
+        _, nsites, policy = spec.grid
+        grid = f'<TetraMeshSpatialGrid {box} policy="{policy}" numSamples="{nsites}" refine="false"/>'
     else:
         raise ValueError(spec.grid)
     instr = "".join(

@@ -13,6 +13,12 @@ Every fixture is data: seeded inputs plus the reference's outputs for them.
   tree_paths.npz   a spiral-disk octree (DensityTreePolicy, levels 2..5): node bounds,
                    children, cell indices, per-wall sorted neighbour lists, per-cell
                    densities, and the same path / tau outputs for seeded rays.
+  voronoi_paths.npz  a Voronoi grid of 1500 density-sampled sites: site coordinates,
+                   neighbour lists (walls as -1..-6), block search lists, densities, the
+                   path / tau outputs for seeded rays and SpatialGrid::cellIndex for points.
+  tetra_paths.npz  a Delaunay tetrahedral grid of 500 density-sampled vertices plus the
+                   domain corners: vertices, tetra vertex indices, face anchors, neighbours
+                   and normals, centroids, densities, paths / tau and cellIndex for points.
   dust.npz         DustMix sections, asymmetry parameters and peel-off phase values
                    (HG and 4-degree mean HG) for a ten-wavelength MeanListDustMix, and the
                    40-wavelength power-law mix of the octree workload.
@@ -105,7 +111,32 @@ def grid_dump(tmp, ski):
     r.magic("SKGD")
     nc = r.i64()
     out = dict(density=r.take("<f8", nc))
-    if r.i32():
+    kind = r.i32()
+    out["kind"] = kind
+    if kind in (2, 3):
+        out["box"] = r.take("<f8", 6)
+        out["eps"] = r.take("<f8", 1)[0]
+    if kind == 2:
+        ns = r.i64()
+        out["sites"] = r.take("<f8", 3 * ns).reshape(ns, 3)
+        out["nb_begin"] = r.take("<i4", ns)
+        out["nb_count"] = r.take("<i4", ns)
+        out["nb_index"] = r.take("<i4", r.i64())
+        nbk = r.i32()
+        out["block_n"] = np.int32(nbk)
+        out["block_begin"] = r.take("<i4", nbk ** 3)
+        out["block_count"] = r.take("<i4", nbk ** 3)
+        out["block_index"] = r.take("<i4", r.i64())
+    if kind == 3:
+        nv = r.i64()
+        out["verts"] = r.take("<f8", 3 * nv).reshape(nv, 3)
+        nt = r.i64()
+        out["tet_verts"] = r.take("<i4", 4 * nt).reshape(nt, 4)
+        out["face_anchor"] = r.take("<i4", 4 * nt).reshape(nt, 4)
+        out["face_nb"] = r.take("<i4", 4 * nt).reshape(nt, 4)
+        out["face_normal"] = r.take("<f8", 12 * nt).reshape(nt, 4, 3)
+        out["centroid"] = r.take("<f8", 3 * nt).reshape(nt, 3)
+    if kind == 1:
         nn = r.i64()
         out["bounds"] = r.take("<f8", 6 * nn).reshape(nn, 6)
         out["child_begin"] = r.take("<i4", nn)
@@ -311,19 +342,100 @@ def fixtures_images(tmp, runs=8, packets=1e6):
     print("image_cfg0.npz", out["mean"].shape, "calib", out["calib_total"])
 
 
+def cells(tmp, ski, pts):
+    pp = os.path.join(tmp, "points.bin")
+    with open(pp, "wb") as f:
+        f.write(struct.pack("<q", len(pts)))
+        f.write(np.ascontiguousarray(pts, dtype="<f8").tobytes())
+    op = os.path.join(tmp, "cells.bin")
+    run("cells", ski, pp, op)
+    r = Reader(op)
+    r.magic("SKCI")
+    return r.take("<i4", r.i64())
+
+
+def fixtures_voronoi(tmp):
+    """Voronoi grid of 1500 density-sampled sites (VoronoiMeshSpatialGrid, DustDensity policy)."""
+    rng = np.random.default_rng(11)
+    h = 10 * PC
+    spec = skigen.SkiSpec(num_packets=1, grid=("voronoi", 1500, "DustDensity"))
+    ski = write_ski(tmp, "vor", spec)
+    g = grid_dump(tmp, ski)
+    rays = make_rays(rng, [-h] * 3, [h] * 3, n_random=600)
+    # rays from sites, towards neighbouring sites (through a shared face) and along bisector planes
+    sites = g["sites"]
+    extra = []
+    for m in rng.choice(len(sites), 60, replace=False):
+        nb = g["nb_index"][g["nb_begin"][m]:g["nb_begin"][m] + g["nb_count"][m]]
+        nb = nb[nb >= 0]
+        j = nb[rng.integers(0, len(nb))]
+        extra.append(np.hstack([sites[m], unit(sites[j] - sites[m])]))
+        mid = 0.5 * (sites[m] + sites[j])
+        t = unit(np.cross(sites[j] - sites[m], rng.normal(size=3)))
+        extra.append(np.hstack([mid, t]))
+    rays = np.vstack([rays, np.array(extra)])
+    p = trace(tmp, ski, rays, 0.55e-6)
+    pts = np.vstack([-h + 2 * h * rng.random((2000, 3)), sites[:200],
+                     0.5 * (sites[:100] + sites[g["nb_index"][g["nb_begin"][:100]].clip(0)])])
+    ci = cells(tmp, ski, pts)
+    dm = dust_tables(tmp, ski)
+    out = {"rays": rays, "off": p["off"], "m": p["m"], "ds": p["ds"], "toff": p["toff"], "tau": p["tau"],
+           "tauobs": p["tauobs"], "density": g["density"], "box": g["box"], "eps": np.array(g["eps"]),
+           "sites": sites, "nb_begin": g["nb_begin"], "nb_count": g["nb_count"], "nb_index": g["nb_index"],
+           "block_n": g["block_n"], "block_begin": g["block_begin"], "block_count": g["block_count"],
+           "block_index": g["block_index"], "points": pts, "point_cell": ci, "sext": dm["sext"]}
+    np.savez_compressed(os.path.join(HERE, "voronoi_paths.npz"), **out)
+    print("voronoi_paths.npz:", {k: v.shape for k, v in out.items()})
+
+
+def fixtures_tetra(tmp):
+    """Delaunay tetrahedral grid of 500 density-sampled vertices plus the 8 domain corners
+    (TetraMeshSpatialGrid, DustDensity policy, no refinement)."""
+    rng = np.random.default_rng(12)
+    h = 10 * PC
+    spec = skigen.SkiSpec(num_packets=1, grid=("tetra", 500, "DustDensity"))
+    ski = write_ski(tmp, "tet", spec)
+    g = grid_dump(tmp, ski)
+    rays = make_rays(rng, [-h] * 3, [h] * 3, n_random=600)
+    # rays from vertices along edges and from face centroids (zero Pluecker products)
+    V, T = g["verts"], g["tet_verts"]
+    extra = []
+    for t in rng.choice(len(T), 60, replace=False):
+        a, b, c, d = V[T[t]]
+        extra.append(np.hstack([a, unit(b - a)]))
+        extra.append(np.hstack([(a + b + c) / 3, unit(d - (a + b + c) / 3)]))
+        extra.append(np.hstack([g["centroid"][t], unit(rng.normal(size=3))]))
+    rays = np.vstack([rays, np.array(extra)])
+    p = trace(tmp, ski, rays, 0.55e-6)
+    pts = np.vstack([-h + 2 * h * rng.random((2000, 3)), g["centroid"][:200]])
+    ci = cells(tmp, ski, pts)
+    dm = dust_tables(tmp, ski)
+    out = {"rays": rays, "off": p["off"], "m": p["m"], "ds": p["ds"], "toff": p["toff"], "tau": p["tau"],
+           "tauobs": p["tauobs"], "density": g["density"], "box": g["box"], "eps": np.array(g["eps"]),
+           "verts": V, "tet_verts": T, "face_anchor": g["face_anchor"], "face_nb": g["face_nb"],
+           "face_normal": g["face_normal"], "centroid": g["centroid"], "points": pts, "point_cell": ci,
+           "sext": dm["sext"]}
+    np.savez_compressed(os.path.join(HERE, "tetra_paths.npz"), **out)
+    print("tetra_paths.npz:", {k: v.shape for k, v in out.items()})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--images", action="store_true")
     ap.add_argument("--only", default="")
     a = ap.parse_args()
     with tempfile.TemporaryDirectory() as tmp:
-        parts = a.only.split(",") if a.only else ["cart", "tree", "dust"]
+        parts = a.only.split(",") if a.only else ["cart", "tree", "dust", "voronoi", "tetra"]
         if "cart" in parts:
             fixtures_cart(tmp)
         if "tree" in parts:
             fixtures_tree(tmp)
         if "dust" in parts:
             fixtures_dust(tmp)
+        if "voronoi" in parts:
+            fixtures_voronoi(tmp)
+        if "tetra" in parts:
+            fixtures_tetra(tmp)
         if a.images:
             fixtures_images(tmp)
 

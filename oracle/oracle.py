@@ -46,7 +46,11 @@ class OrcGrid(C.Structure):
     _fields_ = [("type", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
                 ("xb", f64p), ("yb", f64p), ("zb", f64p),
                 ("nnodes", C.c_int64), ("bounds", f64p), ("child_first", i32p), ("cell_index", i32p),
-                ("nb_off", i64p), ("nb_id", i32p), ("box", C.c_double * 6)]
+                ("nb_off", i64p), ("nb_id", i32p), ("box", C.c_double * 6),
+                ("sites", f64p), ("vnb_begin", i32p), ("vnb_count", i32p), ("vnb_index", i32p),
+                ("block_n", C.c_int32), ("block_begin", i32p), ("block_count", i32p), ("block_index", i32p),
+                ("verts", f64p), ("ntets", C.c_int64), ("tet_verts", i32p), ("face_nb", i32p),
+                ("face_normal", f64p)]
 
 
 class OrcInstrument(C.Structure):
@@ -92,6 +96,26 @@ class Grid:
             g.bounds, g.child_first, g.cell_index = _p(bounds, f64p), _p(cf, i32p), _p(ci, i32p)
             g.nb_off, g.nb_id = _p(nbo, i64p), _p(nbi, i32p)
             box = tuple(bounds[0])
+        elif kind == "voronoi":
+            g.type = 2
+            sites = np.ascontiguousarray(kw["sites"], dtype=np.float64)
+            a = [np.ascontiguousarray(kw[k], dtype=np.int32) for k in
+                 ("nb_begin", "nb_count", "nb_index", "block_begin", "block_count", "block_index")]
+            self.keep += [sites] + a
+            g.sites = _p(sites, f64p)
+            g.vnb_begin, g.vnb_count, g.vnb_index, g.block_begin, g.block_count, g.block_index = (_p(x, i32p) for x in a)
+            g.block_n = int(kw["block_n"])
+            box = tuple(np.asarray(kw["box"], dtype=np.float64))
+        elif kind == "tetra":
+            g.type = 3
+            verts = np.ascontiguousarray(kw["verts"], dtype=np.float64)
+            tv = np.ascontiguousarray(kw["tet_verts"], dtype=np.int32)
+            fn = np.ascontiguousarray(kw["face_nb"], dtype=np.int32)
+            nrm = np.ascontiguousarray(kw["face_normal"], dtype=np.float64)
+            self.keep += [verts, tv, fn, nrm]
+            g.verts, g.tet_verts, g.face_nb, g.face_normal = _p(verts, f64p), _p(tv, i32p), _p(fn, i32p), _p(nrm, f64p)
+            g.ntets = len(tv)
+            box = tuple(np.asarray(kw["box"], dtype=np.float64))
         else:
             raise ValueError(kind)
         g.box = (C.c_double * 6)(*box)
@@ -128,6 +152,14 @@ def tau(grid: Grid, density, section, rays, L=1.0, cap=None):
     if t < 0:
         raise RuntimeError("oracle tau capacity exceeded")
     return toff, tv[:t].copy(), sv[:t].copy(), mv[:t].copy(), tobs
+
+
+def cell_index(grid: Grid, pts):
+    """SpatialGrid::cellIndex of each point (Voronoi and tetra grids)."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    out = np.zeros(len(pts), np.int32)
+    lib().orc_cell_index(C.byref(grid.g), _p(pts, f64p), C.c_int64(len(pts)), _p(out, i32p))
+    return out
 
 
 def philox(ctr, key):

@@ -250,8 +250,177 @@ namespace
         const int64_t* nbOff = nullptr;
         const int32_t* nbId = nullptr;
 
+        // Voronoi (VoronoiMeshSnapshot.hpp:338-349): site coordinates, neighbour lists (walls
+        // as -1..-6) and the block search lists over a vbn^3 block grid
+        const double* sites = nullptr;
+        const int32_t* vnbBegin = nullptr;
+        const int32_t* vnbCount = nullptr;
+        const int32_t* vnbIndex = nullptr;
+        int vbn = 0;
+        const int32_t* vblBegin = nullptr;
+        const int32_t* vblCount = nullptr;
+        const int32_t* vblIndex = nullptr;
+        // tetra (TetraMeshSpatialGrid.hpp:293-303): vertices, per-tetra vertex indices, face
+        // neighbours and outward unit normals; the neighbour's face index and a block search
+        // grid are derived here
+        const double* verts = nullptr;
+        int64_t ntets = 0;
+        const int32_t* tetVerts = nullptr;
+        const int32_t* faceNb = nullptr;
+        const double* faceNormal = nullptr;
+        std::vector<int8_t> faceNbFace;
+        int tbn = 0;
+        std::vector<double> tbx, tby, tbz;
+        std::vector<int64_t> tblOff;
+        std::vector<int32_t> tblIds;
+
         // CartesianSpatialGrid.cpp:200-203
         int index(int i, int j, int k) const { return k + Nz * j + Nz * Ny * i; }
+
+        // Box::cellIndices, Box.hpp:162-167
+        void blockIndices(int& i, int& j, int& k, double x, double y, double z, int n) const
+        {
+            i = std::max(0, std::min(n - 1, static_cast<int>(n * (x - extent.xmin) / (extent.xmax - extent.xmin))));
+            j = std::max(0, std::min(n - 1, static_cast<int>(n * (y - extent.ymin) / (extent.ymax - extent.ymin))));
+            k = std::max(0, std::min(n - 1, static_cast<int>(n * (z - extent.zmin) / (extent.zmax - extent.zmin))));
+        }
+
+        // VoronoiMeshSnapshot::cellIndex, VoronoiMeshSnapshot.cpp:1037-1071: the nearest site
+        // among the block's candidates (squared distance (r - site).norm2(), strict '<', first
+        // wins).  The reference searches blocks with more than 9 candidates with a k-d tree,
+        // which returns the same site except for exact distance ties.
+        int vorCellIndex(double x, double y, double z) const
+        {
+            if (!extent.contains(x, y, z)) return -1;
+            int i, j, k;
+            blockIndices(i, j, k, x, y, z, vbn);
+            int b = i * vbn * vbn + j * vbn + k;
+            int m = -1;
+            double mdist = DBL_MAX;
+            for (int q = vblBegin[b], e = vblBegin[b] + vblCount[b]; q < e; ++q)
+            {
+                int id = vblIndex[q];
+                const double* p = sites + 3 * static_cast<int64_t>(id);
+                double dx = x - p[0], dy = y - p[1], dz = z - p[2];
+                double d = dx * dx + dy * dy + dz * dz;
+                if (d < mdist)
+                {
+                    m = id;
+                    mdist = d;
+                }
+            }
+            return m;
+        }
+
+        const double* tetVertex(int64_t m, int t) const { return verts + 3 * static_cast<int64_t>(tetVerts[4 * m + t]); }
+        const double* tetNormal(int64_t m, int f) const { return faceNormal + 12 * m + 3 * f; }
+
+        // Tetra::contains, TetraMeshSpatialGrid.cpp:110-133 (bounding box first, then the four
+        // face planes with the outward normals; a point on a face is contained)
+        bool tetContains(int64_t m, double x, double y, double z) const
+        {
+            double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+            for (int t = 0; t < 4; ++t)
+            {
+                const double* v = tetVertex(m, t);
+                for (int a = 0; a < 3; ++a)
+                {
+                    lo[a] = std::min(lo[a], v[a]);
+                    hi[a] = std::max(hi[a], v[a]);
+                }
+            }
+            if (!(x >= lo[0] && x <= hi[0] && y >= lo[1] && y <= hi[1] && z >= lo[2] && z <= hi[2])) return false;
+            const double* v = tetVertex(m, 3);
+            for (int f = 0; f <= 2; ++f)
+            {
+                const double* n = tetNormal(m, f);
+                if ((v[0] - x) * n[0] + (v[1] - y) * n[1] + (v[2] - z) * n[2] < 0) return false;
+            }
+            v = tetVertex(m, 0);
+            const double* n = tetNormal(m, 3);
+            if ((v[0] - x) * n[0] + (v[1] - y) * n[1] + (v[2] - z) * n[2] < 0.) return false;
+            return true;
+        }
+
+        // TetraMeshSpatialGrid::cellIndex, TetraMeshSpatialGrid.cpp:615-624: the first candidate
+        // of the point's search block that contains it.  The reference's BoxSearch lists hold
+        // every tetra whose (closed) bounding box meets the block in increasing index order, so
+        // the answer is the lowest-index tetra containing the point; the uniform block grid
+        // used here has the same property.
+        int tetCellIndex(double x, double y, double z) const
+        {
+            if (tbn == 0) return -1;
+            int i = static_cast<int>(std::upper_bound(tbx.begin(), tbx.end(), x) - tbx.begin()) - 1;
+            int j = static_cast<int>(std::upper_bound(tby.begin(), tby.end(), y) - tby.begin()) - 1;
+            int k = static_cast<int>(std::upper_bound(tbz.begin(), tbz.end(), z) - tbz.begin()) - 1;
+            i = std::max(0, std::min(tbn - 1, i));
+            j = std::max(0, std::min(tbn - 1, j));
+            k = std::max(0, std::min(tbn - 1, k));
+            int64_t b = (static_cast<int64_t>(i) * tbn + j) * tbn + k;
+            for (int64_t q = tblOff[b]; q < tblOff[b + 1]; ++q)
+                if (tetContains(tblIds[q], x, y, z)) return tblIds[q];
+            return -1;
+        }
+
+        void buildTetraSearch()
+        {
+            // neighbour face index: the face of the neighbour that points back (TetraMeshSpatialGrid.cpp:508-520)
+            faceNbFace.assign(4 * ntets, -1);
+            for (int64_t m = 0; m < ntets; ++m)
+                for (int f = 0; f < 4; ++f)
+                {
+                    int nt = faceNb[4 * m + f];
+                    if (nt != -1)
+                        for (int fn = 0; fn < 4; ++fn)
+                            if (faceNb[4 * static_cast<int64_t>(nt) + fn] == m)
+                            {
+                                faceNbFace[4 * m + f] = static_cast<int8_t>(fn);
+                                break;
+                            }
+                }
+            tbn = std::max(1, static_cast<int>(std::cbrt(static_cast<double>(ntets)) / 2));
+            tbx.resize(tbn + 1);
+            tby.resize(tbn + 1);
+            tbz.resize(tbn + 1);
+            for (int i = 0; i <= tbn; ++i)
+            {
+                tbx[i] = extent.xmin + (extent.xmax - extent.xmin) * i / tbn;
+                tby[i] = extent.ymin + (extent.ymax - extent.ymin) * i / tbn;
+                tbz[i] = extent.zmin + (extent.zmax - extent.zmin) * i / tbn;
+            }
+            std::vector<std::vector<int32_t>> lists(static_cast<size_t>(tbn) * tbn * tbn);
+            auto lower = [](const std::vector<double>& g, double v) {
+                // first block whose upper border is >= v (closed overlap)
+                int i = static_cast<int>(std::lower_bound(g.begin() + 1, g.end(), v) - (g.begin() + 1));
+                return std::min(i, static_cast<int>(g.size()) - 2);
+            };
+            auto upper = [](const std::vector<double>& g, double v) {
+                // last block whose lower border is <= v
+                int i = static_cast<int>(std::upper_bound(g.begin(), g.end() - 1, v) - g.begin()) - 1;
+                return std::max(0, i);
+            };
+            for (int64_t m = 0; m < ntets; ++m)
+            {
+                double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+                for (int t = 0; t < 4; ++t)
+                    for (int a = 0; a < 3; ++a)
+                    {
+                        lo[a] = std::min(lo[a], tetVertex(m, t)[a]);
+                        hi[a] = std::max(hi[a], tetVertex(m, t)[a]);
+                    }
+                int i1 = lower(tbx, lo[0]), i2 = upper(tbx, hi[0]);
+                int j1 = lower(tby, lo[1]), j2 = upper(tby, hi[1]);
+                int k1 = lower(tbz, lo[2]), k2 = upper(tbz, hi[2]);
+                for (int i = i1; i <= i2; ++i)
+                    for (int j = j1; j <= j2; ++j)
+                        for (int k = k1; k <= k2; ++k)
+                            lists[(static_cast<size_t>(i) * tbn + j) * tbn + k].push_back(static_cast<int32_t>(m));
+            }
+            tblOff.assign(lists.size() + 1, 0);
+            for (size_t b = 0; b < lists.size(); ++b) tblOff[b + 1] = tblOff[b] + static_cast<int64_t>(lists[b].size());
+            tblIds.clear();
+            for (auto& l : lists) tblIds.insert(tblIds.end(), l.begin(), l.end());
+        }
 
         const double* nodeBounds(int n) const { return nb + 6 * static_cast<int64_t>(n); }
         bool nodeContains(int n, double x, double y, double z) const
@@ -290,7 +459,229 @@ namespace
         int node = -1;
         explicit Generator(const Grid* grid) : g(grid) {}
 
-        bool next() { return g->type == 0 ? nextCartesian() : nextTree(); }
+        int mr = -1;            // Voronoi / tetra: current cell
+        int enteringFace = -1;  // tetra: entering face of the current cell
+
+        bool next()
+        {
+            switch (g->type)
+            {
+                case 0: return nextCartesian();
+                case 1: return nextTree();
+                case 2: return nextVoronoi();
+                default: return nextTetra();
+            }
+        }
+
+        // VoronoiMeshSnapshot::MySegmentGenerator::next, VoronoiMeshSnapshot.cpp:1094-1213
+        bool nextVoronoi()
+        {
+            switch (state)
+            {
+                case State::Unknown:
+                {
+                    if (!moveInside(g->extent, g->eps)) return false;
+                    mr = g->vorCellIndex(rx, ry, rz);
+                    if (ds > 0.) return true;
+                }
+                // fall through
+                case State::Inside:
+                {
+                    while (true)
+                    {
+                        const double* pr = g->sites + 3 * static_cast<int64_t>(mr);
+                        double sq = DBL_MAX;
+                        const int NO_INDEX = -99;
+                        int mq = NO_INDEX;
+                        for (int q = g->vnbBegin[mr], e = g->vnbBegin[mr] + g->vnbCount[mr]; q < e; ++q)
+                        {
+                            int mi = g->vnbIndex[q];
+                            double si = 0;
+                            if (mi >= 0)
+                            {
+                                const double* pi = g->sites + 3 * static_cast<int64_t>(mi);
+                                double nx = pi[0] - pr[0], ny = pi[1] - pr[1], nz = pi[2] - pr[2];
+                                double ndotk = nx * kx + ny * ky + nz * kz;
+                                if (ndotk > 0)
+                                {
+                                    double px = 0.5 * (pi[0] + pr[0]), py = 0.5 * (pi[1] + pr[1]), pz = 0.5 * (pi[2] + pr[2]);
+                                    si = (nx * (px - rx) + ny * (py - ry) + nz * (pz - rz)) / ndotk;
+                                }
+                            }
+                            else
+                            {
+                                switch (mi)
+                                {
+                                    case -1: si = (g->extent.xmin - rx) / kx; break;
+                                    case -2: si = (g->extent.xmax - rx) / kx; break;
+                                    case -3: si = (g->extent.ymin - ry) / ky; break;
+                                    case -4: si = (g->extent.ymax - ry) / ky; break;
+                                    case -5: si = (g->extent.zmin - rz) / kz; break;
+                                    case -6: si = (g->extent.zmax - rz) / kz; break;
+                                    default: break;
+                                }
+                            }
+                            if (si > 0 && si < sq)
+                            {
+                                sq = si;
+                                mq = mi;
+                            }
+                        }
+                        if (mq == NO_INDEX)
+                        {
+                            propagater(g->eps);
+                            mr = g->vorCellIndex(rx, ry, rz);
+                            if (mr < 0)
+                            {
+                                state = State::Outside;
+                                return false;
+                            }
+                        }
+                        else
+                        {
+                            propagater(sq + g->eps);
+                            m = mr;
+                            ds = sq;
+                            mr = mq;
+                            if (mr < 0) state = State::Outside;
+                            return true;
+                        }
+                    }
+                }
+                case State::Outside: break;
+            }
+            return false;
+        }
+
+        // TetraMeshSpatialGrid.cpp:44-50
+        static void clockwiseVertices(int face, int* cv)
+        {
+            cv[0] = (face + 3) % 4;
+            cv[1] = (face + 2) % 4;
+            cv[2] = (face + 1) % 4;
+            if (face % 2 == 0) std::swap(cv[0], cv[2]);
+        }
+
+        // Tetra::findEnteringFace, TetraMeshSpatialGrid.cpp:87-108
+        int findEnteringFace(int64_t t) const
+        {
+            static constexpr int etable[6][2] = {{3, 2}, {1, 3}, {2, 1}, {3, 0}, {0, 2}, {1, 0}};
+            int e = 0;
+            for (int t1 = 0; t1 < 3; t1++)
+                for (int t2 = t1 + 1; t2 < 4; t2++)
+                {
+                    const double* a = g->tetVertex(t, t1);
+                    const double* b = g->tetVertex(t, t2);
+                    double wx = rx - a[0], wy = ry - a[1], wz = rz - a[2];
+                    double mx = ky * wz - kz * wy, my = kz * wx - kx * wz, mz = kx * wy - ky * wx;
+                    double prod = mx * (b[0] - a[0]) + my * (b[1] - a[1]) + mz * (b[2] - a[2]);
+                    if (prod != 0.) return prod < 0 ? etable[e][0] : etable[e][1];
+                    e++;
+                }
+            return -1;
+        }
+
+        // TetraMeshSpatialGrid::MySegmentGenerator::next, TetraMeshSpatialGrid.cpp:697-820.
+        // The final segment, in the last tetrahedron before the path leaves the convex hull, is
+        // set but not returned, as in the reference.  Two cases in which the reference does not
+        // terminate end the path here instead: no entering face found (all six Pluecker products
+        // zero; the reference then indexes vertex -1) and a position that is no longer finite (a
+        // 0/0 exit distance, after which the reference walks on forever).
+        bool nextTetra()
+        {
+            if (state == State::Unknown)
+            {
+                if (!moveInside(g->extent, g->eps)) return false;
+                mr = g->tetCellIndex(rx, ry, rz);
+                enteringFace = -1;
+                if (mr == -1) return false;
+                if (ds > 0.) return true;
+            }
+            if (state == State::Inside)
+            {
+                while (true)
+                {
+                    if (!(std::isfinite(rx) && std::isfinite(ry) && std::isfinite(rz)))
+                    {
+                        state = State::Outside;
+                        return false;
+                    }
+                    int leavingFace = -1;
+                    double dsl = DBL_MAX;
+                    if (enteringFace == -1) enteringFace = findEnteringFace(mr);
+                    if (enteringFace == -1)
+                    {
+                        state = State::Outside;
+                        return false;
+                    }
+                    const double* v = g->tetVertex(mr, enteringFace);
+                    double wx = rx - v[0], wy = ry - v[1], wz = rz - v[2];
+                    double mx = ky * wz - kz * wy, my = kz * wx - kx * wz, mz = kx * wy - ky * wx;
+                    int cv[3];
+                    clockwiseVertices(enteringFace, cv);
+                    const double* c0 = g->tetVertex(mr, cv[0]);
+                    double prod0 = mx * (v[0] - c0[0]) + my * (v[1] - c0[1]) + mz * (v[2] - c0[2]);
+                    int clock0 = prod0 < 0;
+                    int i = clock0 ? 1 : 2;
+                    const double* ci = g->tetVertex(mr, cv[i]);
+                    double prodi = mx * (v[0] - ci[0]) + my * (v[1] - ci[1]) + mz * (v[2] - ci[2]);
+                    int cclocki = prodi >= 0;
+                    if (prod0 == 0. || prodi == 0.)
+                    {
+                        for (int q = 0; q < 3; ++q)
+                        {
+                            int face = cv[q];
+                            const double* n = g->tetNormal(mr, face);
+                            double ndotk = n[0] * kx + n[1] * ky + n[2] * kz;
+                            if (ndotk > 0)
+                            {
+                                double dq = (n[0] * (v[0] - rx) + n[1] * (v[1] - ry) + n[2] * (v[2] - rz)) / ndotk;
+                                if (dq < dsl)
+                                {
+                                    dsl = dq;
+                                    leavingFace = face;
+                                }
+                            }
+                        }
+                    }
+                    else
+                    {
+                        static constexpr int dtable[2][2] = {{1, 0}, {0, 2}};
+                        leavingFace = cv[dtable[clock0][cclocki]];
+                        const double* n = g->tetNormal(mr, leavingFace);
+                        double ndotk = n[0] * kx + n[1] * ky + n[2] * kz;
+                        dsl = (n[0] * (v[0] - rx) + n[1] * (v[1] - ry) + n[2] * (v[2] - rz)) / ndotk;
+                    }
+                    if (leavingFace == -1 || dsl < g->eps)
+                    {
+                        propagater(g->eps);
+                        mr = g->tetCellIndex(rx, ry, rz);
+                        if (mr < 0)
+                        {
+                            state = State::Outside;
+                            return false;
+                        }
+                        enteringFace = -1;
+                    }
+                    else
+                    {
+                        propagater(dsl);
+                        m = mr;
+                        ds = dsl;
+                        int64_t fi = 4 * static_cast<int64_t>(mr) + leavingFace;
+                        mr = g->faceNb[fi];
+                        if (mr < 0)
+                        {
+                            state = State::Outside;
+                            return false;
+                        }
+                        enteringFace = g->faceNbFace[fi];
+                        return true;
+                    }
+                }
+            }
+            return false;
+        }
 
         // CartesianSpatialGrid.cpp:77-153
         bool nextCartesian()
@@ -527,7 +918,7 @@ extern "C" {
 
 struct orc_grid
 {
-    int32_t type;  // 0 Cartesian, 1 octree
+    int32_t type;  // 0 Cartesian, 1 octree, 2 Voronoi, 3 tetra
     int32_t nx, ny, nz;
     const double *xb, *yb, *zb;  // Cartesian borders (nx+1, ny+1, nz+1)
     int64_t nnodes;
@@ -537,6 +928,21 @@ struct orc_grid
     const int64_t* nb_off;       // 6*nnodes+1 offsets into nb_id
     const int32_t* nb_id;        // neighbour node ids, per node per wall, sorted
     double box[6];               // domain extent
+    // Voronoi (type 2)
+    const double* sites;         // 3 per site
+    const int32_t* vnb_begin;
+    const int32_t* vnb_count;
+    const int32_t* vnb_index;    // neighbour site ids, walls as -1..-6
+    int32_t block_n;
+    const int32_t* block_begin;  // block_n^3
+    const int32_t* block_count;
+    const int32_t* block_index;
+    // tetra (type 3)
+    const double* verts;         // 3 per vertex
+    int64_t ntets;
+    const int32_t* tet_verts;    // 4 per tetra
+    const int32_t* face_nb;      // 4 per tetra, -1 on the hull
+    const double* face_normal;   // 12 per tetra
 };
 
 struct orc_instrument
@@ -566,7 +972,7 @@ struct orc_sim
 };
 
 // version tag used by the loader
-int orc_version() { return 3; }
+int orc_version() { return 4; }
 
 // Philox4x32-10 block, exposed for the known-answer tests
 void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out)
@@ -586,7 +992,8 @@ static Grid makeGrid(const orc_grid* og)
     Grid g;
     g.type = og->type;
     g.extent = Box{og->box[0], og->box[1], og->box[2], og->box[3], og->box[4], og->box[5]};
-    g.eps = 1e-12 * g.extent.diagonal();  // TreeSpatialGrid.cpp:28
+    // TreeSpatialGrid.cpp:28, VoronoiMeshSnapshot.cpp:396, TetraMeshSpatialGrid.cpp:239
+    g.eps = 1e-12 * g.extent.diagonal();
     g.xv = og->xb;
     g.yv = og->yb;
     g.zv = og->zb;
@@ -598,6 +1005,26 @@ static Grid makeGrid(const orc_grid* og)
     g.cellOf = og->cell_index;
     g.nbOff = og->nb_off;
     g.nbId = og->nb_id;
+    if (g.type == 2)
+    {
+        g.sites = og->sites;
+        g.vnbBegin = og->vnb_begin;
+        g.vnbCount = og->vnb_count;
+        g.vnbIndex = og->vnb_index;
+        g.vbn = og->block_n;
+        g.vblBegin = og->block_begin;
+        g.vblCount = og->block_count;
+        g.vblIndex = og->block_index;
+    }
+    if (g.type == 3)
+    {
+        g.verts = og->verts;
+        g.ntets = og->ntets;
+        g.tetVerts = og->tet_verts;
+        g.faceNb = og->face_nb;
+        g.faceNormal = og->face_normal;
+        g.buildTetraSearch();
+    }
     return g;
 }
 
@@ -673,6 +1100,17 @@ int64_t orc_tau(const orc_grid* og, const double* density, double section, const
         tauobs[i] = to;
     }
     return t;
+}
+
+// SpatialGrid::cellIndex for each point (Voronoi and tetra grids; -1 outside)
+void orc_cell_index(const orc_grid* og, const double* pts, int64_t n, int32_t* out)
+{
+    Grid g = makeGrid(og);
+    for (int64_t i = 0; i < n; ++i)
+    {
+        const double* p = pts + 3 * i;
+        out[i] = g.type == 2 ? g.vorCellIndex(p[0], p[1], p[2]) : g.type == 3 ? g.tetCellIndex(p[0], p[1], p[2]) : -1;
+    }
 }
 
 // HG helpers for the tests

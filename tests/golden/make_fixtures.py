@@ -51,7 +51,7 @@ PC = 3.08567758e16  # parsec in metres (Constants.hpp:50)
 def run(*args):
     res = subprocess.run([EXE, *map(str, args)], capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"skirt_ref {args[0]} failed:\n{res.stdout}\n{res.stderr}")
+        raise RuntimeError(f"skirt_ref {args[0]} failed (status {res.returncode}):\n{res.stdout}\n{res.stderr}")
     return res.stdout
 
 
@@ -388,6 +388,15 @@ def fixtures_voronoi(tmp):
     print("voronoi_paths.npz:", {k: v.shape for k, v in out.items()})
 
 
+def traces_safely(tmp, ski, ray, mem_kib=4_000_000, seconds=20):
+    rp = os.path.join(tmp, "one_ray.bin")
+    with open(rp, "wb") as f:
+        f.write(struct.pack("<q", 1))
+        f.write(np.ascontiguousarray(ray, dtype="<f8").tobytes())
+    cmd = f"ulimit -v {mem_kib}; exec timeout {seconds} {EXE} paths {ski} {rp} {os.path.join(tmp, 'one.bin')} 5.5e-7"
+    return subprocess.run(["bash", "-c", cmd], capture_output=True).returncode == 0
+
+
 def fixtures_tetra(tmp):
     """Delaunay tetrahedral grid of 500 density-sampled vertices plus the 8 domain corners
     (TetraMeshSpatialGrid, DustDensity policy, no refinement)."""
@@ -397,15 +406,21 @@ def fixtures_tetra(tmp):
     ski = write_ski(tmp, "tet", spec)
     g = grid_dump(tmp, ski)
     rays = make_rays(rng, [-h] * 3, [h] * 3, n_random=600)
-    # rays from vertices along edges and from face centroids (zero Pluecker products)
+    # rays from vertices along edges and in random directions, from face centroids and from
+    # tetra centroids (zero Pluecker products).  A ray running exactly along a mesh edge can make
+    # the reference generator loop without end (its path grows until memory runs out); such rays
+    # are detected with a memory and time limit and left out.
     V, T = g["verts"], g["tet_verts"]
     extra = []
     for t in rng.choice(len(T), 60, replace=False):
         a, b, c, d = V[T[t]]
         extra.append(np.hstack([a, unit(b - a)]))
+        extra.append(np.hstack([a, unit(rng.normal(size=3))]))
         extra.append(np.hstack([(a + b + c) / 3, unit(d - (a + b + c) / 3)]))
         extra.append(np.hstack([g["centroid"][t], unit(rng.normal(size=3))]))
-    rays = np.vstack([rays, np.array(extra)])
+    keep = [ray for ray in extra if traces_safely(tmp, ski, ray)]
+    print("tetra: left out", len(extra) - len(keep), "runaway rays of", len(extra))
+    rays = np.vstack([rays, np.array(keep)])
     p = trace(tmp, ski, rays, 0.55e-6)
     pts = np.vstack([-h + 2 * h * rng.random((2000, 3)), g["centroid"][:200]])
     ci = cells(tmp, ski, pts)

@@ -35,6 +35,8 @@ extern "C" {
 
 #define SKG_GRID_CARTESIAN 0
 #define SKG_GRID_OCTREE 1
+#define SKG_GRID_VORONOI 2
+#define SKG_GRID_TETRA 3
 
 #define SKG_MODE_PIPELINE 0   /* wavefront pipeline: event kernel + persistent trace kernel, hipGraph */
 #define SKG_MODE_PERSISTENT 1 /* one persistent kernel running whole packets per lane */
@@ -68,6 +70,25 @@ int skg_grid_cartesian(skg_engine* e, const double* xb, int32_t nx, const double
  * as CSR: nb_off[6*nnodes+1], nb_id[nb_off[6*nnodes]].  Traversal follows TreeSpatialGrid.cpp:143-228. */
 int skg_grid_octree(skg_engine* e, int64_t nnodes, const double* bounds, const int32_t* child_first,
                     const int32_t* cell_index, const int64_t* nb_off, const int32_t* nb_id);
+
+/* Voronoi grid from the traversal tables of VoronoiMeshSpatialGrid (VoronoiMeshSnapshot.hpp:338-349):
+ * sites (3 per cell), per-cell neighbour lists nb_begin/nb_count into nb_index (neighbour cell
+ * ids; domain walls as -1..-6 = xmin,xmax,ymin,ymax,zmin,zmax), the block search grid
+ * (block_n^3 blocks, block_begin/block_count into block_index, VoronoiMeshSnapshot.cpp:783-831)
+ * and the domain box (xmin,ymin,zmin,xmax,ymax,zmax).  cellIndex follows
+ * VoronoiMeshSnapshot.cpp:1037-1071 and traversal VoronoiMeshSnapshot.cpp:1094-1213. */
+int skg_grid_voronoi(skg_engine* e, int64_t nsites, const double* sites, const int32_t* nb_begin,
+                     const int32_t* nb_count, const int32_t* nb_index, int32_t block_n, const int32_t* block_begin,
+                     const int32_t* block_count, const int32_t* block_index, const double* box);
+
+/* Tetrahedral grid from the traversal tables of TetraMeshSpatialGrid (TetraMeshSpatialGrid.hpp:293-303):
+ * vertices (3 per vertex), tet_verts (4 vertex ids per tetra), face_nb (4 per tetra: neighbour
+ * across the face opposite vertex f, -1 on the hull), face_normal (12 per tetra: outward unit
+ * normals) and the domain box.  cellIndex follows TetraMeshSpatialGrid.cpp:615-624 and
+ * traversal TetraMeshSpatialGrid.cpp:697-820, including the final segment before the hull
+ * not being returned. */
+int skg_grid_tetra(skg_engine* e, int64_t nverts, const double* verts, int64_t ntets, const int32_t* tet_verts,
+                   const int32_t* face_nb, const double* face_normal, const double* box);
 
 /* Number of cells of the current grid (SpatialGrid::numCells). */
 int64_t skg_num_cells(const skg_engine* e);

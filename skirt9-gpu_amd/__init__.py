@@ -62,7 +62,8 @@ def lib():
 
 
 EXPORTED = [
-    "skg_create", "skg_destroy", "skg_last_error", "skg_grid_cartesian", "skg_grid_octree", "skg_num_cells",
+    "skg_create", "skg_destroy", "skg_last_error", "skg_grid_cartesian", "skg_grid_octree", "skg_grid_voronoi",
+    "skg_grid_tetra", "skg_num_cells",
     "skg_medium", "skg_dust", "skg_trace_paths", "skg_trace_tau", "skg_frame_add", "skg_frame_remove_all",
     "skg_frame_clear", "skg_frame_read", "skg_source_point", "skg_options", "skg_run_packets",
     "skg_run_packets_async", "skg_sync", "skg_counters", "skg_tune", "skg_last_kernel_ms",
@@ -128,6 +129,26 @@ class Engine:
         nbi = np.ascontiguousarray(nb_id, dtype=np.int32)
         self._chk(lib().skg_grid_octree(self._h, C.c_int64(len(bounds)), _p(bounds, f64p), _p(cf, i32p),
                                         _p(ci, i32p), _p(nbo, i64p), _p(nbi, i32p)))
+
+    def grid_voronoi(self, sites, nb_begin, nb_count, nb_index, block_n, block_begin, block_count, block_index, box):
+        """Voronoi grid from VoronoiMeshSpatialGrid's traversal tables (walls in nb_index as -1..-6)."""
+        sites = _f64(sites).reshape(-1, 3)
+        a = [np.ascontiguousarray(x, dtype=np.int32) for x in
+             (nb_begin, nb_count, nb_index, block_begin, block_count, block_index)]
+        box = _f64(box)
+        self._chk(lib().skg_grid_voronoi(self._h, C.c_int64(len(sites)), _p(sites, f64p), _p(a[0], i32p),
+                                         _p(a[1], i32p), _p(a[2], i32p), C.c_int32(int(block_n)), _p(a[3], i32p),
+                                         _p(a[4], i32p), _p(a[5], i32p), _p(box, f64p)))
+
+    def grid_tetra(self, verts, tet_verts, face_nb, face_normal, box):
+        """Tetrahedral grid from TetraMeshSpatialGrid's traversal tables."""
+        verts = _f64(verts).reshape(-1, 3)
+        tv = np.ascontiguousarray(tet_verts, dtype=np.int32).reshape(-1, 4)
+        fn = np.ascontiguousarray(face_nb, dtype=np.int32).reshape(-1, 4)
+        nrm = _f64(face_normal).reshape(-1, 12)
+        box = _f64(box)
+        self._chk(lib().skg_grid_tetra(self._h, C.c_int64(len(verts)), _p(verts, f64p), C.c_int64(len(tv)),
+                                       _p(tv, i32p), _p(fn, i32p), _p(nrm, f64p), _p(box, f64p)))
 
     def num_cells(self):
         return int(lib().skg_num_cells(self._h))

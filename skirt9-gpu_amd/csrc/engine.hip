@@ -71,11 +71,11 @@ namespace skg
         }
     }
 
-    // copies the cell densities into the octree leaf records
-    __global__ void fill_leaf_density(OctLeaf* leaf, const double* dens, int64_t n)
+    // copies the cell densities into the cell records (octree leaves, Voronoi cells, tetrahedra)
+    template<typename Rec> __global__ void fill_record_density(Rec* rec, const double* dens, int64_t n)
     {
         int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-        if (i < n) leaf[i].dens = dens[i];
+        if (i < n) rec[i].dens = dens[i];
     }
 
     // compensated (Kahan) FP32 accumulator used in SKG_FP32_KAHAN mode
@@ -657,10 +657,15 @@ namespace
 
     int refreshLeafDensity(skg_engine* e)
     {
-        if (e->gridType != SKG_GRID_OCTREE || !e->dens || e->densCells < e->ncells) return SKG_OK;
+        if (e->gridType == SKG_GRID_CARTESIAN || e->gridType < 0 || !e->dens || e->densCells < e->ncells) return SKG_OK;
         int64_t n = e->ncells;
-        hipLaunchKernelGGL(fill_leaf_density, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, e->stream,
-                           const_cast<OctLeaf*>(e->grid.leaf), e->dens, n);
+        dim3 grid(static_cast<unsigned>((n + 255) / 256)), block(256);
+        if (e->gridType == SKG_GRID_OCTREE)
+            hipLaunchKernelGGL(fill_record_density<OctLeaf>, grid, block, 0, e->stream, const_cast<OctLeaf*>(e->grid.leaf), e->dens, n);
+        else if (e->gridType == SKG_GRID_VORONOI)
+            hipLaunchKernelGGL(fill_record_density<VorCell>, grid, block, 0, e->stream, const_cast<VorCell*>(e->grid.vcell), e->dens, n);
+        else
+            hipLaunchKernelGGL(fill_record_density<TetCell>, grid, block, 0, e->stream, const_cast<TetCell*>(e->grid.tcell), e->dens, n);
         HIPCHK(e, hipGetLastError());
         HIPCHK(e, hipStreamSynchronize(e->stream));
         return SKG_OK;
@@ -759,7 +764,65 @@ namespace
 
     template<typename F> int dispatchGrid(skg_engine* e, F&& f)
     {
-        return e->gridType == SKG_GRID_CARTESIAN ? f(std::integral_constant<int, kCartesian>()) : f(std::integral_constant<int, kOctree>());
+        switch (e->gridType)
+        {
+            case SKG_GRID_CARTESIAN: return f(std::integral_constant<int, kCartesian>());
+            case SKG_GRID_OCTREE: return f(std::integral_constant<int, kOctree>());
+            case SKG_GRID_VORONOI: return f(std::integral_constant<int, kVoronoi>());
+            default: return f(std::integral_constant<int, kTetra>());
+        }
+    }
+
+    // block search lists for a tetrahedral grid: tbn^3 equal blocks over the domain; a tetra is
+    // listed (in increasing index order) in every block its closed bounding box meets
+    void buildTetraBlocks(const std::vector<TetCell>& T, const double* box, int& tbn, std::vector<double>& borders,
+                          std::vector<int32_t>& off, std::vector<int32_t>& ids)
+    {
+        const int64_t nt = static_cast<int64_t>(T.size());
+        tbn = std::max(1, std::min(256, static_cast<int>(std::cbrt(static_cast<double>(nt)) / 2)));
+        borders.resize(3 * static_cast<size_t>(tbn + 1));
+        for (int a = 0; a < 3; ++a)
+            for (int i = 0; i <= tbn; ++i)
+                borders[a * (tbn + 1) + i] = box[a] + (box[3 + a] - box[a]) * i / tbn;
+        auto lower = [&](int a, double v) {  // first block whose upper border is >= v
+            const double* b = &borders[a * (tbn + 1)];
+            int i = static_cast<int>(std::lower_bound(b + 1, b + tbn + 1, v) - (b + 1));
+            return std::min(i, tbn - 1);
+        };
+        auto upper = [&](int a, double v) {  // last block whose lower border is <= v
+            const double* b = &borders[a * (tbn + 1)];
+            int i = static_cast<int>(std::upper_bound(b, b + tbn, v) - b) - 1;
+            return std::max(0, i);
+        };
+        const int64_t nb3 = static_cast<int64_t>(tbn) * tbn * tbn;
+        std::vector<int32_t> cnt(nb3 + 1, 0);
+        std::vector<int> rng(6 * nt);
+        for (int64_t m = 0; m < nt; ++m)
+        {
+            for (int a = 0; a < 3; ++a)
+            {
+                double lo = DBL_MAX, hi = -DBL_MAX;
+                for (int t = 0; t < 4; ++t)
+                {
+                    lo = std::min(lo, T[m].v[t][a]);
+                    hi = std::max(hi, T[m].v[t][a]);
+                }
+                rng[6 * m + 2 * a] = lower(a, lo);
+                rng[6 * m + 2 * a + 1] = upper(a, hi);
+            }
+            for (int i = rng[6 * m]; i <= rng[6 * m + 1]; ++i)
+                for (int j = rng[6 * m + 2]; j <= rng[6 * m + 3]; ++j)
+                    for (int k = rng[6 * m + 4]; k <= rng[6 * m + 5]; ++k) cnt[(static_cast<int64_t>(i) * tbn + j) * tbn + k + 1]++;
+        }
+        off.assign(nb3 + 1, 0);
+        for (int64_t b = 0; b < nb3; ++b) off[b + 1] = off[b] + cnt[b + 1];
+        ids.assign(off[nb3], 0);
+        std::vector<int32_t> fill(off.begin(), off.end() - 1);
+        for (int64_t m = 0; m < nt; ++m)
+            for (int i = rng[6 * m]; i <= rng[6 * m + 1]; ++i)
+                for (int j = rng[6 * m + 2]; j <= rng[6 * m + 3]; ++j)
+                    for (int k = rng[6 * m + 4]; k <= rng[6 * m + 5]; ++k)
+                        ids[fill[(static_cast<int64_t>(i) * tbn + j) * tbn + k]++] = static_cast<int32_t>(m);
     }
 }
 
@@ -940,6 +1003,135 @@ int skg_grid_octree(skg_engine* e, int64_t nnodes, const double* bounds, const i
     e->grid = g;
     e->gridType = SKG_GRID_OCTREE;
     e->ncells = ncells;
+    return refreshLeafDensity(e);
+}
+
+int skg_grid_voronoi(skg_engine* e, int64_t nsites, const double* sites, const int32_t* nb_begin, const int32_t* nb_count,
+                     const int32_t* nb_index, int32_t block_n, const int32_t* block_begin, const int32_t* block_count,
+                     const int32_t* block_index, const double* box)
+{
+    if (!e || nsites < 1 || nsites > INT32_MAX || !sites || !nb_begin || !nb_count || !nb_index || block_n < 1
+        || block_n > 1024 || !block_begin || !block_count || !block_index || !box)
+        return fail(e, SKG_ERR_ARGUMENT, "skg_grid_voronoi: bad arguments");
+    if (!(box[0] < box[3] && box[1] < box[4] && box[2] < box[5])) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_voronoi: empty box");
+    std::vector<VorCell> cells(nsites);
+    std::vector<VorNb> nbs;
+    for (int64_t m = 0; m < nsites; ++m)
+    {
+        VorCell& c = cells[m];
+        std::memset(&c, 0, sizeof(c));
+        c.x = sites[3 * m];
+        c.y = sites[3 * m + 1];
+        c.z = sites[3 * m + 2];
+        if (nb_begin[m] < 0 || nb_count[m] < 0) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_voronoi: bad neighbour range");
+        c.begin = static_cast<int32_t>(nbs.size());
+        c.count = nb_count[m];
+        for (int32_t q = nb_begin[m]; q < nb_begin[m] + nb_count[m]; ++q)
+        {
+            int32_t id = nb_index[q];
+            if (id < -6 || id >= nsites) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_voronoi: neighbour id out of range");
+            VorNb E{};
+            E.id = id;
+            if (id >= 0)
+            {
+                E.x = sites[3 * static_cast<int64_t>(id)];
+                E.y = sites[3 * static_cast<int64_t>(id) + 1];
+                E.z = sites[3 * static_cast<int64_t>(id) + 2];
+            }
+            nbs.push_back(E);
+            if (nbs.size() > static_cast<size_t>(INT32_MAX)) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_voronoi: too many neighbours");
+        }
+    }
+    const int64_t nb3 = static_cast<int64_t>(block_n) * block_n * block_n;
+    std::vector<int32_t> boff(nb3 + 1, 0), bids;
+    for (int64_t b = 0; b < nb3; ++b)
+    {
+        if (block_begin[b] < 0 || block_count[b] < 0) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_voronoi: bad block range");
+        for (int32_t q = block_begin[b]; q < block_begin[b] + block_count[b]; ++q)
+        {
+            if (block_index[q] < 0 || block_index[q] >= nsites) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_voronoi: block id out of range");
+            bids.push_back(block_index[q]);
+        }
+        if (bids.size() > static_cast<size_t>(INT32_MAX)) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_voronoi: block lists too long");
+        boff[b + 1] = static_cast<int32_t>(bids.size());
+    }
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    freeGrid(e);
+    GridView g{};
+    int st;
+    if ((st = gridUpload(e, cells.data(), cells.size(), &g.vcell)) || (st = gridUpload(e, boff.data(), boff.size(), &g.vbl_off)))
+        return st;
+    if (!nbs.empty() && (st = gridUpload(e, nbs.data(), nbs.size(), &g.vnb))) return st;
+    if (!bids.empty() && (st = gridUpload(e, bids.data(), bids.size(), &g.vbl_id))) return st;
+    g.vbn = block_n;
+    for (int q = 0; q < 6; ++q) g.box[q] = box[q];
+    double dx = box[3] - box[0], dy = box[4] - box[1], dz = box[5] - box[2];
+    g.eps = 1e-12 * std::sqrt(dx * dx + dy * dy + dz * dz);  // VoronoiMeshSnapshot.cpp:396
+    e->grid = g;
+    e->gridType = SKG_GRID_VORONOI;
+    e->ncells = nsites;
+    return refreshLeafDensity(e);
+}
+
+int skg_grid_tetra(skg_engine* e, int64_t nverts, const double* verts, int64_t ntets, const int32_t* tet_verts,
+                   const int32_t* face_nb, const double* face_normal, const double* box)
+{
+    if (!e || nverts < 4 || ntets < 1 || ntets > INT32_MAX || !verts || !tet_verts || !face_nb || !face_normal || !box)
+        return fail(e, SKG_ERR_ARGUMENT, "skg_grid_tetra: bad arguments");
+    if (!(box[0] < box[3] && box[1] < box[4] && box[2] < box[5])) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_tetra: empty box");
+    std::vector<TetCell> T(ntets);
+    for (int64_t m = 0; m < ntets; ++m)
+    {
+        TetCell& c = T[m];
+        std::memset(&c, 0, sizeof(c));
+        for (int t = 0; t < 4; ++t)
+        {
+            int32_t vi = tet_verts[4 * m + t];
+            if (vi < 0 || vi >= nverts) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_tetra: vertex id out of range");
+            for (int a = 0; a < 3; ++a)
+            {
+                c.v[t][a] = verts[3 * static_cast<int64_t>(vi) + a];
+                c.n[t][a] = face_normal[12 * m + 3 * t + a];
+            }
+            int32_t nt = face_nb[4 * m + t];
+            if (nt < -1 || nt >= ntets) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_tetra: neighbour id out of range");
+            c.nb[t] = nt;
+            c.nface[t] = -1;
+            // the neighbour's face that points back (TetraMeshSpatialGrid.cpp:508-520)
+            if (nt >= 0)
+                for (int fn = 0; fn < 4; ++fn)
+                    if (face_nb[4 * static_cast<int64_t>(nt) + fn] == m)
+                    {
+                        c.nface[t] = static_cast<int8_t>(fn);
+                        break;
+                    }
+            if (nt >= 0 && c.nface[t] < 0) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_tetra: neighbour relation not mutual");
+        }
+    }
+    int tbn = 0;
+    std::vector<double> borders;
+    std::vector<int32_t> off, ids;
+    buildTetraBlocks(T, box, tbn, borders, off, ids);
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    freeGrid(e);
+    GridView g{};
+    int st;
+    const double* dB = nullptr;
+    if ((st = gridUpload(e, T.data(), T.size(), &g.tcell)) || (st = gridUpload(e, borders.data(), borders.size(), &dB))
+        || (st = gridUpload(e, off.data(), off.size(), &g.tbl_off)) || (st = gridUpload(e, ids.data(), ids.size(), &g.tbl_id)))
+        return st;
+    g.tbn = tbn;
+    g.tbx = dB;
+    g.tby = dB + (tbn + 1);
+    g.tbz = dB + 2 * (tbn + 1);
+    for (int q = 0; q < 6; ++q) g.box[q] = box[q];
+    double dx = box[3] - box[0], dy = box[4] - box[1], dz = box[5] - box[2];
+    g.eps = 1e-12 * std::sqrt(dx * dx + dy * dy + dz * dz);  // TetraMeshSpatialGrid.cpp:239
+    e->grid = g;
+    e->gridType = SKG_GRID_TETRA;
+    e->ncells = ntets;
     return refreshLeafDensity(e);
 }
 
@@ -1230,12 +1422,12 @@ static int runPersistent(skg_engine* e, uint64_t first_history, uint64_t count)
     blocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(blocks, needBlocks)));
     HIPCHK(e, hipEventRecord(e->ev0, e->stream));
     hipError_t err;
-    if (e->gridType == SKG_GRID_CARTESIAN)
-        err = e->fpMode == SKG_FP64 ? launchLifecycle<kCartesian, SKG_FP64>(p, blocks, threads, e->stream)
-                                    : launchLifecycle<kCartesian, SKG_FP32_KAHAN>(p, blocks, threads, e->stream);
-    else
-        err = e->fpMode == SKG_FP64 ? launchLifecycle<kOctree, SKG_FP64>(p, blocks, threads, e->stream)
-                                    : launchLifecycle<kOctree, SKG_FP32_KAHAN>(p, blocks, threads, e->stream);
+    dispatchGrid(e, [&](auto gt) {
+        constexpr int GT = decltype(gt)::value;
+        err = e->fpMode == SKG_FP64 ? launchLifecycle<GT, SKG_FP64>(p, blocks, threads, e->stream)
+                                    : launchLifecycle<GT, SKG_FP32_KAHAN>(p, blocks, threads, e->stream);
+        return 0;
+    });
     if (err != hipSuccess) return fail(e, SKG_ERR_DEVICE, std::string("lifecycle launch: ") + hipGetErrorString(err));
     HIPCHK(e, hipEventRecord(e->ev1, e->stream));
     return SKG_OK;
@@ -1417,12 +1609,11 @@ namespace
         const int eventBlocks = static_cast<int>((slots + 255) / 256);
         const int traceBlocks = e->traceBlocks > 0 ? e->traceBlocks : e->cus * 8;
         hipGraphExec_t exec = nullptr;
-        if (e->gridType == SKG_GRID_CARTESIAN)
-            st = e->fpMode == SKG_FP64 ? capturePipeline<kCartesian, SKG_FP64>(e, p, eventBlocks, traceBlocks, iters, &exec)
-                                       : capturePipeline<kCartesian, SKG_FP32_KAHAN>(e, p, eventBlocks, traceBlocks, iters, &exec);
-        else
-            st = e->fpMode == SKG_FP64 ? capturePipeline<kOctree, SKG_FP64>(e, p, eventBlocks, traceBlocks, iters, &exec)
-                                       : capturePipeline<kOctree, SKG_FP32_KAHAN>(e, p, eventBlocks, traceBlocks, iters, &exec);
+        st = dispatchGrid(e, [&](auto gt) {
+            constexpr int GT = decltype(gt)::value;
+            return e->fpMode == SKG_FP64 ? capturePipeline<GT, SKG_FP64>(e, p, eventBlocks, traceBlocks, iters, &exec)
+                                         : capturePipeline<GT, SKG_FP32_KAHAN>(e, p, eventBlocks, traceBlocks, iters, &exec);
+        });
         if (st) return st;
         HIPCHK(e, hipEventRecord(e->ev0, e->stream));
         hipLaunchKernelGGL(pipe_reset, dim3(eventBlocks), dim3(256), 0, e->stream, p.pool, e->ctl);

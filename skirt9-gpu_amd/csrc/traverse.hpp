@@ -5,11 +5,15 @@
 //   moveInside          PathSegmentGenerator.cpp:11-112
 //   Cartesian next()    CartesianSpatialGrid.cpp:77-153
 //   octree next()       TreeSpatialGrid.cpp:143-228, TreeNode.cpp:66-76/103-112, OctTreeNode.cpp:37-42
+//   Voronoi next()      VoronoiMeshSnapshot.cpp:1094-1213, cellIndex :1037-1071
+//   tetra next()        TetraMeshSpatialGrid.cpp:697-820, findEnteringFace :87-108, contains :110-133,
+//                       cellIndex :615-624
 // One lane owns one path; the generator state lives in registers.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <cfloat>
+#include <cstddef>
 #include <cstdint>
 
 namespace skg
@@ -17,8 +21,52 @@ namespace skg
     enum GridType : int
     {
         kCartesian = 0,
-        kOctree = 1
+        kOctree = 1,
+        kVoronoi = 2,
+        kTetra = 3
     };
+
+    // Voronoi cell record (64 bytes, indexed by cell): the site, the cell's number density and
+    // its neighbour list in the entry table.
+    struct alignas(64) VorCell
+    {
+        double x, y, z;
+        double dens;
+        int32_t begin, count;
+        int32_t pad[6];
+    };
+    static_assert(sizeof(VorCell) == 64, "Voronoi cell record must be 64 bytes");
+
+    // One neighbour entry (32 bytes): the neighbour's site is stored with its id so that a
+    // cell's exit search streams through one contiguous run of entries instead of gathering
+    // the neighbour sites.  Domain walls have ids -1..-6 (xmin, xmax, ymin, ymax, zmin, zmax).
+    struct alignas(32) VorNb
+    {
+        double x, y, z;
+        int32_t id;
+        int32_t pad;
+    };
+    static_assert(sizeof(VorNb) == 32, "Voronoi neighbour entry must be 32 bytes");
+
+    // Tetrahedron record (256 bytes, two cache lines, indexed by cell): vertex coordinates,
+    // density and face links in the first line, the outward unit face normals in the second.
+    struct alignas(128) TetCell
+    {
+        double v[4][3];     // vertices in the reference's per-tetra order
+        double dens;        // number density
+        int32_t nb[4];      // neighbour tetra across face f (opposite vertex f), -1 on the hull
+        int8_t nface[4];    // the neighbour's index of the shared face
+        int32_t pad0;
+        double n[4][3];     // outward unit normal of face f
+        double pad1[4];
+    };
+    static_assert(sizeof(TetCell) == 256, "tetra record must be 256 bytes");
+    static_assert(offsetof(TetCell, n) == 128, "tetra normals start the second line");
+
+    // Consecutive "no exit found" steps after which a Voronoi or tetra path is ended.  The
+    // reference steps by eps without limit; a path needs that many only when it has left the
+    // valid geometry (such as a non-finite position).
+    constexpr int kMaxNoExit = 4096;
 
     // One 128-byte record per octree leaf (indexed by cell): everything a cell crossing
     // reads -- the leaf's FP64 bounds, its number density, the first (largest-overlap)
@@ -55,7 +103,127 @@ namespace skg
         const int32_t* cell_of;
         const OctLeaf* leaf;
         const int32_t* nbc;
+        // Voronoi: cell records, neighbour entries and the block search lists (CSR over vbn^3
+        // blocks of cell ids, in the reference's order)
+        const VorCell* vcell;
+        const VorNb* vnb;
+        int vbn;
+        const int32_t* vbl_off;
+        const int32_t* vbl_id;
+        // tetra: records and a block search grid (tbn^3 blocks, borders per axis, CSR of tetra
+        // ids in increasing order)
+        const TetCell* tcell;
+        int tbn;
+        const double* tbx;
+        const double* tby;
+        const double* tbz;
+        const int32_t* tbl_off;
+        const int32_t* tbl_id;
     };
+
+    // Box::cellIndices (Box.hpp:162-167) for the Voronoi block grid
+    __device__ __forceinline__ int vor_block(const GridView& g, double x, double y, double z)
+    {
+        const int n = g.vbn;
+        int i = static_cast<int>(n * (x - g.box[0]) / (g.box[3] - g.box[0]));
+        int j = static_cast<int>(n * (y - g.box[1]) / (g.box[4] - g.box[1]));
+        int k = static_cast<int>(n * (z - g.box[2]) / (g.box[5] - g.box[2]));
+        i = max(0, min(n - 1, i));
+        j = max(0, min(n - 1, j));
+        k = max(0, min(n - 1, k));
+        return i * n * n + j * n + k;
+    }
+
+    // VoronoiMeshSnapshot::cellIndex: nearest site among the block's candidates, squared
+    // distance (r - site).norm2(), strict '<' (the first of equal distances wins)
+    __device__ __forceinline__ int vor_cell_index(const GridView& g, double x, double y, double z)
+    {
+        if (!(x >= g.box[0] && x <= g.box[3] && y >= g.box[1] && y <= g.box[4] && z >= g.box[2] && z <= g.box[5]))
+            return -1;
+        const int b = vor_block(g, x, y, z);
+        int m = -1;
+        double best = DBL_MAX;
+        for (int q = g.vbl_off[b], e = g.vbl_off[b + 1]; q < e; ++q)
+        {
+            const int id = g.vbl_id[q];
+            const VorCell& c = g.vcell[id];
+            double dx = x - c.x, dy = y - c.y, dz = z - c.z;
+            double d = dx * dx + dy * dy + dz * dz;
+            if (d < best)
+            {
+                best = d;
+                m = id;
+            }
+        }
+        return m;
+    }
+
+    // number of borders <= v, minus one, clipped to a block index
+    __device__ __forceinline__ int tet_axis_block(const double* bv, int n, double v)
+    {
+        int lo = 0, hi = n + 1;  // count of borders <= v lies in [lo, hi]
+        while (lo < hi)
+        {
+            int mid = (lo + hi) >> 1;
+            if (bv[mid] <= v)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        return max(0, min(n - 1, lo - 1));
+    }
+
+    // Tetra::contains: bounding box of the vertices, then the face planes (outward normals;
+    // faces 0..2 through vertex 3, face 3 through vertex 0)
+    __device__ __forceinline__ bool tet_contains(const TetCell& T, double x, double y, double z)
+    {
+        double lx = DBL_MAX, ly = DBL_MAX, lz = DBL_MAX, hx = -DBL_MAX, hy = -DBL_MAX, hz = -DBL_MAX;
+        for (int t = 0; t < 4; ++t)
+        {
+            lx = fmin(lx, T.v[t][0]);
+            ly = fmin(ly, T.v[t][1]);
+            lz = fmin(lz, T.v[t][2]);
+            hx = fmax(hx, T.v[t][0]);
+            hy = fmax(hy, T.v[t][1]);
+            hz = fmax(hz, T.v[t][2]);
+        }
+        if (!(x >= lx && x <= hx && y >= ly && y <= hy && z >= lz && z <= hz)) return false;
+        for (int f = 0; f <= 2; ++f)
+            if ((T.v[3][0] - x) * T.n[f][0] + (T.v[3][1] - y) * T.n[f][1] + (T.v[3][2] - z) * T.n[f][2] < 0) return false;
+        if ((T.v[0][0] - x) * T.n[3][0] + (T.v[0][1] - y) * T.n[3][1] + (T.v[0][2] - z) * T.n[3][2] < 0.) return false;
+        return true;
+    }
+
+    // TetraMeshSpatialGrid::cellIndex: the lowest-index tetra containing the point (the
+    // reference's search lists hold, in increasing index order, every tetra whose bounding
+    // box meets the point's block, so its first hit is the lowest index; the block lists here
+    // have the same property)
+    __device__ __forceinline__ int tet_cell_index(const GridView& g, double x, double y, double z)
+    {
+        const int n = g.tbn;
+        int i = tet_axis_block(g.tbx, n, x), j = tet_axis_block(g.tby, n, y), k = tet_axis_block(g.tbz, n, z);
+        int b = (i * n + j) * n + k;
+        for (int q = g.tbl_off[b], e = g.tbl_off[b + 1]; q < e; ++q)
+        {
+            const int id = g.tbl_id[q];
+            if (tet_contains(g.tcell[id], x, y, z)) return id;
+        }
+        return -1;
+    }
+
+    // TetraMeshSpatialGrid.cpp:44-50
+    __device__ __forceinline__ void clockwise_vertices(int face, int& c0, int& c1, int& c2)
+    {
+        c0 = (face + 3) & 3;
+        c1 = (face + 2) & 3;
+        c2 = (face + 1) & 3;
+        if ((face & 1) == 0)
+        {
+            int t = c0;
+            c0 = c2;
+            c2 = t;
+        }
+    }
 
     enum GenState : int
     {
@@ -104,7 +272,8 @@ namespace skg
     {
         double rx, ry, rz, kx, ky, kz;
         int state;
-        int i, j, k;  // Cartesian cell indices; octree: i = current cell
+        int i, j, k;  // Cartesian cell indices; octree, Voronoi, tetra: i = current cell
+                      // (tetra: j = entering face, -1 if unknown)
         double b0, b1, b2, b3, b4, b5, bn;  // octree: bounds and density of the current cell
 
         __device__ __forceinline__ void load_leaf(const GridView& g, int c)
@@ -217,7 +386,7 @@ namespace skg
                     j = locate_clip(g.yb, g.ny + 1, ry);
                     k = locate_clip(g.zb, g.nz + 1, rz);
                 }
-                else
+                else if (GT == kOctree)
                 {
                     i = leaf_cell(g, rx, ry, rz);
                     if (i < 0)
@@ -226,6 +395,25 @@ namespace skg
                         return false;
                     }
                     load_leaf(g, i);
+                }
+                else if (GT == kVoronoi)
+                {
+                    i = vor_cell_index(g, rx, ry, rz);
+                    if (i < 0)
+                    {
+                        state = kOutside;
+                        return false;
+                    }
+                }
+                else
+                {
+                    i = tet_cell_index(g, rx, ry, rz);
+                    j = -1;
+                    if (i < 0)
+                    {
+                        state = kOutside;
+                        return false;
+                    }
                 }
                 if (cumds > 0.)
                 {
@@ -275,6 +463,10 @@ namespace skg
                 }
                 return true;
             }
+            else if constexpr (GT == kVoronoi)
+                return next_voronoi(g, m, ds, n);
+            else if constexpr (GT == kTetra)
+                return next_tetra(g, m, ds, n);
             else
             {
                 const int cur = i;
@@ -349,6 +541,181 @@ namespace skg
                 i = nxt;
                 return true;
             }
+        }
+
+        // exit through the nearest bisector plane or domain wall (VoronoiMeshSnapshot.cpp:1119-1210)
+        __device__ __forceinline__ bool next_voronoi(const GridView& g, int& m, double& ds, double& n)
+        {
+            for (int tries = 0; tries < kMaxNoExit; ++tries)
+            {
+                const VorCell& C = g.vcell[i];
+                const double px0 = C.x, py0 = C.y, pz0 = C.z;
+                double sq = DBL_MAX;
+                int mq = -99;
+                for (int q = C.begin, e = C.begin + C.count; q < e; ++q)
+                {
+                    const VorNb& E = g.vnb[q];
+                    const int mi = E.id;
+                    double si = 0;
+                    if (mi >= 0)
+                    {
+                        double nx = E.x - px0, ny = E.y - py0, nz = E.z - pz0;
+                        double ndotk = nx * kx + ny * ky + nz * kz;
+                        if (ndotk > 0)
+                        {
+                            double qx = 0.5 * (E.x + px0), qy = 0.5 * (E.y + py0), qz = 0.5 * (E.z + pz0);
+                            si = (nx * (qx - rx) + ny * (qy - ry) + nz * (qz - rz)) / ndotk;
+                        }
+                    }
+                    else
+                    {
+                        // walls -1..-6: (box face - r_axis) / k_axis
+                        const int w = -1 - mi;
+                        const int ax = w >> 1;
+                        const double face = g.box[(w & 1) ? 3 + ax : ax];
+                        const double ra = ax == 0 ? rx : (ax == 1 ? ry : rz);
+                        const double ka = ax == 0 ? kx : (ax == 1 ? ky : kz);
+                        if (w < 6) si = (face - ra) / ka;
+                    }
+                    if (si > 0 && si < sq)
+                    {
+                        sq = si;
+                        mq = mi;
+                    }
+                }
+                if (mq == -99)
+                {
+                    rx += kx * g.eps;
+                    ry += ky * g.eps;
+                    rz += kz * g.eps;
+                    i = vor_cell_index(g, rx, ry, rz);
+                    if (i < 0)
+                    {
+                        state = kOutside;
+                        return false;
+                    }
+                    continue;
+                }
+                const double step = sq + g.eps;
+                rx += kx * step;
+                ry += ky * step;
+                rz += kz * step;
+                m = i;
+                ds = sq;
+                n = C.dens;
+                i = mq;
+                if (mq < 0) state = kOutside;
+                return true;
+            }
+            state = kOutside;
+            return false;
+        }
+
+        // Tetra::findEnteringFace: the first of the six edges with a nonzero Pluecker product
+        __device__ __forceinline__ int entering_face(const TetCell& T) const
+        {
+            int e = 0;
+            for (int t1 = 0; t1 < 3; t1++)
+                for (int t2 = t1 + 1; t2 < 4; t2++)
+                {
+                    double wx = rx - T.v[t1][0], wy = ry - T.v[t1][1], wz = rz - T.v[t1][2];
+                    double mx = ky * wz - kz * wy, my = kz * wx - kx * wz, mz = kx * wy - ky * wx;
+                    double prod = mx * (T.v[t2][0] - T.v[t1][0]) + my * (T.v[t2][1] - T.v[t1][1])
+                                  + mz * (T.v[t2][2] - T.v[t1][2]);
+                    if (prod != 0.)
+                    {
+                        // etable {{3,2},{1,3},{2,1},{3,0},{0,2},{1,0}} (TetraMeshSpatialGrid.cpp:91)
+                        constexpr int neg = 3 | (1 << 2) | (2 << 4) | (3 << 6) | (0 << 8) | (1 << 10);
+                        constexpr int pos = 2 | (3 << 2) | (1 << 4) | (0 << 6) | (2 << 8) | (0 << 10);
+                        return ((prod < 0 ? neg : pos) >> (2 * e)) & 3;
+                    }
+                    e++;
+                }
+            return -1;
+        }
+
+        // Pluecker exit-face decision with the plane-intersection fallback
+        // (TetraMeshSpatialGrid.cpp:719-819); the final segment before the hull is not returned.
+        // Ends the path where the reference would not terminate: no entering face (all six
+        // products zero) or a non-finite position.
+        __device__ __forceinline__ bool next_tetra(const GridView& g, int& m, double& ds, double& n)
+        {
+            for (int tries = 0; tries < kMaxNoExit; ++tries)
+            {
+                if (!(isfinite(rx) && isfinite(ry) && isfinite(rz))) break;
+                const TetCell& T = g.tcell[i];
+                if (j < 0) j = entering_face(T);
+                if (j < 0) break;
+                const int ef = j;
+                const double vx = T.v[ef][0], vy = T.v[ef][1], vz = T.v[ef][2];
+                double wx = rx - vx, wy = ry - vy, wz = rz - vz;
+                double mx = ky * wz - kz * wy, my = kz * wx - kx * wz, mz = kx * wy - ky * wx;
+                int c0, c1, c2;
+                clockwise_vertices(ef, c0, c1, c2);
+                double prod0 = mx * (vx - T.v[c0][0]) + my * (vy - T.v[c0][1]) + mz * (vz - T.v[c0][2]);
+                int clock0 = prod0 < 0;
+                int ci = clock0 ? c1 : c2;
+                double prodi = mx * (vx - T.v[ci][0]) + my * (vy - T.v[ci][1]) + mz * (vz - T.v[ci][2]);
+                int cclocki = prodi >= 0;
+                int leaving = -1;
+                double dsl = DBL_MAX;
+                if (prod0 == 0. || prodi == 0.)
+                {
+                    const int cv[3] = {c0, c1, c2};
+                    for (int q = 0; q < 3; ++q)
+                    {
+                        const int f = cv[q];
+                        double ndotk = T.n[f][0] * kx + T.n[f][1] * ky + T.n[f][2] * kz;
+                        if (ndotk > 0)
+                        {
+                            double dq = (T.n[f][0] * (vx - rx) + T.n[f][1] * (vy - ry) + T.n[f][2] * (vz - rz)) / ndotk;
+                            if (dq < dsl)
+                            {
+                                dsl = dq;
+                                leaving = f;
+                            }
+                        }
+                    }
+                }
+                else
+                {
+                    // dtable {{1,0},{0,2}} indexes the clockwise vertices
+                    leaving = clock0 ? (cclocki ? c2 : c0) : (cclocki ? c0 : c1);
+                    double ndotk = T.n[leaving][0] * kx + T.n[leaving][1] * ky + T.n[leaving][2] * kz;
+                    dsl = (T.n[leaving][0] * (vx - rx) + T.n[leaving][1] * (vy - ry) + T.n[leaving][2] * (vz - rz)) / ndotk;
+                }
+                if (leaving == -1 || dsl < g.eps)
+                {
+                    rx += kx * g.eps;
+                    ry += ky * g.eps;
+                    rz += kz * g.eps;
+                    i = tet_cell_index(g, rx, ry, rz);
+                    if (i < 0)
+                    {
+                        state = kOutside;
+                        return false;
+                    }
+                    j = -1;
+                    continue;
+                }
+                rx += kx * dsl;
+                ry += ky * dsl;
+                rz += kz * dsl;
+                const int nxt = T.nb[leaving];
+                if (nxt < 0)
+                {
+                    state = kOutside;
+                    return false;
+                }
+                m = i;
+                ds = dsl;
+                n = T.dens;
+                j = T.nface[leaving];
+                i = nxt;
+                return true;
+            }
+            state = kOutside;
+            return false;
         }
     };
 }

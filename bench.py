@@ -8,12 +8,18 @@ frame at 60 degrees) on the spiral-disk octree (levels 3..10, max dust fraction 
 in HBM before the timed region; `value` is completed histories per second.
 
 Printed JSON line (rank 0, N = 1):
-  roofline      dominant kernel (pipe_trace, the grid walk of the wavefront pipeline):
-                algorithmic bytes per launch (64 B per octree segment crossing + 96 B of
-                request and packet state per traversal) divided by its average duration from
-                HIP events on the engine's stream, against 8 TB/s HBM
+  roofline      dominant kernel: `lifecycle` (persistent engine, the default) or `pipe_trace`
+                (the grid walk of the wavefront pipeline): algorithmic bytes per launch (64 B
+                per octree segment crossing, plus 8 B per detector atomic for the persistent
+                kernel or 96 B of request and packet state per traversal for the pipeline)
+                divided by the kernel's average duration from HIP events on the engine's
+                stream, against 8 TB/s HBM
   cpu_baseline  the reference SKIRT CPU path (oracle/_ref) on a bounded sample of the same
                 workload on this host's cores; falls back to the FP64 oracle ("port")
+  guide         plain HG versus the learned path guide: records from the first 1e7 plain
+                packets, Adam training on the GPU, then 32 independent batches of each sampler;
+                figure of merit 1/(relative variance of the mean x time) for the total frame
+                flux and the median over pixels
 """
 
 from __future__ import annotations
@@ -97,21 +103,111 @@ def cpu_baseline_port(wl, packets, threads, seed):
                 sample=f"{packets} packets of the same workload and grid through the FP64 oracle, {t:.2f} s")
 
 
+def frame_batches(eng, first, batches, per):
+    """Independent batches of `per` packets; returns (per-batch frames (B, nl, ny, nx), GPU
+    wall time summed over the batches, next history index)."""
+    frames, t = [], 0.0
+    for _ in range(batches):
+        eng.frame_clear()
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.run_packets(first, per)
+        eng.sync()
+        t += time.perf_counter() - t0
+        frames.append(eng.frame_read(0))
+        first += per
+    return np.stack(frames), t, first
+
+
+def figure_of_merit(X, t):
+    """1/(relative variance of the mean x time) for the total frame flux and per pixel
+    (median over pixels with signal)."""
+    B = len(X)
+    tot = X.reshape(B, -1).sum(axis=1)
+    rv_tot = tot.var(ddof=1) / B / tot.mean() ** 2
+    m = X.mean(axis=0)
+    v = X.var(axis=0, ddof=1) / B
+    ok = (m > 0) & (v > 0)
+    rv_pix = v[ok] / m[ok] ** 2
+    return dict(fom_total=1.0 / (rv_tot * t), fom_pixel_median=float(np.median(1.0 / (rv_pix * t))),
+                rel_err_total=float(np.sqrt(rv_tot)), pixels=int(ok.sum()), seconds=t, mean_total=float(tot.mean()))
+
+
+def guided_comparison(eng, pkg, a, first):
+    """Plain HG versus the learned guide (50/50 defensive mixture) on the same workload:
+    records from the first `train_packets` plain packets, Adam on the GPU, then batches of
+    both samplers in the pipeline engine and the plain one also in the persistent engine."""
+    out = {}
+    eng.tune_pipeline(pkg.SKG_MODE_PIPELINE, a.pool, a.trace_wgs, a.graph_iters)
+    theta = np.zeros(pkg.GUIDE_PARAMS, np.float32)
+    rng = np.random.default_rng(7)
+    # He-initialised weights, zero biases, small last layer (same layout as skg_guide_params)
+    o = 0
+    for (fan_in, fan_out) in ((9, 64), (64, 64), (64, 64), (64, 40)):
+        w = rng.normal(0, np.sqrt(2.0 / fan_in), fan_in * fan_out) * (0.1 if fan_out == 40 else 1.0)
+        theta[o:o + w.size] = w
+        o += w.size + fan_out
+    eng.guide_params(theta)
+    eng.guide_mode(2, mix=a.mix)
+    t0 = time.perf_counter()
+    eng.guide_collect(int(a.record_capacity))
+    eng.frame_clear()
+    eng.run_packets(first, int(a.train_packets))
+    first += int(a.train_packets)
+    nrec = eng.guide_records()
+    t1 = time.perf_counter()
+    l0, l1 = eng.guide_train(a.train_steps, batch=a.train_batch, lr=a.lr)
+    t2 = time.perf_counter()
+    out.update(train_packets=int(a.train_packets), records=int(nrec), collect_s=t1 - t0, train_s=t2 - t1,
+               train_steps=a.train_steps, loss_first=l0, loss_last=l1, mix=a.mix)
+    print(f"guide: {nrec} records from {a.train_packets:.0f} packets in {t1 - t0:.2f} s, "
+          f"{a.train_steps} Adam steps in {t2 - t1:.2f} s, loss {l0:.4f} -> {l1:.4f}", file=sys.stderr, flush=True)
+    res = {}
+    for name, mode, gm in (("plain_pipeline", pkg.SKG_MODE_PIPELINE, 0), ("guided", pkg.SKG_MODE_PIPELINE, 1),
+                           ("plain_persistent", pkg.SKG_MODE_PERSISTENT, 0)):
+        eng.tune_pipeline(mode, a.pool, a.trace_wgs, a.graph_iters)
+        eng.guide_mode(gm, mix=a.mix)
+        eng.run_packets(first, a.fom_packets)  # warm-up (graph capture, pool)
+        first += a.fom_packets
+        X, t, first = frame_batches(eng, first, a.fom_batches, a.fom_packets)
+        res[name] = figure_of_merit(X, t)
+        print(f"{name}: {a.fom_batches} x {a.fom_packets} packets in {t:.2f} s, rel. error of total "
+              f"{res[name]['rel_err_total']:.3e}, FOM total {res[name]['fom_total']:.3e}, "
+              f"pixel median {res[name]['fom_pixel_median']:.3e}", file=sys.stderr, flush=True)
+    eng.guide_mode(0)
+    plain = max(res["plain_pipeline"], res["plain_persistent"], key=lambda r: r["fom_pixel_median"])
+    g = res["guided"]
+    out.update(batches=a.fom_batches, packets_per_batch=a.fom_packets, runs=res,
+               gain_pixel_median=g["fom_pixel_median"] / plain["fom_pixel_median"],
+               gain_total=g["fom_total"] / plain["fom_total"],
+               guided_vs_plain_mean_total=g["mean_total"] / plain["mean_total"])
+    return out, first
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--packets", type=int, default=1 << 20, help="photon packets per step")
+    ap.add_argument("--packets", type=int, default=1 << 22, help="photon packets per step")
     ap.add_argument("--tau", type=float, default=50.0)
     ap.add_argument("--fp", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--cpu-packets", type=float, default=4e5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--threshold", type=int, default=16)
-    ap.add_argument("--mode", choices=["pipeline", "persistent"], default="pipeline")
+    ap.add_argument("--mode", choices=["pipeline", "persistent"], default="persistent")
     ap.add_argument("--pool", type=int, default=1 << 21, help="packet slots in flight (pipeline)")
     ap.add_argument("--trace-wgs", type=int, default=0)
     ap.add_argument("--graph-iters", type=int, default=8)
+    ap.add_argument("--no-guide", action="store_true", help="skip the guided-versus-plain comparison")
+    ap.add_argument("--train-packets", type=float, default=1e7)
+    ap.add_argument("--record-capacity", type=float, default=2.5e8)
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--train-batch", type=int, default=8192)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--mix", type=float, default=0.5)
+    ap.add_argument("--fom-batches", type=int, default=32)
+    ap.add_argument("--fom-packets", type=int, default=1 << 19)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported on this build "
@@ -176,6 +272,10 @@ def main():
           f"{d[4] / a.steps:.3e} segments and {requests / a.steps:.3e} traversals per step, "
           f"{iterations / a.steps:.0f} pipeline iterations per step", file=sys.stderr, flush=True)
 
+    guide = None
+    if not a.no_guide:
+        guide, first = guided_comparison(eng, pkg, a, first)
+
     cpu = None
     if not a.no_cpu_baseline:
         try:
@@ -213,6 +313,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kname, "kernel_ms": avg_ms, "launches_per_step": launches / a.steps},
         "cpu_baseline": cpu,
+        "guide": guide,
     }
     print(json.dumps(line), flush=True)
     eng.close()

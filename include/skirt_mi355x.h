@@ -225,6 +225,27 @@ int skg_octree_export(const skg_octree* t, double* bounds, int32_t* child_first,
 int skg_octree_upload(skg_engine* e, const skg_octree* t);
 void skg_octree_free(skg_octree* t);
 
+/* Voronoi grid built on the host from a site list: VoronoiMeshSnapshot::buildMesh without
+ * relaxation (VoronoiMeshSnapshot.cpp:491-751: sites outside the box dropped, sorted by x,
+ * near duplicates within eps dropped) with the tessellation computed by clipping each cell
+ * with the bisector planes of its nearby sites (the reference uses Voro++), then the block
+ * search lists of VoronoiMeshSnapshot::buildSearchPerBlock (:783-831).  Cell m is the m-th
+ * retained site in x order. */
+typedef struct skg_voronoi skg_voronoi;
+int skg_voronoi_build(int64_t nsites, const double* sites, const double* box, int32_t threads, skg_voronoi** out);
+int skg_voronoi_info(const skg_voronoi* v, int64_t* ncells, int64_t* nneighbors, int32_t* block_n, int64_t* nblockrefs);
+/* Tables in the layout skg_grid_voronoi takes, plus each cell's volume; any pointer may be NULL. */
+int skg_voronoi_export(const skg_voronoi* v, double* sites, int32_t* nb_begin, int32_t* nb_count, int32_t* nb_index,
+                       int32_t* block_begin, int32_t* block_count, int32_t* block_index, double* volume);
+/* skg_grid_voronoi with the tables of v. */
+int skg_voronoi_upload(skg_engine* e, const skg_voronoi* v);
+/* Cell-averaged lattice field (nx*ny*nz values over the domain, index (i*ny+j)*nz+k, nearest
+ * lattice cell) at `samples` random points per cell, drawn as VoronoiMeshSnapshot::generatePosition
+ * does (:1005-1023: uniform in the cell's extent, kept if the site is the nearest). */
+int skg_sample_density_voronoi(const skg_voronoi* v, const double* lattice, int32_t nx, int32_t ny, int32_t nz,
+                               int32_t samples, uint64_t seed, int32_t threads, double* out);
+void skg_voronoi_free(skg_voronoi* v);
+
 /* Cell-averaged number densities from num_samples random points per cell
  * (MediumSystem.cpp:401-504, PropertySampler :129-155). */
 int skg_sample_density_cartesian(const double* xb, int32_t nx, const double* yb, int32_t ny, const double* zb,

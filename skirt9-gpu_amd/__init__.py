@@ -72,6 +72,8 @@ EXPORTED = [
     "skg_dust_meanlist", "skg_tune_pipeline", "skg_last_run_stats",
     "skg_guide_params", "skg_guide_read_params", "skg_guide_infer", "skg_guide_gradient", "skg_guide_mode",
     "skg_guide_collect", "skg_guide_records", "skg_guide_train",
+    "skg_voronoi_build", "skg_voronoi_info", "skg_voronoi_export", "skg_voronoi_upload",
+    "skg_sample_density_voronoi", "skg_voronoi_free",
 ]
 
 GUIDE_PARAMS = 11560
@@ -309,6 +311,69 @@ class Engine:
 
 
 # ---------------------------------------------------------------------------- host set-up
+class Voronoi:
+    """Host-built Voronoi tessellation (neighbour lists, extents, block lists) in the engine's
+    table layout; cell m is the m-th retained site in x order."""
+
+    def __init__(self, handle, box):
+        self._h = handle
+        self.box = np.array(box, dtype=np.float64)
+
+    @classmethod
+    def build(cls, sites, box, threads=0):
+        s = _f64(sites).reshape(-1, 3)
+        b = _f64(box)
+        h = C.c_void_p()
+        st = lib().skg_voronoi_build(C.c_int64(len(s)), _p(s, f64p), _p(b, f64p), C.c_int32(threads or os.cpu_count() or 1),
+                                     C.byref(h))
+        if st != SKG_OK:
+            raise EngineError(f"skg_voronoi_build failed with status {st}")
+        return cls(h, b)
+
+    def info(self):
+        a, b, c, d = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int64()
+        lib().skg_voronoi_info(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+        return a.value, b.value, c.value, d.value
+
+    def export(self):
+        n, nnb, bn, nbr = self.info()
+        out = dict(sites=np.zeros((n, 3)), nb_begin=np.zeros(n, np.int32), nb_count=np.zeros(n, np.int32),
+                   nb_index=np.zeros(max(nnb, 1), np.int32), block_n=np.int32(bn),
+                   block_begin=np.zeros(bn ** 3, np.int32), block_count=np.zeros(bn ** 3, np.int32),
+                   block_index=np.zeros(max(nbr, 1), np.int32), volume=np.zeros(n), box=self.box.copy())
+        lib().skg_voronoi_export(self._h, _p(out["sites"], f64p), _p(out["nb_begin"], i32p), _p(out["nb_count"], i32p),
+                                 _p(out["nb_index"], i32p), _p(out["block_begin"], i32p), _p(out["block_count"], i32p),
+                                 _p(out["block_index"], i32p), _p(out["volume"], f64p))
+        out["nb_index"] = out["nb_index"][:nnb]
+        out["block_index"] = out["block_index"][:nbr]
+        return out
+
+    def upload(self, engine):
+        engine._chk(lib().skg_voronoi_upload(engine._h, self._h))
+
+    def sample_density(self, lattice, samples=100, seed=1, threads=0):
+        lat = _f64(lattice)
+        nx, ny, nz = lat.shape
+        out = np.zeros(self.info()[0])
+        st = lib().skg_sample_density_voronoi(self._h, _p(lat, f64p), C.c_int32(nx), C.c_int32(ny), C.c_int32(nz),
+                                              C.c_int32(samples), C.c_uint64(seed),
+                                              C.c_int32(threads or os.cpu_count() or 1), _p(out, f64p))
+        if st != SKG_OK:
+            raise EngineError(f"skg_sample_density_voronoi failed with status {st}")
+        return out
+
+    def close(self):
+        if self._h:
+            lib().skg_voronoi_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DiskModel(C.Structure):
     """Exponential disk with optional spiral perturbation, normalised to face-on tau_v."""
     _fields_ = [("scale_length", C.c_double), ("scale_height", C.c_double), ("spiral_arms", C.c_int32),

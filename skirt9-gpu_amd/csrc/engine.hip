@@ -1655,10 +1655,12 @@ namespace
         {
             unsigned long long nrec = 0;
             HIPCHK(e, hipMemcpy(&nrec, e->recCount, sizeof(nrec), hipMemcpyDeviceToHost));
+            // records of earlier collect runs were finalized with their own runs' totals
+            const int64_t n0 = e->recN;
             int64_t n = std::min<int64_t>(static_cast<int64_t>(nrec), e->recCap);
-            if (n > 0)
-                hipLaunchKernelGGL(guide_finalize_records, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
-                                   e->stream, e->rec, e->recHist, e->recA, e->finalLum, n);
+            if (n > n0)
+                hipLaunchKernelGGL(guide_finalize_records, dim3(static_cast<unsigned>((n - n0 + 255) / 256)), dim3(256), 0,
+                                   e->stream, e->rec + 16 * n0, e->recHist + n0, e->recA + n0, e->finalLum, n - n0);
             HIPCHK(e, hipStreamSynchronize(e->stream));
             e->recN = n;
         }
@@ -1894,6 +1896,10 @@ int skg_guide_train(skg_engine* e, int32_t steps, int32_t batch, float lr, doubl
     for (float x : w) sum += x;
     if (!(sum > 0)) return fail(e, SKG_ERR_STATE, "skg_guide_train: the records carry no detector luminosity");
     const float cscale = static_cast<float>(static_cast<double>(e->recN) / (static_cast<double>(batch) * sum));
+    // batch s takes every nb-th record starting at s mod nb, so that each batch samples the
+    // whole collection (records are stored in the order the histories produced them)
+    const int64_t nb = e->recN / batch;
+    if (16 * nb > INT32_MAX) return fail(e, SKG_ERR_CAPACITY, "skg_guide_train: too many records per batch stride");
     double* dL = nullptr;
     HIPCHK(e, hipMalloc(&dL, sizeof(double)));
     float* th = e->gTheta;
@@ -1901,14 +1907,13 @@ int skg_guide_train(skg_engine* e, int32_t steps, int32_t batch, float lr, doubl
     float* m2 = e->gTheta + 2 * kGuideParams;
     float* g = e->gTheta + 3 * kGuideParams;
     HIPCHK(e, hipMemset(g, 0, kGuideParams * sizeof(float)));
-    const int64_t nb = e->recN / batch;
     for (int s = 0; s < steps; ++s)
     {
-        const int64_t off = (s % nb) * batch;
+        const int64_t first = s % nb;
         bool track = s == 0 || s == steps - 1;
         if (track) HIPCHK(e, hipMemsetAsync(dL, 0, sizeof(double), e->stream));
-        hipLaunchKernelGGL(guide_grad_kernel, dim3(batch / 64), dim3(64), 0, e->stream, e->net, e->rec + off * 16, 16,
-                           static_cast<int64_t>(batch), cscale, g, track ? dL : nullptr);
+        hipLaunchKernelGGL(guide_grad_kernel, dim3(batch / 64), dim3(64), 0, e->stream, e->net, e->rec + first * 16,
+                           static_cast<int>(16 * nb), static_cast<int64_t>(batch), cscale, g, track ? dL : nullptr);
         ++e->adamStep;
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.f - std::pow(b1, static_cast<float>(e->adamStep));

@@ -199,6 +199,10 @@ def main():
     ap.add_argument("--pool", type=int, default=1 << 21, help="packet slots in flight (pipeline)")
     ap.add_argument("--trace-wgs", type=int, default=0)
     ap.add_argument("--graph-iters", type=int, default=8)
+    ap.add_argument("--workload", choices=["octree", "voronoi"], default="octree",
+                    help="octree: the spiral disk at face-on tau_V (the measured configuration); voronoi: "
+                         "1e6-site Voronoi grid in a lognormal clumpy medium, mean tau_V 5, three frames")
+    ap.add_argument("--sites", type=int, default=1_000_000)
     ap.add_argument("--no-guide", action="store_true", help="skip the guided-versus-plain comparison")
     ap.add_argument("--train-packets", type=float, default=1e7)
     ap.add_argument("--record-capacity", type=float, default=2.5e8)
@@ -208,7 +212,20 @@ def main():
     ap.add_argument("--mix", type=float, default=0.5)
     ap.add_argument("--fom-batches", type=int, default=32)
     ap.add_argument("--fom-packets", type=int, default=1 << 19)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy kernel of MIB mebibytes three times (the "
+                         "reference reading for the FETCH_SIZE / WRITE_SIZE counters) and exit")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+        n = a.copy_calibration * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported on this build "
               "(packets shard across GPUs with one detector reduction at the end).", file=sys.stderr)
@@ -218,8 +235,20 @@ def main():
     from skirt9_gpu_amd import workloads
     threads = host_threads()
     nlambda, lam_range = 5, (0.3, 2.0)
-    wl = workloads.octree_spiral(tau_v=a.tau, nlambda=nlambda, lam_range_um=lam_range, threads=threads)
-    print(f"setup: {wl.octree.info()[1]} cells, {wl.setup_s:.1f} s", file=sys.stderr, flush=True)
+    if a.workload == "octree":
+        wl = workloads.octree_spiral(tau_v=a.tau, nlambda=nlambda, lam_range_um=lam_range, threads=threads)
+        ncells = int(wl.octree.info()[1])
+        bytes_per_segment = BYTES_PER_SEGMENT
+        wdesc = "octree spiral disk, face-on tauV=%g, 5 wavelengths 0.3-2 um" % a.tau
+    else:
+        wl = workloads.voronoi_clumpy(num_sites=a.sites, threads=threads)
+        ncells, nnb = wl.tables["mesh"].info()[:2]
+        # Voronoi crossing: cell record 64 B + one 32 B entry per neighbour of the cell
+        bytes_per_segment = 64.0 + 32.0 * nnb / ncells
+        wdesc = "Voronoi %d sites, lognormal clumpy medium (P(k)~k^-3.7, sigma_ln 1), mean tauV=5, 3 frames" % ncells
+        a.no_guide = True
+        a.no_cpu_baseline = True
+    print(f"setup: {ncells} cells, {wl.setup_s:.1f} s", file=sys.stderr, flush=True)
 
     eng = pkg.Engine(0)
     wl.apply(eng, npk_norm=1.0)
@@ -260,12 +289,12 @@ def main():
         kname = "pipe_trace"
         launches = max(trace_launches, 1)
         avg_ms = trace_ms / launches
-        total_bytes = BYTES_PER_SEGMENT * d[4] + BYTES_PER_REQUEST * requests
+        total_bytes = bytes_per_segment * d[4] + BYTES_PER_REQUEST * requests
     else:
         kname = "lifecycle"
         launches = a.steps
         avg_ms = float(np.mean(kernel_ms))
-        total_bytes = BYTES_PER_SEGMENT * d[4] + BYTES_PER_ATOMIC * d[5]
+        total_bytes = bytes_per_segment * d[4] + BYTES_PER_ATOMIC * d[5]
     bytes_per_launch = total_bytes / launches
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     print(f"timed: {histories} histories in {dt:.3f} s; {kname}: {launches} launches, avg {avg_ms:.3f} ms, "
@@ -285,11 +314,16 @@ def main():
         if cpu is None:
             cpu = cpu_baseline_port(wl, int(a.cpu_packets // 10), threads, 12345)
 
+    # HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC passes
+    # (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel's reading), when they cover this
+    # kernel, workload and batch size
     traffic = None
-    prof = os.path.join(ROOT, "profiles", "lifecycle_octree_tau50_traffic.json")
+    prof = os.path.join(ROOT, "profiles", f"{kname}_{a.workload}_tau{a.tau:g}_traffic.json")
     if os.path.exists(prof):
         try:
-            traffic = json.load(open(prof)).get("hbm_bytes_per_launch")
+            tj = json.load(open(prof))
+            if kname != "lifecycle" or tj.get("packets_per_launch") == a.packets:
+                traffic = tj.get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
 
@@ -306,9 +340,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if fp == pkg.SKG_FP64 else "f32+kahan/f64",
         "data": "synthetic",
-        "config": {"workload": "octree spiral disk, face-on tauV=%g, 5 wavelengths 0.3-2 um, plain HG (no guide)" % a.tau,
-                   "cells": int(wl.octree.info()[1]), "packets_per_step": a.packets, "frame": "200x200 at 60 deg",
-                   "segments_per_packet": float(d[4] / max(histories, 1))},
+        "config": {"workload": wdesc + ", plain HG in the timed steps",
+                   "cells": ncells, "packets_per_step": a.packets,
+                   "frames": ", ".join("200x200 at %g deg" % np.degrees(f["inclination"]) for f in wl.frames),
+                   "engine": a.mode, "segments_per_packet": float(d[4] / max(histories, 1)),
+                   "bytes_per_segment": bytes_per_segment},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kname, "kernel_ms": avg_ms, "launches_per_step": launches / a.steps},

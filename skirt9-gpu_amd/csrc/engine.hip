@@ -1889,11 +1889,21 @@ int skg_guide_train(skg_engine* e, int32_t steps, int32_t batch, float lr, doubl
     if (!e->gTheta) return fail(e, SKG_ERR_STATE, "skg_guide_train: no guide parameters");
     if (e->recN < batch) return fail(e, SKG_ERR_STATE, "skg_guide_train: fewer records than one batch");
     // mean training weight over all records, so that a batch's weights sum to about one
-    std::vector<float> w(e->recN);
-    HIPCHK(e, hipMemcpy2D(w.data(), sizeof(float), e->rec + 12, 16 * sizeof(float), sizeof(float), e->recN,
-                          hipMemcpyDeviceToHost));
     double sum = 0.;
-    for (float x : w) sum += x;
+    {
+        double* dSum = nullptr;
+        HIPCHK(e, hipMalloc(&dSum, sizeof(double)));
+        hipError_t st = hipMemsetAsync(dSum, 0, sizeof(double), e->stream);
+        if (st == hipSuccess)
+        {
+            hipLaunchKernelGGL(guide_weight_sum, dim3(1024), dim3(256), 0, e->stream, e->rec, e->recN, dSum);
+            st = hipGetLastError();
+        }
+        if (st == hipSuccess) st = hipMemcpyAsync(&sum, dSum, sizeof(double), hipMemcpyDeviceToHost, e->stream);
+        if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+        hipFree(dSum);
+        if (st != hipSuccess) return fail(e, SKG_ERR_DEVICE, std::string("skg_guide_train: ") + hipGetErrorString(st));
+    }
     if (!(sum > 0)) return fail(e, SKG_ERR_STATE, "skg_guide_train: the records carry no detector luminosity");
     const float cscale = static_cast<float>(static_cast<double>(e->recN) / (static_cast<double>(batch) * sum));
     // batch s takes every nb-th record starting at s mod nb, so that each batch samples the

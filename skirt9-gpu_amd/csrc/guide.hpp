@@ -456,6 +456,17 @@ namespace skg
         if (lane == 0 && lossOut) atomicAdd(lossOut, lossv);
     }
 
+    // sum of the training weights (record field 12) over n records, one FP64 atomic per wave
+    __global__ void guide_weight_sum(const float* rec, int64_t n, double* out)
+    {
+        double acc = 0.;
+        for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+             i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+            acc += rec[16 * i + 12];
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+        if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+    }
+
     // Adam step on the FP32 parameters, then zero the gradient
     __global__ void guide_adam_kernel(float* theta, float* m, float* v, float* grad, int n, float lr, float b1, float b2,
                                       float eps, float c1, float c2)

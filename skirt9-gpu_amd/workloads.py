@@ -7,6 +7,10 @@ mix, forced scattering with path-length bias 0.5, distant frame instruments).
   octree_spiral    spiral-perturbed disk on a density-refined octree (levels 3..10, max dust
                    fraction 1e-6, ~2.9e6 leaves), power-law dust kappa ~ lambda^-1.5 with
                    albedo 0.5 -> 0.1 over 0.1..10 um, at a chosen face-on tau_V and wavelengths
+  voronoi_clumpy   Voronoi grid of density-sampled sites in a lognormal clumpy medium: a
+                   Gaussian random field with power spectrum P(k) ~ k^-3.7 on a lattice,
+                   sigma_ln = 1, mean tau_V along z; 0.55 um, albedo 0.6, g 0.6; three
+                   200x200 frames at 0, 45 and 90 deg
 """
 
 from __future__ import annotations
@@ -43,6 +47,8 @@ class Workload:
     def apply(self, eng, npk_norm=1.0):
         if self.grid == "cartesian":
             eng.grid_cartesian(self.tables["xb"], self.tables["yb"], self.tables["zb"])
+        elif self.grid == "voronoi":
+            self.tables["mesh"].upload(eng)
         else:
             self.octree.upload(eng)
         eng.medium(self.density)
@@ -96,3 +102,45 @@ def octree_spiral(tau_v=1.0, nlambda=40, lam_range_um=(0.1, 10.0), min_level=3, 
     dens = tree.sample_density(model, samples, seed + 1, threads)
     return Workload("octree_spiral", "octree", lam, sext, ssca, g, dens, octree=tree,
                     frames=frames if frames is not None else [frame(60.0)], tau_v=tau_v, setup_s=time.time() - t0)
+
+
+def lognormal_field(n=256, slope=-3.7, sigma_ln=1.0, seed=1):
+    """exp(sigma_ln * G - sigma_ln^2 / 2) for a unit-variance Gaussian random field G with
+    power spectrum P(k) ~ k^slope on a periodic n^3 lattice (mean 1)."""
+    rng = np.random.default_rng(seed)
+    kx = np.fft.fftfreq(n)[:, None, None]
+    ky = np.fft.fftfreq(n)[None, :, None]
+    kz = np.fft.rfftfreq(n)[None, None, :]
+    k = np.sqrt(kx * kx + ky * ky + kz * kz)
+    k[0, 0, 0] = 1.0
+    amp = k ** (slope / 2.0)
+    amp[0, 0, 0] = 0.0
+    noise = np.fft.rfftn(rng.standard_normal((n, n, n)))
+    g = np.fft.irfftn(noise * amp, s=(n, n, n))
+    g = (g - g.mean()) / g.std()
+    return np.exp(sigma_ln * g - 0.5 * sigma_ln * sigma_ln)
+
+
+def voronoi_clumpy(num_sites=1_000_000, lattice=256, tau_v=5.0, sigma_ln=1.0, slope=-3.7, samples=20, seed=1,
+                   threads=0, frames=None):
+    import time
+    pkg = _pkg()
+    t0 = time.time()
+    h = 10 * PC
+    box = np.array([-h, -h, -h, h, h, h])
+    rho = lognormal_field(lattice, slope, sigma_ln, seed)
+    # sites distributed as the density: a lattice cell by weight, then uniform inside it
+    rng = np.random.default_rng(seed + 1)
+    w = rho.ravel() / rho.sum()
+    cells = rng.choice(rho.size, size=num_sites, p=w)
+    ijk = np.stack(np.unravel_index(cells, rho.shape), axis=1).astype(np.float64)
+    sites = box[:3] + (ijk + rng.random((num_sites, 3))) * (2 * h / lattice)
+    mesh = pkg.Voronoi.build(sites, box, threads)
+    mean_rho = pkg.Voronoi.sample_density(mesh, rho, samples, seed + 2, threads)
+    sext = MU_DUST * 1.0
+    # mean optical depth along z through the box: tau = sext * n0 * <rho> * 2h with <rho> = 1
+    n0 = tau_v / (sext * 2 * h)
+    return Workload("voronoi_clumpy", "voronoi", np.array([0.55e-6]), np.array([sext]), np.array([0.6 * sext]),
+                    np.array([0.6]), n0 * mean_rho, dict(mesh=mesh), tau_v=tau_v,
+                    frames=frames if frames is not None else [frame(0.0), frame(45.0), frame(90.0)],
+                    setup_s=time.time() - t0)

@@ -201,15 +201,84 @@ namespace skg
     constexpr int kMaxScatterings = 1 << 20;  // guard against non-terminating histories (NaN weights)
     constexpr int kMaxSteps = 1 << 26;        // guard against non-terminating traversals
 
-    template<int GT, int FP> struct Lane
+    // Packet state that only the event transitions read and write.  Between transitions it
+    // lives in LDS as a structure of arrays over the workgroup's lanes (conflict-free, one
+    // 8-byte or 4-byte word per lane and field), so the crossing loop keeps only the
+    // traversal state in registers.
+    enum ColdF64 : int
     {
-        // packet
+        cRx, cRy, cRz, cKx, cKy, cKz, cW, cLam, cLthr, cTaupath, cTauCached, cWp, kColdF64
+    };
+    enum ColdU32 : int
+    {
+        cHist0, cHist1, cDraw, cEll, cNscatt, cF, cPix, cFlags, kColdU32
+    };
+    constexpr int kLcThreads = 256;
+    constexpr int kColdBytes = kLcThreads * (8 * kColdF64 + 4 * kColdU32);  // 32 KB per workgroup
+
+    struct Packet
+    {
         Philox rng;
         double rx, ry, rz, kx, ky, kz;
-        double W, lam, section, Lthr;
-        float section32;
-        int ell, nscatt;
-        // traversal
+        double W, lam, Lthr, taupath, tauCached, Wp;
+        int ell, nscatt, f, pix;
+        bool scat, haveTau;
+
+        __device__ __forceinline__ void load(const double* cd, const uint32_t* cu, int t, uint64_t seed)
+        {
+            rx = cd[cRx * kLcThreads + t];
+            ry = cd[cRy * kLcThreads + t];
+            rz = cd[cRz * kLcThreads + t];
+            kx = cd[cKx * kLcThreads + t];
+            ky = cd[cKy * kLcThreads + t];
+            kz = cd[cKz * kLcThreads + t];
+            W = cd[cW * kLcThreads + t];
+            lam = cd[cLam * kLcThreads + t];
+            Lthr = cd[cLthr * kLcThreads + t];
+            taupath = cd[cTaupath * kLcThreads + t];
+            tauCached = cd[cTauCached * kLcThreads + t];
+            Wp = cd[cWp * kLcThreads + t];
+            uint64_t h = static_cast<uint64_t>(cu[cHist0 * kLcThreads + t])
+                         | (static_cast<uint64_t>(cu[cHist1 * kLcThreads + t]) << 32);
+            rng.init(seed, h);
+            rng.draw = cu[cDraw * kLcThreads + t];
+            ell = static_cast<int>(cu[cEll * kLcThreads + t]);
+            nscatt = static_cast<int>(cu[cNscatt * kLcThreads + t]);
+            f = static_cast<int>(cu[cF * kLcThreads + t]);
+            pix = static_cast<int>(cu[cPix * kLcThreads + t]);
+            uint32_t fl = cu[cFlags * kLcThreads + t];
+            scat = fl & 1u;
+            haveTau = (fl >> 1) & 1u;
+        }
+
+        __device__ __forceinline__ void store(double* cd, uint32_t* cu, int t) const
+        {
+            cd[cRx * kLcThreads + t] = rx;
+            cd[cRy * kLcThreads + t] = ry;
+            cd[cRz * kLcThreads + t] = rz;
+            cd[cKx * kLcThreads + t] = kx;
+            cd[cKy * kLcThreads + t] = ky;
+            cd[cKz * kLcThreads + t] = kz;
+            cd[cW * kLcThreads + t] = W;
+            cd[cLam * kLcThreads + t] = lam;
+            cd[cLthr * kLcThreads + t] = Lthr;
+            cd[cTaupath * kLcThreads + t] = taupath;
+            cd[cTauCached * kLcThreads + t] = tauCached;
+            cd[cWp * kLcThreads + t] = Wp;
+            cu[cHist0 * kLcThreads + t] = rng.h0;
+            cu[cHist1 * kLcThreads + t] = rng.h1;
+            cu[cDraw * kLcThreads + t] = rng.draw;
+            cu[cEll * kLcThreads + t] = static_cast<uint32_t>(ell);
+            cu[cNscatt * kLcThreads + t] = static_cast<uint32_t>(nscatt);
+            cu[cF * kLcThreads + t] = static_cast<uint32_t>(f);
+            cu[cPix * kLcThreads + t] = static_cast<uint32_t>(pix);
+            cu[cFlags * kLcThreads + t] = (scat ? 1u : 0u) | (haveTau ? 2u : 0u);
+        }
+    };
+
+    // traversal state of one lane: the generator, the running sums and the phase
+    template<int GT, int FP> struct Trav
+    {
         Gen<GT> gen;
         int phase;
         bool ended;
@@ -217,18 +286,12 @@ namespace skg
         double tau, s, tprev, sprev;  // FP64 sums
         Kahan32 tau32, s32;           // FP32 sums
         int lastm;
-        // forced propagation
-        double taupath, tint;
+        double section;
+        float section32;
+        double taumax;  // peel-off: early-out optical depth
+        double tint;    // second forced pass: interaction optical depth
         int mint;
         double sint;
-        // peel-off
-        int f;
-        bool scat;
-        bool haveTau;
-        double tauCached, Wp, taumax;
-        int pix;
-        // event counters of this lane (flushed to the workgroup's LDS counters)
-        unsigned int* lds;
 
         __device__ __forceinline__ double tau_value() const
         {
@@ -239,9 +302,9 @@ namespace skg
             return FP == SKG_FP64 ? s : static_cast<double>(s32.sum);
         }
 
-        __device__ __forceinline__ void start_traversal(double dx, double dy, double dz, int ph)
+        __device__ __forceinline__ void start(const Packet& pk, double dx, double dy, double dz, int ph)
         {
-            gen.start(rx, ry, rz, dx, dy, dz);
+            gen.start(pk.rx, pk.ry, pk.rz, dx, dy, dz);
             phase = ph;
             ended = false;
             steps = 0;
@@ -254,104 +317,12 @@ namespace skg
             lastm = -1;
         }
 
-        // MonteCarloSimulation::performLifeCycle launch + SourceSystem::launch
-        __device__ __forceinline__ void launch(const SimParams& p, uint64_t h)
-        {
-            rng.init(p.seed, h);
-            ell = static_cast<int>(rng.uniform() * p.nlambda);
-            if (ell >= p.nlambda) ell = p.nlambda - 1;
-            lam = p.lambda[ell];
-            section = p.sext[ell];
-            section32 = static_cast<float>(section);
-            W = p.lpack[ell] * lam;
-            rx = p.src[0];
-            ry = p.src[1];
-            rz = p.src[2];
-            isotropic(rng, kx, ky, kz);
-            nscatt = 0;
-            if (!(W / lam > 0))
-            {
-                finish();
-                return;
-            }
-            Lthr = (W / lam) / p.min_weight_reduction;
-            begin_peel(p, false);
-        }
-
-        __device__ __forceinline__ void finish()
-        {
-            atomicAdd(lds + 0, 1u);
-            phase = kFetch;
-            ended = false;
-        }
-
-        __device__ __forceinline__ void deposit(const SimParams& p, const FrameGeom& fg, double L, double t)
-        {
-            double v = L * exp(-t);
-            unsafeAtomicAdd(p.frame + fg.offset + pix + static_cast<int64_t>(ell) * fg.nx * fg.ny, v);
-            atomicAdd(lds + 5, 1u);
-        }
-
-        __device__ __forceinline__ void begin_peel(const SimParams& p, bool scattering)
-        {
-            scat = scattering;
-            f = -1;
-            next_peel(p);
-        }
-
-        // peel-off toward each instrument in turn (MonteCarloSimulation.cpp:698-715, :2034-2092;
-        // FluxRecorder::detect skips the optical depth for packets outside the frame, :375)
-        __device__ __forceinline__ void next_peel(const SimParams& p)
-        {
-            while (++f < p.nframes)
-            {
-                const FrameGeom& fg = p.frames[f];
-                if (!fg.same_as_prev)
-                {
-                    haveTau = false;
-                    Wp = W;
-                    if (scat)
-                    {
-                        double c = kx * fg.kobs[0] + ky * fg.kobs[1] + kz * fg.kobs[2];
-                        Wp = W * peel_phase(p.gpar[ell], c);
-                    }
-                }
-                pix = fg.pixel(rx, ry, rz);
-                if (pix < 0) continue;
-                double L = Wp / lam;
-                if (haveTau)
-                {
-                    deposit(p, fg, L, tauCached);
-                    continue;
-                }
-                if (L <= 0)
-                {
-                    haveTau = true;
-                    tauCached = INFINITY;
-                    continue;
-                }
-                taumax = log(L) + 745;
-                atomicAdd(lds + 1, 1u);
-                start_traversal(fg.kobs[0], fg.kobs[1], fg.kobs[2], kPeel);
-                return;
-            }
-            // all instruments done: scatter (after a scattering peel-off) and start a forced path
-            if (scat)
-            {
-                scatter_hg(rng, p.gpar[ell], kx, ky, kz);
-                nscatt++;
-                atomicAdd(lds + 3, 1u);
-            }
-            atomicAdd(lds + 2, 1u);
-            start_traversal(kx, ky, kz, kFwd1);
-        }
-
         // one cell crossing of the current traversal
-        __device__ __forceinline__ void step(const SimParams& p)
+        __device__ __forceinline__ void step(const GridView& grid, unsigned int* lds)
         {
             int m;
             double ds, nd;
-            if (!gen.next(p.grid, m, ds, nd) || ++steps > kMaxSteps)
+            if (!gen.next(grid, m, ds, nd) || ++steps > kMaxSteps)
             {
                 ended = true;
                 atomicAdd(lds + 4, static_cast<unsigned int>(steps));
@@ -411,97 +382,221 @@ namespace skg
                 }
             }
         }
+    };
+
+    // The event transitions of the life cycle, on a packet loaded from LDS.
+    template<int GT, int FP> struct Events
+    {
+        const SimParams& p;
+        Trav<GT, FP>& T;
+        Packet& pk;
+        unsigned int* lds;
+
+        __device__ __forceinline__ void finish()
+        {
+            atomicAdd(lds + 0, 1u);
+            T.phase = kFetch;
+            T.ended = false;
+        }
+
+        __device__ __forceinline__ void deposit(const FrameGeom& fg, double L, double t)
+        {
+            double v = L * exp(-t);
+            unsafeAtomicAdd(p.frame + fg.offset + pk.pix + static_cast<int64_t>(pk.ell) * fg.nx * fg.ny, v);
+            atomicAdd(lds + 5, 1u);
+        }
+
+        // MonteCarloSimulation::performLifeCycle launch + SourceSystem::launch
+        __device__ __forceinline__ void launch(uint64_t h)
+        {
+            pk.rng.init(p.seed, h);
+            pk.ell = static_cast<int>(pk.rng.uniform() * p.nlambda);
+            if (pk.ell >= p.nlambda) pk.ell = p.nlambda - 1;
+            pk.lam = p.lambda[pk.ell];
+            T.section = p.sext[pk.ell];
+            T.section32 = static_cast<float>(T.section);
+            pk.W = p.lpack[pk.ell] * pk.lam;
+            pk.rx = p.src[0];
+            pk.ry = p.src[1];
+            pk.rz = p.src[2];
+            isotropic(pk.rng, pk.kx, pk.ky, pk.kz);
+            pk.nscatt = 0;
+            pk.taupath = 0.;
+            pk.tauCached = 0.;
+            pk.Wp = 0.;
+            pk.pix = -1;
+            pk.haveTau = false;
+            if (!(pk.W / pk.lam > 0))
+            {
+                finish();
+                return;
+            }
+            pk.Lthr = (pk.W / pk.lam) / p.min_weight_reduction;
+            begin_peel(false);
+        }
+
+        __device__ __forceinline__ void begin_peel(bool scattering)
+        {
+            pk.scat = scattering;
+            pk.f = -1;
+            next_peel();
+        }
+
+        // peel-off toward each instrument in turn (MonteCarloSimulation.cpp:698-715, :2034-2092;
+        // FluxRecorder::detect skips the optical depth for packets outside the frame, :375)
+        __device__ __forceinline__ void next_peel()
+        {
+            while (++pk.f < p.nframes)
+            {
+                const FrameGeom& fg = p.frames[pk.f];
+                if (!fg.same_as_prev)
+                {
+                    pk.haveTau = false;
+                    pk.Wp = pk.W;
+                    if (pk.scat)
+                    {
+                        double c = pk.kx * fg.kobs[0] + pk.ky * fg.kobs[1] + pk.kz * fg.kobs[2];
+                        pk.Wp = pk.W * peel_phase(p.gpar[pk.ell], c);
+                    }
+                }
+                pk.pix = fg.pixel(pk.rx, pk.ry, pk.rz);
+                if (pk.pix < 0) continue;
+                double L = pk.Wp / pk.lam;
+                if (pk.haveTau)
+                {
+                    deposit(fg, L, pk.tauCached);
+                    continue;
+                }
+                if (L <= 0)
+                {
+                    pk.haveTau = true;
+                    pk.tauCached = INFINITY;
+                    continue;
+                }
+                T.taumax = log(L) + 745;
+                atomicAdd(lds + 1, 1u);
+                T.start(pk, fg.kobs[0], fg.kobs[1], fg.kobs[2], kPeel);
+                return;
+            }
+            // all instruments done: scatter (after a scattering peel-off) and start a forced path
+            if (pk.scat)
+            {
+                scatter_hg(pk.rng, p.gpar[pk.ell], pk.kx, pk.ky, pk.kz);
+                pk.nscatt++;
+                atomicAdd(lds + 3, 1u);
+            }
+            atomicAdd(lds + 2, 1u);
+            T.start(pk, pk.kx, pk.ky, pk.kz, kFwd1);
+        }
 
         // event handling once a traversal ended
-        __device__ __forceinline__ void transition(const SimParams& p)
+        __device__ __forceinline__ void transition()
         {
-            if (phase == kPeel)
+            if (T.phase == kPeel)
             {
-                haveTau = true;
-                tauCached = tau_value();
-                const FrameGeom& fg = p.frames[f];
-                deposit(p, fg, Wp / lam, tauCached);
-                next_peel(p);
+                pk.haveTau = true;
+                pk.tauCached = T.tau_value();
+                const FrameGeom& fg = p.frames[pk.f];
+                deposit(fg, pk.Wp / pk.lam, pk.tauCached);
+                next_peel();
             }
-            else if (phase == kFwd1)
+            else if (T.phase == kFwd1)
             {
                 // MonteCarloSimulation::simulateForcedPropagation (:1492-1544)
-                taupath = tau_value();
-                if (taupath <= 0.)
+                pk.taupath = T.tau_value();
+                if (pk.taupath <= 0.)
                 {
-                    W *= 0.;
+                    pk.W *= 0.;
                     finish();
                     return;
                 }
                 double xi = p.xi;
+                double tint;
                 if (xi == 0.)
-                    tint = expon_cutoff(rng, taupath);
+                    tint = expon_cutoff(pk.rng, pk.taupath);
                 else
                 {
-                    tint = rng.uniform() < xi ? rng.uniform() * taupath : expon_cutoff(rng, taupath);
-                    double pp = -exp(-tint) / expm1(-taupath);
-                    double q = (1.0 - xi) * pp + xi / taupath;
-                    W *= pp / q;
+                    tint = pk.rng.uniform() < xi ? pk.rng.uniform() * pk.taupath : expon_cutoff(pk.rng, pk.taupath);
+                    double pp = -exp(-tint) / expm1(-pk.taupath);
+                    double q = (1.0 - xi) * pp + xi / pk.taupath;
+                    pk.W *= pp / q;
                 }
-                mint = -1;
                 atomicAdd(lds + 6, 1u);
-                start_traversal(kx, ky, kz, kFwd2);
+                T.start(pk, pk.kx, pk.ky, pk.kz, kFwd2);
+                T.tint = tint;
+                T.mint = -1;
             }
-            else if (phase == kFwd2)
+            else if (T.phase == kFwd2)
             {
                 // SpatialGridPath::findInteractionPoint (SpatialGridPath.cpp:161-203): beyond the
                 // last segment, use the last segment's cell and exit distance
-                if (mint < 0 && !(tint < tau_value()))
+                if (T.mint < 0 && !(T.tint < T.tau_value()))
                 {
-                    mint = lastm;
-                    sint = s_value();
+                    T.mint = T.lastm;
+                    T.sint = T.s_value();
                 }
-                int m = mint;
+                int m = T.mint;
                 double n = m >= 0 ? p.dens[m] : 0.;
-                double ksca = n > 0. ? n * p.ssca[ell] : 0.;
-                double kext = n > 0. ? n * p.sext[ell] : 0.;
+                double ksca = n > 0. ? n * p.ssca[pk.ell] : 0.;
+                double kext = n > 0. ? n * p.sext[pk.ell] : 0.;
                 double albedo = kext > 0. ? ksca / kext : 0.;
-                W *= -expm1(-taupath) * albedo;
-                rx += sint * kx;
-                ry += sint * ky;
-                rz += sint * kz;
-                double L = W / lam;
-                if (L <= 0 || (L <= Lthr && nscatt >= p.min_scatt) || nscatt >= kMaxScatterings)
+                pk.W *= -expm1(-pk.taupath) * albedo;
+                pk.rx += T.sint * pk.kx;
+                pk.ry += T.sint * pk.ky;
+                pk.rz += T.sint * pk.kz;
+                double L = pk.W / pk.lam;
+                if (L <= 0 || (L <= pk.Lthr && pk.nscatt >= p.min_scatt) || pk.nscatt >= kMaxScatterings)
                 {
                     finish();
                     return;
                 }
-                begin_peel(p, true);
+                begin_peel(true);
             }
         }
     };
 
     __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
-    template<int GT, int FP> __global__ void __launch_bounds__(256, 4) lifecycle(SimParams p)
+// Minimum waves per SIMD the persistent kernel is compiled for.  Two waves give it 256 VGPRs:
+// the event transitions (FP64 transcendentals on a loaded packet) need far more than the
+// crossing loop, and at three or four waves the register spills to scratch cost more than the
+// extra latency hiding gains (measured on the octree workload: 22.9 M packets/s at two waves,
+// 22.1 M at three, 14.0 M at four).
+#ifndef SKG_LIFECYCLE_WAVES
+#define SKG_LIFECYCLE_WAVES 2
+#endif
+    template<int GT, int FP> __global__ void __launch_bounds__(kLcThreads, SKG_LIFECYCLE_WAVES) lifecycle(SimParams p)
     {
         __shared__ unsigned int wgcnt[8];
+        __shared__ double coldD[kColdF64 * kLcThreads];
+        __shared__ uint32_t coldU[kColdU32 * kLcThreads];
         if (threadIdx.x < 8) wgcnt[threadIdx.x] = 0;
         __syncthreads();
-        Lane<GT, FP> L;
-        L.phase = kFetch;
-        L.ended = false;
-        L.lds = wgcnt;
+        const int t = threadIdx.x;
+        Trav<GT, FP> T;
+        T.phase = kFetch;
+        T.ended = false;
         const int lane = lane_id();
         while (true)
         {
             // advance every busy lane by one cell crossing
-            bool busy = (L.phase == kPeel || L.phase == kFwd1 || L.phase == kFwd2) && !L.ended;
-            if (busy) L.step(p);
+            bool busy = (T.phase == kPeel || T.phase == kFwd1 || T.phase == kFwd2) && !T.ended;
+            if (busy) T.step(p.grid, wgcnt);
 
             // wave-wide transition round once enough lanes wait
-            unsigned long long waiting = __ballot(L.ended || L.phase == kFetch);
-            unsigned long long active = __ballot(L.phase != kDone);
+            unsigned long long waiting = __ballot(T.ended || T.phase == kFetch);
+            unsigned long long active = __ballot(T.phase != kDone);
             if (active == 0) break;
             if (__popcll(waiting) >= p.threshold || waiting == active)
             {
-                if (L.ended) L.transition(p);
-                unsigned long long need = __ballot(L.phase == kFetch);
+                if (T.ended)
+                {
+                    Packet pk;
+                    pk.load(coldD, coldU, t, p.seed);
+                    Events<GT, FP>{p, T, pk, wgcnt}.transition();
+                    pk.store(coldD, coldU, t);
+                }
+                unsigned long long need = __ballot(T.phase == kFetch);
                 if (need)
                 {
                     int cntNeed = __popcll(need);
@@ -509,14 +604,18 @@ namespace skg
                     unsigned long long base = 0;
                     if (lane == leader) base = atomicAdd(p.queue, static_cast<unsigned long long>(cntNeed));
                     base = __shfl(base, leader);
-                    if (L.phase == kFetch)
+                    if (T.phase == kFetch)
                     {
                         unsigned long long below = need & ((1ull << lane) - 1ull);
                         unsigned long long mine = base + __popcll(below);
                         if (mine < p.count)
-                            L.launch(p, p.first + mine);
+                        {
+                            Packet pk;
+                            Events<GT, FP>{p, T, pk, wgcnt}.launch(p.first + mine);
+                            pk.store(coldD, coldU, t);
+                        }
                         else
-                            L.phase = kDone;
+                            T.phase = kDone;
                     }
                 }
             }
@@ -575,7 +674,7 @@ struct skg_engine
     unsigned long long* queue = nullptr;
     unsigned long long* counters = nullptr;
     // launch shape
-    int workgroups = 0, threads = 256, threshold = 16;
+    int workgroups = 0, threads = 256, threshold = 32;
     float lastMs = 0.f;
     // wavefront pipeline
     int mode = SKG_MODE_PIPELINE;

@@ -275,6 +275,7 @@ namespace skg
         int i, j, k;  // Cartesian cell indices; octree, Voronoi, tetra: i = current cell
                       // (tetra: j = entering face, -1 if unknown)
         double b0, b1, b2, b3, b4, b5, bn;  // octree: bounds and density of the current cell
+        int f0, f1, f2, f3, f4, f5;         // octree: first neighbour of the current cell per wall
 
         __device__ __forceinline__ void load_leaf(const GridView& g, int c)
         {
@@ -286,6 +287,12 @@ namespace skg
             b4 = L.hi[1];
             b5 = L.hi[2];
             bn = L.dens;
+            f0 = L.first[0];
+            f1 = L.first[1];
+            f2 = L.first[2];
+            f3 = L.first[3];
+            f4 = L.first[4];
+            f5 = L.first[5];
         }
 
         __device__ __forceinline__ int leaf_cell(const GridView& g, double x, double y, double z) const
@@ -500,17 +507,43 @@ namespace skg
                 m = cur;
                 ds = d;
                 n = bn;
-                // first neighbour on the crossed wall whose closed box contains r
-                const OctLeaf& R = g.leaf[cur];
+                // first neighbour on the crossed wall whose closed box contains r.  The first
+                // (largest-overlap) neighbour's whole record is read in one go, so the common
+                // crossing costs one dependent cache-line access and no reload.
+                const int f = wall == 0 ? f0 : wall == 1 ? f1 : wall == 2 ? f2 : wall == 3 ? f3 : wall == 4 ? f4 : f5;
                 int nxt = -1;
-                int f = R.first[wall];
+                bool loaded = false;
                 if (f >= 0)
                 {
                     const OctLeaf& F = g.leaf[f];
-                    if (rx >= F.lo[0] && rx <= F.hi[0] && ry >= F.lo[1] && ry <= F.hi[1] && rz >= F.lo[2] && rz <= F.hi[2])
+                    const double l0 = F.lo[0], l1 = F.lo[1], l2 = F.lo[2], h0 = F.hi[0], h1 = F.hi[1], h2 = F.hi[2];
+                    const double dn = F.dens;
+                    const int n0 = F.first[0], n1 = F.first[1], n2 = F.first[2], n3 = F.first[3], n4 = F.first[4],
+                              n5 = F.first[5];
+                    if (rx >= l0 && rx <= h0 && ry >= l1 && ry <= h1 && rz >= l2 && rz <= h2)
+                    {
                         nxt = f;
+                        if (f != cur)
+                        {
+                            b0 = l0;
+                            b1 = l1;
+                            b2 = l2;
+                            b3 = h0;
+                            b4 = h1;
+                            b5 = h2;
+                            bn = dn;
+                            f0 = n0;
+                            f1 = n1;
+                            f2 = n2;
+                            f3 = n3;
+                            f4 = n4;
+                            f5 = n5;
+                            loaded = true;
+                        }
+                    }
                     else
                     {
+                        const OctLeaf& R = g.leaf[cur];
                         int e0 = R.off[wall] + 1;
                         int e1 = wall < 5 ? R.off[wall + 1] : R.off_end;
                         for (int q = e0; q < e1; ++q)
@@ -533,10 +566,11 @@ namespace skg
                     ry = nextafter(ry, (ky < 0.) ? -DBL_MAX : DBL_MAX);
                     rz = nextafter(rz, (kz < 0.) ? -DBL_MAX : DBL_MAX);
                     nxt = leaf_cell(g, rx, ry, rz);
+                    loaded = false;
                 }
                 if (nxt < 0 || nxt == cur)
                     state = kOutside;
-                else
+                else if (!loaded)
                     load_leaf(g, nxt);
                 i = nxt;
                 return true;

@@ -475,6 +475,8 @@ namespace skg
     }
 
     // ------------------------------------------------------------------ trace kernel
+    constexpr unsigned kTraceChunk = 256;  // request indices a wavefront reserves at once
+    constexpr int kTraceRefill = 8;        // idle lanes that trigger a refill round
     template<int GT, int FP> __global__ void __launch_bounds__(256) pipe_trace(PipeParams p, int in)
     {
         const int lane = __lane_id();
@@ -491,14 +493,34 @@ namespace skg
         double section = 0.;
         float section32 = 0.f;
         unsigned long long segs = 0;
+        // the wavefront's reserved run of request indices [cnext, cend), refilled with one
+        // atomic per kTraceChunk requests; idle lanes are refilled once kTraceRefill of them
+        // wait (or when no lane is busy), so most crossings run without a refill round
+        unsigned long long cnext = 0, cend = 0;
         while (true)
         {
-            // idle lanes take the next requests, one atomic per wavefront
             bool idle = !busy && !done;
-            unsigned long long want = __ballot(idle);
-            if (want)
+            const unsigned long long want = __ballot(idle);
+            const int nidle = __popcll(want);
+            if (want && (nidle >= kTraceRefill || __ballot(busy) == 0))
             {
-                unsigned long long base = wave_reserve(p.ctl + kCtlHead + in, idle ? 1u : 0u);
+                const unsigned long long avail = cend - cnext;
+                unsigned long long nb = 0;
+                if (static_cast<unsigned long long>(nidle) > avail)
+                {
+                    const int leader = __ffsll(static_cast<long long>(want)) - 1;
+                    if (lane == leader) nb = atomicAdd(p.ctl + kCtlHead + in, static_cast<unsigned long long>(kTraceChunk));
+                    nb = __shfl(nb, leader);
+                }
+                const unsigned idx = __popcll(want & ((1ull << lane) - 1ull));
+                const unsigned long long base = idx < avail ? cnext + idx : nb + (idx - avail);
+                if (static_cast<unsigned long long>(nidle) > avail)
+                {
+                    cnext = nb + (nidle - avail);
+                    cend = nb + kTraceChunk;
+                }
+                else
+                    cnext += nidle;
                 if (idle)
                 {
                     if (base < count && base < static_cast<unsigned long long>(p.qcap))

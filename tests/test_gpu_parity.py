@@ -89,6 +89,42 @@ def test_octree_tau(eng):
     np.testing.assert_allclose(to32, to, rtol=1e-5, atol=1e-12)
 
 
+def _random_rays(rng, n, h):
+    r = rng.uniform(-1.2 * h, 1.2 * h, (n, 3))
+    k = rng.normal(size=(n, 3))
+    # a share of axis-aligned and in-plane directions (exact ties between exit distances)
+    m = rng.random(n) < 0.1
+    k[m] = np.eye(3)[rng.integers(0, 3, m.sum())] * rng.choice([-1.0, 1.0], (m.sum(), 1))
+    m2 = (rng.random(n) < 0.1) & ~m
+    k[m2, rng.integers(0, 3, m2.sum())] = 0.0
+    k /= np.linalg.norm(k, axis=1, keepdims=True)
+    return np.hstack([r, k])
+
+
+@pytest.mark.parametrize("grid", ["cartesian", "octree"])
+def test_random_rays_match_oracle(eng, grid):
+    """1e5 random rays (inside and outside starts, 20% with axis-aligned or in-plane
+    directions): the device paths equal the oracle's bit for bit."""
+    rng = np.random.default_rng(17)
+    if grid == "octree":
+        t = _tree(eng)
+        og = O.Grid("octree", bounds=t["bounds"], child_first=child_first(t["child_begin"], t["child_count"], t["child_index"]),
+                    cell_index=t["cell_index"], nb_off=t["nb_off"], nb_id=t["nb_id"])
+        h = float(t["half"])
+    else:
+        f = _cart(eng, "c64")
+        h = float(f["c64_half"])
+        b = cart_borders(64, -h, h)
+        og = O.Grid("cartesian", xb=b, yb=b, zb=b)
+    rays = _random_rays(rng, 100000, h)
+    print("case random rays", grid, flush=True)
+    off, m, ds = eng.trace_paths(rays)
+    ooff, om, ods = O.trace(og, rays)
+    np.testing.assert_array_equal(off, ooff)
+    np.testing.assert_array_equal(m, om)
+    np.testing.assert_array_equal(ds, ods)
+
+
 def test_empty_and_outside_rays(eng):
     _cart(eng, "c8")
     print("case empty rays", flush=True)

@@ -291,7 +291,13 @@ namespace skg
                 double c = kxo * kx + kyo * ky + kzo * kz;
                 double phi = (fabs(g) < 1e-6 ? 1.0 : value_hg(g, c)) / (4 * M_PI);
                 double q = guide_pdf(Lb, kx, ky, kz);
-                W *= phi / ((1.0 - p.guideMix) * phi + p.guideMix * q);
+                const double ratio = phi / ((1.0 - p.guideMix) * phi + p.guideMix * q);
+                W *= ratio;
+                // the termination rule (MonteCarloSimulation.cpp:643-660) must see the weight
+                // without the guide's factors, or packets the guide down-weights would be cut
+                // early and their later deposits lost (a bias); scaling the threshold with the
+                // same factor applies the rule to W / (product of the factors)
+                P.lthr[slot] *= ratio;
             }
         }
         else if (doScatter)
@@ -477,7 +483,10 @@ namespace skg
     // ------------------------------------------------------------------ trace kernel
     constexpr unsigned kTraceChunk = 256;  // request indices a wavefront reserves at once
     constexpr int kTraceRefill = 8;        // idle lanes that trigger a refill round
-    template<int GT, int FP> __global__ void __launch_bounds__(256) pipe_trace(PipeParams p, int in)
+#ifndef SKG_TRACE_WAVES
+#define SKG_TRACE_WAVES 4  // minimum waves per SIMD of the trace kernel
+#endif
+    template<int GT, int FP> __global__ void __launch_bounds__(256, SKG_TRACE_WAVES) pipe_trace(PipeParams p, int in)
     {
         const int lane = __lane_id();
         const unsigned long long count = p.ctl[kCtlCount + in];

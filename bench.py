@@ -203,6 +203,8 @@ def main():
                     help="octree: the spiral disk at face-on tau_V (the measured configuration); voronoi: "
                          "1e6-site Voronoi grid in a lognormal clumpy medium, mean tau_V 5, three frames")
     ap.add_argument("--sites", type=int, default=1_000_000)
+    ap.add_argument("--nlambda", type=int, default=5, help="octree workload: number of wavelengths")
+    ap.add_argument("--lam-range", type=float, nargs=2, default=(0.3, 2.0), metavar=("MIN_UM", "MAX_UM"))
     ap.add_argument("--no-guide", action="store_true", help="skip the guided-versus-plain comparison")
     ap.add_argument("--train-packets", type=float, default=1e7)
     ap.add_argument("--record-capacity", type=float, default=2.5e8)
@@ -234,12 +236,12 @@ def main():
     pkg = load_pkg()
     from skirt9_gpu_amd import workloads
     threads = host_threads()
-    nlambda, lam_range = 5, (0.3, 2.0)
+    nlambda, lam_range = a.nlambda, tuple(a.lam_range)
     if a.workload == "octree":
         wl = workloads.octree_spiral(tau_v=a.tau, nlambda=nlambda, lam_range_um=lam_range, threads=threads)
         ncells = int(wl.octree.info()[1])
         bytes_per_segment = BYTES_PER_SEGMENT
-        wdesc = "octree spiral disk, face-on tauV=%g, 5 wavelengths 0.3-2 um" % a.tau
+        wdesc = "octree spiral disk, face-on tauV=%g, %d wavelengths %g-%g um" % (a.tau, nlambda, *lam_range)
     else:
         wl = workloads.voronoi_clumpy(num_sites=a.sites, threads=threads)
         ncells, nnb = wl.tables["mesh"].info()[:2]
@@ -322,7 +324,7 @@ def main():
     if os.path.exists(prof):
         try:
             tj = json.load(open(prof))
-            if kname != "lifecycle" or tj.get("packets_per_launch") == a.packets:
+            if (kname != "lifecycle" or tj.get("packets_per_launch") == a.packets) and tj.get("nlambda", 5) == nlambda:
                 traffic = tj.get("hbm_bytes_per_launch")
         except Exception:
             traffic = None

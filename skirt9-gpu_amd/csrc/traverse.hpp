@@ -586,9 +586,9 @@ namespace skg
                 const double px0 = C.x, py0 = C.y, pz0 = C.z;
                 double sq = DBL_MAX;
                 int mq = -99;
-                for (int q = C.begin, e = C.begin + C.count; q < e; ++q)
-                {
-                    const VorNb& E = g.vnb[q];
+                // exit distance through one neighbour entry; the smallest positive one wins,
+                // strict '<' (the first of equal distances wins)
+                auto consider = [&](const VorNb E) {
                     const int mi = E.id;
                     double si = 0;
                     if (mi >= 0)
@@ -616,7 +616,19 @@ namespace skg
                         sq = si;
                         mq = mi;
                     }
+                };
+                // entries four at a time so that their loads are in flight together
+                int q = C.begin;
+                const int e = C.begin + C.count;
+                for (; q + 4 <= e; q += 4)
+                {
+                    const VorNb E0 = g.vnb[q], E1 = g.vnb[q + 1], E2 = g.vnb[q + 2], E3 = g.vnb[q + 3];
+                    consider(E0);
+                    consider(E1);
+                    consider(E2);
+                    consider(E3);
                 }
+                for (; q < e; ++q) consider(g.vnb[q]);
                 if (mq == -99)
                 {
                     rx += kx * g.eps;

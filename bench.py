@@ -205,6 +205,8 @@ def main():
     ap.add_argument("--sites", type=int, default=1_000_000)
     ap.add_argument("--nlambda", type=int, default=5, help="octree workload: number of wavelengths")
     ap.add_argument("--lam-range", type=float, nargs=2, default=(0.3, 2.0), metavar=("MIN_UM", "MAX_UM"))
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep --mode/--threshold/--pool/... instead of timing the candidate launch configurations")
     ap.add_argument("--no-guide", action="store_true", help="skip the guided-versus-plain comparison")
     ap.add_argument("--train-packets", type=float, default=1e7)
     ap.add_argument("--record-capacity", type=float, default=2.5e8)
@@ -259,6 +261,12 @@ def main():
     eng.tune(0, 256, a.threshold)
     mode = pkg.SKG_MODE_PIPELINE if a.mode == "pipeline" else pkg.SKG_MODE_PERSISTENT
     eng.tune_pipeline(mode, a.pool, a.trace_wgs, a.graph_iters)
+    tuned = None
+    if not a.no_autotune:
+        cfg, rate = eng.autotune(1 << 40, trial_packets=1 << 20)
+        tuned = dict(cfg, trial_packets_per_s=rate)
+        mode = cfg["mode"]
+        print(f"autotune: {cfg}, {rate:.3e} packets/s in the trial", file=sys.stderr, flush=True)
     eng.frame_clear()
     first = 0
     for _ in range(a.warmup):
@@ -345,7 +353,8 @@ def main():
         "config": {"workload": wdesc + ", plain HG in the timed steps",
                    "cells": ncells, "packets_per_step": a.packets,
                    "frames": ", ".join("200x200 at %g deg" % np.degrees(f["inclination"]) for f in wl.frames),
-                   "engine": a.mode, "segments_per_packet": float(d[4] / max(histories, 1)),
+                   "engine": "pipeline" if mode == pkg.SKG_MODE_PIPELINE else "persistent",
+                   "autotuned": tuned, "segments_per_packet": float(d[4] / max(histories, 1)),
                    "bytes_per_segment": bytes_per_segment},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -73,7 +73,7 @@ EXPORTED = [
     "skg_guide_params", "skg_guide_read_params", "skg_guide_infer", "skg_guide_gradient", "skg_guide_mode",
     "skg_guide_collect", "skg_guide_records", "skg_guide_train",
     "skg_voronoi_build", "skg_voronoi_info", "skg_voronoi_export", "skg_voronoi_upload",
-    "skg_sample_density_voronoi", "skg_voronoi_free",
+    "skg_sample_density_voronoi", "skg_voronoi_free", "skg_autotune",
 ]
 
 GUIDE_PARAMS = 11560
@@ -291,6 +291,41 @@ class Engine:
         self._chk(lib().skg_guide_train(self._h, C.c_int32(steps), C.c_int32(batch), C.c_float(lr), C.byref(a),
                                         C.byref(b)))
         return a.value, b.value
+
+    def autotune(self, first, trial_packets=1 << 20, engines=3, cache=None, key=None):
+        """Picks the fastest launch configuration (engine, transition threshold, workgroups,
+        pool slots, trace workgroups, graph length) from timed trial runs of histories
+        [first, ...) and keeps it; the detector contents are restored.  With `cache` (a JSON
+        file) and `key`, a configuration stored under the key is applied without trials and
+        new results are stored.  Returns (config dict, trial packets/s or None)."""
+        import json
+        names = ("mode", "threshold", "workgroups", "pool_slots", "trace_workgroups", "graph_iterations")
+        if cache and key and os.path.exists(cache):
+            try:
+                entry = json.load(open(cache)).get(key)
+            except (OSError, ValueError):
+                entry = None
+            if entry:
+                self.tune(int(entry["workgroups"]), 256, int(entry["threshold"]))
+                self.tune_pipeline(int(entry["mode"]), int(entry["pool_slots"]), int(entry["trace_workgroups"]),
+                                   int(entry["graph_iterations"]))
+                return {k: int(entry[k]) for k in names}, None
+        best = np.zeros(6, np.int64)
+        rate = C.c_double()
+        self._chk(lib().skg_autotune(self._h, C.c_uint64(first), C.c_uint64(trial_packets), C.c_int32(engines),
+                                     _p(best, i64p), C.byref(rate)))
+        cfg = {k: int(v) for k, v in zip(names, best)}
+        if cache and key:
+            data = {}
+            if os.path.exists(cache):
+                try:
+                    data = json.load(open(cache))
+                except (OSError, ValueError):
+                    data = {}
+            data[key] = dict(cfg, rate=rate.value)
+            with open(cache, "w") as f:
+                json.dump(data, f, indent=1)
+        return cfg, rate.value
 
     def run_packets(self, first, count):
         self._chk(lib().skg_run_packets(self._h, C.c_uint64(first), C.c_uint64(count)))

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -2065,6 +2066,95 @@ int skg_run_packets(skg_engine* e, uint64_t first_history, uint64_t count)
     int st = skg_run_packets_async(e, first_history, count);
     if (st) return st;
     return skg_sync(e);
+}
+
+int skg_autotune(skg_engine* e, uint64_t first_history, uint64_t trial_packets, int32_t engines, int64_t* best6,
+                 double* best_rate)
+{
+    if (!e || trial_packets < 1024 || !(engines >= 1 && engines <= 3))
+        return fail(e, SKG_ERR_ARGUMENT, "skg_autotune: bad arguments");
+    if (e->guideMode == 2) return fail(e, SKG_ERR_STATE, "skg_autotune: not in guide collect mode");
+    // guided sampling runs in the pipeline engine only
+    if (e->guideMode == 1) engines &= 2;
+    if (!engines) return fail(e, SKG_ERR_ARGUMENT, "skg_autotune: guided sampling needs the pipeline engine");
+    int st = SKG_OK;
+    // keep the caller's detector contents and counters
+    double* saved = nullptr;
+    unsigned long long savedCnt[8];
+    {
+        std::lock_guard<std::mutex> lk(e->mu);
+        hipSetDevice(e->device);
+        if ((st = ensureFrames(e))) return st;
+        if (e->frameSize) HIPCHK(e, hipMalloc(&saved, e->frameSize * sizeof(double)));
+        if (saved) HIPCHK(e, hipMemcpy(saved, e->frameBuf, e->frameSize * sizeof(double), hipMemcpyDeviceToDevice));
+        HIPCHK(e, hipMemcpy(savedCnt, e->counters, sizeof(savedCnt), hipMemcpyDeviceToHost));
+    }
+    struct Cand
+    {
+        int mode, threshold, workgroups;
+        int64_t pool;
+        int traceWgs, iters;
+    };
+    std::vector<Cand> cands;
+    const int cus = e->cus;
+    if (engines & 1)
+        for (int thr : {16, 32, 48})
+            for (int wg : {2 * cus, 4 * cus, 8 * cus}) cands.push_back({SKG_MODE_PERSISTENT, thr, wg, e->poolSlots, e->traceBlocks, e->graphIters});
+    if (engines & 2)
+        for (int64_t pool : {int64_t(1) << 20, int64_t(1) << 21, int64_t(1) << 22})
+            for (int it : {8, 16})
+                for (int twg : {4 * cus, 8 * cus}) cands.push_back({SKG_MODE_PIPELINE, e->threshold, e->workgroups, pool, twg, it});
+    double bestRate = -1.;
+    Cand best = cands[0];
+    uint64_t h = first_history;
+    for (const Cand& c : cands)
+    {
+        e->mode = c.mode;
+        e->threshold = c.threshold;
+        e->workgroups = c.workgroups;
+        e->poolSlots = c.pool;
+        e->traceBlocks = c.traceWgs;
+        e->graphIters = c.iters;
+        // one warm-up run (pool allocation, graph capture), then the timed trial
+        if ((st = skg_run_packets(e, h, trial_packets / 4))) break;
+        h += trial_packets / 4;
+        auto t0 = std::chrono::steady_clock::now();
+        if ((st = skg_run_packets(e, h, trial_packets))) break;
+        double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        h += trial_packets;
+        double rate = static_cast<double>(trial_packets) / dt;
+        if (rate > bestRate)
+        {
+            bestRate = rate;
+            best = c;
+        }
+    }
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    if (saved)
+    {
+        hipMemcpy(e->frameBuf, saved, e->frameSize * sizeof(double), hipMemcpyDeviceToDevice);
+        hipFree(saved);
+    }
+    hipMemcpy(e->counters, savedCnt, sizeof(savedCnt), hipMemcpyHostToDevice);
+    if (st) return st;
+    e->mode = best.mode;
+    e->threshold = best.threshold;
+    e->workgroups = best.workgroups;
+    e->poolSlots = best.pool;
+    e->traceBlocks = best.traceWgs;
+    e->graphIters = best.iters;
+    if (best6)
+    {
+        best6[0] = best.mode;
+        best6[1] = best.threshold;
+        best6[2] = best.workgroups;
+        best6[3] = best.pool;
+        best6[4] = best.traceWgs;
+        best6[5] = best.iters;
+    }
+    if (best_rate) *best_rate = bestRate;
+    return SKG_OK;
 }
 
 int skg_counters(skg_engine* e, int64_t* out8)

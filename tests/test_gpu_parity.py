@@ -223,3 +223,18 @@ def test_pipeline_multiple_observers(eng):
     for i in range(3):
         img = eng.frame_read(i)[0]
         assert abs(img.sum() / ofr[i][0].sum() - 1) < 1e-6, i
+
+
+def test_autotune_keeps_detector_and_picks_valid_config(eng):
+    """The autotuner's trial runs leave the frame and the counters as they were, the chosen
+    configuration is one of the candidates, and a run after tuning still equals the oracle."""
+    img0, cnt0, _, _ = _lifecycle_case(eng, 2000, 23, with_oracle=False)
+    print("case autotune", flush=True)
+    cfg, rate = eng.autotune(10_000_000, trial_packets=20000)
+    assert rate > 0
+    assert cfg["mode"] in (0, 1) and cfg["threshold"] in (16, 32, 48)
+    np.testing.assert_array_equal(eng.frame_read(0)[0], img0)
+    np.testing.assert_array_equal(eng.counters(), cnt0)
+    eng.frame_clear()
+    eng.run_packets(0, 2000)
+    np.testing.assert_allclose(eng.frame_read(0)[0], img0, rtol=1e-12, atol=0)

@@ -36,6 +36,7 @@ namespace skg
     template<int GT> __global__ void trace_count(GridView g, const double* __restrict__ rays, int64_t n, int64_t* counts)
     {
         int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+        stage_top(g);
         if (t >= n) return;
         const double* r = rays + 6 * t;
         Gen<GT> gen;
@@ -57,6 +58,7 @@ namespace skg
                                 double* dso)
     {
         int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+        stage_top(g);
         if (t >= n) return;
         const double* r = rays + 6 * t;
         Gen<GT> gen;
@@ -103,6 +105,7 @@ namespace skg
                               double* tau_path, double* tau_obs, const int64_t* toff, double* tau_cum)
     {
         int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+        stage_top(g);
         if (t >= n) return;
         const double* r = rays + 6 * t;
         Gen<GT> gen;
@@ -572,7 +575,7 @@ namespace skg
         __shared__ double coldD[kColdF64 * kLcThreads];
         __shared__ uint32_t coldU[kColdU32 * kLcThreads];
         if (threadIdx.x < 8) wgcnt[threadIdx.x] = 0;
-        __syncthreads();
+        stage_top(p.grid);
         const int t = threadIdx.x;
         Trav<GT, FP> T;
         T.phase = kFetch;
@@ -1096,6 +1099,23 @@ int skg_grid_octree(skg_engine* e, int64_t nnodes, const double* bounds, const i
     if (!nbc.empty())
     {
         if ((st = gridUpload(e, nbc.data(), nbc.size(), &g.nbc))) return st;
+    }
+    // the nodes kernels stage in LDS for the top-down search (ids below kTopNodes; the
+    // level-by-level construction numbers the top levels first)
+    {
+        const int topN = static_cast<int>(std::min<int64_t>(nnodes, kTopNodes));
+        std::vector<TopNode> top(topN);
+        for (int n = 0; n < topN; ++n)
+        {
+            const int32_t c = child_first[n];
+            top[n].child = c;
+            top[n].pad = 0;
+            top[n].sx = c >= 0 ? bounds[6 * static_cast<int64_t>(c) + 3] : 0.;
+            top[n].sy = c >= 0 ? bounds[6 * static_cast<int64_t>(c) + 4] : 0.;
+            top[n].sz = c >= 0 ? bounds[6 * static_cast<int64_t>(c) + 5] : 0.;
+        }
+        if ((st = gridUpload(e, top.data(), top.size(), &g.top))) return st;
+        g.topN = topN;
     }
     for (int q = 0; q < 6; ++q) g.box[q] = bounds[q];
     double dx = g.box[3] - g.box[0], dy = g.box[4] - g.box[1], dz = g.box[5] - g.box[2];

@@ -488,6 +488,7 @@ namespace skg
 #endif
     template<int GT, int FP> __global__ void __launch_bounds__(256, SKG_TRACE_WAVES) pipe_trace(PipeParams p, int in)
     {
+        stage_top(p.grid);
         const int lane = __lane_id();
         const unsigned long long count = p.ctl[kCtlCount + in];
         const Request* q = p.q[in];

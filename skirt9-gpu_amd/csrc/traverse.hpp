@@ -85,6 +85,17 @@ namespace skg
     };
     static_assert(sizeof(OctLeaf) == 128, "leaf record must be one 128-byte line");
 
+    // An octree node near the root as staged in LDS: the split point (upper corner of the first
+    // child, TreeNode::leafChild's test point) and the first child id (-1 for a leaf).
+    struct TopNode
+    {
+        double sx, sy, sz;
+        int32_t child;
+        int32_t pad;
+    };
+    static_assert(sizeof(TopNode) == 32, "top node must be 32 bytes");
+    constexpr int kTopNodes = 1024;  // 32 KB of LDS: levels 0-3 of a full octree and part of level 4
+
     // Read-only grid tables resident in HBM.  Octree: node-indexed bounds, first child
     // (children are 8 consecutive ids) and cell index for the top-down search, plus the
     // cell-indexed leaf records and their neighbour lists (cell ids, sorted by decreasing
@@ -103,6 +114,8 @@ namespace skg
         const int32_t* cell_of;
         const OctLeaf* leaf;
         const int32_t* nbc;
+        const TopNode* top;  // the nodes with id < topN in TopNode form (global copy)
+        int topN;            // 0 disables the LDS stage (non-octree grids)
         // Voronoi: cell records, neighbour entries and the block search lists (CSR over vbn^3
         // blocks of cell ids, in the reference's order)
         const VorCell* vcell;
@@ -253,17 +266,46 @@ namespace skg
         return x >= b[0] && x <= b[3] && y >= b[1] && y <= b[4] && z >= b[2] && z <= b[5];
     }
 
+    // The top of the octree staged in LDS for the top-down search every path start makes:
+    // for node ids below GridView::topN, the split point (upper corner of the first child) and
+    // the first child id.  Every kernel that walks an octree fills it with stage_top() first;
+    // with topN = 0 the search reads only global memory.
+    __shared__ TopNode skg_top_lds[kTopNodes];
+
+    __device__ __forceinline__ void stage_top(const GridView& g)
+    {
+        for (int t = threadIdx.x; t < g.topN; t += blockDim.x) skg_top_lds[t] = g.top[t];
+        __syncthreads();
+    }
+
     // top-down search for the leaf containing r; -1 outside the root (closed box)
     __device__ __forceinline__ int leaf_child(const GridView& g, double x, double y, double z)
     {
         if (!node_contains(g.bounds, x, y, z)) return -1;
         int node = 0;
-        int c0;
-        while ((c0 = g.child_first[node]) >= 0)
+        while (true)
         {
-            const double* b = g.bounds + 6 * static_cast<int64_t>(c0);
-            int l = (x < b[3] ? 0 : 1) + (y < b[4] ? 0 : 2) + (z < b[5] ? 0 : 4);
-            node = c0 + l;
+            int c0;
+            double sx, sy, sz;
+            if (node < g.topN)
+            {
+                const TopNode& t = skg_top_lds[node];
+                c0 = t.child;
+                sx = t.sx;
+                sy = t.sy;
+                sz = t.sz;
+            }
+            else
+            {
+                c0 = g.child_first[node];
+                if (c0 < 0) break;
+                const double* b = g.bounds + 6 * static_cast<int64_t>(c0);
+                sx = b[3];
+                sy = b[4];
+                sz = b[5];
+            }
+            if (c0 < 0) break;
+            node = c0 + (x < sx ? 0 : 1) + (y < sy ? 0 : 2) + (z < sz ? 0 : 4);
         }
         return node;
     }

@@ -673,9 +673,36 @@ namespace skg
         if (lane == 0 && segs) atomicAdd(p.ctl + kCtlCnt + 4, segs);
     }
 
+    // Adds v at base[key] for every lane with key >= 0, called by all 64 lanes of the wave.
+    // The lanes whose key equals the first such lane's combine into one atomic (a wave of
+    // emission peel-offs from a point source all land in the source's pixel); the others add
+    // on their own.
+    __device__ __forceinline__ void wave_add(double* base, int64_t key, double v, unsigned long long& atomics)
+    {
+        const unsigned long long has = __ballot(key >= 0);
+        if (!has) return;
+        const int leader = __ffsll(static_cast<long long>(has)) - 1;
+        const int64_t lk = __shfl(key, leader);
+        const bool same = key == lk;
+        double sum = same ? v : 0.;
+        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+        if (__lane_id() == leader)
+        {
+            unsafeAtomicAdd(base + lk, sum);
+            ++atomics;
+        }
+        else if (key >= 0 && !same)
+        {
+            unsafeAtomicAdd(base + key, v);
+            ++atomics;
+        }
+    }
+
     // ------------------------------------------------------------------ detect kernel
     // FluxRecorder::detect (FluxRecorder.cpp:372-418) for the peel-off requests of one trace
     // launch: L exp(-tau) into every instrument of the observer group that sees the packet.
+    // The loop runs with whole wavefronts so that the first instrument's deposits can be
+    // combined across the wave before the atomics.
     __global__ void __launch_bounds__(256) pipe_detect(PipeParams p, int in)
     {
         const unsigned long long count = p.ctl[kCtlCount + in];
@@ -683,26 +710,46 @@ namespace skg
         unsigned long long atomics = 0;
         const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
         const unsigned long long n = count < static_cast<unsigned long long>(p.qcap) ? count : p.qcap;
-        for (unsigned long long i = blockIdx.x * static_cast<unsigned long long>(blockDim.x) + threadIdx.x; i < n;
-             i += stride)
+        for (unsigned long long i0 = blockIdx.x * static_cast<unsigned long long>(blockDim.x); i0 < n; i0 += stride)
         {
-            const Request& r = p.q[in][i];
-            if (r.kind != kReqPeel) continue;
-            double v = r.L * exp(-p.qtau[in][i]);
-            int slot = r.slot;
-            double x = P.rx[slot], y = P.ry[slot], z = P.rz[slot];
-            int ell = P.ell[slot];
-            double dep = 0.;
-            for (int f = r.frame; f < p.nframes && (f == r.frame || p.frames[f].same_as_prev); ++f)
+            const unsigned long long i = i0 + threadIdx.x;
+            bool valid = i < n;
+            int kind = valid ? p.q[in][i].kind : -1;
+            valid = valid && kind == kReqPeel;
+            int64_t key = -1;
+            double v = 0.;
+            int slot = 0, f0 = 0, ell = 0;
+            double x = 0., y = 0., z = 0.;
+            if (valid)
             {
-                const FrameGeom& fg = p.frames[f];
+                const Request& r = p.q[in][i];
+                v = r.L * exp(-p.qtau[in][i]);
+                slot = r.slot;
+                f0 = r.frame;
+                x = P.rx[slot];
+                y = P.ry[slot];
+                z = P.rz[slot];
+                ell = P.ell[slot];
+                const FrameGeom& fg = p.frames[f0];
                 int pix = fg.pixel(x, y, z);
-                if (pix < 0) continue;
-                unsafeAtomicAdd(p.frame + fg.offset + pix + static_cast<int64_t>(ell) * fg.nx * fg.ny, v);
-                dep += v;
-                ++atomics;
+                if (pix >= 0) key = fg.offset + pix + static_cast<int64_t>(ell) * fg.nx * fg.ny;
             }
-            if (p.guideMode == 2 && dep > 0.) atomicAdd(p.contrib + slot, dep);
+            wave_add(p.frame, key, v, atomics);
+            if (valid)
+            {
+                double dep = key >= 0 ? v : 0.;
+                // further instruments of the same observer (same direction, shared optical depth)
+                for (int f = f0 + 1; f < p.nframes && p.frames[f].same_as_prev; ++f)
+                {
+                    const FrameGeom& fg = p.frames[f];
+                    int pix = fg.pixel(x, y, z);
+                    if (pix < 0) continue;
+                    unsafeAtomicAdd(p.frame + fg.offset + pix + static_cast<int64_t>(ell) * fg.nx * fg.ny, v);
+                    dep += v;
+                    ++atomics;
+                }
+                if (p.guideMode == 2 && dep > 0.) atomicAdd(p.contrib + slot, dep);
+            }
         }
         for (int o = 32; o > 0; o >>= 1) atomics += __shfl_xor(atomics, o);
         if (__lane_id() == 0 && atomics) atomicAdd(p.ctl + kCtlCnt + 5, atomics);

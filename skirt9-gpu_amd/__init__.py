@@ -420,9 +420,14 @@ class DiskModel(C.Structure):
 PC = 3.08567758e16  # parsec (m)
 
 
-def disk_model(tau_v=1.0, sext_v=1.5e-29, spiral=False, hR=3 * PC, hz=0.3 * PC):
-    """The workloads' disk: h_R = 3 pc, h_z = 0.3 pc; spiral: 2 arms, pitch 15 deg, R0 = 3 pc, weight 0.5."""
-    return DiskModel(hR, hz, 2 if spiral else 0, 1, np.radians(15.0), 3 * PC, 0.0, 0.5, tau_v, sext_v)
+def disk_model(tau_v=1.0, sext_v=1.5e-29, spiral=False, hR=3 * PC, hz=0.3 * PC, scale_length=None, scale_height=None,
+               arms=2, pitch=np.radians(15.0), radius_zero=3 * PC, phase_zero=0.0, weight=0.5, index=1):
+    """Exponential disk normalised to face-on tau_V (the workloads' h_R = 3 pc, h_z = 0.3 pc), with
+    an optional logarithmic spiral perturbation (default 2 arms, pitch 15 deg, R0 = 3 pc,
+    weight 0.5, index 1)."""
+    hR = scale_length if scale_length is not None else hR
+    hz = scale_height if scale_height is not None else hz
+    return DiskModel(hR, hz, arms if spiral else 0, index, pitch, radius_zero, phase_zero, weight, tau_v, sext_v)
 
 
 def disk_density(model, x, y, z):

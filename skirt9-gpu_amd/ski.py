@@ -1,0 +1,351 @@
+"""Ski-file front end and FITS output for the engine.
+
+`load(path)` reads the subset of SKIRT's ski configuration (SMILE XML) that the engine runs:
+an oligochromatic `MonteCarloSimulation` with one `PointSource` (`BlackBodySED` with a
+`SpecificLuminosityNormalization`), one `GeometricMedium` (`ExpDiskGeometry`, optionally
+inside a `SpiralStructureGeometryDecorator`) with a `MeanListDustMix` and an
+`OpticalDepthMaterialNormalization`, a `CartesianSpatialGrid` (`LinMesh` axes) or a
+`PolicyTreeSpatialGrid` with a `DensityTreePolicy`, `PhotonPacketOptions` and
+`FrameInstrument`s.  Anything else raises `UnsupportedSki`, so a caller can fall back to
+the reference.  `run(sim, outdir, prefix)` sets the engine up, runs the photon packets and
+writes `<prefix>_<instrument>_total.fits` as `FrameInstrument`/`FluxRecorder` do
+(`FluxRecorder.cpp:876-960`): surface brightness in MJy/sr (ExtragalacticUnits with
+`fluxOutputStyle="Frequency"`), float32, one plane per wavelength.
+
+Calibration (FluxRecorder::calibrateAndWrite): value = recorded luminosity
+/ (4 pi d^2) / omega / (bin width) * lambda^2 / c * 1e20, with omega = 4 atan(dx / 2d)
+atan(dy / 2d) per pixel and the oligochromatic bin width 2 * 1e-3 * lambda_min
+(OligoWavelengthGrid.cpp:20-25).  The recorded luminosity of a bin is its share of the
+source luminosity, L_lambda(lambda) * (bin width), with L_lambda the black body scaled to
+the normalization's specific luminosity (BlackBodySED, SpecificLuminosityNormalization).
+"""
+
+from __future__ import annotations
+
+import math
+import os
+import time
+import xml.etree.ElementTree as ET
+from dataclasses import dataclass, field
+from typing import List
+
+import numpy as np
+
+# SKIRT's constants (Constants.hpp) and the units the supported items use (SkirtUnitDef.cpp)
+C_LIGHT = 2.99792458e8
+H_PLANCK = 6.62606957e-34
+K_BOLTZ = 1.3806488e-23
+PC = 3.08567758e16
+LSUN = 3.839e26
+AU = 1.495978707e11
+
+_UNITS = {
+    "m": 1.0, "cm": 1e-2, "mm": 1e-3, "km": 1e3, "AU": AU, "pc": PC, "kpc": 1e3 * PC, "Mpc": 1e6 * PC,
+    "micron": 1e-6, "nm": 1e-9, "Angstrom": 1e-10,
+    "deg": math.pi / 180.0, "rad": 1.0, "arcsec": math.pi / 180.0 / 3600.0,
+    "K": 1.0, "km/s": 1e3, "m/s": 1.0, "uG": 1e-10, "nT": 1e-9,
+    "Lsun/micron": LSUN * 1e6, "W/m": 1.0, "W/micron": 1e6, "erg/s/Angstrom": 1e-7 * 1e10,
+    "m2/kg": 1.0, "cm2/g": 0.1, "pc2/Msun": PC * PC / 1.9891e30,
+}
+
+
+class UnsupportedSki(ValueError):
+    """The ski file asks for something the engine does not run."""
+
+
+def quantity(text: str) -> float:
+    """'10 pc' -> metres, '60 deg' -> radians, '1' -> 1.0 (SI values, as SKIRT stores them)."""
+    parts = text.strip().split()
+    if len(parts) == 1:
+        return float(parts[0])
+    if len(parts) != 2 or parts[1] not in _UNITS:
+        raise UnsupportedSki(f"unsupported quantity {text!r}")
+    return float(parts[0]) * _UNITS[parts[1]]
+
+
+def quantities(text: str) -> List[float]:
+    return [quantity(t) for t in text.split(",")]
+
+
+@dataclass
+class Frame:
+    name: str
+    distance: float
+    inclination: float
+    azimuth: float
+    roll: float
+    fovx: float
+    fovy: float
+    nx: int
+    ny: int
+    cx: float
+    cy: float
+
+
+@dataclass
+class Simulation:
+    num_packets: int
+    seed: int
+    wavelengths: np.ndarray          # m, sorted
+    source_position: tuple
+    temperature: float               # black body temperature (K)
+    norm_wavelength: float           # m
+    norm_specific_luminosity: float  # W/m
+    spiral: bool
+    scale_length: float
+    scale_height: float
+    spiral_params: dict
+    dust_table: np.ndarray           # (n, 4): lambda (m), kappa_ext (m2/kg), albedo, g
+    tau_v: float
+    tau_wavelength: float
+    grid: dict
+    box: tuple
+    frames: List[Frame] = field(default_factory=list)
+    force_scattering: bool = True
+    min_weight_reduction: float = 1e4
+    min_scatt_events: int = 0
+    path_length_bias: float = 0.5
+    num_density_samples: int = 100
+
+
+def _one(root, tag):
+    found = root.findall(".//" + tag)
+    if len(found) != 1:
+        raise UnsupportedSki(f"expected exactly one {tag}, found {len(found)}")
+    return found[0]
+
+
+def load(path: str) -> Simulation:
+    root = ET.parse(path).getroot()
+    sim = _one(root, "MonteCarloSimulation")
+    if sim.get("simulationMode") != "OligoExtinctionOnly":
+        raise UnsupportedSki("only simulationMode OligoExtinctionOnly is supported")
+    for tag in ("GeometricMedium", "PointSource", "FrameInstrument"):
+        if not root.findall(".//" + tag):
+            raise UnsupportedSki(f"no {tag}")
+    media = root.findall(".//media/*")
+    sources = root.findall(".//sources/*")
+    if len(media) != 1 or len(sources) != 1:
+        raise UnsupportedSki("exactly one medium and one source are supported")
+    instruments = root.findall(".//instruments/*")
+    if any(i.tag != "FrameInstrument" for i in instruments):
+        raise UnsupportedSki("only FrameInstrument instruments are supported")
+    seed = int(_one(root, "Random").get("seed", "0"))
+    ss = _one(root, "SourceSystem")
+    lam = np.array(sorted(quantities(ss.get("wavelengths"))))
+    ps = _one(root, "PointSource")
+    pos = tuple(quantity(ps.get(k, "0 pc")) for k in ("positionX", "positionY", "positionZ"))
+    bb = _one(root, "BlackBodySED")
+    norm = _one(root, "SpecificLuminosityNormalization")
+    if norm.get("unitStyle", "wavelengthmonluminosity") != "wavelengthmonluminosity":
+        raise UnsupportedSki("only wavelengthmonluminosity normalizations are supported")
+    opts = _one(root, "PhotonPacketOptions")
+    disk = _one(root, "ExpDiskGeometry")
+    for k in ("minRadius", "maxRadius", "maxZ"):
+        if quantity(disk.get(k, "0 pc")) != 0:
+            raise UnsupportedSki("truncated exponential disks are not supported")
+    spiral_el = root.findall(".//SpiralStructureGeometryDecorator")
+    spiral = dict()
+    if spiral_el:
+        s = spiral_el[0]
+        spiral = dict(arms=int(s.get("numArms")), pitch=quantity(s.get("pitchAngle")),
+                      radius_zero=quantity(s.get("radiusZeroPoint")), phase_zero=quantity(s.get("phaseZeroPoint")),
+                      weight=float(s.get("perturbationWeight")), index=int(s.get("index")))
+    mix = _one(root, "MeanListDustMix")
+    tab = np.column_stack([quantities(mix.get("wavelengths")), quantities(mix.get("extinctionCoefficients")),
+                           [float(x) for x in mix.get("albedos").split(",")],
+                           [float(x) for x in mix.get("asymmetryParameters").split(",")]])
+    mn = _one(root, "OpticalDepthMaterialNormalization")
+    if mn.get("axis", "Z") != "Z":
+        raise UnsupportedSki("only optical depth normalizations along Z are supported")
+    samp = root.findall(".//SamplingOptions")
+    nds = int(samp[0].get("numDensitySamples", "100")) if samp else 100
+    grid_el = root.find(".//grid/*")
+    if grid_el is None:
+        raise UnsupportedSki("no spatial grid")
+    box = tuple(quantity(grid_el.get(k)) for k in ("minX", "minY", "minZ", "maxX", "maxY", "maxZ"))
+    if grid_el.tag == "CartesianSpatialGrid":
+        ns = []
+        for ax in ("meshX", "meshY", "meshZ"):
+            m = grid_el.find(f"{ax}/LinMesh")
+            if m is None:
+                raise UnsupportedSki("only LinMesh axes are supported")
+            ns.append(int(m.get("numBins")))
+        grid = dict(kind="cartesian", n=ns)
+    elif grid_el.tag == "PolicyTreeSpatialGrid":
+        if grid_el.get("treeType", "OctTree") != "OctTree":
+            raise UnsupportedSki("only octrees are supported")
+        pol = grid_el.find(".//DensityTreePolicy")
+        if pol is None:
+            raise UnsupportedSki("only DensityTreePolicy is supported")
+        if float(pol.get("maxDustOpticalDepth", "0")) or float(pol.get("maxDustDensityDispersion", "0")):
+            raise UnsupportedSki("only maxDustFraction refinement is supported")
+        grid = dict(kind="octree", min_level=int(pol.get("minLevel")), max_level=int(pol.get("maxLevel")),
+                    max_fraction=float(pol.get("maxDustFraction")))
+    else:
+        raise UnsupportedSki(f"grid {grid_el.tag} is not supported by the ski front end")
+    frames = []
+    for fi in instruments:
+        frames.append(Frame(fi.get("instrumentName"), quantity(fi.get("distance")), quantity(fi.get("inclination")),
+                            quantity(fi.get("azimuth")), quantity(fi.get("roll")),
+                            quantity(fi.get("fieldOfViewX")), quantity(fi.get("fieldOfViewY")),
+                            int(fi.get("numPixelsX")), int(fi.get("numPixelsY")),
+                            quantity(fi.get("centerX")), quantity(fi.get("centerY"))))
+    return Simulation(
+        num_packets=int(float(sim.get("numPackets"))), seed=seed, wavelengths=lam, source_position=pos,
+        temperature=quantity(bb.get("temperature")), norm_wavelength=quantity(norm.get("wavelength")),
+        norm_specific_luminosity=quantity(norm.get("specificLuminosity")), spiral=bool(spiral_el),
+        scale_length=quantity(disk.get("scaleLength")), scale_height=quantity(disk.get("scaleHeight")),
+        spiral_params=spiral, dust_table=tab, tau_v=float(mn.get("opticalDepth")),
+        tau_wavelength=quantity(mn.get("wavelength")), grid=grid, box=box, frames=frames,
+        force_scattering=opts.get("forceScattering", "true") == "true",
+        min_weight_reduction=float(opts.get("minWeightReduction", "1e4")),
+        min_scatt_events=int(opts.get("minScattEvents", "0")),
+        path_length_bias=float(opts.get("pathLengthBias", "0.5")), num_density_samples=nds)
+
+
+def planck(lam, T):
+    """Black-body specific intensity shape B_lambda(T) (PlanckFunction)."""
+    x = H_PLANCK * C_LIGHT / (lam * K_BOLTZ * T)
+    return 2 * H_PLANCK * C_LIGHT ** 2 / lam ** 5 / np.expm1(x)
+
+
+def bin_widths(lam):
+    """Oligochromatic bins: constant half width 1e-3 * lambda_min (OligoWavelengthGrid.cpp:24)."""
+    delta = lam[0] * 1e-3
+    return (lam + delta) - (lam - delta)
+
+
+def bin_luminosities(sim: Simulation):
+    """Source luminosity (W) per wavelength bin."""
+    lam = sim.wavelengths
+    Llam = sim.norm_specific_luminosity * planck(lam, sim.temperature) / planck(sim.norm_wavelength, sim.temperature)
+    return Llam * bin_widths(lam)
+
+
+def calibration(sim: Simulation, frame: Frame):
+    """Recorded luminosity -> MJy/sr, per wavelength bin (FluxRecorder::calibrateAndWrite)."""
+    lam = sim.wavelengths
+    d = frame.distance
+    dx, dy = frame.fovx / frame.nx, frame.fovy / frame.ny
+    omega = 4.0 * math.atan(0.5 * dx / d) * math.atan(0.5 * dy / d)
+    return 1.0 / (4.0 * math.pi * d * d) / omega / bin_widths(lam) * (lam * lam / C_LIGHT) * 1e20
+
+
+def write_fits(path, cube, frame: Frame, lam):
+    """Float32 FITS cube (x fastest, then y, then wavelength) with the reference's main cards."""
+    cube = np.asarray(cube, dtype=np.float64)
+    nz, ny, nx = cube.shape
+    cards = [("SIMPLE", "T"), ("BITPIX", -32), ("NAXIS", 3), ("NAXIS1", nx), ("NAXIS2", ny), ("NAXIS3", nz),
+             ("BSCALE", 1.0), ("BZERO", 0.0), ("ORIGIN", "'skirt9-gpu_amd'"), ("BUNIT", "'MJy/sr'"),
+             ("CRPIX1", (nx + 1) / 2.0), ("CRVAL1", frame.cx / PC), ("CDELT1", frame.fovx / nx / PC),
+             ("CUNIT1", "'pc'"), ("CRPIX2", (ny + 1) / 2.0), ("CRVAL2", frame.cy / PC),
+             ("CDELT2", frame.fovy / ny / PC), ("CUNIT2", "'pc'"), ("CUNIT3", "'micron'"),
+             ("CROTA1", math.degrees(frame.inclination)), ("CROTA2", math.degrees(frame.azimuth)),
+             ("CROTA3", math.degrees(frame.roll)), ("DISTLUMI", frame.distance / (1e6 * PC)),
+             ("DISTUNIT", "'Mpc'")]
+    head = ""
+    for k, v in cards:
+        if isinstance(v, bool) or v == "T":
+            s = "T"
+        elif isinstance(v, str):
+            s = v
+        elif isinstance(v, int):
+            s = str(v)
+        else:
+            s = repr(float(v)).upper()
+        head += f"{k:<8}= {s:>20}".ljust(80)
+    head += "END".ljust(80)
+    head += " " * ((-len(head)) % 2880)
+    data = cube.astype(">f4").tobytes()
+    data += b"\0" * ((-len(data)) % 2880)
+    with open(path, "wb") as f:
+        f.write(head.encode("ascii"))
+        f.write(data)
+
+
+def read_fits(path):
+    """Reads the primary image of a float32 FITS file as (nz, ny, nx) or (ny, nx) float64."""
+    raw = open(path, "rb").read()
+    hdr, pos = {}, 0
+    while True:
+        card = raw[pos:pos + 80].decode("ascii")
+        pos += 80
+        if card[:8].strip() == "END":
+            break
+        if card[8:10] == "= ":
+            hdr[card[:8].strip()] = card[10:].split("/")[0].strip()
+    pos = (pos + 2879) // 2880 * 2880
+    dims = [int(hdr[f"NAXIS{i + 1}"]) for i in range(int(hdr["NAXIS"]))]
+    n = int(np.prod(dims))
+    return np.frombuffer(raw, dtype=">f4", count=n, offset=pos).astype(np.float64).reshape(dims[::-1])
+
+
+def setup(sim: Simulation, engine, threads=0, npk_norm=None):
+    """Builds the grid and medium on the host, uploads them and configures the engine."""
+    from . import workloads as W
+    pkg = W._pkg()
+    if not sim.force_scattering:
+        raise UnsupportedSki("the engine runs the forced-scattering life cycle only")
+    if sim.source_position != (0.0, 0.0, 0.0) or any(abs(b) != abs(sim.box[0]) for b in sim.box):
+        raise UnsupportedSki("a point source at the centre of a cubic domain centred on the origin is supported")
+    mu = W.MU_DUST
+    tab = sim.dust_table
+    sext, ssca, g = pkg.dust_meanlist(tab[:, 0], tab[:, 1], tab[:, 2], tab[:, 3], sim.wavelengths, mu)
+    sv, _, _ = pkg.dust_meanlist(tab[:, 0], tab[:, 1], tab[:, 2], tab[:, 3], [sim.tau_wavelength], mu)
+    sp = sim.spiral_params
+    model = pkg.disk_model(tau_v=sim.tau_v, sext_v=float(sv[0]), spiral=sim.spiral, scale_length=sim.scale_length,
+                           scale_height=sim.scale_height, **({} if not sp else dict(
+                               arms=sp["arms"], pitch=sp["pitch"], radius_zero=sp["radius_zero"],
+                               phase_zero=sp["phase_zero"], weight=sp["weight"], index=sp["index"])))
+    h = sim.box[3]
+    if sim.grid["kind"] == "cartesian":
+        borders = []
+        for n in sim.grid["n"]:
+            t = np.array([0.0 + i * ((1.0 - 0.0) / n) for i in range(n + 1)])
+            borders.append(t * (h - (-h)) + (-h))
+        engine.grid_cartesian(*borders)
+        dens = pkg.sample_density_cartesian(*borders, model, sim.num_density_samples, sim.seed + 1, threads)
+    else:
+        gr = sim.grid
+        tree = pkg.Octree.build(list(sim.box), model, gr["min_level"], gr["max_level"], gr["max_fraction"],
+                                sim.num_density_samples, sim.seed + 1, threads)
+        tree.upload(engine)
+        dens = tree.sample_density(model, sim.num_density_samples, sim.seed + 2, threads)
+    engine.medium(dens)
+    engine.dust(sim.wavelengths, sext, ssca, g)
+    engine.frame_remove_all()
+    for f in sim.frames:
+        engine.frame_add(f.inclination, f.azimuth, f.roll, f.nx, f.ny, f.fovx, f.fovy, f.cx, f.cy)
+    # the bin is chosen uniformly, so each packet of bin ell carries nlambda * L_ell / N
+    npk = npk_norm or sim.num_packets
+    engine.source_point(sim.source_position, bin_luminosities(sim) * len(sim.wavelengths) / npk)
+    engine.options(sim.path_length_bias, sim.min_weight_reduction, sim.min_scatt_events, 0, sim.seed)
+
+
+def run(sim: Simulation, outdir: str, prefix: str, engine=None, chunk=1 << 22, threads=0, log=print):
+    """Runs the simulation on the GPU and writes one FITS cube per instrument; returns the
+    calibrated cubes {instrument name: (nlambda, ny, nx)}."""
+    from . import Engine
+    own = engine is None
+    eng = engine or Engine(0)
+    t0 = time.time()
+    setup(sim, eng, threads)
+    log(f"set-up {time.time() - t0:.1f} s")
+    eng.frame_clear()
+    t1 = time.time()
+    done = 0
+    while done < sim.num_packets:
+        n = min(chunk, sim.num_packets - done)
+        eng.run_packets(done, n)
+        done += n
+    log(f"{sim.num_packets} photon packets in {time.time() - t1:.2f} s")
+    os.makedirs(outdir, exist_ok=True)
+    out = {}
+    for i, f in enumerate(sim.frames):
+        cube = eng.frame_read(i) * calibration(sim, f)[:, None, None]
+        write_fits(os.path.join(outdir, f"{prefix}_{f.name}_total.fits"), cube, f, sim.wavelengths)
+        out[f.name] = cube
+    if own:
+        eng.close()
+    return out

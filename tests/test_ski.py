@@ -1,0 +1,87 @@
+"""The ski-file front end and FITS writer (CPU): parsing of the reference's configuration
+items, the absolute calibration against the reference's own FITS output, and the FITS
+round trip."""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import GOLDEN, PC, ROOT, golden, load_pkg
+
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import skigen  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def ski():
+    load_pkg()
+    from skirt9_gpu_amd import ski as S
+    return S
+
+
+def _write(tmp_path, name, spec):
+    p = os.path.join(tmp_path, name + ".ski")
+    with open(p, "w") as f:
+        f.write(skigen.render(spec))
+    return p
+
+
+def test_parse_config0(ski, tmp_path):
+    sim = ski.load(_write(str(tmp_path), "c0", skigen.config0(num_packets=1e6, seed=3)))
+    assert sim.num_packets == 1000000 and sim.seed == 3
+    np.testing.assert_allclose(sim.wavelengths, [0.55e-6])
+    assert sim.grid == dict(kind="cartesian", n=[64, 64, 64])
+    assert sim.box == (-10 * PC, -10 * PC, -10 * PC, 10 * PC, 10 * PC, 10 * PC)
+    assert sim.tau_v == 1.0 and not sim.spiral
+    assert abs(sim.scale_length - 3 * PC) < 1e-6 * PC and abs(sim.scale_height - 0.3 * PC) < 1e-6 * PC
+    (f,) = sim.frames
+    assert f.name == "i60" and f.nx == f.ny == 200
+    assert abs(f.inclination - np.radians(60)) < 1e-15 and abs(f.distance / (1e7 * PC) - 1) < 1e-15
+    assert sim.min_weight_reduction == 1e4 and sim.path_length_bias == 0.5 and sim.force_scattering
+
+
+def test_parse_config1(ski, tmp_path):
+    sim = ski.load(_write(str(tmp_path), "c1", skigen.config1(num_packets=1e8, tau_v=50, nlambda=5, lam_range=(0.3, 2.0))))
+    assert len(sim.wavelengths) == 5 and sim.spiral
+    assert sim.grid == dict(kind="octree", min_level=3, max_level=10, max_fraction=1e-6)
+    assert sim.spiral_params["arms"] == 2 and abs(sim.spiral_params["pitch"] - np.radians(15)) < 1e-15
+
+
+def test_calibration_matches_reference_fits(ski, tmp_path):
+    """A near-transparent reference run puts the whole source luminosity into the frame: its
+    FITS total equals our bin luminosity times our calibration factor."""
+    sim = ski.load(_write(str(tmp_path), "c0", skigen.config0()))
+    total = ski.bin_luminosities(sim)[0] * ski.calibration(sim, sim.frames[0])[0]
+    ref = float(golden("image_cfg0.npz")["calib_total"])
+    assert abs(total / ref - 1) < 1e-6
+
+
+def test_multiwavelength_bins_follow_black_body(ski, tmp_path):
+    """Bin luminosities: constant bin width 2e-3 lambda_min and the 5800 K black body shape,
+    equal to the normalisation at the normalisation wavelength."""
+    sim = ski.load(_write(str(tmp_path), "c1", skigen.config1(nlambda=5, lam_range=(0.3, 2.0))))
+    w = ski.bin_widths(sim.wavelengths)
+    np.testing.assert_allclose(w, 2e-3 * sim.wavelengths[0], rtol=1e-9)
+    L = ski.bin_luminosities(sim)
+    lam = sim.wavelengths
+    h, c, k, T = 6.62606957e-34, 2.99792458e8, 1.3806488e-23, 5800.0
+    B = lambda l: 1 / l ** 5 / np.expm1(h * c / (l * k * T))  # noqa: E731
+    np.testing.assert_allclose(L / w, 3.839e26 * 1e6 * B(lam) / B(0.55e-6), rtol=1e-12)
+
+
+def test_fits_round_trip(ski, tmp_path):
+    sim = ski.load(_write(str(tmp_path), "c0", skigen.config0()))
+    cube = np.random.default_rng(1).random((3, 4, 5))
+    p = os.path.join(str(tmp_path), "x.fits")
+    ski.write_fits(p, cube, sim.frames[0], np.array([1e-6, 2e-6, 3e-6]))
+    back = ski.read_fits(p)
+    np.testing.assert_array_equal(back, cube.astype(np.float32))
+    assert os.path.getsize(p) % 2880 == 0
+
+
+def test_unsupported_items_raise(ski, tmp_path):
+    spec = skigen.SkiSpec(num_packets=1, grid=("voronoi", 100, "DustDensity"))
+    with pytest.raises(ski.UnsupportedSki):
+        ski.load(_write(str(tmp_path), "v", spec))

@@ -258,6 +258,21 @@ int skg_sample_density_voronoi(const skg_voronoi* v, const double* lattice, int3
                                int32_t samples, uint64_t seed, int32_t threads, double* out);
 void skg_voronoi_free(skg_voronoi* v);
 
+/* Tetrahedral grid built on the host: TetraMeshSpatialGrid's vertex preparation (the 8 domain
+ * corners added, points outside dropped, sorted by x, near duplicates within eps dropped;
+ * TetraMeshSpatialGrid.cpp:300-380) and a Delaunay tetrahedralisation (the reference uses
+ * TetGen, :386-452) by Bowyer-Watson insertion with exact predicates on a lattice of 2^23 steps;
+ * tetrahedra positively oriented, neighbour f opposite vertex f, outward unit normals as
+ * storeTetrahedra (:455-545). */
+typedef struct skg_tetra skg_tetra;
+int skg_tetra_build(int64_t npoints, const double* points, const double* box, skg_tetra** out);
+int skg_tetra_info(const skg_tetra* t, int64_t* nverts, int64_t* ntets);
+/* Tables in the layout skg_grid_tetra takes; any pointer may be NULL. */
+int skg_tetra_export(const skg_tetra* t, double* verts, int32_t* tet_verts, int32_t* face_nb, double* face_normal);
+/* skg_grid_tetra with the tables of t. */
+int skg_tetra_upload(skg_engine* e, const skg_tetra* t);
+void skg_tetra_free(skg_tetra* t);
+
 /* Cell-averaged number densities from num_samples random points per cell
  * (MediumSystem.cpp:401-504, PropertySampler :129-155). */
 int skg_sample_density_cartesian(const double* xb, int32_t nx, const double* yb, int32_t ny, const double* zb,

@@ -74,6 +74,7 @@ EXPORTED = [
     "skg_guide_collect", "skg_guide_records", "skg_guide_train",
     "skg_voronoi_build", "skg_voronoi_info", "skg_voronoi_export", "skg_voronoi_upload",
     "skg_sample_density_voronoi", "skg_voronoi_free", "skg_autotune",
+    "skg_tetra_build", "skg_tetra_info", "skg_tetra_export", "skg_tetra_upload", "skg_tetra_free",
 ]
 
 GUIDE_PARAMS = 11560
@@ -492,6 +493,51 @@ class Octree:
     def close(self):
         if self._h:
             lib().skg_octree_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Tetra:
+    """Host-built Delaunay tetrahedral grid (the 8 domain corners added) in the engine's layout."""
+
+    def __init__(self, handle, box):
+        self._h = handle
+        self.box = np.array(box, dtype=np.float64)
+
+    @classmethod
+    def build(cls, points, box):
+        pts = _f64(points).reshape(-1, 3)
+        b = _f64(box)
+        h = C.c_void_p()
+        st = lib().skg_tetra_build(C.c_int64(len(pts)), _p(pts, f64p), _p(b, f64p), C.byref(h))
+        if st != SKG_OK:
+            raise EngineError(f"skg_tetra_build failed with status {st}")
+        return cls(h, b)
+
+    def info(self):
+        a, b = C.c_int64(), C.c_int64()
+        lib().skg_tetra_info(self._h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def export(self):
+        nv, nt = self.info()
+        out = dict(verts=np.zeros((nv, 3)), tet_verts=np.zeros((nt, 4), np.int32), face_nb=np.zeros((nt, 4), np.int32),
+                   face_normal=np.zeros((nt, 4, 3)), box=self.box.copy())
+        lib().skg_tetra_export(self._h, _p(out["verts"], f64p), _p(out["tet_verts"], i32p), _p(out["face_nb"], i32p),
+                               _p(out["face_normal"], f64p))
+        return out
+
+    def upload(self, engine):
+        engine._chk(lib().skg_tetra_upload(engine._h, self._h))
+
+    def close(self):
+        if self._h:
+            lib().skg_tetra_free(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

@@ -564,3 +564,421 @@ int skg_sample_density_voronoi(const skg_voronoi* v, const double* lattice, int3
 void skg_voronoi_free(skg_voronoi* v) { delete v; }
 
 }  // extern "C"
+
+
+// ==================================================================== Delaunay tetrahedra
+//
+// The tetrahedral grid of TetraMeshSpatialGrid (TetraMeshSpatialGrid.cpp:300-452: sampled
+// vertices plus the 8 domain corners, outside points dropped, sorted by x, near duplicates
+// dropped, then TetGen's Delaunay tetrahedralisation) built here by Bowyer-Watson insertion
+// with exact predicates: the vertices are mapped to an integer lattice of 2^23 steps along the
+// longest axis (only for the predicates; the output keeps the input coordinates), where the orientation
+// and in-sphere determinants are exact in 128-bit integers.  Starting from a Delaunay
+// triangulation of the domain cube, each vertex (in Morton order) is located by a visibility
+// walk and its cavity -- the tetrahedra whose circumsphere strictly contains it -- replaced
+// by the fan from the vertex to the cavity boundary.  Output tetrahedra are positively
+// oriented (det[b-a, c-a, d-a] > 0, the reference's convention) with neighbour f opposite
+// vertex f and outward face normals computed as storeTetrahedra does.
+
+struct skg_tetra
+{
+    double box[6];
+    std::vector<double> verts;    // 3 per vertex
+    std::vector<int32_t> tv;      // 4 per tetra
+    std::vector<int32_t> nb;      // 4 per tetra, -1 on the hull
+    std::vector<double> normals;  // 12 per tetra
+};
+
+namespace
+{
+    using i128 = __int128;
+    struct LP
+    {
+        int64_t x, y, z;
+    };
+
+    inline i128 orient3(const LP& a, const LP& b, const LP& c, const LP& d)
+    {
+        const i128 bx = b.x - a.x, by = b.y - a.y, bz = b.z - a.z;
+        const i128 cx = c.x - a.x, cy = c.y - a.y, cz = c.z - a.z;
+        const i128 dx = d.x - a.x, dy = d.y - a.y, dz = d.z - a.z;
+        return bx * (cy * dz - cz * dy) - by * (cx * dz - cz * dx) + bz * (cx * dy - cy * dx);
+    }
+
+    // < 0 when e lies strictly inside the sphere through a, b, c, d of a positively oriented tetra
+    inline i128 insphere(const LP& a, const LP& b, const LP& c, const LP& d, const LP& e)
+    {
+        const LP* q[4] = {&a, &b, &c, &d};
+        i128 m[4][4];
+        for (int r = 0; r < 4; ++r)
+        {
+            const i128 x = q[r]->x - e.x, y = q[r]->y - e.y, z = q[r]->z - e.z;
+            m[r][0] = x;
+            m[r][1] = y;
+            m[r][2] = z;
+            m[r][3] = x * x + y * y + z * z;
+        }
+        // Laplace expansion along the last column with 3x3 minors
+        auto det3 = [&](int r0, int r1, int r2) {
+            return m[r0][0] * (m[r1][1] * m[r2][2] - m[r1][2] * m[r2][1]) -
+                   m[r0][1] * (m[r1][0] * m[r2][2] - m[r1][2] * m[r2][0]) +
+                   m[r0][2] * (m[r1][0] * m[r2][1] - m[r1][1] * m[r2][0]);
+        };
+        return -m[0][3] * det3(1, 2, 3) + m[1][3] * det3(0, 2, 3) - m[2][3] * det3(0, 1, 3) + m[3][3] * det3(0, 1, 2);
+    }
+
+    struct Tet
+    {
+        int32_t v[4];
+        int32_t n[4];  // neighbour across the face opposite v[i]
+        bool alive;
+    };
+
+    uint64_t morton3(uint64_t x, uint64_t y, uint64_t z)
+    {
+        auto spread = [](uint64_t a) {
+            a &= 0x1fffff;
+            a = (a | a << 32) & 0x1f00000000ffffULL;
+            a = (a | a << 16) & 0x1f0000ff0000ffULL;
+            a = (a | a << 8) & 0x100f00f00f00f00fULL;
+            a = (a | a << 4) & 0x10c30c30c30c30c3ULL;
+            a = (a | a << 2) & 0x1249249249249249ULL;
+            return a;
+        };
+        return spread(x) | spread(y) << 1 | spread(z) << 2;
+    }
+}
+
+extern "C" {
+
+int skg_tetra_build(int64_t npoints, const double* points, const double* box, skg_tetra** out)
+{
+    if (!out || npoints < 0 || (npoints && !points) || !box || !(box[0] < box[3] && box[1] < box[4] && box[2] < box[5]))
+        return SKG_ERR_ARGUMENT;
+    *out = nullptr;
+    try
+    {
+        auto M = std::make_unique<skg_tetra>();
+        std::memcpy(M->box, box, sizeof(M->box));
+        const double dx = box[3] - box[0], dy = box[4] - box[1], dz = box[5] - box[2];
+        const double eps = 1e-12 * std::sqrt(dx * dx + dy * dy + dz * dz);  // TetraMeshSpatialGrid.cpp:239
+        // TetraMeshSpatialGrid::addCorners and removeInvalid (:300-380)
+        std::vector<std::array<double, 3>> v;
+        v.reserve(npoints + 8);
+        for (int64_t i = 0; i < npoints; ++i) v.push_back({points[3 * i], points[3 * i + 1], points[3 * i + 2]});
+        for (int c = 0; c < 8; ++c)
+            v.push_back({(c & 4) ? box[3] : box[0], (c & 2) ? box[4] : box[1], (c & 1) ? box[5] : box[2]});
+        v.erase(std::remove_if(v.begin(), v.end(),
+                               [&](const std::array<double, 3>& p) {
+                                   return !(p[0] >= box[0] && p[0] <= box[3] && p[1] >= box[1] && p[1] <= box[4]
+                                            && p[2] >= box[2] && p[2] <= box[5]);
+                               }),
+                v.end());
+        std::sort(v.begin(), v.end(), [](const std::array<double, 3>& a, const std::array<double, 3>& b) { return a[0] < b[0]; });
+        {
+            const int n0 = static_cast<int>(v.size());
+            std::vector<int> toRemove;
+            for (int i = 0; i < n0; ++i)
+                for (int j = i + 1; j < n0 && v[j][0] - v[i][0] < eps; ++j)
+                {
+                    double ex = v[j][0] - v[i][0], ey = v[j][1] - v[i][1], ez = v[j][2] - v[i][2];
+                    if (ex * ex + ey * ey + ez * ez < eps * eps)
+                    {
+                        toRemove.push_back(i);
+                        break;
+                    }
+                }
+            for (auto it = toRemove.rbegin(); it != toRemove.rend(); ++it) v.erase(v.begin() + *it);
+        }
+        const int nv = static_cast<int>(v.size());
+        // lattice coordinates for the predicates, one step h for all axes (the in-sphere test is
+        // not invariant under anisotropic scaling); 2^23 steps along the longest axis keep the
+        // in-sphere determinant below 2^127; interior points stay off the domain faces
+        const double ext = std::max(dx, std::max(dy, dz));
+        const double h = ext / static_cast<double>(int64_t(1) << 23);
+        int64_t Sa[3];
+        for (int a = 0; a < 3; ++a) Sa[a] = std::max<int64_t>(2, std::llround((box[3 + a] - box[a]) / h));
+        std::vector<LP> L(nv);
+        std::vector<int> corner(8, -1);
+        for (int i = 0; i < nv; ++i)
+        {
+            int64_t q[3];
+            bool onCorner = true;
+            for (int a = 0; a < 3; ++a)
+            {
+                q[a] = static_cast<int64_t>(std::llround((v[i][a] - box[a]) / h));
+                if (v[i][a] != box[a] && v[i][a] != box[3 + a]) onCorner = false;
+            }
+            if (onCorner)
+            {
+                int c = (v[i][0] == box[3] ? 4 : 0) | (v[i][1] == box[4] ? 2 : 0) | (v[i][2] == box[5] ? 1 : 0);
+                corner[c] = i;
+                for (int a = 0; a < 3; ++a) q[a] = (v[i][a] == box[a]) ? 0 : Sa[a];
+            }
+            else
+                for (int a = 0; a < 3; ++a) q[a] = std::max<int64_t>(1, std::min<int64_t>(Sa[a] - 1, q[a]));
+            L[i] = LP{q[0], q[1], q[2]};
+        }
+        for (int c = 0; c < 8; ++c)
+            if (corner[c] < 0) return SKG_ERR_STATE;
+        // the cube as 6 positively oriented tetrahedra around the diagonal corner 0 - corner 7
+        std::vector<Tet> T;
+        T.reserve(static_cast<size_t>(7.5 * nv) + 16);
+        {
+            const int c0 = corner[0], c7 = corner[7];
+            const int ring[6] = {corner[1], corner[3], corner[2], corner[6], corner[4], corner[5]};
+            for (int k = 0; k < 6; ++k)
+            {
+                Tet t;
+                t.v[0] = c0;
+                t.v[1] = c7;
+                t.v[2] = ring[k];
+                t.v[3] = ring[(k + 1) % 6];
+                if (orient3(L[t.v[0]], L[t.v[1]], L[t.v[2]], L[t.v[3]]) < 0) std::swap(t.v[2], t.v[3]);
+                for (int f = 0; f < 4; ++f) t.n[f] = -1;
+                t.alive = true;
+                T.push_back(t);
+            }
+        }
+        // adjacency by shared faces (sorted vertex triples)
+        auto faceKey = [](int a, int b, int c) {
+            int x[3] = {a, b, c};
+            std::sort(x, x + 3);
+            return (static_cast<uint64_t>(x[0]) << 42) | (static_cast<uint64_t>(x[1]) << 21) | static_cast<uint64_t>(x[2]);
+        };
+        auto link = [&](std::vector<int>& ids) {
+            std::vector<std::pair<uint64_t, int>> fk;
+            for (int id : ids)
+                for (int f = 0; f < 4; ++f)
+                {
+                    const Tet& t = T[id];
+                    int o[3], m = 0;
+                    for (int q = 0; q < 4; ++q)
+                        if (q != f) o[m++] = t.v[q];
+                    fk.emplace_back(faceKey(o[0], o[1], o[2]), id * 4 + f);
+                }
+            std::sort(fk.begin(), fk.end());
+            for (size_t q = 0; q + 1 < fk.size(); ++q)
+                if (fk[q].first == fk[q + 1].first)
+                {
+                    int a = fk[q].second, b = fk[q + 1].second;
+                    T[a / 4].n[a % 4] = b / 4;
+                    T[b / 4].n[b % 4] = a / 4;
+                }
+        };
+        {
+            std::vector<int> ids{0, 1, 2, 3, 4, 5};
+            link(ids);
+        }
+        // insertion order: Morton order of the lattice points, corners excluded
+        std::vector<std::pair<uint64_t, int>> order;
+        for (int i = 0; i < nv; ++i)
+        {
+            bool isCorner = false;
+            for (int c = 0; c < 8; ++c) isCorner |= corner[c] == i;
+            if (!isCorner) order.emplace_back(morton3(L[i].x, L[i].y, L[i].z), i);
+        }
+        std::sort(order.begin(), order.end());
+        std::vector<int32_t> freeList, cavity, stamp, newIds;
+        std::vector<int32_t> mark;
+        int32_t last = 0;
+        int32_t epoch = 0;
+        struct BFace
+        {
+            int a, b, c, outer, outerFace;
+        };
+        std::vector<BFace> bfaces;
+        for (auto& oi : order)
+        {
+            const int p = oi.second;
+            const LP& P = L[p];
+            // visibility walk
+            int cur = last;
+            if (!T[cur].alive)
+                for (cur = 0; !T[cur].alive; ++cur) {}
+            for (int guard = 0;; ++guard)
+            {
+                if (guard > 100 * (nv + 10)) return SKG_ERR_STATE;
+                const Tet& t = T[cur];
+                int next = -1;
+                for (int f = 0; f < 4 && next < 0; ++f)
+                {
+                    LP q[4] = {L[t.v[0]], L[t.v[1]], L[t.v[2]], L[t.v[3]]};
+                    q[f] = P;
+                    if (orient3(q[0], q[1], q[2], q[3]) < 0) next = t.n[f];
+                }
+                if (next < 0) break;  // inside (or on the boundary of) cur
+                cur = next;
+            }
+            // cavity: tetrahedra whose circumsphere strictly contains p, grown from cur
+            ++epoch;
+            if (mark.size() < T.size()) mark.resize(T.size() * 2, 0);
+            cavity.clear();
+            cavity.push_back(cur);
+            mark[cur] = epoch;
+            bfaces.clear();
+            for (size_t q = 0; q < cavity.size(); ++q)
+            {
+                const int id = cavity[q];
+                for (int f = 0; f < 4; ++f)
+                {
+                    const int nb = T[id].n[f];
+                    if (nb >= 0 && mark[nb] == epoch) continue;
+                    bool inside = false;
+                    if (nb >= 0)
+                    {
+                        const Tet& u = T[nb];
+                        inside = insphere(L[u.v[0]], L[u.v[1]], L[u.v[2]], L[u.v[3]], P) < 0;
+                    }
+                    if (inside)
+                    {
+                        mark[nb] = epoch;
+                        cavity.push_back(nb);
+                    }
+                    else
+                    {
+                        int o[3], m = 0;
+                        for (int r = 0; r < 4; ++r)
+                            if (r != f) o[m++] = T[id].v[r];
+                        int outerFace = -1;
+                        if (nb >= 0)
+                            for (int r = 0; r < 4; ++r)
+                                if (T[nb].n[r] == id) outerFace = r;
+                        bfaces.push_back({o[0], o[1], o[2], nb, outerFace});
+                    }
+                }
+            }
+            // a boundary face already counted from the other side cannot occur: the neighbour
+            // across it is either in the cavity (skipped) or outside (counted once)
+            for (int id : cavity)
+            {
+                T[id].alive = false;
+                freeList.push_back(id);
+            }
+            newIds.clear();
+            for (const BFace& bf : bfaces)
+            {
+                Tet t;
+                t.v[0] = bf.a;
+                t.v[1] = bf.b;
+                t.v[2] = bf.c;
+                t.v[3] = p;
+                i128 o = orient3(L[t.v[0]], L[t.v[1]], L[t.v[2]], L[t.v[3]]);
+                if (o == 0) return SKG_ERR_STATE;  // degenerate (coplanar) insertion
+                if (o < 0) std::swap(t.v[0], t.v[1]);
+                for (int f = 0; f < 4; ++f) t.n[f] = -1;
+                t.n[3] = bf.outer;  // face opposite p is the cavity boundary face
+                t.alive = true;
+                int id;
+                if (!freeList.empty())
+                {
+                    id = freeList.back();
+                    freeList.pop_back();
+                    T[id] = t;
+                }
+                else
+                {
+                    id = static_cast<int>(T.size());
+                    T.push_back(t);
+                }
+                if (bf.outer >= 0) T[bf.outer].n[bf.outerFace] = id;
+                newIds.push_back(id);
+            }
+            // link the new tetrahedra among themselves (faces through p)
+            {
+                std::vector<std::pair<uint64_t, int>> fk;
+                fk.reserve(newIds.size() * 3);
+                for (int id : newIds)
+                    for (int f = 0; f < 3; ++f)
+                    {
+                        const Tet& t = T[id];
+                        int o[3], m = 0;
+                        for (int q = 0; q < 4; ++q)
+                            if (q != f) o[m++] = t.v[q];
+                        fk.emplace_back(faceKey(o[0], o[1], o[2]), id * 4 + f);
+                    }
+                std::sort(fk.begin(), fk.end());
+                for (size_t q = 0; q + 1 < fk.size(); ++q)
+                    if (fk[q].first == fk[q + 1].first)
+                    {
+                        int a = fk[q].second, b = fk[q + 1].second;
+                        T[a / 4].n[a % 4] = b / 4;
+                        T[b / 4].n[b % 4] = a / 4;
+                        ++q;
+                    }
+            }
+            last = newIds.empty() ? last : newIds.back();
+        }
+        // compact, then faces and normals as storeTetrahedra (TetraMeshSpatialGrid.cpp:500-540)
+        std::vector<int32_t> newId(T.size(), -1);
+        int32_t nt = 0;
+        for (size_t i = 0; i < T.size(); ++i)
+            if (T[i].alive) newId[i] = nt++;
+        M->verts.resize(3 * static_cast<size_t>(nv));
+        for (int i = 0; i < nv; ++i)
+            for (int a = 0; a < 3; ++a) M->verts[3 * i + a] = v[i][a];
+        M->tv.resize(4 * static_cast<size_t>(nt));
+        M->nb.resize(4 * static_cast<size_t>(nt));
+        M->normals.resize(12 * static_cast<size_t>(nt));
+        for (size_t i = 0; i < T.size(); ++i)
+        {
+            if (!T[i].alive) continue;
+            const int32_t m = newId[i];
+            for (int q = 0; q < 4; ++q)
+            {
+                M->tv[4 * m + q] = T[i].v[q];
+                M->nb[4 * m + q] = T[i].n[q] >= 0 ? newId[T[i].n[q]] : -1;
+            }
+            for (int f = 0; f < 4; ++f)
+            {
+                int cv[3] = {(f + 3) % 4, (f + 2) % 4, (f + 1) % 4};
+                if (f % 2 == 0) std::swap(cv[0], cv[2]);
+                const double* v0 = &M->verts[3 * static_cast<size_t>(T[i].v[cv[0]])];
+                const double* v1 = &M->verts[3 * static_cast<size_t>(T[i].v[cv[1]])];
+                const double* v2 = &M->verts[3 * static_cast<size_t>(T[i].v[cv[2]])];
+                double e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+                double e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+                double nx = e1[1] * e2[2] - e1[2] * e2[1], ny = e1[2] * e2[0] - e1[0] * e2[2], nz = e1[0] * e2[1] - e1[1] * e2[0];
+                double nn = std::sqrt(nx * nx + ny * ny + nz * nz);
+                M->normals[12 * m + 3 * f] = nx / nn;
+                M->normals[12 * m + 3 * f + 1] = ny / nn;
+                M->normals[12 * m + 3 * f + 2] = nz / nn;
+            }
+        }
+        *out = M.release();
+        return SKG_OK;
+    }
+    catch (const std::bad_alloc&)
+    {
+        return SKG_ERR_CAPACITY;
+    }
+}
+
+int skg_tetra_info(const skg_tetra* t, int64_t* nverts, int64_t* ntets)
+{
+    if (!t) return SKG_ERR_ARGUMENT;
+    if (nverts) *nverts = static_cast<int64_t>(t->verts.size() / 3);
+    if (ntets) *ntets = static_cast<int64_t>(t->tv.size() / 4);
+    return SKG_OK;
+}
+
+int skg_tetra_export(const skg_tetra* t, double* verts, int32_t* tet_verts, int32_t* face_nb, double* face_normal)
+{
+    if (!t) return SKG_ERR_ARGUMENT;
+    if (verts) std::memcpy(verts, t->verts.data(), t->verts.size() * sizeof(double));
+    if (tet_verts) std::memcpy(tet_verts, t->tv.data(), t->tv.size() * sizeof(int32_t));
+    if (face_nb) std::memcpy(face_nb, t->nb.data(), t->nb.size() * sizeof(int32_t));
+    if (face_normal) std::memcpy(face_normal, t->normals.data(), t->normals.size() * sizeof(double));
+    return SKG_OK;
+}
+
+int skg_tetra_upload(skg_engine* e, const skg_tetra* t)
+{
+    if (!e || !t) return SKG_ERR_ARGUMENT;
+    return skg_grid_tetra(e, static_cast<int64_t>(t->verts.size() / 3), t->verts.data(),
+                          static_cast<int64_t>(t->tv.size() / 4), t->tv.data(), t->nb.data(), t->normals.data(), t->box);
+}
+
+void skg_tetra_free(skg_tetra* t) { delete t; }
+
+}  // extern "C"

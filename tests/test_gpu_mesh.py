@@ -102,3 +102,31 @@ def test_mesh_lifecycle_matches_oracle(eng, kind, mode):
     rel = np.abs(img[nz] - oimg[nz]) / oimg[nz]
     assert np.median(rel) < 1e-9
     assert np.mean(rel < 1e-6) > 0.99
+
+
+def test_built_meshes_trace_like_the_oracle(eng):
+    """Meshes built by the library's host builders (skg_voronoi_build, skg_tetra_build) and
+    uploaded from their handles: GPU paths bit-exact against the oracle on the exported tables."""
+    pkg = load_pkg()
+    rng = np.random.default_rng(21)
+    box = np.array([-1.0, -2.0, -0.5, 1.0, 2.0, 0.5])
+    pts = box[:3] + (box[3:] - box[:3]) * rng.random((3000, 3))
+    k = rng.normal(size=(2000, 3))
+    k /= np.linalg.norm(k, axis=1, keepdims=True)
+    rays = np.hstack([box[:3] + (box[3:] - box[:3]) * rng.random((2000, 3)), k])
+    for kind in KINDS:
+        b = (pkg.Voronoi.build(pts, box, threads=4) if kind == "voronoi" else pkg.Tetra.build(pts, box))
+        t = b.export()
+        b.upload(eng)
+        b.close()
+        print("case built mesh paths", kind, flush=True)
+        off, m, ds = eng.trace_paths(rays)
+        if kind == "voronoi":
+            g = O.Grid(kind, **{q: t[q] for q in ("sites", "nb_begin", "nb_count", "nb_index", "block_n", "block_begin",
+                                                 "block_count", "block_index", "box")})
+        else:
+            g = O.Grid(kind, **t)
+        ooff, om, ods = O.trace(g, rays)
+        np.testing.assert_array_equal(off, ooff)
+        np.testing.assert_array_equal(m, om)
+        np.testing.assert_array_equal(ds, ods)

@@ -75,3 +75,111 @@ def test_voronoi_builder_invariants_random_sites():
     ci = O.cell_index(g, pts)
     d2 = ((pts[:, None, :] - t["sites"][None, :, :]) ** 2).sum(axis=2)
     np.testing.assert_array_equal(ci, d2.argmin(axis=1))
+
+
+# ---------------------------------------------------------------- tetrahedra
+def _tetra_ref():
+    f = golden("tetra_paths.npz")
+    return {k: f[k] for k in f.files}
+
+
+def test_tetra_builder_matches_reference_delaunay():
+    """Our Delaunay tetrahedralisation of the reference's vertices equals TetGen's except for
+    the tetrahedra with three domain corners: the 4 corners of a box face are cocircular, so
+    the face's diagonal is a free (degenerate) choice."""
+    pkg = load_pkg()
+    d = _tetra_ref()
+    V, box = d["verts"], d["box"]
+    corner = np.all((V == box[:3]) | (V == box[3:]), axis=1)
+    t = pkg.Tetra.build(V[~corner], box)
+    e = t.export()
+    t.close()
+    assert len(e["tet_verts"]) == len(d["tet_verts"])
+    np.testing.assert_array_equal(np.sort(e["verts"], axis=0), np.sort(V, axis=0))
+
+    def keyed(W, T):
+        out = set()
+        for tt in T:
+            pts = [tuple(W[i]) for i in tt]
+            if sum(all(c == box[0] or c == box[3] for c in p) for p in pts) >= 3:
+                continue
+            out.add(tuple(sorted(pts)))
+        return out
+
+    a, b = keyed(e["verts"], e["tet_verts"]), keyed(V, d["tet_verts"])
+    assert a == b and len(a) > 0.99 * len(d["tet_verts"])
+
+
+def _tetra_checks(e, box):
+    W, T, nb = e["verts"], e["tet_verts"], e["face_nb"]
+    a, b, c, d = (W[T[:, i]] for i in range(4))
+    vol = np.einsum("ij,ij->i", np.cross(b - a, c - a), d - a) / 6
+    assert vol.min() > 0
+    assert abs(vol.sum() / np.prod(box[3:] - box[:3]) - 1) < 1e-10
+    for t in range(len(T)):
+        for f in range(4):
+            n = nb[t, f]
+            if n >= 0:
+                assert (nb[n] == t).sum() == 1
+    assert (nb < 0).sum() == 12  # the two triangles of each box face
+    # outward normals: the vertex opposite face f lies behind it
+    for f in range(4):
+        cv = [(f + 3) % 4, (f + 2) % 4, (f + 1) % 4]
+        s = np.einsum("ij,ij->i", e["face_normal"][:, f], W[T[:, f]] - W[T[:, cv[0]]])
+        assert np.all(s < 0)
+
+
+def test_tetra_builder_invariants_random_points():
+    """5e3 random points: positive volumes tiling the box, mutual neighbours, outward normals,
+    and the empty-circumsphere property (brute force on a sample of tetrahedra)."""
+    pkg = load_pkg()
+    rng = np.random.default_rng(8)
+    box = np.array([-1.0, -2.0, -0.5, 1.0, 2.0, 0.5])
+    pts = box[:3] + (box[3:] - box[:3]) * rng.random((5000, 3))
+    t = pkg.Tetra.build(pts, box)
+    e = t.export()
+    t.close()
+    _tetra_checks(e, box)
+    W, T = e["verts"], e["tet_verts"]
+    for tt in rng.choice(len(T), 200, replace=False):
+        P = W[T[tt]]
+        A = 2 * (P[1:] - P[0])
+        rhs = (P[1:] ** 2).sum(1) - (P[0] ** 2).sum()
+        cen = np.linalg.solve(A, rhs)
+        r2 = ((P[0] - cen) ** 2).sum()
+        d2 = ((W - cen) ** 2).sum(1)
+        d2[T[tt]] = np.inf
+        assert d2.min() > r2 * (1 - 1e-5)  # exact on the predicate lattice (step 2^-23 of the box)
+
+
+def test_tetra_builder_traversal_invariants():
+    """Paths through our mesh (oracle generator): segment midpoints inside their tetrahedron,
+    and cellIndex agrees with a brute-force containment search."""
+    pkg = load_pkg()
+    d = _tetra_ref()
+    V, box = d["verts"], d["box"]
+    corner = np.all((V == box[:3]) | (V == box[3:]), axis=1)
+    t = pkg.Tetra.build(V[~corner], box)
+    e = t.export()
+    t.close()
+    g = O.Grid("tetra", verts=e["verts"], tet_verts=e["tet_verts"], face_nb=e["face_nb"], face_normal=e["face_normal"],
+               box=box)
+    rays = d["rays"][:300]
+    off, m, ds = O.trace(g, rays)
+    W, T = e["verts"], e["tet_verts"]
+    for i in range(len(rays)):
+        r, k = rays[i, :3], rays[i, 3:]
+        s = 0.0
+        for q in range(off[i], off[i + 1]):
+            mid = r + k * (s + 0.5 * ds[q])
+            s += ds[q]
+            if m[q] < 0:
+                continue
+            A = np.column_stack([W[T[m[q], j]] - W[T[m[q], 0]] for j in (1, 2, 3)])
+            lam = np.linalg.solve(A, mid - W[T[m[q], 0]])
+            assert lam.min() > -1e-9 and lam.sum() < 1 + 1e-9
+    ci = O.cell_index(g, d["points"][:300])
+    for p, c in zip(d["points"][:300], ci):
+        A = np.column_stack([W[T[c, j]] - W[T[c, 0]] for j in (1, 2, 3)])
+        lam = np.linalg.solve(A, p - W[T[c, 0]])
+        assert lam.min() > -1e-9 and lam.sum() < 1 + 1e-9

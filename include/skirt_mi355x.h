@@ -128,7 +128,7 @@ int skg_trace_tau(skg_engine* e, const double* rays, int64_t n, int32_t ell, dou
 /* ---- instruments -------------------------------------------------------------------- */
 
 /* Adds a distant frame instrument (FrameInstrument.cpp:12-33, DistantInstrument.cpp:13-63);
- * returns its index (>= 0) or a negative status.  At most 8 instruments. */
+ * returns its index (>= 0) or a negative status.  At most 1024 instruments. */
 int skg_frame_add(skg_engine* e, double inclination, double azimuth, double roll, int32_t nx, int32_t ny,
                   double fovx, double fovy, double cx, double cy);
 int skg_frame_remove_all(skg_engine* e);
@@ -156,6 +156,20 @@ int skg_options(skg_engine* e, double path_length_bias, double min_weight_reduct
  * every instrument, accumulating into the frames on the device (MonteCarloSimulation.cpp:614-694,
  * :1492-1544, :2034-2092).  Synchronous. */
 int skg_run_packets(skg_engine* e, uint64_t first_history, uint64_t count);
+
+/* Batched independent instances of the configured simulation in one launch (no counterpart
+ * in the reference, which runs one simulation per process; its per-simulation inputs are the
+ * ones varied here).  Instance i has packets[i] histories with their own random stream
+ * (seeds[i]; history j of the instance draws as history j of a single run with that seed),
+ * optical depths scaled by tau_scale[i] (sections sext*tau_scale, ssca*tau_scale, as a
+ * medium normalised to tau_scale times the optical depth), launch luminosities
+ * lpack[i*nlambda .. +nlambda) as skg_source_point's, and peel-off to the frames
+ * [frame_begin[i], frame_begin[i]+frame_count[i]) only.  While instances are set,
+ * skg_run_packets(first, count) runs histories [first, first+count) of the concatenation of
+ * the instances (sum of packets[i] in all), in the persistent engine; n = 0 removes them.
+ * Each instance's images equal those of a single run with its parameters. */
+int skg_instances(skg_engine* e, int32_t n, const uint64_t* packets, const uint64_t* seeds, const double* tau_scale,
+                  const double* lpack, const int32_t* frame_begin, const int32_t* frame_count);
 
 /* Same, asynchronous on the engine's stream; skg_sync waits for completion. */
 int skg_run_packets_async(skg_engine* e, uint64_t first_history, uint64_t count);
@@ -280,6 +294,10 @@ int skg_sample_density_cartesian(const double* xb, int32_t nx, const double* yb,
                                  int32_t threads, double* out);
 int skg_sample_density_octree(const skg_octree* t, const skg_disk_model* model, int32_t num_samples,
                               uint64_t seed, int32_t threads, double* out);
+/* The same for a tetrahedral grid given by its tables (as skg_grid_tetra's), positions drawn
+ * uniformly in each tetra as Tetra::generatePosition does (TetraMeshSpatialGrid.cpp:164-173). */
+int skg_sample_density_tetra(int64_t ntets, const double* verts, const int32_t* tet_verts, const skg_disk_model* model,
+                             int32_t num_samples, uint64_t seed, int32_t threads, double* out);
 
 /* Dust cross sections and asymmetry parameters at the simulation wavelengths from a
  * MeanListDustMix table (TabulatedDustMix::getOpticalProperties, TabulatedDustMix.cpp:12-43;

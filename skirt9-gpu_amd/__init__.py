@@ -75,6 +75,7 @@ EXPORTED = [
     "skg_voronoi_build", "skg_voronoi_info", "skg_voronoi_export", "skg_voronoi_upload",
     "skg_sample_density_voronoi", "skg_voronoi_free", "skg_autotune",
     "skg_tetra_build", "skg_tetra_info", "skg_tetra_export", "skg_tetra_upload", "skg_tetra_free",
+    "skg_sample_density_tetra", "skg_instances",
 ]
 
 GUIDE_PARAMS = 11560
@@ -230,6 +231,26 @@ class Engine:
         lp = _f64(lpack)
         self._chk(lib().skg_source_point(self._h, C.c_double(pos[0]), C.c_double(pos[1]), C.c_double(pos[2]),
                                          _p(lp, f64p)))
+
+    def instances(self, packets, seeds=None, tau_scale=None, lpack=None, frame_begin=None, frame_count=None):
+        """Batched independent instances in one persistent launch (skg_instances); an empty
+        `packets` removes them.  lpack: (n, nlambda) launch luminosities per instance."""
+        pk = np.ascontiguousarray(packets, dtype=np.uint64).ravel()
+        n = len(pk)
+        if n == 0:
+            self._chk(lib().skg_instances(self._h, C.c_int32(0), None, None, None, None, None, None))
+            return 0
+        sd = np.ascontiguousarray(seeds, dtype=np.uint64).ravel()
+        ts = _f64(tau_scale).ravel()
+        lp = _f64(lpack).reshape(n, -1)
+        fb = np.ascontiguousarray(frame_begin, dtype=np.int32).ravel()
+        fc = np.ascontiguousarray(frame_count, dtype=np.int32).ravel()
+        if not (len(sd) == len(ts) == len(fb) == len(fc) == n):
+            raise EngineError("instances: all per-instance arrays need one entry per instance")
+        u64p = P(C.c_uint64)
+        self._chk(lib().skg_instances(self._h, C.c_int32(n), _p(pk, u64p), _p(sd, u64p), _p(ts, f64p), _p(lp, f64p),
+                                      _p(fb, i32p), _p(fc, i32p)))
+        return int(pk.sum())
 
     def options(self, path_length_bias=0.5, min_weight_reduction=1e4, min_scatt_events=0, fp_mode=SKG_FP64, seed=1):
         self._chk(lib().skg_options(self._h, C.c_double(path_length_bias), C.c_double(min_weight_reduction),
@@ -545,6 +566,20 @@ class Tetra:
             self.close()
         except Exception:
             pass
+
+
+def sample_density_tetra(verts, tet_verts, model, num_samples=100, seed=1, threads=0):
+    """Cell-averaged number density of a disk model in each tetrahedron (skg_sample_density_tetra)."""
+    v = _f64(verts).reshape(-1, 3)
+    tv = np.ascontiguousarray(tet_verts, dtype=np.int32).reshape(-1, 4)
+    if len(tv) and (tv.min() < 0 or tv.max() >= len(v)):
+        raise EngineError("sample_density_tetra: vertex index out of range")
+    out = np.zeros(len(tv))
+    st = lib().skg_sample_density_tetra(C.c_int64(len(tv)), _p(v, f64p), _p(tv, i32p), C.byref(model),
+                                        C.c_int32(num_samples), C.c_uint64(seed), C.c_int32(threads), _p(out, f64p))
+    if st != SKG_OK:
+        raise EngineError(f"skg_sample_density_tetra failed with status {st}")
+    return out
 
 
 def sample_density_cartesian(xb, yb, zb, model, num_samples=100, seed=1, threads=0):

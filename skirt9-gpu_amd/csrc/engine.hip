@@ -191,6 +191,9 @@ namespace skg
         unsigned long long* counters;  // 8 event counters
         double* frame;
         const FrameGeom* frames;  // device copy of the instrument geometry
+        const Instance* inst;     // batched instances (ninst > 0) or none
+        int ninst;
+        const double* ilpack;     // launch luminosities per instance (ninst x nlambda)
     };
 
     enum Phase : int
@@ -215,20 +218,21 @@ namespace skg
     };
     enum ColdU32 : int
     {
-        cHist0, cHist1, cDraw, cEll, cNscatt, cF, cPix, cFlags, kColdU32
+        cHist0, cHist1, cDraw, cEll, cNscatt, cF, cPix, cFlags, cInst, kColdU32
     };
     constexpr int kLcThreads = 256;
-    constexpr int kColdBytes = kLcThreads * (8 * kColdF64 + 4 * kColdU32);  // 32 KB per workgroup
+    constexpr int kColdBytes = kLcThreads * (8 * kColdF64 + 4 * kColdU32);  // 33 KB per workgroup
 
     struct Packet
     {
         Philox rng;
         double rx, ry, rz, kx, ky, kz;
         double W, lam, Lthr, taupath, tauCached, Wp;
-        int ell, nscatt, f, pix;
+        int ell, nscatt, f, pix, inst;
         bool scat, haveTau;
 
-        __device__ __forceinline__ void load(const double* cd, const uint32_t* cu, int t, uint64_t seed)
+        template<bool INST>
+        __device__ __forceinline__ void load(const double* cd, const uint32_t* cu, int t, const SimParams& p)
         {
             rx = cd[cRx * kLcThreads + t];
             ry = cd[cRy * kLcThreads + t];
@@ -244,7 +248,8 @@ namespace skg
             Wp = cd[cWp * kLcThreads + t];
             uint64_t h = static_cast<uint64_t>(cu[cHist0 * kLcThreads + t])
                          | (static_cast<uint64_t>(cu[cHist1 * kLcThreads + t]) << 32);
-            rng.init(seed, h);
+            inst = static_cast<int>(cu[cInst * kLcThreads + t]);
+            rng.init(INST ? p.inst[inst].seed : p.seed, h);
             rng.draw = cu[cDraw * kLcThreads + t];
             ell = static_cast<int>(cu[cEll * kLcThreads + t]);
             nscatt = static_cast<int>(cu[cNscatt * kLcThreads + t]);
@@ -277,6 +282,7 @@ namespace skg
             cu[cF * kLcThreads + t] = static_cast<uint32_t>(f);
             cu[cPix * kLcThreads + t] = static_cast<uint32_t>(pix);
             cu[cFlags * kLcThreads + t] = (scat ? 1u : 0u) | (haveTau ? 2u : 0u);
+            cu[cInst * kLcThreads + t] = static_cast<uint32_t>(inst);
         }
     };
 
@@ -389,7 +395,7 @@ namespace skg
     };
 
     // The event transitions of the life cycle, on a packet loaded from LDS.
-    template<int GT, int FP> struct Events
+    template<int GT, int FP, bool INST> struct Events
     {
         const SimParams& p;
         Trav<GT, FP>& T;
@@ -410,16 +416,36 @@ namespace skg
             atomicAdd(lds + 5, 1u);
         }
 
-        // MonteCarloSimulation::performLifeCycle launch + SourceSystem::launch
+        // MonteCarloSimulation::performLifeCycle launch + SourceSystem::launch; in a batched
+        // launch, history h is history h - offset of its instance
         __device__ __forceinline__ void launch(uint64_t h)
         {
-            pk.rng.init(p.seed, h);
+            uint64_t seed = p.seed;
+            const double* lp = p.lpack;
+            pk.inst = 0;
+            if constexpr (INST)
+            {
+                int lo = 0, hi = p.ninst - 1;  // last instance whose offset is <= h
+                while (lo < hi)
+                {
+                    int mid = (lo + hi + 1) >> 1;
+                    if (p.inst[mid].offset <= h)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                pk.inst = lo;
+                h -= p.inst[lo].offset;
+                seed = p.inst[lo].seed;
+                lp = p.ilpack + static_cast<int64_t>(lo) * p.nlambda;
+            }
+            pk.rng.init(seed, h);
             pk.ell = static_cast<int>(pk.rng.uniform() * p.nlambda);
             if (pk.ell >= p.nlambda) pk.ell = p.nlambda - 1;
             pk.lam = p.lambda[pk.ell];
-            T.section = p.sext[pk.ell];
+            T.section = INST ? p.sext[pk.ell] * p.inst[pk.inst].scale : p.sext[pk.ell];
             T.section32 = static_cast<float>(T.section);
-            pk.W = p.lpack[pk.ell] * pk.lam;
+            pk.W = lp[pk.ell] * pk.lam;
             pk.rx = p.src[0];
             pk.ry = p.src[1];
             pk.rz = p.src[2];
@@ -442,7 +468,7 @@ namespace skg
         __device__ __forceinline__ void begin_peel(bool scattering)
         {
             pk.scat = scattering;
-            pk.f = -1;
+            pk.f = (INST ? p.inst[pk.inst].fbeg : 0) - 1;
             next_peel();
         }
 
@@ -450,10 +476,16 @@ namespace skg
         // FluxRecorder::detect skips the optical depth for packets outside the frame, :375)
         __device__ __forceinline__ void next_peel()
         {
-            while (++pk.f < p.nframes)
+            int fb = 0, fe = p.nframes;
+            if constexpr (INST)
+            {
+                fb = p.inst[pk.inst].fbeg;
+                fe = p.inst[pk.inst].fend;
+            }
+            while (++pk.f < fe)
             {
                 const FrameGeom& fg = p.frames[pk.f];
-                if (!fg.same_as_prev)
+                if (!fg.same_as_prev || pk.f == fb)
                 {
                     pk.haveTau = false;
                     pk.Wp = pk.W;
@@ -541,8 +573,9 @@ namespace skg
                 }
                 int m = T.mint;
                 double n = m >= 0 ? p.dens[m] : 0.;
-                double ksca = n > 0. ? n * p.ssca[pk.ell] : 0.;
-                double kext = n > 0. ? n * p.sext[pk.ell] : 0.;
+                double ss = INST ? p.ssca[pk.ell] * p.inst[pk.inst].scale : p.ssca[pk.ell];
+                double ksca = n > 0. ? n * ss : 0.;
+                double kext = n > 0. ? n * T.section : 0.;
                 double albedo = kext > 0. ? ksca / kext : 0.;
                 pk.W *= -expm1(-pk.taupath) * albedo;
                 pk.rx += T.sint * pk.kx;
@@ -569,7 +602,8 @@ namespace skg
 #ifndef SKG_LIFECYCLE_WAVES
 #define SKG_LIFECYCLE_WAVES 2
 #endif
-    template<int GT, int FP> __global__ void __launch_bounds__(kLcThreads, SKG_LIFECYCLE_WAVES) lifecycle(SimParams p)
+    template<int GT, int FP, bool INST>
+    __global__ void __launch_bounds__(kLcThreads, SKG_LIFECYCLE_WAVES) lifecycle(SimParams p)
     {
         __shared__ unsigned int wgcnt[8];
         __shared__ double coldD[kColdF64 * kLcThreads];
@@ -596,8 +630,8 @@ namespace skg
                 if (T.ended)
                 {
                     Packet pk;
-                    pk.load(coldD, coldU, t, p.seed);
-                    Events<GT, FP>{p, T, pk, wgcnt}.transition();
+                    pk.load<INST>(coldD, coldU, t, p);
+                    Events<GT, FP, INST>{p, T, pk, wgcnt}.transition();
                     pk.store(coldD, coldU, t);
                 }
                 unsigned long long need = __ballot(T.phase == kFetch);
@@ -615,7 +649,7 @@ namespace skg
                         if (mine < p.count)
                         {
                             Packet pk;
-                            Events<GT, FP>{p, T, pk, wgcnt}.launch(p.first + mine);
+                            Events<GT, FP, INST>{p, T, pk, wgcnt}.launch(p.first + mine);
                             pk.store(coldD, coldU, t);
                         }
                         else
@@ -677,6 +711,11 @@ struct skg_engine
     bool frameDirty = true;
     unsigned long long* queue = nullptr;
     unsigned long long* counters = nullptr;
+    // batched instances
+    std::vector<Instance> hInst;
+    Instance* inst = nullptr;
+    double* ilpack = nullptr;
+    uint64_t instTotal = 0;
     // launch shape
     int workgroups = 0, threads = 256, threshold = 32;
     float lastMs = 0.f;
@@ -852,7 +891,11 @@ namespace
 
     template<int GT, int FP> hipError_t launchLifecycle(const SimParams& p, int blocks, int threads, hipStream_t s)
     {
-        hipLaunchKernelGGL((lifecycle<GT, FP>), dim3(blocks), dim3(threads), 0, s, p);
+        // batched instances take their own instantiation; the single run carries no instance code
+        if (p.ninst)
+            hipLaunchKernelGGL((lifecycle<GT, FP, true>), dim3(blocks), dim3(threads), 0, s, p);
+        else
+            hipLaunchKernelGGL((lifecycle<GT, FP, false>), dim3(blocks), dim3(threads), 0, s, p);
         return hipGetLastError();
     }
 }
@@ -983,7 +1026,7 @@ void skg_destroy(skg_engine* e)
                     static_cast<void*>(e->gBf), static_cast<void*>(e->gBias), static_cast<void*>(e->rec),
                     static_cast<void*>(e->recHist), static_cast<void*>(e->recA), static_cast<void*>(e->recCount),
                     static_cast<void*>(e->recSum), static_cast<void*>(e->contrib), static_cast<void*>(e->recLast),
-                    static_cast<void*>(e->finalLum)})
+                    static_cast<void*>(e->finalLum), static_cast<void*>(e->inst), static_cast<void*>(e->ilpack)})
         if (p) hipFree(p);
     if (e->hostCtl) hipHostFree(e->hostCtl);
     for (auto ev : e->traceEv) hipEventDestroy(ev);
@@ -1416,7 +1459,7 @@ int skg_frame_add(skg_engine* e, double inclination, double azimuth, double roll
     if (!e || nx < 1 || ny < 1 || !(fovx > 0) || !(fovy > 0)) return fail(e, SKG_ERR_ARGUMENT, "skg_frame_add: bad arguments");
     if (inclination < -1e-8 || inclination > M_PI + 1e-8) return fail(e, SKG_ERR_ARGUMENT, "skg_frame_add: inclination outside [0, pi]");
     std::lock_guard<std::mutex> lk(e->mu);
-    if (e->frames.size() >= static_cast<size_t>(kMaxFrames)) return fail(e, SKG_ERR_CAPACITY, "skg_frame_add: at most 8 instruments");
+    if (e->frames.size() >= static_cast<size_t>(kMaxFrames)) return fail(e, SKG_ERR_CAPACITY, "skg_frame_add: at most 1024 instruments");
     e->frames.push_back({inclination, azimuth, roll, nx, ny, fovx, fovy, cx, cy});
     e->frameDirty = true;
     return static_cast<int>(e->frames.size()) - 1;
@@ -1493,6 +1536,41 @@ int skg_options(skg_engine* e, double path_length_bias, double min_weight_reduct
     return SKG_OK;
 }
 
+int skg_instances(skg_engine* e, int32_t n, const uint64_t* packets, const uint64_t* seeds, const double* tau_scale,
+                  const double* lpack, const int32_t* frame_begin, const int32_t* frame_count)
+{
+    if (!e || n < 0 || (n > 0 && (!packets || !seeds || !tau_scale || !lpack || !frame_begin || !frame_count)))
+        return fail(e, SKG_ERR_ARGUMENT, "skg_instances: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    if (n > 0 && e->nlambda < 1) return fail(e, SKG_ERR_STATE, "skg_instances: upload the dust wavelengths first");
+    std::vector<Instance> h(n);
+    uint64_t total = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        if (!(tau_scale[i] >= 0) || !std::isfinite(tau_scale[i]) || frame_begin[i] < 0 || frame_count[i] < 0
+            || frame_begin[i] + frame_count[i] > kMaxFrames || packets[i] > (uint64_t(1) << 52) - total)
+            return fail(e, SKG_ERR_ARGUMENT, "skg_instances: bad parameters of instance " + std::to_string(i));
+        h[i] = Instance{total, seeds[i], tau_scale[i], frame_begin[i], frame_begin[i] + frame_count[i]};
+        total += packets[i];
+    }
+    for (void** b : {reinterpret_cast<void**>(&e->inst), reinterpret_cast<void**>(&e->ilpack)})
+        if (*b)
+        {
+            hipFree(*b);
+            *b = nullptr;
+        }
+    e->hInst.clear();
+    e->instTotal = 0;
+    if (n == 0) return SKG_OK;
+    int st = upload(e, &e->inst, h.data(), h.size());
+    if (!st) st = upload(e, &e->ilpack, lpack, static_cast<size_t>(n) * e->nlambda);
+    if (st) return st;
+    e->hInst = std::move(h);
+    e->instTotal = total;
+    return SKG_OK;
+}
+
 int skg_tune(skg_engine* e, int32_t workgroups, int32_t threads, int32_t transition_threshold)
 {
     if (!e || workgroups < 0 || threads < 64 || threads > 256 || threads % 64 || transition_threshold < 1
@@ -1533,6 +1611,9 @@ static int runPersistent(skg_engine* e, uint64_t first_history, uint64_t count)
     p.frame = e->frameBuf;
     p.frames = e->frameGeom;
     p.nframes = static_cast<int>(e->frames.size());
+    p.ninst = static_cast<int>(e->hInst.size());
+    p.inst = e->inst;
+    p.ilpack = e->ilpack;
     HIPCHK(e, hipMemsetAsync(e->queue, 0, sizeof(unsigned long long), e->stream));
     (void)hipGetLastError();
     int threads = e->threads;
@@ -1803,6 +1884,16 @@ int skg_run_packets_async(skg_engine* e, uint64_t first_history, uint64_t count)
         return fail(e, SKG_ERR_STATE, "skg_run_packets: source luminosities do not match the dust wavelength bins");
     int st = ensureFrames(e);
     if (st) return st;
+    if (!e->hInst.empty())
+    {
+        if (e->mode != SKG_MODE_PERSISTENT || e->guideMode != 0)
+            return fail(e, SKG_ERR_STATE, "skg_run_packets: batched instances run in the persistent engine without the guide");
+        if (first_history > e->instTotal || count > e->instTotal - first_history)
+            return fail(e, SKG_ERR_ARGUMENT, "skg_run_packets: histories beyond the instances' total");
+        for (const Instance& I : e->hInst)
+            if (I.fend > static_cast<int>(e->frames.size()))
+                return fail(e, SKG_ERR_STATE, "skg_run_packets: an instance names a frame that does not exist");
+    }
     if (count == 0) return SKG_OK;
     if (e->mode == SKG_MODE_PIPELINE) return runPipeline(e, first_history, count);
     return runPersistent(e, first_history, count);
@@ -2094,9 +2185,12 @@ int skg_autotune(skg_engine* e, uint64_t first_history, uint64_t trial_packets, 
     if (!e || trial_packets < 1024 || !(engines >= 1 && engines <= 3))
         return fail(e, SKG_ERR_ARGUMENT, "skg_autotune: bad arguments");
     if (e->guideMode == 2) return fail(e, SKG_ERR_STATE, "skg_autotune: not in guide collect mode");
-    // guided sampling runs in the pipeline engine only
+    // guided sampling runs in the pipeline engine only, batched instances in the persistent one
     if (e->guideMode == 1) engines &= 2;
-    if (!engines) return fail(e, SKG_ERR_ARGUMENT, "skg_autotune: guided sampling needs the pipeline engine");
+    if (!e->hInst.empty()) engines &= 1;
+    if (!engines) return fail(e, SKG_ERR_ARGUMENT, "skg_autotune: no engine supports this configuration");
+    if (!e->hInst.empty() && e->instTotal < trial_packets + trial_packets / 4)
+        return fail(e, SKG_ERR_ARGUMENT, "skg_autotune: fewer instance histories than one trial");
     int st = SKG_OK;
     // keep the caller's detector contents and counters
     double* saved = nullptr;
@@ -2136,6 +2230,7 @@ int skg_autotune(skg_engine* e, uint64_t first_history, uint64_t trial_packets, 
         e->traceBlocks = c.traceWgs;
         e->graphIters = c.iters;
         // one warm-up run (pool allocation, graph capture), then the timed trial
+        if (!e->hInst.empty() && h + trial_packets / 4 + trial_packets > e->instTotal) h = 0;
         if ((st = skg_run_packets(e, h, trial_packets / 4))) break;
         h += trial_packets / 4;
         auto t0 = std::chrono::steady_clock::now();

@@ -447,6 +447,54 @@ int skg_sample_density_octree(const skg_octree* t, const skg_disk_model* model, 
     return SKG_OK;
 }
 
+// tetrahedral cells from their vertex tables: num_samples points per tetra drawn uniformly by
+// the folding method of Rocchini & Cignoni (J. Graphics Tools 5(4), 2000): a point of the unit
+// cube is folded into the corner simplex s + t + u <= 1 and used as barycentric weights
+int skg_sample_density_tetra(int64_t ntets, const double* verts, const int32_t* tet_verts, const skg_disk_model* model,
+                             int32_t num_samples, uint64_t seed, int32_t threads, double* out)
+{
+    if (ntets < 0 || (ntets > 0 && (!verts || !tet_verts || !out)) || !model || num_samples < 1) return SKG_ERR_ARGUMENT;
+    Model md(*model);
+    parallelFor(ntets, defaultThreads(threads), [&](int64_t b, int64_t e) {
+        for (int64_t m = b; m < e; ++m)
+        {
+            const double* v[4];
+            for (int q = 0; q < 4; ++q) v[q] = verts + 3 * static_cast<int64_t>(tet_verts[4 * m + q]);
+            skg::Philox rng;
+            rng.init(seed, static_cast<uint64_t>(m), kTagDensity);
+            double sum = 0.;
+            for (int i = 0; i < num_samples; ++i)
+            {
+                double s = rng.uniform(), t = rng.uniform(), u = rng.uniform();
+                if (s + t > 1.)
+                {
+                    s = 1. - s;
+                    t = 1. - t;
+                }
+                if (t + u > 1.)
+                {
+                    double tt = u;
+                    u = 1. - s - t;
+                    t = 1. - tt;
+                }
+                else if (s + t + u > 1.)
+                {
+                    double tt = u;
+                    u = s + t + u - 1.;
+                    s = 1. - t - tt;
+                }
+                double a = 1. - s - t - u;
+                double x = a * v[0][0] + s * v[1][0] + t * v[2][0] + u * v[3][0];
+                double y = a * v[0][1] + s * v[1][1] + t * v[2][1] + u * v[3][1];
+                double z = a * v[0][2] + s * v[1][2] + t * v[2][2] + u * v[3][2];
+                sum += md.geom(x, y, z);
+            }
+            out[m] = md.nnorm * sum / num_samples;
+        }
+    });
+    return SKG_OK;
+}
+
 // TabulatedDustMix::getOpticalProperties + NR::clampedResample (TabulatedDustMix.cpp:12-43,
 // NR.hpp:338-434): sigma_abs = mu kappa (1-a) and sigma_sca = mu kappa a resampled log-log,
 // g resampled log-lin, values clamped beyond the table; sigma_ext = sigma_abs + sigma_sca.

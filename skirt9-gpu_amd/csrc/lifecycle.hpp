@@ -148,5 +148,14 @@ namespace skg
         }
     };
 
-    constexpr int kMaxFrames = 8;
+    constexpr int kMaxFrames = 1024;
+
+    // one instance of a batched launch: its histories are [offset, next offset) of the launch
+    struct Instance
+    {
+        uint64_t offset;
+        uint64_t seed;
+        double scale;  // optical-depth scale: sections multiplied by it
+        int32_t fbeg, fend;
+    };
 }

@@ -194,6 +194,17 @@ namespace skg
         const Instance* inst;     // batched instances (ninst > 0) or none
         int ninst;
         const double* ilpack;     // launch luminosities per instance (ninst x nlambda)
+        GuideNet net;             // guided variant: the network, its input frame and the mixture
+        GuideFrame gframe;
+        double guideMix;
+    };
+
+    // kernel variants of the life cycle
+    enum Variant : int
+    {
+        kPlain = 0,      // one run
+        kBatched = 1,    // batched independent instances
+        kGuided = 2      // scattering directions from the defensive guide / HG mixture
     };
 
     enum Phase : int
@@ -202,7 +213,8 @@ namespace skg
         kPeel = 1,
         kFwd1 = 2,
         kFwd2 = 3,
-        kDone = 4
+        kDone = 4,
+        kScatter = 5  // guided variant: peel-offs done, waiting for the wave's guided scattering
     };
 
     constexpr int kMaxScatterings = 1 << 20;  // guard against non-terminating histories (NaN weights)
@@ -395,8 +407,9 @@ namespace skg
     };
 
     // The event transitions of the life cycle, on a packet loaded from LDS.
-    template<int GT, int FP, bool INST> struct Events
+    template<int GT, int FP, int V> struct Events
     {
+        static constexpr bool INST = V == kBatched;
         const SimParams& p;
         Trav<GT, FP>& T;
         Packet& pk;
@@ -517,10 +530,45 @@ namespace skg
             // all instruments done: scatter (after a scattering peel-off) and start a forced path
             if (pk.scat)
             {
+                if constexpr (V == kGuided)
+                {
+                    // the wave's guided scattering round follows (guided_scatter)
+                    T.phase = kScatter;
+                    T.ended = false;
+                    return;
+                }
                 scatter_hg(pk.rng, p.gpar[pk.ell], pk.kx, pk.ky, pk.kz);
                 pk.nscatt++;
                 atomicAdd(lds + 3, 1u);
             }
+            atomicAdd(lds + 2, 1u);
+            T.start(pk, pk.kx, pk.ky, pk.kz, kFwd1);
+        }
+
+        // Guided scattering with the network outputs o of this packet's event: a direction from
+        // the defensive mixture (1 - mix) HG + mix guide, and the weight factor
+        // Phi_HG / p_mix that keeps the estimator unbiased.  The termination threshold takes the
+        // same factor, so the rule (MonteCarloSimulation.cpp:643-660) applies to the weight
+        // without the guide's factors (packets the guide down-weights are not cut early).
+        __device__ __forceinline__ void guided_scatter(const float* o)
+        {
+            Lobes Lb;
+            guide_lobes(o, Lb);
+            const double g = p.gpar[pk.ell];
+            const double kxo = pk.kx, kyo = pk.ky, kzo = pk.kz;
+            const bool useGuide = pk.rng.uniform() < p.guideMix;
+            if (useGuide)
+                guide_sample(Lb, pk.rng, pk.kx, pk.ky, pk.kz);
+            else
+                scatter_hg(pk.rng, g, pk.kx, pk.ky, pk.kz);
+            const double c = kxo * pk.kx + kyo * pk.ky + kzo * pk.kz;
+            const double phi = (fabs(g) < 1e-6 ? 1.0 : value_hg(g, c)) / (4 * M_PI);
+            const double q = guide_pdf(Lb, pk.kx, pk.ky, pk.kz);
+            const double ratio = phi / ((1.0 - p.guideMix) * phi + p.guideMix * q);
+            pk.W *= ratio;
+            pk.Lthr *= ratio;
+            pk.nscatt++;
+            atomicAdd(lds + 3, 1u);
             atomicAdd(lds + 2, 1u);
             T.start(pk, pk.kx, pk.ky, pk.kz, kFwd1);
         }
@@ -602,9 +650,10 @@ namespace skg
 #ifndef SKG_LIFECYCLE_WAVES
 #define SKG_LIFECYCLE_WAVES 2
 #endif
-    template<int GT, int FP, bool INST>
+    template<int GT, int FP, int V>
     __global__ void __launch_bounds__(kLcThreads, SKG_LIFECYCLE_WAVES) lifecycle(SimParams p)
     {
+        constexpr bool INST = V == kBatched;
         __shared__ unsigned int wgcnt[8];
         __shared__ double coldD[kColdF64 * kLcThreads];
         __shared__ uint32_t coldU[kColdU32 * kLcThreads];
@@ -631,8 +680,31 @@ namespace skg
                 {
                     Packet pk;
                     pk.load<INST>(coldD, coldU, t, p);
-                    Events<GT, FP, INST>{p, T, pk, wgcnt}.transition();
+                    Events<GT, FP, V>{p, T, pk, wgcnt}.transition();
                     pk.store(coldD, coldU, t);
+                }
+                if constexpr (V == kGuided)
+                {
+                    // one network evaluation for every lane of the wave that waits to scatter
+                    const bool sc = T.phase == kScatter;
+                    if (__ballot(sc))
+                    {
+                        Packet pk;
+                        float feat[kGuideIn];
+                        for (int q = 0; q < kGuideIn; ++q) feat[q] = 0.f;
+                        if (sc)
+                        {
+                            pk.load<false>(coldD, coldU, t, p);
+                            guide_features(p.gframe, pk.rx, pk.ry, pk.rz, pk.kx, pk.ky, pk.kz, pk.lam, feat);
+                        }
+                        float o[kGuideOut];
+                        guide_wave_infer_regs(p.net, feat, sc, o);
+                        if (sc)
+                        {
+                            Events<GT, FP, V>{p, T, pk, wgcnt}.guided_scatter(o);
+                            pk.store(coldD, coldU, t);
+                        }
+                    }
                 }
                 unsigned long long need = __ballot(T.phase == kFetch);
                 if (need)
@@ -649,7 +721,7 @@ namespace skg
                         if (mine < p.count)
                         {
                             Packet pk;
-                            Events<GT, FP, INST>{p, T, pk, wgcnt}.launch(p.first + mine);
+                            Events<GT, FP, V>{p, T, pk, wgcnt}.launch(p.first + mine);
                             pk.store(coldD, coldU, t);
                         }
                         else
@@ -889,13 +961,17 @@ namespace
         return SKG_OK;
     }
 
-    template<int GT, int FP> hipError_t launchLifecycle(const SimParams& p, int blocks, int threads, hipStream_t s)
+    template<int GT, int FP>
+    hipError_t launchLifecycle(const SimParams& p, bool guided, int blocks, int threads, hipStream_t s)
     {
-        // batched instances take their own instantiation; the single run carries no instance code
+        // batched instances and guided sampling take their own instantiations; the plain run
+        // carries neither
         if (p.ninst)
-            hipLaunchKernelGGL((lifecycle<GT, FP, true>), dim3(blocks), dim3(threads), 0, s, p);
+            hipLaunchKernelGGL((lifecycle<GT, FP, kBatched>), dim3(blocks), dim3(threads), 0, s, p);
+        else if (guided)
+            hipLaunchKernelGGL((lifecycle<GT, FP, kGuided>), dim3(blocks), dim3(threads), 0, s, p);
         else
-            hipLaunchKernelGGL((lifecycle<GT, FP, false>), dim3(blocks), dim3(threads), 0, s, p);
+            hipLaunchKernelGGL((lifecycle<GT, FP, kPlain>), dim3(blocks), dim3(threads), 0, s, p);
         return hipGetLastError();
     }
 }
@@ -1614,6 +1690,13 @@ static int runPersistent(skg_engine* e, uint64_t first_history, uint64_t count)
     p.ninst = static_cast<int>(e->hInst.size());
     p.inst = e->inst;
     p.ilpack = e->ilpack;
+    p.net = e->net;
+    p.gframe = e->gframe;
+    p.guideMix = e->guideMix;
+    if (e->guideMode == 2)
+        return fail(e, SKG_ERR_STATE, "skg_run_packets: training records are collected by the pipeline engine");
+    const bool guided = e->guideMode == 1;
+    if (guided && !e->gTheta) return fail(e, SKG_ERR_STATE, "guided run without guide parameters");
     HIPCHK(e, hipMemsetAsync(e->queue, 0, sizeof(unsigned long long), e->stream));
     (void)hipGetLastError();
     int threads = e->threads;
@@ -1625,8 +1708,8 @@ static int runPersistent(skg_engine* e, uint64_t first_history, uint64_t count)
     hipError_t err;
     dispatchGrid(e, [&](auto gt) {
         constexpr int GT = decltype(gt)::value;
-        err = e->fpMode == SKG_FP64 ? launchLifecycle<GT, SKG_FP64>(p, blocks, threads, e->stream)
-                                    : launchLifecycle<GT, SKG_FP32_KAHAN>(p, blocks, threads, e->stream);
+        err = e->fpMode == SKG_FP64 ? launchLifecycle<GT, SKG_FP64>(p, guided, blocks, threads, e->stream)
+                                    : launchLifecycle<GT, SKG_FP32_KAHAN>(p, guided, blocks, threads, e->stream);
         return 0;
     });
     if (err != hipSuccess) return fail(e, SKG_ERR_DEVICE, std::string("lifecycle launch: ") + hipGetErrorString(err));
@@ -2185,8 +2268,7 @@ int skg_autotune(skg_engine* e, uint64_t first_history, uint64_t trial_packets, 
     if (!e || trial_packets < 1024 || !(engines >= 1 && engines <= 3))
         return fail(e, SKG_ERR_ARGUMENT, "skg_autotune: bad arguments");
     if (e->guideMode == 2) return fail(e, SKG_ERR_STATE, "skg_autotune: not in guide collect mode");
-    // guided sampling runs in the pipeline engine only, batched instances in the persistent one
-    if (e->guideMode == 1) engines &= 2;
+    // batched instances run in the persistent engine only
     if (!e->hInst.empty()) engines &= 1;
     if (!engines) return fail(e, SKG_ERR_ARGUMENT, "skg_autotune: no engine supports this configuration");
     if (!e->hInst.empty() && e->instTotal < trial_packets + trial_packets / 4)

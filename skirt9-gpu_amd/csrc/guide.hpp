@@ -191,6 +191,7 @@ namespace skg
         double u = rng.uniform();
         int j = kGuideLobes - 1;
         double cum = 0.;
+#pragma unroll
         for (int q = 0; q < kGuideLobes; ++q)
         {
             cum += L.w[q];
@@ -200,12 +201,21 @@ namespace skg
                 break;
             }
         }
-        double kap = L.kap[j];
+        // the chosen lobe's parameters by selection (no dynamically indexed registers)
+        double kap = L.kap[0], mx = L.mu[0][0], my = L.mu[0][1], mz = L.mu[0][2];
+#pragma unroll
+        for (int q = 1; q < kGuideLobes; ++q)
+            if (q == j)
+            {
+                kap = L.kap[q];
+                mx = L.mu[q][0];
+                my = L.mu[q][1];
+                mz = L.mu[q][2];
+            }
         double w = 1. + log(rng.uniform() * (-expm1(-2 * kap)) + exp(-2 * kap)) / kap;
         w = fmin(1., fmax(-1., w));
         double phi = 2 * M_PI * rng.uniform();
         double s = sqrt(fmax(0., 1. - w * w));
-        double mx = L.mu[j][0], my = L.mu[j][1], mz = L.mu[j][2];
         // orthonormal frame around mu
         double ax = fabs(mz) < 0.9 ? 0. : 1., az = fabs(mz) < 0.9 ? 1. : 0.;
         double e1x = my * az, e1y = mz * ax - mx * az, e1z = -my * ax;
@@ -268,6 +278,51 @@ namespace skg
         }
         __syncthreads();
         for (int q = 0; q < kGuideOut; ++q) out40[q] = lds_out[lane * 48 + q];
+    }
+
+    // Wave-level inference held in registers: every lane supplies its 9 features and, where
+    // `want` is set, receives its 40 raw outputs.  Inputs are gathered into the B-operand
+    // layout and outputs returned to their lanes with cross-lane permutes (ds_bpermute), so
+    // no LDS is allocated and no workgroup barrier is needed; 16-lane tiles without a wanting
+    // lane are skipped.  Every lane of the wavefront must call it (converged control flow).
+    __device__ __forceinline__ void guide_wave_infer_regs(const GuideNet& net, const float* f, bool want, float* out40)
+    {
+        const int lane = __lane_id();
+        const int row = lane & 15, grp = lane >> 4;
+        const unsigned long long need = __ballot(want);
+        for (int t = 0; t < 4; ++t)
+        {
+            if (!((need >> (16 * t)) & 0xFFFFull)) continue;
+            // features 4*grp .. 4*grp+3 of sample t*16 + row (zero beyond the 9 features)
+            const int src = t * 16 + row;
+            float xs[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < kGuideIn; ++k)
+            {
+                float v = __shfl(f[k], src);
+                if (grp == (k >> 2)) xs[k & 3] = v;
+            }
+            bf16x4 x;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) x[q] = static_cast<short>(f2bf(xs[q]));
+            f32x4 o[3];
+            guide_tile_forward(net, x, o, nullptr, nullptr, nullptr);
+            // output 16*ot + 4*g + q of sample t*16 + r is o[ot][q] of lane r + 16*g
+#pragma unroll
+            for (int ot = 0; ot < 3; ++ot)
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                    {
+                        const int j = 16 * ot + 4 * g + q;
+                        if (j < kGuideOut)
+                        {
+                            float v = __shfl(o[ot][q], row + 16 * g);
+                            if (grp == t) out40[j] = v;
+                        }
+                    }
+        }
     }
 
     // ---------------------------------------------------------------- kernels

@@ -168,6 +168,7 @@ namespace skg
         unsigned peelMask = 0;
         bool peelScat = false;
         double kxo = 0, kyo = 0, kzo = 0;
+        double Wpeel = 0.;  // weight of this event's peel-offs (before the guide's factor)
         bool terminated = false;
         unsigned cntHist = 0, cntScat = 0, cntFwd2 = 0;
 
@@ -256,6 +257,7 @@ namespace skg
                     peelMask |= 1u << f;
                 }
                 peelScat = true;
+                Wpeel = W;
                 kxo = kx;
                 kyo = ky;
                 kzo = kz;
@@ -421,9 +423,11 @@ namespace skg
                 double Wp = W;
                 if (peelScat)
                 {
+                    // the peel-off belongs to the event, before the new direction is drawn: the
+                    // guide's factor of that direction applies to the continuing packet only
                     const FrameGeom& fg = p.frames[f];
                     double c = kxo * fg.kobs[0] + kyo * fg.kobs[1] + kzo * fg.kobs[2];
-                    Wp = W * peel_phase(p.gpar[ell], c);
+                    Wp = Wpeel * peel_phase(p.gpar[ell], c);
                 }
                 if (o < static_cast<unsigned long long>(p.qcap))
                 {

@@ -11,6 +11,10 @@ mix, forced scattering with path-length bias 0.5, distant frame instruments).
                    Gaussian random field with power spectrum P(k) ~ k^-3.7 on a lattice,
                    sigma_ln = 1, mean tau_V along z; 0.55 um, albedo 0.6, g 0.6; three
                    200x200 frames at 0, 45 and 90 deg
+  tetra_batch      256 independent variants of cartesian_disk in one launch, on a Delaunay
+                   tetrahedral mesh of 2e5 points distributed as the disk density
+                   (~1.3e6 tetrahedra): per instance tau_V log-uniform in 0.1..50,
+                   inclination uniform in 0..90 deg, 1e5..1e7 packets (log-uniform)
 """
 
 from __future__ import annotations
@@ -43,11 +47,12 @@ class Workload:
     frames: list = field(default_factory=list)
     tau_v: float = 1.0
     setup_s: float = 0.0
+    instances: dict = None  # batched variants: packets, seeds, tau_scale, frame per instance
 
     def apply(self, eng, npk_norm=1.0):
         if self.grid == "cartesian":
             eng.grid_cartesian(self.tables["xb"], self.tables["yb"], self.tables["zb"])
-        elif self.grid == "voronoi":
+        elif self.grid in ("voronoi", "tetra"):
             self.tables["mesh"].upload(eng)
         else:
             self.octree.upload(eng)
@@ -57,6 +62,16 @@ class Workload:
         for f in self.frames:
             eng.frame_add(f["inclination"], f["azimuth"], f["roll"], f["nx"], f["ny"], f["fovx"], f["fovy"])
         eng.source_point((0.0, 0.0, 0.0), np.full(len(self.lam), 1.0 / npk_norm))
+        if self.instances is not None:
+            I = self.instances
+            n = len(I["packets"])
+            lp = np.repeat((1.0 / np.maximum(I["packets"], 1))[:, None], len(self.lam), axis=1)
+            eng.instances(I["packets"], I["seeds"], I["tau_scale"], lp, np.arange(n), np.ones(n))
+        else:
+            eng.instances([])
+
+    def total_packets(self):
+        return int(np.sum(self.instances["packets"])) if self.instances is not None else 0
 
 
 def frame(inc_deg, npix=200, fov=20 * PC, az_deg=0.0):
@@ -144,3 +159,56 @@ def voronoi_clumpy(num_sites=1_000_000, lattice=256, tau_v=5.0, sigma_ln=1.0, sl
                     np.array([0.6]), n0 * mean_rho, dict(mesh=mesh), tau_v=tau_v,
                     frames=frames if frames is not None else [frame(0.0), frame(45.0), frame(90.0)],
                     setup_s=time.time() - t0)
+
+
+def disk_points(n, box, hR=3 * PC, hz=0.3 * PC, seed=1):
+    """n positions distributed as the exponential disk: the distributions of
+    SepAxGeometry::generatePosition (SepAxGeometry.cpp:11-19) with ExpDiskGeometry's
+    randomCylRadius / randomZ (ExpDiskGeometry.cpp:46-68) -- cylindrical radius with density
+    R exp(-R/hR), uniform azimuth, Laplace z -- drawn with numpy's samplers; positions outside
+    the box are discarded and redrawn, as TetraMeshSpatialGrid's vertex sampling does
+    (TetraMeshSpatialGrid.cpp:23-40)."""
+    rng = np.random.default_rng(seed)
+    box = np.asarray(box, dtype=np.float64)
+    out = np.empty((0, 3))
+    while len(out) < n:
+        m = 2 * (n - len(out)) + 1024
+        R = rng.gamma(2.0, hR, size=m)
+        phi = 2 * np.pi * rng.random(m)
+        z = rng.exponential(hz, size=m) * np.where(rng.random(m) < 0.5, -1.0, 1.0)
+        p = np.stack([R * np.cos(phi), R * np.sin(phi), z], axis=1)
+        p = p[np.all((p > box[:3]) & (p < box[3:]), axis=1)]
+        out = np.concatenate([out, p])
+    return out[:n]
+
+
+def batch_instances(n=256, packet_range=(1e5, 1e7), tau_range=(0.1, 50.0), inc_range_deg=(0.0, 90.0), seed=3,
+                    packet_scale=1.0):
+    """Per-instance parameters of the batched workload from a fixed seed: tau_V log-uniform,
+    inclination uniform, packet count log-uniform (times packet_scale, at least 1)."""
+    rng = np.random.default_rng(seed)
+    tau = np.exp(rng.uniform(np.log(tau_range[0]), np.log(tau_range[1]), n))
+    inc = rng.uniform(inc_range_deg[0], inc_range_deg[1], n)
+    npk = np.exp(rng.uniform(np.log(packet_range[0]), np.log(packet_range[1]), n))
+    npk = np.maximum(1, np.round(npk * packet_scale)).astype(np.uint64)
+    seeds = (1000 + np.arange(n)).astype(np.uint64)
+    return dict(packets=npk, seeds=seeds, tau_scale=tau, inclination_deg=inc)
+
+
+def tetra_batch(num_points=200_000, num_instances=256, packet_scale=1.0, npix=200, samples=100, seed=1, threads=0):
+    import time
+    pkg = _pkg()
+    t0 = time.time()
+    h = 10 * PC
+    box = np.array([-h, -h, -h, h, h, h])
+    mesh = pkg.Tetra.build(disk_points(num_points, box, seed=seed), box)
+    ex = mesh.export()
+    sext = MU_DUST * 1.0
+    # the disk normalised to face-on tau_V = 1; instance i scales the optical depths to its tau_V
+    model = pkg.disk_model(tau_v=1.0, sext_v=sext)
+    dens = pkg.sample_density_tetra(ex["verts"], ex["tet_verts"], model, samples, seed + 1, threads)
+    inst = batch_instances(num_instances, seed=seed + 2, packet_scale=packet_scale)
+    frames = [frame(i, npix=npix) for i in inst["inclination_deg"]]
+    return Workload("tetra_batch", "tetra", np.array([0.55e-6]), np.array([sext]), np.array([0.6 * sext]),
+                    np.array([0.6]), dens, dict(mesh=mesh, ntets=len(ex["tet_verts"])), frames=frames, tau_v=1.0,
+                    instances=inst, setup_s=time.time() - t0)

@@ -105,8 +105,9 @@ def test_guide_trains_on_collected_records(eng):
     assert l1 < l0 - 0.05, (l0, l1)
 
 
+@pytest.mark.parametrize("engine", [0, 1], ids=["pipeline", "persistent"])
 @pytest.mark.parametrize("mwr", [1e4, 10.0], ids=["mwr1e4", "mwr10"])
-def test_guided_estimator_is_unbiased(eng, mwr):
+def test_guided_estimator_is_unbiased(eng, mwr, engine):
     """Guided sampling (defensive mixture, weight Phi_HG / p_mix) reproduces the plain total
     flux within Monte Carlo error, with an untrained (arbitrary) guide.  With an aggressive
     termination threshold (minWeightReduction 10) the rule must be applied to the weight
@@ -122,7 +123,7 @@ def test_guided_estimator_is_unbiased(eng, mwr):
     nb, per = 24, 100000
     eng.source_point((0.0, 0.0, 0.0), [1.0 / per])
     eng.options(0.5, mwr, 0, 0, 9)
-    eng.tune_pipeline(0, 1 << 18, 0, 8)
+    eng.tune_pipeline(engine, 1 << 18, 0, 8)
     eng.guide_params(G.flatten(_params(11)).astype(np.float32))
     totals = {}
     for mode in (0, 1):
@@ -130,7 +131,7 @@ def test_guided_estimator_is_unbiased(eng, mwr):
         t = []
         for bi in range(nb):
             eng.frame_clear()
-            print("case guided unbiased", mode, bi, flush=True)
+            print("case guided unbiased", engine, mode, bi, flush=True)
             eng.run_packets(bi * per, per)
             t.append(eng.frame_read(0).sum())
         totals[mode] = np.array(t)
@@ -139,3 +140,39 @@ def test_guided_estimator_is_unbiased(eng, mwr):
     e0 = totals[0].std(ddof=1) / np.sqrt(nb)
     e1 = totals[1].std(ddof=1) / np.sqrt(nb)
     assert abs(m1 - m0) < 4 * np.hypot(e0, e1), (m0, m1, e0, e1)
+
+
+def test_guided_persistent_matches_pipeline(eng):
+    """The persistent engine's guided scattering (network evaluated in registers with
+    cross-lane permutes) against the pipeline engine's (LDS staging) on the same histories:
+    same random streams, same BF16 MFMA arithmetic, so the same images up to the order of
+    the FP64 detector atomics."""
+    b, dens = disk_density_cartesian(n=32, samples=10, seed=9, tau_v=5.0)
+    eng.grid_cartesian(b, b, b)
+    eng.medium(dens)
+    s = 1.5e-29
+    eng.dust([0.55e-6], [s], [0.6 * s], [0.6])
+    eng.frame_remove_all()
+    fr = frame60()
+    eng.frame_add(fr["inclination"], fr["azimuth"], fr["roll"], 50, 50, fr["fovx"], fr["fovy"])
+    npk = 50000
+    eng.source_point((0.0, 0.0, 0.0), [1.0 / npk])
+    eng.options(0.5, 1e4, 0, 0, 17)
+    eng.guide_params(G.flatten(_params(11)).astype(np.float32))
+    eng.guide_mode(1, mix=0.5)
+    imgs, cnts = [], []
+    for engine in (0, 1):
+        eng.tune_pipeline(engine, 1 << 18, 0, 8)
+        eng.frame_clear()
+        print("case guided engines", engine, flush=True)
+        eng.run_packets(0, npk)
+        imgs.append(eng.frame_read(0))
+        cnts.append(eng.counters())
+    eng.guide_mode(0)
+    assert cnts[0][0] == cnts[1][0] == npk
+    assert cnts[0][3] == cnts[1][3]  # the same scattering events
+    a, c = imgs
+    assert abs(c.sum() / a.sum() - 1) < 1e-9
+    nz = a > 0
+    np.testing.assert_array_equal(nz, c > 0)
+    assert np.max(np.abs(c[nz] - a[nz]) / a[nz]) < 1e-9

@@ -217,6 +217,7 @@ namespace skg
         kScatter = 5  // guided variant: peel-offs done, waiting for the wave's guided scattering
     };
 
+    constexpr int kGuideLanes = 16;            // guided variant: lanes per network evaluation round
     constexpr int kMaxScatterings = 1 << 20;  // guard against non-terminating histories (NaN weights)
     constexpr int kMaxSteps = 1 << 26;        // guard against non-terminating traversals
 
@@ -552,18 +553,18 @@ namespace skg
         // without the guide's factors (packets the guide down-weights are not cut early).
         __device__ __forceinline__ void guided_scatter(const float* o)
         {
-            Lobes Lb;
-            guide_lobes(o, Lb);
+            LobesF Lb;
+            guide_lobes_f(o, Lb);
             const double g = p.gpar[pk.ell];
             const double kxo = pk.kx, kyo = pk.ky, kzo = pk.kz;
             const bool useGuide = pk.rng.uniform() < p.guideMix;
             if (useGuide)
-                guide_sample(Lb, pk.rng, pk.kx, pk.ky, pk.kz);
+                guide_sample_f(Lb, pk.rng, pk.kx, pk.ky, pk.kz);
             else
                 scatter_hg(pk.rng, g, pk.kx, pk.ky, pk.kz);
             const double c = kxo * pk.kx + kyo * pk.ky + kzo * pk.kz;
             const double phi = (fabs(g) < 1e-6 ? 1.0 : value_hg(g, c)) / (4 * M_PI);
-            const double q = guide_pdf(Lb, pk.kx, pk.ky, pk.kz);
+            const double q = guide_pdf_f(Lb, pk.kx, pk.ky, pk.kz);
             const double ratio = phi / ((1.0 - p.guideMix) * phi + p.guideMix * q);
             pk.W *= ratio;
             pk.Lthr *= ratio;
@@ -671,7 +672,7 @@ namespace skg
             if (busy) T.step(p.grid, wgcnt);
 
             // wave-wide transition round once enough lanes wait
-            unsigned long long waiting = __ballot(T.ended || T.phase == kFetch);
+            unsigned long long waiting = __ballot(T.ended || T.phase == kFetch || (V == kGuided && T.phase == kScatter));
             unsigned long long active = __ballot(T.phase != kDone);
             if (active == 0) break;
             if (__popcll(waiting) >= p.threshold || waiting == active)
@@ -685,9 +686,13 @@ namespace skg
                 }
                 if constexpr (V == kGuided)
                 {
-                    // one network evaluation for every lane of the wave that waits to scatter
+                    // one network evaluation for the lanes of the wave that wait to scatter, once
+                    // there are kGuideLanes of them or no lane of the wave is still traversing
                     const bool sc = T.phase == kScatter;
-                    if (__ballot(sc))
+                    const unsigned long long scm = __ballot(sc);
+                    const unsigned long long trav =
+                        __ballot((T.phase == kPeel || T.phase == kFwd1 || T.phase == kFwd2) && !T.ended);
+                    if (scm && (__popcll(scm) >= kGuideLanes || trav == 0))
                     {
                         Packet pk;
                         float feat[kGuideIn];

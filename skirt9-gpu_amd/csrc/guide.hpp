@@ -172,7 +172,7 @@ namespace skg
         for (int j = 0; j < kGuideLobes; ++j) L.w[j] /= s;
     }
 
-    // mixture density per steradian at unit vector k
+    // mixture density per steradian at unit vector k (FP64, for the training loss)
     __device__ __forceinline__ double guide_pdf(const Lobes& L, double kx, double ky, double kz)
     {
         double q = 0.;
@@ -185,53 +185,104 @@ namespace skg
         return q;
     }
 
-    // samples the mixture: lobe by weight, then the 3-D vMF by w = 1 + ln(u + (1-u) e^{-2 kappa}) / kappa
-    template<typename Rng> __device__ __forceinline__ void guide_sample(const Lobes& L, Rng& rng, double& kx, double& ky, double& kz)
+    // The sampler inside the life cycle works in FP32: lobe parameters, mixture density and
+    // direction sampling.  The weight factor uses the density of this same sampler, so the
+    // estimator stays unbiased up to the FP32 rounding of that density (~1e-7 relative); the
+    // sampled direction is renormalised in FP64.
+    struct LobesF
     {
-        double u = rng.uniform();
-        int j = kGuideLobes - 1;
-        double cum = 0.;
+        float mu[kGuideLobes][3];
+        float kap[kGuideLobes];
+        float w[kGuideLobes];
+        float norm[kGuideLobes];  // kappa / (2 pi (1 - exp(-2 kappa))), the vMF normalisation
+    };
+
+    __device__ __forceinline__ void guide_lobes_f(const float* o, LobesF& L)
+    {
+        float mx = -3.0e38f;
 #pragma unroll
-        for (int q = 0; q < kGuideLobes; ++q)
+        for (int j = 0; j < kGuideLobes; ++j) mx = fmaxf(mx, o[5 * j + 4]);
+        float s = 0.f;
+#pragma unroll
+        for (int j = 0; j < kGuideLobes; ++j)
+        {
+            const float a = o[5 * j], b = o[5 * j + 1], c = o[5 * j + 2];
+            float n = sqrtf(a * a + b * b + c * c);
+            n = n > 1e-12f ? n : 1e-12f;
+            L.mu[j][0] = a / n;
+            L.mu[j][1] = b / n;
+            L.mu[j][2] = c / n;
+            const float r = o[5 * j + 3];
+            const float sp = r > 20.f ? r : log1pf(expf(r));
+            const float kap = fminf(static_cast<float>(kGuideKMin) + sp, static_cast<float>(kGuideKMax));
+            L.kap[j] = kap;
+            L.norm[j] = kap / (2.f * static_cast<float>(M_PI) * (-expm1f(-2.f * kap)));
+            L.w[j] = expf(o[5 * j + 4] - mx);
+            s += L.w[j];
+        }
+        const float inv = 1.f / s;
+#pragma unroll
+        for (int j = 0; j < kGuideLobes; ++j) L.w[j] *= inv;
+    }
+
+    // mixture density per steradian at unit vector k
+    __device__ __forceinline__ double guide_pdf_f(const LobesF& L, double kx, double ky, double kz)
+    {
+        const float x = static_cast<float>(kx), y = static_cast<float>(ky), z = static_cast<float>(kz);
+        float q = 0.f;
+#pragma unroll
+        for (int j = 0; j < kGuideLobes; ++j)
+        {
+            const float c = L.mu[j][0] * x + L.mu[j][1] * y + L.mu[j][2] * z;
+            q += L.w[j] * L.norm[j] * expf(L.kap[j] * (c - 1.f));
+        }
+        return static_cast<double>(q);
+    }
+
+    // samples the mixture: lobe by weight, then the 3-D vMF by w = 1 + ln(u + (1-u) e^{-2 kappa}) / kappa
+    template<typename Rng> __device__ __forceinline__ void guide_sample_f(const LobesF& L, Rng& rng, double& kx, double& ky, double& kz)
+    {
+        const float u = static_cast<float>(rng.uniform());
+        // the chosen lobe's parameters by selection (no dynamically indexed registers)
+        float kap = L.kap[kGuideLobes - 1], mx = L.mu[kGuideLobes - 1][0], my = L.mu[kGuideLobes - 1][1],
+              mz = L.mu[kGuideLobes - 1][2];
+        float cum = 0.f;
+        bool found = false;
+#pragma unroll
+        for (int q = 0; q < kGuideLobes - 1; ++q)
         {
             cum += L.w[q];
-            if (u < cum)
+            if (!found && u < cum)
             {
-                j = q;
-                break;
-            }
-        }
-        // the chosen lobe's parameters by selection (no dynamically indexed registers)
-        double kap = L.kap[0], mx = L.mu[0][0], my = L.mu[0][1], mz = L.mu[0][2];
-#pragma unroll
-        for (int q = 1; q < kGuideLobes; ++q)
-            if (q == j)
-            {
+                found = true;
                 kap = L.kap[q];
                 mx = L.mu[q][0];
                 my = L.mu[q][1];
                 mz = L.mu[q][2];
             }
-        double w = 1. + log(rng.uniform() * (-expm1(-2 * kap)) + exp(-2 * kap)) / kap;
-        w = fmin(1., fmax(-1., w));
-        double phi = 2 * M_PI * rng.uniform();
-        double s = sqrt(fmax(0., 1. - w * w));
+        }
+        const float e = expf(-2.f * kap);
+        float w = 1.f + logf(static_cast<float>(rng.uniform()) * (1.f - e) + e) / kap;
+        w = fminf(1.f, fmaxf(-1.f, w));
+        const float phi = 2.f * static_cast<float>(M_PI) * static_cast<float>(rng.uniform());
+        const float s = sqrtf(fmaxf(0.f, 1.f - w * w));
         // orthonormal frame around mu
-        double ax = fabs(mz) < 0.9 ? 0. : 1., az = fabs(mz) < 0.9 ? 1. : 0.;
-        double e1x = my * az, e1y = mz * ax - mx * az, e1z = -my * ax;
-        double n1 = sqrt(e1x * e1x + e1y * e1y + e1z * e1z);
-        e1x /= n1;
-        e1y /= n1;
-        e1z /= n1;
-        double e2x = my * e1z - mz * e1y, e2y = mz * e1x - mx * e1z, e2z = mx * e1y - my * e1x;
-        double cp = cos(phi), sp = sin(phi);
-        kx = w * mx + s * (cp * e1x + sp * e2x);
-        ky = w * my + s * (cp * e1y + sp * e2y);
-        kz = w * mz + s * (cp * e1z + sp * e2z);
-        double n = sqrt(kx * kx + ky * ky + kz * kz);
-        kx /= n;
-        ky /= n;
-        kz /= n;
+        const float ax = fabsf(mz) < 0.9f ? 0.f : 1.f, az = fabsf(mz) < 0.9f ? 1.f : 0.f;
+        float e1x = my * az, e1y = mz * ax - mx * az, e1z = -my * ax;
+        const float n1 = 1.f / sqrtf(e1x * e1x + e1y * e1y + e1z * e1z);
+        e1x *= n1;
+        e1y *= n1;
+        e1z *= n1;
+        const float e2x = my * e1z - mz * e1y, e2y = mz * e1x - mx * e1z, e2z = mx * e1y - my * e1x;
+        float sp, cp;
+        sincosf(phi, &sp, &cp);
+        const double dx = w * mx + s * (cp * e1x + sp * e2x);
+        const double dy = w * my + s * (cp * e1y + sp * e2y);
+        const double dz = w * mz + s * (cp * e1z + sp * e2z);
+        const double n = sqrt(dx * dx + dy * dy + dz * dz);
+        kx = dx / n;
+        ky = dy / n;
+        kz = dz / n;
     }
 
     // guide features of a scattering event (9 values): position / half-size, incoming

@@ -283,16 +283,16 @@ namespace skg
             guide_wave_infer(p.net, feat, o, gIn[wv], gOut[wv]);
             if (doScatter)
             {
-                Lobes Lb;
-                guide_lobes(o, Lb);
+                LobesF Lb;
+                guide_lobes_f(o, Lb);
                 const double g = p.gpar[ell];
                 if (useGuide)
-                    guide_sample(Lb, rng, kx, ky, kz);
+                    guide_sample_f(Lb, rng, kx, ky, kz);
                 else
                     scatter_hg(rng, g, kx, ky, kz);
                 double c = kxo * kx + kyo * ky + kzo * kz;
                 double phi = (fabs(g) < 1e-6 ? 1.0 : value_hg(g, c)) / (4 * M_PI);
-                double q = guide_pdf(Lb, kx, ky, kz);
+                double q = guide_pdf_f(Lb, kx, ky, kz);
                 const double ratio = phi / ((1.0 - p.guideMix) * phi + p.guideMix * q);
                 W *= ratio;
                 // the termination rule (MonteCarloSimulation.cpp:643-660) must see the weight

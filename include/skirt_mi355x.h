@@ -190,7 +190,7 @@ int skg_tune(skg_engine* e, int32_t workgroups, int32_t threads, int32_t transit
  * counters are restored afterwards) for each candidate launch configuration -- persistent
  * engine: transition threshold 16/32/48 x 2/4/8 workgroups per CU; pipeline engine: 2^20/2^21/
  * 2^22 packet slots x 8/16 graph iterations x 4/8 trace workgroups per CU -- and keeps the
- * fastest.  engines: bit 0 persistent, bit 1 pipeline (guided sampling: pipeline only).
+ * fastest.  engines: bit 0 persistent, bit 1 pipeline (batched instances: persistent only).
  * best6 (may be NULL) receives {mode, threshold, workgroups, pool_slots, trace_workgroups,
  * graph_iterations}; best_rate the trial packets/s.  Not to be called concurrently with other
  * calls on the same engine. */
@@ -325,8 +325,9 @@ int skg_guide_infer(skg_engine* e, const float* features, int64_t n, float* out)
 /* Gradient of sum_i -w_i ln q(k_i | x_i) with respect to the parameters for n records of 16
  * floats (9 features, direction k (3), weight w, 3 unused), and the loss. */
 int skg_guide_gradient(skg_engine* e, const float* records, int64_t n, float* grad, double* loss);
-/* mode 0: plain HG; 1: guided (defensive mixture with probability mix of sampling the guide);
- * 2: plain HG that records training data at every scattering.  half_size, lambda_center (m)
+/* mode 0: plain HG; 1: guided (defensive mixture with probability mix of sampling the guide;
+ * both engines, with the same results for the same histories); 2: plain HG that records
+ * training data at every scattering (pipeline engine only).  half_size, lambda_center (m)
  * and lambda_scale set the feature normalisation. */
 int skg_guide_mode(skg_engine* e, int32_t mode, double mix, double half_size, double lambda_center,
                    double lambda_scale);

@@ -171,6 +171,20 @@ int skg_run_packets(skg_engine* e, uint64_t first_history, uint64_t count);
 int skg_instances(skg_engine* e, int32_t n, const uint64_t* packets, const uint64_t* seeds, const double* tau_scale,
                   const double* lpack, const int32_t* frame_begin, const int32_t* frame_count);
 
+/* Radiation-field storage of the primary emission (MonteCarloSimulation::storeRadiationField,
+ * MonteCarloSimulation.cpp:888-927, with MediumSystem::storeRadiationField :2488-2494): while
+ * enabled, every forced path adds L * lnmean(e^-tau_end, e^-tau_begin) * ds of each of its
+ * stored segments in a cell (SpecialFunctions::lnmean, SpecialFunctions.cpp:860-880) to the
+ * table entry m * nlambda + ell (the layout of MediumSystem's _rf1(m, ell)), L being the
+ * packet's luminosity at the start of the path and ell its wavelength bin (the oligochromatic
+ * radiation-field grid, Configuration.cpp:243).  FP64 accumulation on the device, persistent
+ * engine, not combined with instances or the guide.  skg_rf_enable allocates and clears the
+ * table for the current grid and wavelengths (on = 0 frees it); skg_rf_read copies it out
+ * (count >= ncells * nlambda). */
+int skg_rf_enable(skg_engine* e, int32_t on);
+int skg_rf_clear(skg_engine* e);
+int skg_rf_read(skg_engine* e, double* out, int64_t count);
+
 /* Same, asynchronous on the engine's stream; skg_sync waits for completion. */
 int skg_run_packets_async(skg_engine* e, uint64_t first_history, uint64_t count);
 int skg_sync(skg_engine* e);

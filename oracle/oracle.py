@@ -64,7 +64,7 @@ class OrcSim(C.Structure):
                 ("lam", f64p), ("sext", f64p), ("ssca", f64p), ("g", f64p), ("lpack", f64p),
                 ("src", C.c_double * 3), ("xi", C.c_double), ("min_weight_reduction", C.c_double),
                 ("min_scatt_events", C.c_int32), ("ninstr", C.c_int32), ("instr", P(OrcInstrument)),
-                ("seed", C.c_uint64)]
+                ("seed", C.c_uint64), ("ncells", C.c_int64), ("rf", f64p)]
 
 
 def _p(a, t):
@@ -194,9 +194,17 @@ def expon_cutoff(seed, history, xmax, n):
     return out
 
 
+def lnmean(x1, x2):
+    """SpecialFunctions::lnmean as the oracle states it (radiation-field storage)."""
+    f = lib().orc_lnmean
+    f.restype = C.c_double
+    return f(C.c_double(x1), C.c_double(x2))
+
+
 def simulate(grid: Grid, density, lam, sext, ssca, g, lpack, instruments, first, count, threads=8, seed=1,
-             xi=0.5, min_weight_reduction=1e4, min_scatt_events=0, src=(0.0, 0.0, 0.0)):
-    """Run histories [first, first+count); returns (frames list, counters).
+             xi=0.5, min_weight_reduction=1e4, min_scatt_events=0, src=(0.0, 0.0, 0.0), store_rf=False):
+    """Run histories [first, first+count); returns (frames list, counters), and the stored
+    radiation field (ncells, nlambda) as a third item when store_rf.
 
     instruments: list of dicts with inclination, azimuth, roll (rad), nx, ny, fovx, fovy, cx, cy (m).
     Each frame has shape (nlambda, ny, nx)."""
@@ -217,6 +225,9 @@ def simulate(grid: Grid, density, lam, sext, ssca, g, lpack, instruments, first,
     s.ninstr = len(instruments)
     s.instr = ins
     s.seed = seed
+    s.ncells = len(arrs[0])
+    rf = np.zeros((len(arrs[0]), len(arrs[1]))) if store_rf else None
+    s.rf = _p(rf, f64p) if store_rf else None
     n = lib().orc_frame_size(C.byref(s))
     frame = np.zeros(n)
     cnt = np.zeros(4, np.int64)
@@ -229,6 +240,8 @@ def simulate(grid: Grid, density, lam, sext, ssca, g, lpack, instruments, first,
         sz = d["nx"] * d["ny"] * nl
         frames.append(frame[o:o + sz].reshape(nl, d["ny"], d["nx"]))
         o += sz
+    if store_rf:
+        return frames, cnt, rf
     return frames, cnt
 
 

@@ -969,10 +969,15 @@ struct orc_sim
     int32_t ninstr;
     const orc_instrument* instr;
     uint64_t seed;
+    int64_t ncells;         // cells of the grid (radiation-field table rows)
+    double* rf;             // radiation field ncells x nlambda to add into, or NULL
 };
 
 // version tag used by the loader
-int orc_version() { return 4; }
+int orc_version() { return 5; }
+
+// the logarithmic mean of the radiation-field storage, exposed for the known-answer tests
+double orc_lnmean(double x1, double x2);
 
 // Philox4x32-10 block, exposed for the known-answer tests
 void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out)
@@ -1158,8 +1163,24 @@ namespace
     struct Seg
     {
         int m;
-        double s, tau;
+        double s, tau, ds;
     };
+
+    // SpecialFunctions::lnmean(x1, x2, lnx1, lnx2), SpecialFunctions.cpp:860-880
+    double lnMean(double x1, double x2, double lnx1, double lnx2)
+    {
+        if (x1 > x2)
+        {
+            std::swap(x1, x2);
+            std::swap(lnx1, lnx2);
+        }
+        if (x1 <= 0) return 0.;
+        double x = x2 / x1 - 1.;
+        if (x < 1e-3)
+            return x1 / (1. - 1. / 2. * x + 1. / 3. * x * x - 1. / 4. * x * x * x + 1. / 5. * x * x * x * x
+                         - 1. / 6. * x * x * x * x * x);
+        return (x2 - x1) / (lnx2 - lnx1);
+    }
 
     struct Sim
     {
@@ -1224,7 +1245,8 @@ namespace
 
         // MonteCarloSimulation::performLifeCycle forced-scattering branch (:626-665),
         // simulateForcedPropagation (:1492-1544), MediumSystem::simulateScattering (:1607-1653)
-        void history(uint64_t h, double* frame, std::vector<Seg>& segs, Generator& gen, int64_t* counters) const
+        void history(uint64_t h, double* frame, double* rf, std::vector<Seg>& segs, Generator& gen,
+                     int64_t* counters) const
         {
             Stream rng(p->seed, h, 0);
             // launch: uniform wavelength bin (OligoWavelengthDistribution.cpp:34-38), isotropic direction
@@ -1255,10 +1277,27 @@ namespace
                     {
                         s += gen.ds;
                         if (gen.m >= 0) tau += section * p->density[gen.m] * gen.ds;
-                        segs.push_back(Seg{gen.m, s, tau});
+                        segs.push_back(Seg{gen.m, s, tau, gen.ds});
                     }
                 }
                 counters[2]++;
+                // MonteCarloSimulation::storeRadiationField (:888-927), oligochromatic branch:
+                // the bin of the radiation-field grid is the packet's bin
+                if (rf)
+                {
+                    const double L = W / lambda;
+                    double lnExtBeg = 0., extBeg = 1.;
+                    for (const Seg& sg : segs)
+                    {
+                        double lnExtEnd = -sg.tau;
+                        double extEnd = std::exp(lnExtEnd);
+                        if (sg.m >= 0)
+                            rf[static_cast<int64_t>(sg.m) * p->nlambda + ell] +=
+                                L * lnMean(extEnd, extBeg, lnExtEnd, lnExtBeg) * sg.ds;
+                        lnExtBeg = lnExtEnd;
+                        extBeg = extEnd;
+                    }
+                }
                 double taupath = segs.empty() ? 0. : segs.back().tau;
                 if (taupath <= 0.)
                 {
@@ -1320,6 +1359,8 @@ namespace
     };
 }
 
+double orc_lnmean(double x1, double x2) { return lnMean(x1, x2, std::log(x1), std::log(x2)); }
+
 int64_t orc_frame_size(const orc_sim* p)
 {
     int64_t t = 0;
@@ -1362,6 +1403,8 @@ int orc_simulate(const orc_sim* p, uint64_t first, uint64_t count, int threads, 
     }
     if (threads < 1) threads = 1;
     std::vector<std::vector<double>> frames(threads, std::vector<double>(sim.npix_total, 0.));
+    const int64_t nrf = p->rf ? p->ncells * p->nlambda : 0;
+    std::vector<std::vector<double>> rfs(threads, std::vector<double>(nrf, 0.));
     std::vector<std::array<int64_t, 4>> cnt(threads, std::array<int64_t, 4>{0, 0, 0, 0});
     std::atomic<uint64_t> next{0};
     const uint64_t chunk = 256;
@@ -1374,7 +1417,8 @@ int orc_simulate(const orc_sim* p, uint64_t first, uint64_t count, int threads, 
             uint64_t b = next.fetch_add(chunk);
             if (b >= count) break;
             uint64_t e = std::min(count, b + chunk);
-            for (uint64_t i = b; i < e; ++i) sim.history(first + i, frames[t].data(), segs, gen, cnt[t].data());
+            for (uint64_t i = b; i < e; ++i)
+                sim.history(first + i, frames[t].data(), nrf ? rfs[t].data() : nullptr, segs, gen, cnt[t].data());
         }
     };
     std::vector<std::thread> pool;
@@ -1383,6 +1427,8 @@ int orc_simulate(const orc_sim* p, uint64_t first, uint64_t count, int threads, 
     for (auto& th : pool) th.join();
     for (int t = 0; t < threads; ++t)
         for (int64_t i = 0; i < sim.npix_total; ++i) frame[i] += frames[t][i];
+    for (int t = 0; t < threads; ++t)
+        for (int64_t i = 0; i < nrf; ++i) p->rf[i] += rfs[t][i];
     if (counters)
         for (int t = 0; t < threads; ++t)
             for (int c = 0; c < 4; ++c) counters[c] += cnt[t][c];

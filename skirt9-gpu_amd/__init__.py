@@ -75,7 +75,7 @@ EXPORTED = [
     "skg_voronoi_build", "skg_voronoi_info", "skg_voronoi_export", "skg_voronoi_upload",
     "skg_sample_density_voronoi", "skg_voronoi_free", "skg_autotune",
     "skg_tetra_build", "skg_tetra_info", "skg_tetra_export", "skg_tetra_upload", "skg_tetra_free",
-    "skg_sample_density_tetra", "skg_instances",
+    "skg_sample_density_tetra", "skg_instances", "skg_rf_enable", "skg_rf_clear", "skg_rf_read",
 ]
 
 GUIDE_PARAMS = 11560
@@ -251,6 +251,19 @@ class Engine:
         self._chk(lib().skg_instances(self._h, C.c_int32(n), _p(pk, u64p), _p(sd, u64p), _p(ts, f64p), _p(lp, f64p),
                                       _p(fb, i32p), _p(fc, i32p)))
         return int(pk.sum())
+
+    def rf_enable(self, on=True):
+        """Radiation-field storage (skg_rf_enable): a cleared ncells x nlambda table."""
+        self._chk(lib().skg_rf_enable(self._h, C.c_int32(1 if on else 0)))
+
+    def rf_clear(self):
+        self._chk(lib().skg_rf_clear(self._h))
+
+    def rf_read(self):
+        """The stored radiation field, shape (ncells, nlambda): sum of L * mean extinction * ds."""
+        out = np.zeros(self.num_cells() * self.nlambda)
+        self._chk(lib().skg_rf_read(self._h, _p(out, f64p), C.c_int64(len(out))))
+        return out.reshape(-1, self.nlambda)
 
     def options(self, path_length_bias=0.5, min_weight_reduction=1e4, min_scatt_events=0, fp_mode=SKG_FP64, seed=1):
         self._chk(lib().skg_options(self._h, C.c_double(path_length_bias), C.c_double(min_weight_reduction),

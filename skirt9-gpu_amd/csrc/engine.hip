@@ -197,6 +197,7 @@ namespace skg
         GuideNet net;             // guided variant: the network, its input frame and the mixture
         GuideFrame gframe;
         double guideMix;
+        double* rf;               // radiation-field variant: ncells x nlambda table
     };
 
     // kernel variants of the life cycle
@@ -204,8 +205,31 @@ namespace skg
     {
         kPlain = 0,      // one run
         kBatched = 1,    // batched independent instances
-        kGuided = 2      // scattering directions from the defensive guide / HG mixture
+        kGuided = 2,     // scattering directions from the defensive guide / HG mixture
+        kStoreRF = 3     // one run that also stores the radiation field of the forced paths
     };
+
+    // SpecialFunctions::lnmean(x1, x2, lnx1, lnx2) (SpecialFunctions.cpp:860-880): the
+    // logarithmic mean (x2 - x1) / ln(x2 / x1), by its series for nearly equal arguments
+    __device__ __forceinline__ double ln_mean(double x1, double x2, double lnx1, double lnx2)
+    {
+        if (x1 > x2)
+        {
+            const double t = x1, lt = lnx1;
+            x1 = x2;
+            x2 = t;
+            lnx1 = lnx2;
+            lnx2 = lt;
+        }
+        if (x1 <= 0) return 0.;
+        const double x = x2 / x1 - 1.;
+        if (x < 1e-3)
+        {
+            const double c3 = 1. / 3., c5 = 1. / 5., c6 = 1. / 6.;
+            return x1 / (1. - 0.5 * x + c3 * x * x - 0.25 * x * x * x + c5 * x * x * x * x - c6 * x * x * x * x * x);
+        }
+        return (x2 - x1) / (lnx2 - lnx1);
+    }
 
     enum Phase : int
     {
@@ -315,6 +339,9 @@ namespace skg
         double tint;    // second forced pass: interaction optical depth
         int mint;
         double sint;
+        int ell;        // radiation-field variant: the packet's bin, luminosity and the
+        double rfL;     // extinction factor at the start of the current segment
+        double rfExt;
 
         __device__ __forceinline__ double tau_value() const
         {
@@ -338,10 +365,18 @@ namespace skg
             tau32.reset();
             s32.reset();
             lastm = -1;
+            // radiation-field variant: the luminosity at the start of a forced path and the
+            // extinction factor before its first segment (unused, and removed, elsewhere)
+            ell = pk.ell;
+            rfL = pk.W / pk.lam;
+            rfExt = 1.;
         }
 
-        // one cell crossing of the current traversal
-        __device__ __forceinline__ void step(const GridView& grid, unsigned int* lds)
+        // one cell crossing of the current traversal; with RF, the first forced pass also
+        // stores the radiation field (MonteCarloSimulation::storeRadiationField, :909-925):
+        // L * lnmean(e^-tau_end, e^-tau_begin) * ds per stored segment in a cell
+        template<bool RF>
+        __device__ __forceinline__ void step(const GridView& grid, unsigned int* lds, double* rf, int nl)
         {
             int m;
             double ds, nd;
@@ -394,6 +429,20 @@ namespace skg
                     if (m >= 0) tau32.add(section32 * static_cast<float>(nd) * static_cast<float>(ds));
                 }
                 lastm = m;
+                if constexpr (RF)
+                {
+                    if (phase == kFwd1)
+                    {
+                        const double lnEnd = -tau_value();
+                        const double extEnd = exp(lnEnd);
+                        if (m >= 0)
+                        {
+                            const double em = ln_mean(extEnd, rfExt, lnEnd, -tprev);
+                            unsafeAtomicAdd(rf + static_cast<int64_t>(m) * nl + ell, rfL * em * ds);
+                        }
+                        rfExt = extEnd;
+                    }
+                }
                 // second pass: stop at the first segment whose exit optical depth exceeds tau_int
                 if (phase == kFwd2 && tint < tau_value())
                 {
@@ -669,7 +718,7 @@ namespace skg
         {
             // advance every busy lane by one cell crossing
             bool busy = (T.phase == kPeel || T.phase == kFwd1 || T.phase == kFwd2) && !T.ended;
-            if (busy) T.step(p.grid, wgcnt);
+            if (busy) T.template step<V == kStoreRF>(p.grid, wgcnt, p.rf, p.nlambda);
 
             // wave-wide transition round once enough lanes wait
             unsigned long long waiting = __ballot(T.ended || T.phase == kFetch || (V == kGuided && T.phase == kScatter));
@@ -788,6 +837,10 @@ struct skg_engine
     bool frameDirty = true;
     unsigned long long* queue = nullptr;
     unsigned long long* counters = nullptr;
+    // radiation field (ncells x nlambda FP64 table) and the shape it was allocated for
+    double* rf = nullptr;
+    int64_t rfCells = 0;
+    int rfLambda = 0;
     // batched instances
     std::vector<Instance> hInst;
     Instance* inst = nullptr;
@@ -969,10 +1022,12 @@ namespace
     template<int GT, int FP>
     hipError_t launchLifecycle(const SimParams& p, bool guided, int blocks, int threads, hipStream_t s)
     {
-        // batched instances and guided sampling take their own instantiations; the plain run
-        // carries neither
+        // batched instances, guided sampling and radiation-field storage take their own
+        // instantiations; the plain run carries none of them
         if (p.ninst)
             hipLaunchKernelGGL((lifecycle<GT, FP, kBatched>), dim3(blocks), dim3(threads), 0, s, p);
+        else if (p.rf)
+            hipLaunchKernelGGL((lifecycle<GT, FP, kStoreRF>), dim3(blocks), dim3(threads), 0, s, p);
         else if (guided)
             hipLaunchKernelGGL((lifecycle<GT, FP, kGuided>), dim3(blocks), dim3(threads), 0, s, p);
         else
@@ -1107,7 +1162,8 @@ void skg_destroy(skg_engine* e)
                     static_cast<void*>(e->gBf), static_cast<void*>(e->gBias), static_cast<void*>(e->rec),
                     static_cast<void*>(e->recHist), static_cast<void*>(e->recA), static_cast<void*>(e->recCount),
                     static_cast<void*>(e->recSum), static_cast<void*>(e->contrib), static_cast<void*>(e->recLast),
-                    static_cast<void*>(e->finalLum), static_cast<void*>(e->inst), static_cast<void*>(e->ilpack)})
+                    static_cast<void*>(e->finalLum), static_cast<void*>(e->inst), static_cast<void*>(e->ilpack),
+                    static_cast<void*>(e->rf)})
         if (p) hipFree(p);
     if (e->hostCtl) hipHostFree(e->hostCtl);
     for (auto ev : e->traceEv) hipEventDestroy(ev);
@@ -1652,6 +1708,49 @@ int skg_instances(skg_engine* e, int32_t n, const uint64_t* packets, const uint6
     return SKG_OK;
 }
 
+int skg_rf_enable(skg_engine* e, int32_t on)
+{
+    if (!e) return SKG_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    if (e->rf) hipFree(e->rf);
+    e->rf = nullptr;
+    e->rfCells = 0;
+    e->rfLambda = 0;
+    if (!on) return SKG_OK;
+    if (e->gridType < 0 || e->nlambda < 1) return fail(e, SKG_ERR_STATE, "skg_rf_enable: upload the grid and the dust first");
+    const size_t n = static_cast<size_t>(e->ncells) * e->nlambda;
+    HIPCHK(e, hipMalloc(&e->rf, n * sizeof(double)));
+    HIPCHK(e, hipMemset(e->rf, 0, n * sizeof(double)));
+    e->rfCells = e->ncells;
+    e->rfLambda = e->nlambda;
+    return SKG_OK;
+}
+
+int skg_rf_clear(skg_engine* e)
+{
+    if (!e) return SKG_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    if (!e->rf) return fail(e, SKG_ERR_STATE, "skg_rf_clear: radiation-field storage is not enabled");
+    HIPCHK(e, hipMemsetAsync(e->rf, 0, static_cast<size_t>(e->rfCells) * e->rfLambda * sizeof(double), e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return SKG_OK;
+}
+
+int skg_rf_read(skg_engine* e, double* out, int64_t count)
+{
+    if (!e || !out) return SKG_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    if (!e->rf) return fail(e, SKG_ERR_STATE, "skg_rf_read: radiation-field storage is not enabled");
+    const int64_t n = e->rfCells * e->rfLambda;
+    if (count < n) return fail(e, SKG_ERR_CAPACITY, "skg_rf_read: the output holds fewer than ncells x nlambda values");
+    HIPCHK(e, hipMemcpyAsync(out, e->rf, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    return SKG_OK;
+}
+
 int skg_tune(skg_engine* e, int32_t workgroups, int32_t threads, int32_t transition_threshold)
 {
     if (!e || workgroups < 0 || threads < 64 || threads > 256 || threads % 64 || transition_threshold < 1
@@ -1698,6 +1797,11 @@ static int runPersistent(skg_engine* e, uint64_t first_history, uint64_t count)
     p.net = e->net;
     p.gframe = e->gframe;
     p.guideMix = e->guideMix;
+    p.rf = e->rf;
+    if (e->rf && (e->rfCells != e->ncells || e->rfLambda != e->nlambda))
+        return fail(e, SKG_ERR_STATE, "skg_run_packets: the radiation-field table does not match the grid / wavelengths");
+    if (e->rf && (!e->hInst.empty() || e->guideMode != 0))
+        return fail(e, SKG_ERR_STATE, "skg_run_packets: radiation-field storage runs without instances and without the guide");
     if (e->guideMode == 2)
         return fail(e, SKG_ERR_STATE, "skg_run_packets: training records are collected by the pipeline engine");
     const bool guided = e->guideMode == 1;
@@ -1983,6 +2087,8 @@ int skg_run_packets_async(skg_engine* e, uint64_t first_history, uint64_t count)
                 return fail(e, SKG_ERR_STATE, "skg_run_packets: an instance names a frame that does not exist");
     }
     if (count == 0) return SKG_OK;
+    if (e->mode == SKG_MODE_PIPELINE && e->rf)
+        return fail(e, SKG_ERR_STATE, "skg_run_packets: radiation-field storage runs in the persistent engine");
     if (e->mode == SKG_MODE_PIPELINE) return runPipeline(e, first_history, count);
     return runPersistent(e, first_history, count);
 }

@@ -255,10 +255,10 @@ namespace skg
     };
     enum ColdU32 : int
     {
-        cHist0, cHist1, cDraw, cEll, cNscatt, cF, cPix, cFlags, cInst, kColdU32
+        cHist0, cHist1, cDraw, cEll, cNscatt, cF, cPix, cFlags, cInst, cHint, kColdU32
     };
     constexpr int kLcThreads = 256;
-    constexpr int kColdBytes = kLcThreads * (8 * kColdF64 + 4 * kColdU32);  // 33 KB per workgroup
+    constexpr int kColdBytes = kLcThreads * (8 * kColdF64 + 4 * kColdU32);  // 34 KB per workgroup
 
     struct Packet
     {
@@ -266,6 +266,7 @@ namespace skg
         double rx, ry, rz, kx, ky, kz;
         double W, lam, Lthr, taupath, tauCached, Wp;
         int ell, nscatt, f, pix, inst;
+        int hint;  // a cell that probably holds the packet's position (-1: none)
         bool scat, haveTau;
 
         template<bool INST>
@@ -286,6 +287,7 @@ namespace skg
             uint64_t h = static_cast<uint64_t>(cu[cHist0 * kLcThreads + t])
                          | (static_cast<uint64_t>(cu[cHist1 * kLcThreads + t]) << 32);
             inst = static_cast<int>(cu[cInst * kLcThreads + t]);
+            hint = static_cast<int>(cu[cHint * kLcThreads + t]);
             rng.init(INST ? p.inst[inst].seed : p.seed, h);
             rng.draw = cu[cDraw * kLcThreads + t];
             ell = static_cast<int>(cu[cEll * kLcThreads + t]);
@@ -320,6 +322,7 @@ namespace skg
             cu[cPix * kLcThreads + t] = static_cast<uint32_t>(pix);
             cu[cFlags * kLcThreads + t] = (scat ? 1u : 0u) | (haveTau ? 2u : 0u);
             cu[cInst * kLcThreads + t] = static_cast<uint32_t>(inst);
+            cu[cHint * kLcThreads + t] = static_cast<uint32_t>(hint);
         }
     };
 
@@ -354,7 +357,7 @@ namespace skg
 
         __device__ __forceinline__ void start(const Packet& pk, double dx, double dy, double dz, int ph)
         {
-            gen.start(pk.rx, pk.ry, pk.rz, dx, dy, dz);
+            gen.start(pk.rx, pk.ry, pk.rz, dx, dy, dz, pk.hint);
             phase = ph;
             ended = false;
             steps = 0;
@@ -518,6 +521,7 @@ namespace skg
             pk.tauCached = 0.;
             pk.Wp = 0.;
             pk.pix = -1;
+            pk.hint = -1;
             pk.haveTau = false;
             if (!(pk.W / pk.lam > 0))
             {
@@ -626,6 +630,10 @@ namespace skg
         // event handling once a traversal ended
         __device__ __forceinline__ void transition()
         {
+            // every traversal starts at the packet's position: its start cell is the hint for
+            // the next traversal from there (tetrahedral grids)
+            if constexpr (GT == kTetra)
+                if (T.gen.hint >= 0) pk.hint = T.gen.hint;
             if (T.phase == kPeel)
             {
                 pk.haveTau = true;
@@ -676,6 +684,7 @@ namespace skg
                 double kext = n > 0. ? n * T.section : 0.;
                 double albedo = kext > 0. ? ksca / kext : 0.;
                 pk.W *= -expm1(-pk.taupath) * albedo;
+                if constexpr (GT == kTetra) pk.hint = m;  // the interaction point lies in the cell of its segment
                 pk.rx += T.sint * pk.kx;
                 pk.ry += T.sint * pk.ky;
                 pk.rz += T.sint * pk.kz;

@@ -631,8 +631,8 @@ namespace skg
         __device__ __forceinline__ void transition()
         {
             // every traversal starts at the packet's position: its start cell is the hint for
-            // the next traversal from there (tetrahedral grids)
-            if constexpr (GT == kTetra)
+            // the next traversal from there (Voronoi and tetrahedral grids)
+            if constexpr (GT == kTetra || GT == kVoronoi)
                 if (T.gen.hint >= 0) pk.hint = T.gen.hint;
             if (T.phase == kPeel)
             {
@@ -684,7 +684,8 @@ namespace skg
                 double kext = n > 0. ? n * T.section : 0.;
                 double albedo = kext > 0. ? ksca / kext : 0.;
                 pk.W *= -expm1(-pk.taupath) * albedo;
-                if constexpr (GT == kTetra) pk.hint = m;  // the interaction point lies in the cell of its segment
+                if constexpr (GT == kTetra || GT == kVoronoi)
+                    pk.hint = m;  // the interaction point lies in the cell of its segment
                 pk.rx += T.sint * pk.kx;
                 pk.ry += T.sint * pk.ky;
                 pk.rz += T.sint * pk.kz;

@@ -171,6 +171,29 @@ namespace skg
         return m;
     }
 
+    // True when the point lies in the domain and is nearer to the site of cell h than to each
+    // of h's neighbouring sites by a relative margin far above the rounding of the squared
+    // distances.  Inside the domain, the cell is the set of points nearer to its site than to
+    // the sites across its faces, so h's site is then the unique nearest site: the cell that
+    // VoronoiMeshSnapshot::cellIndex (and vor_cell_index) returns.
+    __device__ __forceinline__ bool vor_nearest_strict(const GridView& g, int h, double x, double y, double z)
+    {
+        if (!(x >= g.box[0] && x <= g.box[3] && y >= g.box[1] && y <= g.box[4] && z >= g.box[2] && z <= g.box[5]))
+            return false;
+        const VorCell& C = g.vcell[h];
+        const double hx = x - C.x, hy = y - C.y, hz = z - C.z;
+        const double dh = hx * hx + hy * hy + hz * hz;
+        for (int q = C.begin, e = C.begin + C.count; q < e; ++q)
+        {
+            const VorNb E = g.vnb[q];
+            if (E.id < 0) continue;
+            const double ex = x - E.x, ey = y - E.y, ez = z - E.z;
+            const double dj = ex * ex + ey * ey + ez * ez;
+            if (!(dj - dh > 1e-12 * (dj + dh))) return false;
+        }
+        return true;
+    }
+
     // number of borders <= v, minus one, clipped to a block index
     __device__ __forceinline__ int tet_axis_block(const double* bv, int n, double v)
     {
@@ -332,8 +355,8 @@ namespace skg
                       // (tetra: j = entering face, -1 if unknown)
         double b0, b1, b2, b3, b4, b5, bn;  // octree: bounds and density of the current cell
         int f0, f1, f2, f3, f4, f5;         // octree: first neighbour of the current cell per wall
-        int hint;  // tetra: a cell that probably contains the start point (-1: none); after
-                   // the first step, the start cell
+        int hint;  // Voronoi, tetra: a cell that probably contains the start point (-1: none);
+                   // after the first step, the start cell
 
         __device__ __forceinline__ void load_leaf(const GridView& g, int c)
         {
@@ -467,7 +490,11 @@ namespace skg
                 }
                 else if (GT == kVoronoi)
                 {
-                    i = vor_cell_index(g, rx, ry, rz);
+                    if (hint >= 0 && vor_nearest_strict(g, hint, rx, ry, rz))
+                        i = hint;
+                    else
+                        i = vor_cell_index(g, rx, ry, rz);
+                    hint = i;
                     if (i < 0)
                     {
                         state = kOutside;

@@ -131,6 +131,12 @@ int skg_trace_tau(skg_engine* e, const double* rays, int64_t n, int32_t ell, dou
  * returns its index (>= 0) or a negative status.  At most 1024 instruments. */
 int skg_frame_add(skg_engine* e, double inclination, double azimuth, double roll, int32_t nx, int32_t ny,
                   double fovx, double fovy, double cx, double cy);
+/* Adds an SED instrument (SEDInstrument, SEDInstrument.cpp:14-25): one value per wavelength
+ * bin, the luminosity peeled off toward the direction (inclination, azimuth) from positions
+ * whose projection on the sky lies within `radius` (m) of the centre (ApertureInstrument.cpp:
+ * 24-43; 0 = no aperture).  It shares the instrument list and indices with the frames
+ * (skg_frame_read returns its nlambda values); returns its index or a negative status. */
+int skg_sed_add(skg_engine* e, double inclination, double azimuth, double radius);
 int skg_frame_remove_all(skg_engine* e);
 /* Zeroes all frame accumulators. */
 int skg_frame_clear(skg_engine* e);

@@ -56,7 +56,8 @@ class OrcGrid(C.Structure):
 class OrcInstrument(C.Structure):
     _fields_ = [("inclination", C.c_double), ("azimuth", C.c_double), ("roll", C.c_double),
                 ("nx", C.c_int32), ("ny", C.c_int32),
-                ("fovx", C.c_double), ("fovy", C.c_double), ("cx", C.c_double), ("cy", C.c_double)]
+                ("fovx", C.c_double), ("fovy", C.c_double), ("cx", C.c_double), ("cy", C.c_double),
+                ("sed", C.c_int32), ("radius", C.c_double)]
 
 
 class OrcSim(C.Structure):
@@ -206,13 +207,16 @@ def simulate(grid: Grid, density, lam, sext, ssca, g, lpack, instruments, first,
     """Run histories [first, first+count); returns (frames list, counters), and the stored
     radiation field (ncells, nlambda) as a third item when store_rf.
 
-    instruments: list of dicts with inclination, azimuth, roll (rad), nx, ny, fovx, fovy, cx, cy (m).
-    Each frame has shape (nlambda, ny, nx)."""
+    instruments: list of dicts with inclination, azimuth, roll (rad), nx, ny, fovx, fovy, cx, cy (m),
+    or, for an SED instrument, sed=True, inclination, azimuth and an aperture radius (m, 0 = none).
+    Each frame has shape (nlambda, ny, nx) (an SED: (nlambda, 1, 1))."""
     arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (density, lam, sext, ssca, g, lpack)]
     ins = (OrcInstrument * len(instruments))()
     for i, d in enumerate(instruments):
-        ins[i] = OrcInstrument(d["inclination"], d["azimuth"], d.get("roll", 0.0), d["nx"], d["ny"],
-                               d["fovx"], d["fovy"], d.get("cx", 0.0), d.get("cy", 0.0))
+        sed = bool(d.get("sed", False))
+        ins[i] = OrcInstrument(d["inclination"], d["azimuth"], d.get("roll", 0.0), 1 if sed else d["nx"],
+                               1 if sed else d["ny"], d.get("fovx", 1.0), d.get("fovy", 1.0), d.get("cx", 0.0),
+                               d.get("cy", 0.0), 1 if sed else 0, d.get("radius", 0.0))
     s = OrcSim()
     s.grid = C.pointer(grid.g)
     s.density = _p(arrs[0], f64p)
@@ -237,8 +241,9 @@ def simulate(grid: Grid, density, lam, sext, ssca, g, lpack, instruments, first,
     o = 0
     nl = len(arrs[1])
     for d in instruments:
-        sz = d["nx"] * d["ny"] * nl
-        frames.append(frame[o:o + sz].reshape(nl, d["ny"], d["nx"]))
+        nx, ny = (1, 1) if d.get("sed", False) else (d["nx"], d["ny"])
+        sz = nx * ny * nl
+        frames.append(frame[o:o + sz].reshape(nl, ny, nx))
         o += sz
     if store_rf:
         return frames, cnt, rf

@@ -27,6 +27,8 @@ class SkiSpec:
     half_size_pc: float = 10.0
     # instruments: (name, inclination deg, azimuth deg)
     instruments: Sequence[Tuple[str, float, float]] = (("i60", 60.0, 0.0),)
+    # SED instruments: (name, inclination deg, azimuth deg, aperture radius pc; 0 = none)
+    sed_instruments: Sequence[Tuple[str, float, float, float]] = ()
     num_pixels: int = 200
     fov_pc: float = 20.0
     seed: int = 0
@@ -84,7 +86,11 @@ def render(spec: SkiSpec) -> str:
         f'azimuth="{az:.9g} deg" roll="0 deg" fieldOfViewX="{spec.fov_pc:.9g} pc" numPixelsX="{spec.num_pixels}" '
         f'centerX="0 pc" fieldOfViewY="{spec.fov_pc:.9g} pc" numPixelsY="{spec.num_pixels}" centerY="0 pc" '
         'recordComponents="false" numScatteringLevels="0" recordPolarization="false" recordStatistics="false"/>'
-        for (name, inc, az) in spec.instruments)
+        for (name, inc, az) in spec.instruments) + "".join(
+        f'<SEDInstrument instrumentName="{name}" distance="10 Mpc" inclination="{inc:.9g} deg" '
+        f'azimuth="{az:.9g} deg" roll="0 deg" radius="{rad:.9g} pc" recordComponents="false" '
+        'numScatteringLevels="0" recordPolarization="false" recordStatistics="false"/>'
+        for (name, inc, az, rad) in spec.sed_instruments)
     force = "true" if spec.force_scattering else "false"
     return f"""<?xml version="1.0" encoding="UTF-8"?>
 <skirt-simulation-hierarchy type="MonteCarloSimulation" format="9" producer="skigen" time="2026-01-01T00:00:00">

@@ -950,6 +950,8 @@ struct orc_instrument
     double inclination, azimuth, roll;  // radians
     int32_t nx, ny;
     double fovx, fovy, cx, cy;          // metres
+    int32_t sed;                        // SEDInstrument (nx = ny = 1) with an aperture radius (m)
+    double radius;
 };
 
 struct orc_sim
@@ -1145,10 +1147,23 @@ namespace
         double xpmin, xpsiz, ypmin, ypsiz;
         double kobs[3];
         bool sameAsPrev;
+        bool sed;
+        double radius2;
 
         int pixel(const double* r) const
         {
             double x = r[0], y = r[1], z = r[2];
+            if (sed)
+            {
+                // SEDInstrument::detect with ApertureInstrument::isInsideAperture (ApertureInstrument.cpp:24-43)
+                if (radius2)
+                {
+                    double xpp = -sinphi * x + cosphi * y;
+                    double ypp = -cosphi * costheta * x - sinphi * costheta * y + sintheta * z;
+                    if (xpp * xpp + ypp * ypp > radius2) return -1;
+                }
+                return 0;
+            }
             double xpp = -sinphi * x + cosphi * y;
             double ypp = -cosphi * costheta * x - sinphi * costheta * y + sintheta * z;
             double xp = cosomega * xpp - sinomega * ypp;
@@ -1395,6 +1410,8 @@ int orc_simulate(const orc_sim* p, uint64_t first, uint64_t count, int threads, 
         fr.ypsiz = in.fovy / in.ny;
         directionFromAngles(in.inclination, in.azimuth, fr.kobs);  // DistantInstrument.cpp:47
         // DistantInstrument.cpp:55-63
+        fr.sed = in.sed != 0;
+        fr.radius2 = in.radius * in.radius;
         fr.sameAsPrev = f > 0 && in.inclination == p->instr[f - 1].inclination
                         && in.azimuth == p->instr[f - 1].azimuth && in.roll == p->instr[f - 1].roll;
         sim.frameOff.push_back(sim.npix_total);

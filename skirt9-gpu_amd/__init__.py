@@ -76,6 +76,7 @@ EXPORTED = [
     "skg_sample_density_voronoi", "skg_voronoi_free", "skg_autotune",
     "skg_tetra_build", "skg_tetra_info", "skg_tetra_export", "skg_tetra_upload", "skg_tetra_free",
     "skg_sample_density_tetra", "skg_instances", "skg_rf_enable", "skg_rf_clear", "skg_rf_read",
+    "skg_sed_add",
 ]
 
 GUIDE_PARAMS = 11560
@@ -211,6 +212,12 @@ class Engine:
                                             C.c_int32(nx), C.c_int32(ny), C.c_double(fovx), C.c_double(fovy),
                                             C.c_double(cx), C.c_double(cy)))
         self.frames.append((nx, ny))
+        return idx
+
+    def sed_add(self, inclination, azimuth, radius=0.0):
+        """SED instrument (skg_sed_add): frame_read returns its (nlambda, 1, 1) values."""
+        idx = self._chk(lib().skg_sed_add(self._h, C.c_double(inclination), C.c_double(azimuth), C.c_double(radius)))
+        self.frames.append((1, 1))
         return idx
 
     def frame_remove_all(self):

@@ -839,6 +839,8 @@ struct skg_engine
         double inc, az, roll;
         int nx, ny;
         double fovx, fovy, cx, cy;
+        int sed = 0;
+        double radius = 0.;
     };
     std::vector<FrameDesc> frames;
     double* frameBuf = nullptr;
@@ -1006,6 +1008,8 @@ namespace
             }
             g.same_as_prev = f > 0 && d.inc == e->frames[f - 1].inc && d.az == e->frames[f - 1].az
                              && d.roll == e->frames[f - 1].roll;
+            g.sed = d.sed;
+            g.radius2 = d.radius * d.radius;
             g.offset = off;
             off += static_cast<int64_t>(d.nx) * d.ny * e->nlambda;
         }
@@ -1608,6 +1612,20 @@ int skg_frame_add(skg_engine* e, double inclination, double azimuth, double roll
     std::lock_guard<std::mutex> lk(e->mu);
     if (e->frames.size() >= static_cast<size_t>(kMaxFrames)) return fail(e, SKG_ERR_CAPACITY, "skg_frame_add: at most 1024 instruments");
     e->frames.push_back({inclination, azimuth, roll, nx, ny, fovx, fovy, cx, cy});
+    e->frameDirty = true;
+    return static_cast<int>(e->frames.size()) - 1;
+}
+
+int skg_sed_add(skg_engine* e, double inclination, double azimuth, double radius)
+{
+    if (!e || !(radius >= 0) || !std::isfinite(radius)) return fail(e, SKG_ERR_ARGUMENT, "skg_sed_add: bad arguments");
+    if (inclination < -1e-8 || inclination > M_PI + 1e-8) return fail(e, SKG_ERR_ARGUMENT, "skg_sed_add: inclination outside [0, pi]");
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (e->frames.size() >= static_cast<size_t>(kMaxFrames)) return fail(e, SKG_ERR_CAPACITY, "skg_sed_add: at most 1024 instruments");
+    skg_engine::FrameDesc d{inclination, azimuth, 0.0, 1, 1, 1.0, 1.0, 0.0, 0.0};
+    d.sed = 1;
+    d.radius = radius;
+    e->frames.push_back(d);
     e->frameDirty = true;
     return static_cast<int>(e->frames.size()) - 1;
 }

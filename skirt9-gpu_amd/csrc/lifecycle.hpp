@@ -131,12 +131,25 @@ namespace skg
         double kobs[3];
         int nx, ny;
         int same_as_prev;
-        int pad;
-        int64_t offset;  // first element of this instrument in the frame buffer
+        int sed;          // SEDInstrument: one pixel, inside the aperture radius (0: no aperture)
+        int64_t offset;   // first element of this instrument in the frame buffer
+        double radius2;
 
-        // FrameInstrument::pixelOnDetector (FrameInstrument.cpp:117-136)
+        // FrameInstrument::pixelOnDetector (FrameInstrument.cpp:117-136); for an SED instrument
+        // ApertureInstrument::isInsideAperture (ApertureInstrument.cpp:24-43: the projected
+        // radius without the roll) and pixel 0
         __device__ __forceinline__ int pixel(double x, double y, double z) const
         {
+            if (sed)
+            {
+                if (radius2 != 0.)
+                {
+                    double xpp = -sinphi * x + cosphi * y;
+                    double ypp = -cosphi * costheta * x - sinphi * costheta * y + sintheta * z;
+                    if (xpp * xpp + ypp * ypp > radius2) return -1;
+                }
+                return 0;
+            }
             double xpp = -sinphi * x + cosphi * y;
             double ypp = -cosphi * costheta * x - sinphi * costheta * y + sintheta * z;
             double xp = cosomega * xpp - sinomega * ypp;

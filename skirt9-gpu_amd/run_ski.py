@@ -45,7 +45,8 @@ def main():
     prefix = os.path.splitext(os.path.basename(a.ski))[0]
     S.run(sim, a.outdir, prefix, log=lambda m: print(m, flush=True))
     for f in sim.frames:
-        print(os.path.join(a.outdir, f"{prefix}_{f.name}_total.fits"))
+        suffix = "sed.dat" if isinstance(f, S.Sed) else "total.fits"
+        print(os.path.join(a.outdir, f"{prefix}_{f.name}_{suffix}"))
 
 
 if __name__ == "__main__":

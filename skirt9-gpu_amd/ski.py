@@ -5,12 +5,15 @@ an oligochromatic `MonteCarloSimulation` with one `PointSource` (`BlackBodySED` 
 `SpecificLuminosityNormalization`), one `GeometricMedium` (`ExpDiskGeometry`, optionally
 inside a `SpiralStructureGeometryDecorator`) with a `MeanListDustMix` and an
 `OpticalDepthMaterialNormalization`, a `CartesianSpatialGrid` (`LinMesh` axes) or a
-`PolicyTreeSpatialGrid` with a `DensityTreePolicy`, `PhotonPacketOptions` and
-`FrameInstrument`s.  Anything else raises `UnsupportedSki`, so a caller can fall back to
-the reference.  `run(sim, outdir, prefix)` sets the engine up, runs the photon packets and
-writes `<prefix>_<instrument>_total.fits` as `FrameInstrument`/`FluxRecorder` do
-(`FluxRecorder.cpp:876-960`): surface brightness in MJy/sr (ExtragalacticUnits with
-`fluxOutputStyle="Frequency"`), float32, one plane per wavelength.
+`PolicyTreeSpatialGrid` with a `DensityTreePolicy`, `PhotonPacketOptions`,
+`FrameInstrument`s and `SEDInstrument`s.  Anything else raises `UnsupportedSki`, so a caller
+can fall back to the reference.  `run(sim, outdir, prefix)` sets the engine up, runs the
+photon packets and writes `<prefix>_<instrument>_total.fits` per frame as
+`FrameInstrument`/`FluxRecorder` do (`FluxRecorder.cpp:876-960`): surface brightness in
+MJy/sr (ExtragalacticUnits with `fluxOutputStyle="Frequency"`), float32, one plane per
+wavelength; and `<prefix>_<instrument>_sed.dat` per SED instrument, the text columns of
+`FluxRecorder.cpp:1093-1115` (wavelength in micron, total flux density F_nu in Jy =
+recorded luminosity / (4 pi d^2) / (bin width) * lambda^2 / c * 1e26).
 
 Calibration (FluxRecorder::calibrateAndWrite): value = recorded luminosity
 / (4 pi d^2) / omega / (bin width) * lambda^2 / c * 1e20, with omega = 4 atan(dx / 2d)
@@ -36,6 +39,7 @@ C_LIGHT = 2.99792458e8
 H_PLANCK = 6.62606957e-34
 K_BOLTZ = 1.3806488e-23
 PC = 3.08567758e16
+MPC = 1e6 * PC
 LSUN = 3.839e26
 AU = 1.495978707e11
 
@@ -83,6 +87,17 @@ class Frame:
 
 
 @dataclass
+class Sed:
+    """SEDInstrument: distance, direction and aperture radius (0 = none)."""
+    name: str
+    distance: float
+    inclination: float
+    azimuth: float
+    roll: float
+    radius: float
+
+
+@dataclass
 class Simulation:
     num_packets: int
     seed: int
@@ -120,7 +135,7 @@ def load(path: str) -> Simulation:
     sim = _one(root, "MonteCarloSimulation")
     if sim.get("simulationMode") != "OligoExtinctionOnly":
         raise UnsupportedSki("only simulationMode OligoExtinctionOnly is supported")
-    for tag in ("GeometricMedium", "PointSource", "FrameInstrument"):
+    for tag in ("GeometricMedium", "PointSource"):
         if not root.findall(".//" + tag):
             raise UnsupportedSki(f"no {tag}")
     media = root.findall(".//media/*")
@@ -128,8 +143,8 @@ def load(path: str) -> Simulation:
     if len(media) != 1 or len(sources) != 1:
         raise UnsupportedSki("exactly one medium and one source are supported")
     instruments = root.findall(".//instruments/*")
-    if any(i.tag != "FrameInstrument" for i in instruments):
-        raise UnsupportedSki("only FrameInstrument instruments are supported")
+    if any(i.tag not in ("FrameInstrument", "SEDInstrument") for i in instruments):
+        raise UnsupportedSki("only FrameInstrument and SEDInstrument instruments are supported")
     seed = int(_one(root, "Random").get("seed", "0"))
     ss = _one(root, "SourceSystem")
     lam = np.array(sorted(quantities(ss.get("wavelengths"))))
@@ -184,8 +199,15 @@ def load(path: str) -> Simulation:
                     max_fraction=float(pol.get("maxDustFraction")))
     else:
         raise UnsupportedSki(f"grid {grid_el.tag} is not supported by the ski front end")
+    if not instruments:
+        raise UnsupportedSki("no instruments")
     frames = []
     for fi in instruments:
+        if fi.tag == "SEDInstrument":
+            frames.append(Sed(fi.get("instrumentName"), quantity(fi.get("distance")), quantity(fi.get("inclination")),
+                              quantity(fi.get("azimuth")), quantity(fi.get("roll", "0 deg")),
+                              quantity(fi.get("radius", "0 pc"))))
+            continue
         frames.append(Frame(fi.get("instrumentName"), quantity(fi.get("distance")), quantity(fi.get("inclination")),
                             quantity(fi.get("azimuth")), quantity(fi.get("roll")),
                             quantity(fi.get("fieldOfViewX")), quantity(fi.get("fieldOfViewY")),
@@ -230,6 +252,36 @@ def calibration(sim: Simulation, frame: Frame):
     dx, dy = frame.fovx / frame.nx, frame.fovy / frame.ny
     omega = 4.0 * math.atan(0.5 * dx / d) * math.atan(0.5 * dy / d)
     return 1.0 / (4.0 * math.pi * d * d) / omega / bin_widths(lam) * (lam * lam / C_LIGHT) * 1e20
+
+
+def sed_calibration(sim: Simulation, sed: Sed):
+    """Recorded luminosity -> F_nu in Jy, per wavelength bin (FluxRecorder::calibrateAndWrite)."""
+    lam = sim.wavelengths
+    d = sed.distance
+    return 1.0 / (4.0 * math.pi * d * d) / bin_widths(lam) * (lam * lam / C_LIGHT) * 1e26
+
+
+def _smart(v):
+    """StringUtils::toString(double): %1.10g without exponent zero padding or '+'."""
+    t = f"{v:1.10g}"
+    return t.replace("e-0", "e-").replace("e+0", "e").replace("e+", "e")
+
+
+def write_sed(path, values, sed: Sed, lam):
+    """The SED text file of FluxRecorder (header, column lines, '%.9e' values)."""
+    with open(path, "w") as f:
+        f.write(f"# SED at inclination {_smart(math.degrees(sed.inclination))} deg, azimuth "
+                f"{_smart(math.degrees(sed.azimuth))} deg, distance {_smart(sed.distance / MPC)} Mpc\n")
+        f.write("# column 1: wavelength; lambda (micron)\n")
+        f.write("# column 2: total flux; F_nu (Jy)\n")
+        for l, v in zip(lam, values):
+            f.write(f"{l * 1e6:.9e} {v:.9e}\n")
+
+
+def read_sed(path):
+    """(wavelength in micron, F_nu in Jy) columns of an SED text file."""
+    rows = [list(map(float, ln.split())) for ln in open(path) if ln.strip() and not ln.startswith("#")]
+    return np.array(rows).reshape(-1, 2)
 
 
 def write_fits(path, cube, frame: Frame, lam):
@@ -316,7 +368,10 @@ def setup(sim: Simulation, engine, threads=0, npk_norm=None):
     engine.dust(sim.wavelengths, sext, ssca, g)
     engine.frame_remove_all()
     for f in sim.frames:
-        engine.frame_add(f.inclination, f.azimuth, f.roll, f.nx, f.ny, f.fovx, f.fovy, f.cx, f.cy)
+        if isinstance(f, Sed):
+            engine.sed_add(f.inclination, f.azimuth, f.radius)
+        else:
+            engine.frame_add(f.inclination, f.azimuth, f.roll, f.nx, f.ny, f.fovx, f.fovy, f.cx, f.cy)
     # the bin is chosen uniformly, so each packet of bin ell carries nlambda * L_ell / N
     npk = npk_norm or sim.num_packets
     engine.source_point(sim.source_position, bin_luminosities(sim) * len(sim.wavelengths) / npk)
@@ -343,6 +398,11 @@ def run(sim: Simulation, outdir: str, prefix: str, engine=None, chunk=1 << 22, t
     os.makedirs(outdir, exist_ok=True)
     out = {}
     for i, f in enumerate(sim.frames):
+        if isinstance(f, Sed):
+            vals = eng.frame_read(i)[:, 0, 0] * sed_calibration(sim, f)
+            write_sed(os.path.join(outdir, f"{prefix}_{f.name}_sed.dat"), vals, f, sim.wavelengths)
+            out[f.name] = vals
+            continue
         cube = eng.frame_read(i) * calibration(sim, f)[:, None, None]
         write_fits(os.path.join(outdir, f"{prefix}_{f.name}_total.fits"), cube, f, sim.wavelengths)
         out[f.name] = cube

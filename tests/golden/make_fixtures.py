@@ -26,6 +26,9 @@ Every fixture is data: seeded inputs plus the reference's outputs for them.
                    per-pixel mean and standard error over independent runs, plus a
                    near-transparent calibration run giving the per-wavelength factor
                    from recorded luminosity to the FITS surface brightness.
+  sed_cfg0.npz     (--seds) the same workload with two SEDInstruments at 60 deg (no
+                   aperture, 3 pc aperture): F_nu mean and standard error over independent
+                   runs; sed_cfg0_sed60.dat is one run's SED text output as written.
 """
 
 from __future__ import annotations
@@ -434,9 +437,36 @@ def fixtures_tetra(tmp):
     print("tetra_paths.npz:", {k: v.shape for k, v in out.items()})
 
 
+def sed_values(path):
+    return np.array([float(ln.split()[1]) for ln in open(path) if ln.strip() and not ln.startswith("#")])
+
+
+def fixtures_seds(tmp, runs=8, packets=1e6):
+    outdir = os.path.join(tmp, "sed")
+    os.makedirs(outdir, exist_ok=True)
+    names = ("sed60", "ap60")
+    vals = {n: [] for n in names}
+    for s in range(runs):
+        spec = skigen.config0(num_packets=packets, seed=2000 + s)
+        spec.sed_instruments = (("sed60", 60.0, 0.0, 0.0), ("ap60", 60.0, 0.0, 3.0))
+        ski = write_ski(tmp, f"sed{s}", spec)
+        run("run", ski, outdir, 8)
+        for n in names:
+            vals[n].append(sed_values(os.path.join(outdir, f"sed{s}_{n}_sed.dat")))
+    out = {"runs": np.array(runs), "packets": np.array(packets)}
+    for n in names:
+        v = np.array(vals[n])
+        out[n + "_mean"] = v.mean(axis=0)
+        out[n + "_sem"] = v.std(axis=0, ddof=1) / np.sqrt(runs)
+    np.savez(os.path.join(HERE, "sed_cfg0.npz"), **out)
+    with open(os.path.join(outdir, "sed0_sed60_sed.dat")) as f, open(os.path.join(HERE, "sed_cfg0_sed60.dat"), "w") as g:
+        g.write(f.read())
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--images", action="store_true")
+    ap.add_argument("--seds", action="store_true")
     ap.add_argument("--only", default="")
     a = ap.parse_args()
     with tempfile.TemporaryDirectory() as tmp:
@@ -453,6 +483,8 @@ def main():
             fixtures_tetra(tmp)
         if a.images:
             fixtures_images(tmp)
+        if a.seds:
+            fixtures_seds(tmp)
 
 
 if __name__ == "__main__":

@@ -85,3 +85,29 @@ def test_unsupported_items_raise(ski, tmp_path):
     spec = skigen.SkiSpec(num_packets=1, grid=("voronoi", 100, "DustDensity"))
     with pytest.raises(ski.UnsupportedSki):
         ski.load(_write(str(tmp_path), "v", spec))
+
+
+def test_sed_instruments_parse_and_write(ski, tmp_path):
+    """SEDInstrument items: parsed with their aperture; the SED file reproduces the reference's
+    header, column lines and number format (a reference output kept as data in
+    tests/golden/sed_cfg0_sed60.dat), and reads back; the F_nu calibration is
+    L / (4 pi d^2) / (bin width) * lambda^2 / c * 1e26."""
+    spec = skigen.config0(num_packets=1e5)
+    spec.sed_instruments = (("sed60", 60.0, 0.0, 0.0), ("ap60", 60.0, 0.0, 3.0))
+    p = os.path.join(str(tmp_path), "s.ski")
+    with open(p, "w") as f:
+        f.write(skigen.render(spec))
+    sim = ski.load(p)
+    names = [f.name for f in sim.frames]
+    assert names == ["i60", "sed60", "ap60"]
+    s0, s1 = sim.frames[1], sim.frames[2]
+    assert isinstance(s0, ski.Sed) and s0.radius == 0.0 and abs(s1.radius / (3 * ski.PC) - 1) < 1e-15
+    ref = open(os.path.join(GOLDEN, "sed_cfg0_sed60.dat")).read().splitlines()
+    out = os.path.join(str(tmp_path), "x_sed.dat")
+    v = float(ref[3].split()[1])
+    ski.write_sed(out, [v], s0, sim.wavelengths)
+    assert open(out).read().splitlines() == ref
+    np.testing.assert_array_equal(ski.read_sed(out), [[0.55, v]])
+    lam = sim.wavelengths[0]
+    want = 1 / (4 * np.pi * s0.distance ** 2) / ski.bin_widths(sim.wavelengths)[0] * lam * lam / 2.99792458e8 * 1e26
+    assert abs(ski.sed_calibration(sim, s0)[0] / want - 1) < 1e-15

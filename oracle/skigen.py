@@ -29,6 +29,8 @@ class SkiSpec:
     instruments: Sequence[Tuple[str, float, float]] = (("i60", 60.0, 0.0),)
     # SED instruments: (name, inclination deg, azimuth deg, aperture radius pc; 0 = none)
     sed_instruments: Sequence[Tuple[str, float, float, float]] = ()
+    # store the radiation field and probe its mean intensity per cell (RadiationFieldProbe)
+    store_rf: bool = False
     num_pixels: int = 200
     fov_pc: float = 20.0
     seed: int = 0
@@ -92,6 +94,11 @@ def render(spec: SkiSpec) -> str:
         'numScatteringLevels="0" recordPolarization="false" recordStatistics="false"/>'
         for (name, inc, az, rad) in spec.sed_instruments)
     force = "true" if spec.force_scattering else "false"
+    rfopt = ('                <radiationFieldOptions type="RadiationFieldOptions">'
+             '<RadiationFieldOptions storeRadiationField="true"/></radiationFieldOptions>\n') if spec.store_rf else ""
+    probes = ('<ProbeSystem><probes type="Probe"><RadiationFieldProbe probeName="rf" writeWavelengthGrid="false">'
+              '<form type="Form"><PerCellForm/></form></RadiationFieldProbe></probes></ProbeSystem>'
+              if spec.store_rf else "<ProbeSystem/>")
     return f"""<?xml version="1.0" encoding="UTF-8"?>
 <skirt-simulation-hierarchy type="MonteCarloSimulation" format="9" producer="skigen" time="2026-01-01T00:00:00">
     <MonteCarloSimulation userLevel="Expert" simulationMode="OligoExtinctionOnly" iteratePrimaryEmission="false" numPackets="{spec.num_packets:.9g}">
@@ -126,7 +133,7 @@ def render(spec: SkiSpec) -> str:
                         </normalization>
                     </GeometricMedium>
                 </media>
-                <samplingOptions type="SamplingOptions">
+{rfopt}                <samplingOptions type="SamplingOptions">
                     <SamplingOptions numDensitySamples="{spec.num_density_samples}" numPropertySamples="1" aggregateVelocity="Average"/>
                 </samplingOptions>
                 <grid type="SpatialGrid">{grid}</grid>
@@ -135,7 +142,7 @@ def render(spec: SkiSpec) -> str:
         <instrumentSystem type="InstrumentSystem">
             <InstrumentSystem><instruments type="Instrument">{instr}</instruments></InstrumentSystem>
         </instrumentSystem>
-        <probeSystem type="ProbeSystem"><ProbeSystem/></probeSystem>
+        <probeSystem type="ProbeSystem">{probes}</probeSystem>
     </MonteCarloSimulation>
 </skirt-simulation-hierarchy>
 """

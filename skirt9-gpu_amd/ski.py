@@ -11,7 +11,9 @@ can fall back to the reference.  `run(sim, outdir, prefix)` sets the engine up, 
 photon packets and writes `<prefix>_<instrument>_total.fits` per frame as
 `FrameInstrument`/`FluxRecorder` do (`FluxRecorder.cpp:876-960`): surface brightness in
 MJy/sr (ExtragalacticUnits with `fluxOutputStyle="Frequency"`), float32, one plane per
-wavelength; and `<prefix>_<instrument>_sed.dat` per SED instrument, the text columns of
+wavelength; `<prefix>_<probe>_J.dat` per `RadiationFieldProbe` (`PerCellForm`) when the
+radiation field is stored (mean intensity J_nu per cell); and
+`<prefix>_<instrument>_sed.dat` per SED instrument, the text columns of
 `FluxRecorder.cpp:1093-1115` (wavelength in micron, total flux density F_nu in Jy =
 recorded luminosity / (4 pi d^2) / (bin width) * lambda^2 / c * 1e26).
 
@@ -116,11 +118,13 @@ class Simulation:
     grid: dict
     box: tuple
     frames: List[Frame] = field(default_factory=list)
+    rf_probes: List[str] = field(default_factory=list)  # RadiationFieldProbe names (PerCellForm)
     force_scattering: bool = True
     min_weight_reduction: float = 1e4
     min_scatt_events: int = 0
     path_length_bias: float = 0.5
     num_density_samples: int = 100
+    cell_volumes: object = None  # set by setup(): volume per cell (m3)
 
 
 def _one(root, tag):
@@ -201,6 +205,14 @@ def load(path: str) -> Simulation:
         raise UnsupportedSki(f"grid {grid_el.tag} is not supported by the ski front end")
     if not instruments:
         raise UnsupportedSki("no instruments")
+    # radiation field: RadiationFieldOptions storeRadiationField and per-cell probes of it
+    rfo = root.findall(".//RadiationFieldOptions")
+    store_rf = bool(rfo) and rfo[0].get("storeRadiationField", "false") == "true"
+    rf_probes = []
+    for pr in root.findall(".//probes/*"):
+        if pr.tag != "RadiationFieldProbe" or pr.find(".//PerCellForm") is None or not store_rf:
+            raise UnsupportedSki("only RadiationFieldProbe with PerCellForm (and a stored radiation field) is supported")
+        rf_probes.append(pr.get("probeName"))
     frames = []
     for fi in instruments:
         if fi.tag == "SEDInstrument":
@@ -219,7 +231,7 @@ def load(path: str) -> Simulation:
         norm_specific_luminosity=quantity(norm.get("specificLuminosity")), spiral=bool(spiral_el),
         scale_length=quantity(disk.get("scaleLength")), scale_height=quantity(disk.get("scaleHeight")),
         spiral_params=spiral, dust_table=tab, tau_v=float(mn.get("opticalDepth")),
-        tau_wavelength=quantity(mn.get("wavelength")), grid=grid, box=box, frames=frames,
+        tau_wavelength=quantity(mn.get("wavelength")), grid=grid, box=box, frames=frames, rf_probes=rf_probes,
         force_scattering=opts.get("forceScattering", "true") == "true",
         min_weight_reduction=float(opts.get("minWeightReduction", "1e4")),
         min_scatt_events=int(opts.get("minScattEvents", "0")),
@@ -276,6 +288,24 @@ def write_sed(path, values, sed: Sed, lam):
         f.write("# column 2: total flux; F_nu (Jy)\n")
         for l, v in zip(lam, values):
             f.write(f"{l * 1e6:.9e} {v:.9e}\n")
+
+
+def write_mean_intensity(path, J, lam):
+    """RadiationFieldProbe with PerCellForm: J_nu per cell and wavelength (W/m2/Hz/sr)."""
+    with open(path, "w") as f:
+        f.write("# Mean intensity per spatial cell\n")
+        f.write("# column 1: spatial cell index (1)\n")
+        for c, l in enumerate(lam):
+            f.write(f"# column {c + 2}: J_nu at lambda = {_smart(l * 1e6)} micron (W/m2/Hz/sr)\n")
+        for m in range(J.shape[0]):
+            f.write(str(m) + "".join(f" {v:.9e}" for v in J[m]) + "\n")
+
+
+def mean_intensity(sim: Simulation, rf):
+    """MediumSystem::meanIntensity (MediumSystem.cpp:2670-2680): the stored field / (4 pi V
+    Δλ), as J_nu = J_lambda * lambda^2 / c."""
+    lam = sim.wavelengths
+    return rf / (4.0 * math.pi * sim.cell_volumes[:, None]) / bin_widths(lam)[None, :] * (lam * lam / C_LIGHT)[None, :]
 
 
 def read_sed(path):
@@ -358,12 +388,21 @@ def setup(sim: Simulation, engine, threads=0, npk_norm=None):
             borders.append(t * (h - (-h)) + (-h))
         engine.grid_cartesian(*borders)
         dens = pkg.sample_density_cartesian(*borders, model, sim.num_density_samples, sim.seed + 1, threads)
+        # cell volumes, cell index k + nz*j + nz*ny*i (CartesianSpatialGrid.cpp:200-203)
+        dx, dy, dz = (np.diff(bb) for bb in borders)
+        sim.cell_volumes = (dx[:, None, None] * dy[None, :, None] * dz[None, None, :]).ravel()
     else:
         gr = sim.grid
         tree = pkg.Octree.build(list(sim.box), model, gr["min_level"], gr["max_level"], gr["max_fraction"],
                                 sim.num_density_samples, sim.seed + 1, threads)
         tree.upload(engine)
         dens = tree.sample_density(model, sim.num_density_samples, sim.seed + 2, threads)
+        ex = tree.export()
+        leaf = ex["cell_index"] >= 0
+        bnd = ex["bounds"].reshape(-1, 6)[leaf]
+        vol = np.empty(int(leaf.sum()))
+        vol[ex["cell_index"][leaf]] = np.prod(bnd[:, 3:] - bnd[:, :3], axis=1)
+        sim.cell_volumes = vol
     engine.medium(dens)
     engine.dust(sim.wavelengths, sext, ssca, g)
     engine.frame_remove_all()
@@ -376,6 +415,8 @@ def setup(sim: Simulation, engine, threads=0, npk_norm=None):
     npk = npk_norm or sim.num_packets
     engine.source_point(sim.source_position, bin_luminosities(sim) * len(sim.wavelengths) / npk)
     engine.options(sim.path_length_bias, sim.min_weight_reduction, sim.min_scatt_events, 0, sim.seed)
+    # the persistent engine: the faster one here, and the one that stores the radiation field
+    engine.tune_pipeline(pkg.SKG_MODE_PERSISTENT, 1 << 21, 0, 8)
 
 
 def run(sim: Simulation, outdir: str, prefix: str, engine=None, chunk=1 << 22, threads=0, log=print):
@@ -388,6 +429,8 @@ def run(sim: Simulation, outdir: str, prefix: str, engine=None, chunk=1 << 22, t
     setup(sim, eng, threads)
     log(f"set-up {time.time() - t0:.1f} s")
     eng.frame_clear()
+    if sim.rf_probes:
+        eng.rf_enable(True)
     t1 = time.time()
     done = 0
     while done < sim.num_packets:
@@ -406,6 +449,12 @@ def run(sim: Simulation, outdir: str, prefix: str, engine=None, chunk=1 << 22, t
         cube = eng.frame_read(i) * calibration(sim, f)[:, None, None]
         write_fits(os.path.join(outdir, f"{prefix}_{f.name}_total.fits"), cube, f, sim.wavelengths)
         out[f.name] = cube
+    if sim.rf_probes:
+        J = mean_intensity(sim, eng.rf_read())
+        eng.rf_enable(False)
+        for name in sim.rf_probes:
+            write_mean_intensity(os.path.join(outdir, f"{prefix}_{name}_J.dat"), J, sim.wavelengths)
+        out["J"] = J
     if own:
         eng.close()
     return out

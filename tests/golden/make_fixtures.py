@@ -29,6 +29,10 @@ Every fixture is data: seeded inputs plus the reference's outputs for them.
   sed_cfg0.npz     (--seds) the same workload with two SEDInstruments at 60 deg (no
                    aperture, 3 pc aperture): F_nu mean and standard error over independent
                    runs; sed_cfg0_sed60.dat is one run's SED text output as written.
+  rf_cfg0.npz      (--rf) the same workload on a 16^3 grid (1e4 density samples per cell)
+                   storing the radiation field:
+                   the RadiationFieldProbe's mean intensity J_nu per cell (PerCellForm),
+                   mean and standard error over independent runs, and the file's header.
 """
 
 from __future__ import annotations
@@ -463,10 +467,33 @@ def fixtures_seds(tmp, runs=8, packets=1e6):
         g.write(f.read())
 
 
+def fixtures_rf(tmp, runs=8, packets=1e6):
+    outdir = os.path.join(tmp, "rf")
+    os.makedirs(outdir, exist_ok=True)
+    Js, header = [], None
+    for s in range(runs):
+        spec = skigen.config0(num_packets=packets, seed=4000 + s)
+        spec.grid = ("cartesian", 16)
+        spec.store_rf = True
+        # many density samples per cell: the sampled densities of the coarse cells around the
+        # thin disk's midplane otherwise differ enough from run to run to shift the field of
+        # a whole half of the grid together
+        spec.num_density_samples = 10000
+        ski = write_ski(tmp, f"rf{s}", spec)
+        run("run", ski, outdir, 8)
+        lines = open(os.path.join(outdir, f"rf{s}_rf_J.dat")).read().splitlines()
+        header = [ln for ln in lines if ln.startswith("#")]
+        Js.append(np.array([[float(x) for x in ln.split()[1:]] for ln in lines if not ln.startswith("#")]))
+    Js = np.array(Js)
+    np.savez(os.path.join(HERE, "rf_cfg0.npz"), mean=Js.mean(axis=0), sem=Js.std(axis=0, ddof=1) / np.sqrt(runs),
+             runs=np.array(runs), packets=np.array(packets), header=np.array("\n".join(header)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--images", action="store_true")
     ap.add_argument("--seds", action="store_true")
+    ap.add_argument("--rf", action="store_true")
     ap.add_argument("--only", default="")
     a = ap.parse_args()
     with tempfile.TemporaryDirectory() as tmp:
@@ -485,6 +512,8 @@ def main():
             fixtures_images(tmp)
         if a.seds:
             fixtures_seds(tmp)
+        if a.rf:
+            fixtures_rf(tmp)
 
 
 if __name__ == "__main__":

@@ -56,3 +56,48 @@ def test_config0_ski_run_matches_reference_fits(tmp_path):
     assert m.sum() > 300
     assert np.mean(z ** 2) < 1.5
     assert np.mean(np.abs(z) > 4) < 0.01
+
+
+def test_config0_mean_intensity_matches_reference(tmp_path):
+    """The stored radiation field through the ski front end (RadiationFieldProbe, PerCellForm)
+    on the config-0 disk with a 16^3 grid: J_nu per cell against the reference's probe output
+    (8 runs of 1e6 packets): per cell, the difference of the means over the standard error
+    from the run-to-run variance pooled over both sets (a t variable with 12 degrees of
+    freedom, E[t^2] = 1.2): mean t^2 < 1.5, few cells beyond 4; the total within 4 sigma.
+    Both sides sample 1e4 densities per cell, so that the run-to-run noise of the coarse
+    cells around the thin disk's midplane does not shift whole halves of the grid together."""
+    pkg = load_pkg()
+    from skirt9_gpu_amd import ski as S
+    spec = skigen.config0(num_packets=1e6)
+    spec.grid = ("cartesian", 16)
+    spec.store_rf = True
+    spec.num_density_samples = 10000  # as the reference runs (see make_fixtures.fixtures_rf)
+    path = os.path.join(str(tmp_path), "rf.ski")
+    with open(path, "w") as f:
+        f.write(skigen.render(spec))
+    sim = S.load(path)
+    eng = pkg.Engine(0)
+    nb = 6
+    Js = []
+    for b in range(nb):
+        sim.seed = 5000 + b
+        print("case ski rf batch", b, flush=True)
+        out = S.run(sim, str(tmp_path), f"b{b}", engine=eng, log=lambda *a: None)
+        Js.append(out["J"][:, 0])
+    eng.close()
+    back = np.loadtxt(os.path.join(str(tmp_path), "b0_rf_J.dat"))
+    np.testing.assert_allclose(back[:, 1], Js[0], rtol=1e-9)
+    f = golden("rf_cfg0.npz")
+    ref, rsem = f["mean"][:, 0], f["sem"][:, 0]
+    Js = np.array(Js)
+    # run-to-run variance pooled over both sets of runs (same packets per run)
+    nref = int(f["runs"])
+    pooled = ((nref - 1) * rsem ** 2 * nref + (nb - 1) * Js.var(axis=0, ddof=1)) / (nref + nb - 2)
+    sem = np.sqrt(pooled * (1.0 / nb + 1.0 / nref))
+    ok = sem > 0
+    z = (Js.mean(0) - ref)[ok] / sem[ok]
+    assert ok.sum() > 3000
+    assert np.mean(z ** 2) < 1.5, np.mean(z ** 2)
+    assert np.mean(np.abs(z) > 4) < 0.01
+    vt = Js.sum(axis=1)
+    assert abs(vt.mean() - ref.sum()) < 4 * np.hypot(vt.std(ddof=1) / np.sqrt(nb), np.sqrt((rsem ** 2).sum()))

@@ -111,3 +111,31 @@ def test_sed_instruments_parse_and_write(ski, tmp_path):
     lam = sim.wavelengths[0]
     want = 1 / (4 * np.pi * s0.distance ** 2) / ski.bin_widths(sim.wavelengths)[0] * lam * lam / 2.99792458e8 * 1e26
     assert abs(ski.sed_calibration(sim, s0)[0] / want - 1) < 1e-15
+
+
+def test_radiation_field_probe_parse_and_header(ski, tmp_path):
+    """RadiationFieldOptions storeRadiationField with a RadiationFieldProbe (PerCellForm):
+    parsed; the mean-intensity file's header equals the reference's (tests/golden/rf_cfg0.npz);
+    J_nu = rf / (4 pi V dlambda) * lambda^2 / c; other probes are refused."""
+    spec = skigen.config0(num_packets=1e5)
+    spec.grid = ("cartesian", 16)
+    spec.store_rf = True
+    p = os.path.join(str(tmp_path), "rf.ski")
+    with open(p, "w") as f:
+        f.write(skigen.render(spec))
+    sim = ski.load(p)
+    assert sim.rf_probes == ["rf"]
+    out = os.path.join(str(tmp_path), "rf_J.dat")
+    ski.write_mean_intensity(out, np.array([[1.5e-26], [2.0e-27]]), sim.wavelengths)
+    lines = open(out).read().splitlines()
+    assert "\n".join(lines[:3]) == str(golden("rf_cfg0.npz")["header"])
+    assert lines[3:] == ["0 1.500000000e-26", "1 2.000000000e-27"]
+    sim.cell_volumes = np.array([2.0, 4.0])
+    lam = sim.wavelengths[0]
+    J = ski.mean_intensity(sim, np.array([[3.0], [3.0]]))
+    want = 3.0 / (4 * np.pi * np.array([2.0, 4.0])) / ski.bin_widths(sim.wavelengths)[0] * lam * lam / 2.99792458e8
+    np.testing.assert_allclose(J[:, 0], want, rtol=1e-15)
+    with open(p, "w") as f:
+        f.write(skigen.render(spec).replace("<PerCellForm/>", "<DefaultCutsForm/>"))
+    with pytest.raises(ski.UnsupportedSki):
+        ski.load(p)

@@ -702,23 +702,33 @@ namespace skg
 
     __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
-// Minimum waves per SIMD the persistent kernel is compiled for.  Two waves give it 256 VGPRs:
-// the event transitions (FP64 transcendentals on a loaded packet) need far more than the
-// crossing loop, and at three or four waves the register spills to scratch cost more than the
-// extra latency hiding gains (measured on the octree workload: 22.9 M packets/s at two waves,
-// 22.1 M at three, 14.0 M at four).
+// Minimum waves per SIMD of the persistent kernel on Cartesian and octree grids.  Two waves
+// give it 256 VGPRs: the event transitions (FP64 transcendentals on a loaded packet) need far
+// more than the crossing loop, and on the octree at three or four waves the register spills to
+// scratch cost more than the extra latency hiding gains.
 #ifndef SKG_LIFECYCLE_WAVES
 #define SKG_LIFECYCLE_WAVES 2
 #endif
+
+    // Minimum waves per SIMD per grid type.  The Voronoi and tetra kernels hold less
+    // traversal state and stage no octree nodes in LDS (35 KB per workgroup): measured on the
+    // clumpy Voronoi workload 1735 / 1703 / 1800 ms per launch at 2 / 3 / 4 waves, on the
+    // batched tetra workload 6.72 / 5.80 / 5.27 s.  The octree kernel gains nothing from a
+    // third wave (147.4 vs 148.2 ms, with the top-node table halved to fit three workgroups).
+    template<int GT> constexpr int lifecycle_waves()
+    {
+        return GT == kTetra ? 4 : GT == kVoronoi ? 3 : SKG_LIFECYCLE_WAVES;
+    }
+
     template<int GT, int FP, int V>
-    __global__ void __launch_bounds__(kLcThreads, SKG_LIFECYCLE_WAVES) lifecycle(SimParams p)
+    __global__ void __launch_bounds__(kLcThreads, lifecycle_waves<GT>()) lifecycle(SimParams p)
     {
         constexpr bool INST = V == kBatched;
         __shared__ unsigned int wgcnt[8];
         __shared__ double coldD[kColdF64 * kLcThreads];
         __shared__ uint32_t coldU[kColdU32 * kLcThreads];
         if (threadIdx.x < 8) wgcnt[threadIdx.x] = 0;
-        stage_top(p.grid);
+        if constexpr (GT == kOctree) stage_top(p.grid);  // other grids allocate no top-node LDS
         const int t = threadIdx.x;
         Trav<GT, FP> T;
         T.phase = kFetch;

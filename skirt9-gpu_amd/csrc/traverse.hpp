@@ -94,7 +94,10 @@ namespace skg
         int32_t pad;
     };
     static_assert(sizeof(TopNode) == 32, "top node must be 32 bytes");
-    constexpr int kTopNodes = 1024;  // 32 KB of LDS: levels 0-3 of a full octree and part of level 4
+#ifndef SKG_TOP_NODES
+#define SKG_TOP_NODES 1024
+#endif
+    constexpr int kTopNodes = SKG_TOP_NODES;  // 1024: 32 KB of LDS, levels 0-3 of a full octree and part of level 4
 
     // Read-only grid tables resident in HBM.  Octree: node-indexed bounds, first child
     // (children are 8 consecutive ids) and cell index for the top-down search, plus the

@@ -63,15 +63,18 @@ def host_threads():
     return max(1, min(16, n))
 
 
-def cpu_baseline_reference(tau_v, nlambda, lam_range, packets, threads):
-    """Times the reference SKIRT build on the same workload (its own tree and densities);
+def cpu_baseline_reference(tau_v, nlambda, lam_range, packets, threads, cartesian=False):
+    """Times the reference SKIRT build on the same workload (its own grid and densities);
     packets/s over the primary emission phase, as logged by the reference."""
     exe = os.path.join(ROOT, "oracle", "_ref", "bin", "skirt", "ref", "skirt_ref")
     if not os.path.exists(exe):
         return None
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import skigen
-    spec = skigen.config1(num_packets=packets, tau_v=tau_v, nlambda=nlambda, lam_range=lam_range)
+    if cartesian:
+        spec = skigen.config0(num_packets=packets)
+    else:
+        spec = skigen.config1(num_packets=packets, tau_v=tau_v, nlambda=nlambda, lam_range=lam_range)
     with tempfile.TemporaryDirectory() as tmp:
         ski = os.path.join(tmp, "bench.ski")
         with open(ski, "w") as f:
@@ -85,16 +88,20 @@ def cpu_baseline_reference(tau_v, nlambda, lam_range, packets, threads):
         return None
     t = float(m.group(1))
     return dict(value=packets / t, unit="packets/s", cores=threads, kind="reference",
-                sample=f"{packets:.0f} packets of the same workload (reference-built tree), primary emission "
+                sample=f"{packets:.0f} packets of the same workload (grid and densities built by the reference), "
+                       f"primary emission "
                        f"{t:.2f} s")
 
 
 def cpu_baseline_port(wl, packets, threads, seed):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    ex = wl.octree.export()
-    g = O.Grid("octree", bounds=ex["bounds"], child_first=ex["child_first"], cell_index=ex["cell_index"],
-               nb_off=ex["nb_off"], nb_id=ex["nb_id"])
+    if wl.grid == "cartesian":
+        g = O.Grid("cartesian", xb=wl.tables["xb"], yb=wl.tables["yb"], zb=wl.tables["zb"])
+    else:
+        ex = wl.octree.export()
+        g = O.Grid("octree", bounds=ex["bounds"], child_first=ex["child_first"], cell_index=ex["cell_index"],
+                   nb_off=ex["nb_off"], nb_id=ex["nb_id"])
     t0 = time.perf_counter()
     O.simulate(g, wl.density, wl.lam, wl.sext, wl.ssca, wl.g, np.ones(len(wl.lam)), wl.frames, 0, packets,
                threads=threads, seed=seed)
@@ -201,9 +208,10 @@ def main():
     ap.add_argument("--pool", type=int, default=1 << 21, help="packet slots in flight (pipeline)")
     ap.add_argument("--trace-wgs", type=int, default=0)
     ap.add_argument("--graph-iters", type=int, default=8)
-    ap.add_argument("--workload", choices=["octree", "voronoi", "tetra-batch"], default="octree",
+    ap.add_argument("--workload", choices=["octree", "cartesian", "voronoi", "tetra-batch"], default="octree",
                     help="octree: the spiral disk at face-on tau_V (the measured configuration); voronoi: "
                          "1e6-site Voronoi grid in a lognormal clumpy medium, mean tau_V 5, three frames; "
+                         "cartesian: the 64^3 disk at face-on tau_V=1, 0.55 um, the reference's CPU-runnable case; "
                          "tetra-batch: 256 independent disk instances (tau_V 0.1..50, inclination 0..90 deg) on "
                          "a 2e5-point Delaunay mesh in one launch per step")
     ap.add_argument("--batch-scale", type=float, default=0.1,
@@ -250,6 +258,14 @@ def main():
         ncells = int(wl.octree.info()[1])
         bytes_per_segment = BYTES_PER_SEGMENT
         wdesc = "octree spiral disk, face-on tauV=%g, %d wavelengths %g-%g um" % (a.tau, nlambda, *lam_range)
+    elif a.workload == "cartesian":
+        wl = workloads.cartesian_disk(threads=threads)
+        ncells = 64 ** 3
+        # Cartesian crossing: the cell's density (borders are cached)
+        bytes_per_segment = 8.0
+        nlambda, lam_range = 1, (0.55, 0.55)
+        wdesc = "Cartesian 64^3 exponential disk, face-on tauV=1, 0.55 um, albedo 0.6, g 0.6"
+        a.no_guide = True
     elif a.workload == "tetra-batch":
         wl = workloads.tetra_batch(packet_scale=a.batch_scale, threads=threads)
         ncells = int(wl.tables["ntets"])
@@ -337,7 +353,8 @@ def main():
     cpu = None
     if not a.no_cpu_baseline:
         try:
-            cpu = cpu_baseline_reference(a.tau, nlambda, lam_range, a.cpu_packets, threads)
+            cpu = cpu_baseline_reference(a.tau, nlambda, lam_range, a.cpu_packets, threads,
+                                         cartesian=a.workload == "cartesian")
         except Exception as ex:  # the reference build may be absent or fail on this host
             print(f"reference CPU baseline unavailable: {ex}", file=sys.stderr)
         if cpu is None:

@@ -197,7 +197,8 @@ int skg_sync(skg_engine* e);
 
 /* Device counters accumulated since the last skg_frame_clear: [0] histories, [1] peel-off
  * traversals, [2] forced-path traversals (first pass), [3] scattering events,
- * [4] segment crossings, [5] detector atomics, [6] second-pass traversals, [7] reserved. */
+ * [4] segment crossings, [5] detector atomics, [6] second-pass traversals, [7] crossings of
+ * the second passes (persistent engine; 0 in the pipeline engine). */
 int skg_counters(skg_engine* e, int64_t* out8);
 
 /* Launch shape of the persistent life-cycle kernel: workgroups (0 = automatic), threads per

@@ -387,6 +387,7 @@ namespace skg
             {
                 ended = true;
                 atomicAdd(lds + 4, static_cast<unsigned int>(steps));
+                if (phase == kFwd2) atomicAdd(lds + 7, static_cast<unsigned int>(steps));
                 return;
             }
             if (phase == kPeel)
@@ -454,6 +455,7 @@ namespace skg
                     sint = sprev + ((tint - tprev) / (tv - tprev)) * (sv - sprev);
                     ended = true;
                     atomicAdd(lds + 4, static_cast<unsigned int>(steps));
+                    atomicAdd(lds + 7, static_cast<unsigned int>(steps));
                 }
             }
         }

@@ -241,7 +241,10 @@ namespace skg
         kScatter = 5  // guided variant: peel-offs done, waiting for the wave's guided scattering
     };
 
-    constexpr int kGuideLanes = 16;            // guided variant: lanes per network evaluation round
+#ifndef SKG_GUIDE_LANES
+#define SKG_GUIDE_LANES 16
+#endif
+    constexpr int kGuideLanes = SKG_GUIDE_LANES;  // guided variant: lanes per network evaluation round
     constexpr int kMaxScatterings = 1 << 20;  // guard against non-terminating histories (NaN weights)
     constexpr int kMaxSteps = 1 << 26;        // guard against non-terminating traversals
 

@@ -315,6 +315,12 @@ int skg_sample_density_cartesian(const double* xb, int32_t nx, const double* yb,
                                  int32_t threads, double* out);
 int skg_sample_density_octree(const skg_octree* t, const skg_disk_model* model, int32_t num_samples,
                               uint64_t seed, int32_t threads, double* out);
+/* The same on the GPU for the engine's resident Cartesian, octree or tetrahedral grid: the
+ * same random positions per cell as the host samplers above (Philox stream per cell), the
+ * model evaluated on the device (results equal to the host's to FP64 rounding of the
+ * transcendental functions).  out receives ncells values (count >= ncells). */
+int skg_sample_density(skg_engine* e, const skg_disk_model* model, int32_t num_samples, uint64_t seed, double* out,
+                       int64_t count);
 /* The same for a tetrahedral grid given by its tables (as skg_grid_tetra's), positions drawn
  * uniformly in each tetra as Tetra::generatePosition does (TetraMeshSpatialGrid.cpp:164-173). */
 int skg_sample_density_tetra(int64_t ntets, const double* verts, const int32_t* tet_verts, const skg_disk_model* model,

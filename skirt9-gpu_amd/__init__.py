@@ -76,7 +76,7 @@ EXPORTED = [
     "skg_sample_density_voronoi", "skg_voronoi_free", "skg_autotune",
     "skg_tetra_build", "skg_tetra_info", "skg_tetra_export", "skg_tetra_upload", "skg_tetra_free",
     "skg_sample_density_tetra", "skg_instances", "skg_rf_enable", "skg_rf_clear", "skg_rf_read",
-    "skg_sed_add",
+    "skg_sed_add", "skg_sample_density",
 ]
 
 GUIDE_PARAMS = 11560
@@ -258,6 +258,14 @@ class Engine:
         self._chk(lib().skg_instances(self._h, C.c_int32(n), _p(pk, u64p), _p(sd, u64p), _p(ts, f64p), _p(lp, f64p),
                                       _p(fb, i32p), _p(fc, i32p)))
         return int(pk.sum())
+
+    def sample_density(self, model, num_samples=100, seed=1):
+        """Cell-averaged number densities of the resident grid's cells on the GPU
+        (skg_sample_density; Cartesian, octree and tetrahedral grids)."""
+        out = np.zeros(self.num_cells())
+        self._chk(lib().skg_sample_density(self._h, C.byref(model), C.c_int32(num_samples), C.c_uint64(seed),
+                                           _p(out, f64p), C.c_int64(len(out))))
+        return out
 
     def rf_enable(self, on=True):
         """Radiation-field storage (skg_rf_enable): a cleared ncells x nlambda table."""

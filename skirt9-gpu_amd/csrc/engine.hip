@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "../../include/skirt_mi355x.h"
+#include "disk_model.hpp"
 #include "lifecycle.hpp"
 #include "pipeline.hpp"
 #include "traverse.hpp"
@@ -166,6 +167,85 @@ namespace skg
         }
         res = (FP == SKG_FP64) ? tau : static_cast<double>(k32.sum);
         tau_obs[t] = inf ? INFINITY : res;
+    }
+
+    // ------------------------------------------------------------------ cell densities
+    constexpr uint32_t kTagDensity = 0x44454e53u;  // the host samplers' stream tag for cell densities
+
+    // Cell-averaged number density of the disk model in every cell of the resident grid from
+    // n random points per cell, drawn from the same Philox stream (seed, cell, tag) and placed
+    // the same way as the host samplers do (uniform in the Cartesian / octree cell box, x then
+    // y then z; folded barycentric weights in a tetra), as MediumSystem's PropertySampler
+    // (MediumSystem.cpp:129-155) averages a medium over random positions in each cell.
+    template<int GT>
+    __global__ void sample_density_kernel(GridView g, DiskModel md, int n, uint64_t seed, int64_t ncells, double* out)
+    {
+        const int64_t m = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+        if (m >= ncells) return;
+        Philox rng;
+        rng.init(seed, static_cast<uint64_t>(m), kTagDensity);
+        double sum = 0.;
+        if constexpr (GT == kTetra)
+        {
+            const TetCell& T = g.tcell[m];
+            for (int q = 0; q < n; ++q)
+            {
+                double s = rng.uniform(), t = rng.uniform(), u = rng.uniform();
+                if (s + t > 1.)
+                {
+                    s = 1. - s;
+                    t = 1. - t;
+                }
+                if (t + u > 1.)
+                {
+                    const double tt = u;
+                    u = 1. - s - t;
+                    t = 1. - tt;
+                }
+                else if (s + t + u > 1.)
+                {
+                    const double tt = u;
+                    u = s + t + u - 1.;
+                    s = 1. - t - tt;
+                }
+                const double a = 1. - s - t - u;
+                const double x = a * T.v[0][0] + s * T.v[1][0] + t * T.v[2][0] + u * T.v[3][0];
+                const double y = a * T.v[0][1] + s * T.v[1][1] + t * T.v[2][1] + u * T.v[3][1];
+                const double z = a * T.v[0][2] + s * T.v[1][2] + t * T.v[2][2] + u * T.v[3][2];
+                sum += md.geom(x, y, z);
+            }
+            out[m] = md.nnorm * sum / n;
+            return;
+        }
+        double b[6];
+        if constexpr (GT == kCartesian)
+        {
+            // cell index k + nz*j + nz*ny*i (CartesianSpatialGrid.cpp:200-203)
+            const int64_t i = m / (static_cast<int64_t>(g.nz) * g.ny), j = (m / g.nz) % g.ny, k = m % g.nz;
+            b[0] = g.xb[i];
+            b[1] = g.yb[j];
+            b[2] = g.zb[k];
+            b[3] = g.xb[i + 1];
+            b[4] = g.yb[j + 1];
+            b[5] = g.zb[k + 1];
+        }
+        else
+        {
+            const OctLeaf& L = g.leaf[m];
+            for (int a = 0; a < 3; ++a)
+            {
+                b[a] = L.lo[a];
+                b[3 + a] = L.hi[a];
+            }
+        }
+        for (int q = 0; q < n; ++q)
+        {
+            const double x = b[0] + rng.uniform() * (b[3] - b[0]);
+            const double y = b[1] + rng.uniform() * (b[4] - b[1]);
+            const double z = b[2] + rng.uniform() * (b[5] - b[2]);
+            sum += md.geom(x, y, z);
+        }
+        out[m] = md.nnorm * (sum / n);
     }
 
     // ------------------------------------------------------------------ the life cycle
@@ -1752,6 +1832,34 @@ int skg_instances(skg_engine* e, int32_t n, const uint64_t* packets, const uint6
     if (st) return st;
     e->hInst = std::move(h);
     e->instTotal = total;
+    return SKG_OK;
+}
+
+int skg_sample_density(skg_engine* e, const skg_disk_model* model, int32_t num_samples, uint64_t seed, double* out,
+                       int64_t count)
+{
+    if (!e || !model || num_samples < 1 || !out) return fail(e, SKG_ERR_ARGUMENT, "skg_sample_density: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    if (e->gridType < 0) return fail(e, SKG_ERR_STATE, "skg_sample_density: no grid");
+    if (e->gridType == SKG_GRID_VORONOI)
+        return fail(e, SKG_ERR_STATE, "skg_sample_density: Voronoi cells are sampled on the host");
+    if (count < e->ncells) return fail(e, SKG_ERR_CAPACITY, "skg_sample_density: the output holds fewer values than cells");
+    const DiskModel md(*model);
+    double* d = nullptr;
+    HIPCHK(e, hipMalloc(&d, static_cast<size_t>(e->ncells) * sizeof(double)));
+    dim3 grid(static_cast<unsigned>((e->ncells + 255) / 256)), block(256);
+    if (e->gridType == SKG_GRID_CARTESIAN)
+        hipLaunchKernelGGL(sample_density_kernel<kCartesian>, grid, block, 0, e->stream, e->grid, md, num_samples, seed, e->ncells, d);
+    else if (e->gridType == SKG_GRID_OCTREE)
+        hipLaunchKernelGGL(sample_density_kernel<kOctree>, grid, block, 0, e->stream, e->grid, md, num_samples, seed, e->ncells, d);
+    else
+        hipLaunchKernelGGL(sample_density_kernel<kTetra>, grid, block, 0, e->stream, e->grid, md, num_samples, seed, e->ncells, d);
+    hipError_t st = hipGetLastError();
+    if (st == hipSuccess) st = hipMemcpyAsync(out, d, static_cast<size_t>(e->ncells) * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+    if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+    hipFree(d);
+    if (st != hipSuccess) return fail(e, SKG_ERR_DEVICE, std::string("skg_sample_density: ") + hipGetErrorString(st));
     return SKG_OK;
 }
 

@@ -253,8 +253,10 @@ def main():
     from skirt9_gpu_amd import workloads
     threads = host_threads()
     nlambda, lam_range = a.nlambda, tuple(a.lam_range)
+    eng = pkg.Engine(0)
     if a.workload == "octree":
-        wl = workloads.octree_spiral(tau_v=a.tau, nlambda=nlambda, lam_range_um=lam_range, threads=threads)
+        # tree refinement estimates and cell densities on the GPU (the host builder's positions)
+        wl = workloads.octree_spiral(tau_v=a.tau, nlambda=nlambda, lam_range_um=lam_range, threads=threads, engine=eng)
         ncells = int(wl.octree.info()[1])
         bytes_per_segment = BYTES_PER_SEGMENT
         wdesc = "octree spiral disk, face-on tauV=%g, %d wavelengths %g-%g um" % (a.tau, nlambda, *lam_range)
@@ -287,7 +289,6 @@ def main():
         a.no_cpu_baseline = True
     print(f"setup: {ncells} cells, {wl.setup_s:.1f} s", file=sys.stderr, flush=True)
 
-    eng = pkg.Engine(0)
     wl.apply(eng, npk_norm=1.0)
     fp = pkg.SKG_FP64 if a.fp == "fp64" else pkg.SKG_FP32_KAHAN
     eng.options(0.5, 1e4, 0, fp, 12345)

@@ -315,6 +315,12 @@ int skg_sample_density_cartesian(const double* xb, int32_t nx, const double* yb,
                                  int32_t threads, double* out);
 int skg_sample_density_octree(const skg_octree* t, const skg_disk_model* model, int32_t num_samples,
                               uint64_t seed, int32_t threads, double* out);
+/* skg_octree_build with the per-node density estimates of each level computed on the
+ * engine's GPU (the same random positions per node; the subdivision and the neighbour lists
+ * on the host as before). */
+int skg_octree_build_gpu(skg_engine* e, const double* box, const skg_disk_model* model, int32_t min_level,
+                         int32_t max_level, double max_dust_fraction, int32_t num_samples, uint64_t seed,
+                         int32_t threads, skg_octree** out);
 /* The same on the GPU for the engine's resident Cartesian, octree or tetrahedral grid: the
  * same random positions per cell as the host samplers above (Philox stream per cell), the
  * model evaluated on the device (results equal to the host's to FP64 rounding of the

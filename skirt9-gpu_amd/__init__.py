@@ -76,7 +76,7 @@ EXPORTED = [
     "skg_sample_density_voronoi", "skg_voronoi_free", "skg_autotune",
     "skg_tetra_build", "skg_tetra_info", "skg_tetra_export", "skg_tetra_upload", "skg_tetra_free",
     "skg_sample_density_tetra", "skg_instances", "skg_rf_enable", "skg_rf_clear", "skg_rf_read",
-    "skg_sed_add", "skg_sample_density",
+    "skg_sed_add", "skg_sample_density", "skg_octree_build_gpu",
 ]
 
 GUIDE_PARAMS = 11560
@@ -490,6 +490,17 @@ class Octree:
 
     def __init__(self, handle):
         self._h = handle
+
+    @classmethod
+    def build_gpu(cls, engine, box, model, min_level, max_level, max_dust_fraction, num_samples=100, seed=1, threads=0):
+        """The same tree with each level's density estimates computed on the engine's GPU."""
+        b = _f64(box)
+        h = C.c_void_p()
+        engine._chk(lib().skg_octree_build_gpu(engine._h, _p(b, f64p), C.byref(model), C.c_int32(min_level),
+                                               C.c_int32(max_level), C.c_double(max_dust_fraction),
+                                               C.c_int32(num_samples), C.c_uint64(seed), C.c_int32(threads),
+                                               C.byref(h)))
+        return cls(h)
 
     @classmethod
     def build(cls, box, model, min_level, max_level, max_dust_fraction, num_samples=100, seed=1, threads=0):

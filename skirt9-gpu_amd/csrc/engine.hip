@@ -28,6 +28,7 @@
 #include "../../include/skirt_mi355x.h"
 #include "disk_model.hpp"
 #include "lifecycle.hpp"
+#include "octree_build.hpp"
 #include "pipeline.hpp"
 #include "traverse.hpp"
 
@@ -246,6 +247,28 @@ namespace skg
             sum += md.geom(x, y, z);
         }
         out[m] = md.nnorm * (sum / n);
+    }
+
+    // Mean normalised density of n boxes (6 bounds each) over `samples` uniform points, from the
+    // Philox stream (seed, id, tree tag): the per-node estimate of the octree refinement, the
+    // same positions as the host builder's
+    __global__ void box_sample_kernel(DiskModel md, int64_t n, const double* __restrict__ boxes,
+                                      const uint64_t* __restrict__ ids, int samples, uint64_t seed, double* out)
+    {
+        const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+        if (i >= n) return;
+        const double* b = boxes + 6 * i;
+        Philox rng;
+        rng.init(seed, ids[i], kTagTree);
+        double sum = 0.;
+        for (int q = 0; q < samples; ++q)
+        {
+            const double x = b[0] + rng.uniform() * (b[3] - b[0]);
+            const double y = b[1] + rng.uniform() * (b[4] - b[1]);
+            const double z = b[2] + rng.uniform() * (b[5] - b[2]);
+            sum += md.geom(x, y, z);
+        }
+        out[i] = sum / samples;
     }
 
     // ------------------------------------------------------------------ the life cycle
@@ -1861,6 +1884,56 @@ int skg_sample_density(skg_engine* e, const skg_disk_model* model, int32_t num_s
     hipFree(d);
     if (st != hipSuccess) return fail(e, SKG_ERR_DEVICE, std::string("skg_sample_density: ") + hipGetErrorString(st));
     return SKG_OK;
+}
+
+int skg_octree_build_gpu(skg_engine* e, const double* box, const skg_disk_model* model, int32_t min_level,
+                         int32_t max_level, double max_dust_fraction, int32_t num_samples, uint64_t seed,
+                         int32_t threads, skg_octree** out)
+{
+    if (!e || !model) return fail(e, SKG_ERR_ARGUMENT, "skg_octree_build_gpu: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    const DiskModel md(*model);
+    double* dBox = nullptr;
+    uint64_t* dIds = nullptr;
+    double* dRho = nullptr;
+    int64_t cap = 0;
+    auto release = [&]() {
+        if (dBox) hipFree(dBox);
+        if (dIds) hipFree(dIds);
+        if (dRho) hipFree(dRho);
+        dBox = nullptr;
+        dIds = nullptr;
+        dRho = nullptr;
+    };
+    // the builder's per-level estimates on the GPU; everything else as the host builder
+    auto sampler = [&](const double* boxes, const uint64_t* ids, int64_t n, double* rho) -> int {
+        if (n > cap)
+        {
+            release();
+            cap = std::max<int64_t>(n, 1 << 16);
+            if (hipMalloc(&dBox, cap * 6 * sizeof(double)) != hipSuccess || hipMalloc(&dIds, cap * sizeof(uint64_t)) != hipSuccess
+                || hipMalloc(&dRho, cap * sizeof(double)) != hipSuccess)
+                return fail(e, SKG_ERR_DEVICE, "skg_octree_build_gpu: allocation failed");
+        }
+        hipError_t st = hipMemcpyAsync(dBox, boxes, n * 6 * sizeof(double), hipMemcpyHostToDevice, e->stream);
+        if (st == hipSuccess) st = hipMemcpyAsync(dIds, ids, n * sizeof(uint64_t), hipMemcpyHostToDevice, e->stream);
+        if (st == hipSuccess)
+        {
+            hipLaunchKernelGGL(box_sample_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, e->stream, md, n,
+                               dBox, dIds, num_samples, seed, dRho);
+            st = hipGetLastError();
+        }
+        if (st == hipSuccess) st = hipMemcpyAsync(rho, dRho, n * sizeof(double), hipMemcpyDeviceToHost, e->stream);
+        if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+        if (st != hipSuccess) return fail(e, SKG_ERR_DEVICE, std::string("skg_octree_build_gpu: ") + hipGetErrorString(st));
+        return SKG_OK;
+    };
+    int st = skg::octree_build_with(box, model, min_level, max_level, max_dust_fraction, num_samples, seed, threads, out,
+                                    sampler);
+    release();
+    if (st != SKG_OK && e->err.empty()) e->err = "skg_octree_build_gpu failed";
+    return st;
 }
 
 int skg_rf_enable(skg_engine* e, int32_t on)

@@ -20,11 +20,11 @@
 
 #include "../../include/skirt_mi355x.h"
 #include "disk_model.hpp"
+#include "octree_build.hpp"
 #include "philox.hpp"
 
 namespace
 {
-    constexpr uint32_t kTagTree = 0x54524545u;     // stream tag for tree refinement samples
     constexpr uint32_t kTagDensity = 0x44454e53u;  // stream tag for cell density samples
 
     using Model = skg::DiskModel;
@@ -203,24 +203,16 @@ namespace
     }
 }
 
-extern "C" {
-
-double skg_disk_density(const skg_disk_model* model, double x, double y, double z)
-{
-    if (!model) return 0.;
-    Model md(*model);
-    return md.nnorm * md.geom(x, y, z);
-}
-
-int skg_octree_build(const double* box, const skg_disk_model* model, int32_t min_level, int32_t max_level,
-                     double max_dust_fraction, int32_t num_samples, uint64_t seed, int32_t threads, skg_octree** out)
+// the builder itself (C++ linkage; the host and GPU entry points pass their sampler)
+int skg::octree_build_with(const double* box, const skg_disk_model* model, int32_t min_level, int32_t max_level,
+                           double max_dust_fraction, int32_t num_samples, uint64_t seed, int32_t threads,
+                           skg_octree** out, const OctreeSampler& sampler)
 {
     if (!box || !model || !out || min_level < 0 || max_level < min_level || max_level > 20 || num_samples < 1
         || !(box[3] > box[0] && box[4] > box[1] && box[5] > box[2]))
         return SKG_ERR_ARGUMENT;
     try
     {
-        Model md(*model);
         threads = defaultThreads(threads);
         auto t = std::make_unique<skg_octree>();
         auto& v = t->nodes;
@@ -234,23 +226,35 @@ int skg_octree_build(const double* box, const skg_disk_model* model, int32_t min
         {
             const size_t n = lend - lbeg;
             std::vector<uint8_t> divide(n, 0);
-            parallelFor(static_cast<int64_t>(n), threads, [&](int64_t b, int64_t e) {
-                for (int64_t i = b; i < e; ++i)
+            // DensityTreePolicy::needsSubdivide (:82-170), maxDustFraction criterion: the nodes
+            // between the minimum and maximum level are sampled (stream id = node id)
+            std::vector<double> boxes;
+            std::vector<uint64_t> ids;
+            std::vector<size_t> at;
+            for (size_t i = 0; i < n; ++i)
+            {
+                const Node& nd = v[lbeg + i];
+                if (nd.level < min_level)
+                    divide[i] = 1;
+                else if (nd.level < max_level)
                 {
-                    const Node& nd = v[lbeg + i];
-                    // DensityTreePolicy::needsSubdivide (:82-170), maxDustFraction criterion
-                    if (nd.level < min_level)
-                        divide[i] = 1;
-                    else if (nd.level >= max_level)
-                        divide[i] = 0;
-                    else
-                    {
-                        double rho = sampleMean(md, nd.b, num_samples, seed, lbeg + i, kTagTree);
-                        double vol = (nd.b[3] - nd.b[0]) * (nd.b[4] - nd.b[1]) * (nd.b[5] - nd.b[2]);
-                        divide[i] = rho * vol > max_dust_fraction ? 1 : 0;
-                    }
+                    boxes.insert(boxes.end(), nd.b, nd.b + 6);
+                    ids.push_back(lbeg + i);
+                    at.push_back(i);
                 }
-            });
+            }
+            if (!ids.empty())
+            {
+                std::vector<double> rho(ids.size());
+                int st = sampler(boxes.data(), ids.data(), static_cast<int64_t>(ids.size()), rho.data());
+                if (st != SKG_OK) return st;
+                for (size_t q = 0; q < ids.size(); ++q)
+                {
+                    const double* b = boxes.data() + 6 * q;
+                    double vol = (b[3] - b[0]) * (b[4] - b[1]) * (b[5] - b[2]);
+                    divide[at[q]] = rho[q] * vol > max_dust_fraction ? 1 : 0;
+                }
+            }
             for (size_t i = 0; i < n; ++i)
                 if (divide[i]) subdivide(v, static_cast<int32_t>(lbeg + i));
             lbeg = lend;
@@ -284,6 +288,33 @@ int skg_octree_build(const double* box, const skg_disk_model* model, int32_t min
     {
         return SKG_ERR_CAPACITY;
     }
+}
+
+
+extern "C" {
+
+double skg_disk_density(const skg_disk_model* model, double x, double y, double z)
+{
+    if (!model) return 0.;
+    Model md(*model);
+    return md.nnorm * md.geom(x, y, z);
+}
+
+int skg_octree_build(const double* box, const skg_disk_model* model, int32_t min_level, int32_t max_level,
+                     double max_dust_fraction, int32_t num_samples, uint64_t seed, int32_t threads, skg_octree** out)
+{
+    if (!model) return SKG_ERR_ARGUMENT;
+    Model md(*model);
+    const int nthreads = defaultThreads(threads);
+    // the host sampler: the mean normalised density of each box over num_samples points
+    return skg::octree_build_with(box, model, min_level, max_level, max_dust_fraction, num_samples, seed, threads, out,
+                                  [&](const double* boxes, const uint64_t* ids, int64_t n, double* rho) {
+                                      parallelFor(n, nthreads, [&](int64_t b, int64_t e) {
+                                          for (int64_t i = b; i < e; ++i)
+                                              rho[i] = sampleMean(md, boxes + 6 * i, num_samples, seed, ids[i], skg::kTagTree);
+                                      });
+                                      return SKG_OK;
+                                  });
 }
 
 int skg_octree_info(const skg_octree* t, int64_t* nnodes, int64_t* ncells, int64_t* nneighbors)

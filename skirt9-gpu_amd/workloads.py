@@ -101,7 +101,9 @@ def cartesian_disk(n=64, tau_v=1.0, samples=100, seed=1, threads=0):
 
 
 def octree_spiral(tau_v=1.0, nlambda=40, lam_range_um=(0.1, 10.0), min_level=3, max_level=10, max_frac=1e-6,
-                  samples=100, seed=1, threads=0, frames=None):
+                  samples=100, seed=1, threads=0, frames=None, engine=None):
+    """With an engine, the tree's per-level density estimates and the cell densities are
+    computed on its GPU (the same random positions as on the host)."""
     import time
     pkg = _pkg()
     t0 = time.time()
@@ -113,8 +115,13 @@ def octree_spiral(tau_v=1.0, nlambda=40, lam_range_um=(0.1, 10.0), min_level=3, 
     sext, ssca, g = pkg.dust_meanlist(tab[:, 0], tab[:, 1], tab[:, 2], tab[:, 3], lam, MU_DUST)
     sv, _, _ = pkg.dust_meanlist(tab[:, 0], tab[:, 1], tab[:, 2], tab[:, 3], [0.55e-6], MU_DUST)
     model = pkg.disk_model(tau_v=tau_v, sext_v=float(sv[0]), spiral=True)
-    tree = pkg.Octree.build(box, model, min_level, max_level, max_frac, samples, seed, threads)
-    dens = tree.sample_density(model, samples, seed + 1, threads)
+    if engine is not None:
+        tree = pkg.Octree.build_gpu(engine, box, model, min_level, max_level, max_frac, samples, seed, threads)
+        tree.upload(engine)
+        dens = engine.sample_density(model, samples, seed + 1)
+    else:
+        tree = pkg.Octree.build(box, model, min_level, max_level, max_frac, samples, seed, threads)
+        dens = tree.sample_density(model, samples, seed + 1, threads)
     return Workload("octree_spiral", "octree", lam, sext, ssca, g, dens, octree=tree,
                     frames=frames if frames is not None else [frame(60.0)], tau_v=tau_v, setup_s=time.time() - t0)
 

@@ -291,6 +291,10 @@ int skg_voronoi_upload(skg_engine* e, const skg_voronoi* v);
  * does (:1005-1023: uniform in the cell's extent, kept if the site is the nearest). */
 int skg_sample_density_voronoi(const skg_voronoi* v, const double* lattice, int32_t nx, int32_t ny, int32_t nz,
                                int32_t samples, uint64_t seed, int32_t threads, double* out);
+/* The same for the density model: number densities averaged over `samples` positions per
+ * cell drawn the same way. */
+int skg_sample_density_voronoi_model(const skg_voronoi* v, const skg_disk_model* model, int32_t samples, uint64_t seed,
+                                     int32_t threads, double* out);
 void skg_voronoi_free(skg_voronoi* v);
 
 /* Tetrahedral grid built on the host: TetraMeshSpatialGrid's vertex preparation (the 8 domain

@@ -76,7 +76,7 @@ EXPORTED = [
     "skg_sample_density_voronoi", "skg_voronoi_free", "skg_autotune",
     "skg_tetra_build", "skg_tetra_info", "skg_tetra_export", "skg_tetra_upload", "skg_tetra_free",
     "skg_sample_density_tetra", "skg_instances", "skg_rf_enable", "skg_rf_clear", "skg_rf_read",
-    "skg_sed_add", "skg_sample_density", "skg_octree_build_gpu",
+    "skg_sed_add", "skg_sample_density", "skg_octree_build_gpu", "skg_sample_density_voronoi_model",
 ]
 
 GUIDE_PARAMS = 11560
@@ -435,6 +435,16 @@ class Voronoi:
 
     def upload(self, engine):
         engine._chk(lib().skg_voronoi_upload(engine._h, self._h))
+
+    def sample_density_model(self, model, samples=100, seed=1, threads=0):
+        """Cell-averaged number densities of a disk model (skg_sample_density_voronoi_model)."""
+        n = self.info()[0]
+        out = np.zeros(n)
+        st = lib().skg_sample_density_voronoi_model(self._h, C.byref(model), C.c_int32(samples), C.c_uint64(seed),
+                                                    C.c_int32(threads), _p(out, f64p))
+        if st != SKG_OK:
+            raise EngineError(f"skg_sample_density_voronoi_model failed with status {st}")
+        return out
 
     def sample_density(self, lattice, samples=100, seed=1, threads=0):
         lat = _f64(lattice)

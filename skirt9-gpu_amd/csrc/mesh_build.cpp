@@ -24,6 +24,7 @@
 #include <thread>
 #include <vector>
 
+#include "disk_model.hpp"
 #include "../../include/skirt_mi355x.h"
 #include "philox.hpp"
 
@@ -556,6 +557,48 @@ int skg_sample_density_voronoi(const skg_voronoi* v, const double* lattice, int3
                 ++got;
             }
             out[m] = got ? sum / got : 0.;
+        }
+    });
+    return SKG_OK;
+}
+
+int skg_sample_density_voronoi_model(const skg_voronoi* v, const skg_disk_model* model, int32_t samples, uint64_t seed,
+                                     int32_t threads, double* out)
+{
+    if (!v || !model || samples < 1 || !out) return SKG_ERR_ARGUMENT;
+    const skg::DiskModel md(*model);
+    const int64_t n = static_cast<int64_t>(v->sites.size() / 3);
+    parallelChunks(n, threads, [&](int64_t c0, int64_t c1) {
+        for (int64_t m = c0; m < c1; ++m)
+        {
+            // positions uniform in the cell: uniform in its extent, kept when the site is the
+            // nearest of the cell's neighbours (VoronoiMeshSnapshot::generatePosition, :1005-1023)
+            skg::Philox rng;
+            rng.init(seed, static_cast<uint64_t>(m), 0x564f524du);
+            const double* lo = &v->lo[3 * m];
+            const double* hi = &v->hi[3 * m];
+            const double* p = &v->sites[3 * m];
+            double sum = 0;
+            int got = 0;
+            for (int tries = 0; got < samples && tries < 100 * samples; ++tries)
+            {
+                double x = lo[0] + (hi[0] - lo[0]) * rng.uniform();
+                double y = lo[1] + (hi[1] - lo[1]) * rng.uniform();
+                double z = lo[2] + (hi[2] - lo[2]) * rng.uniform();
+                double d0 = (x - p[0]) * (x - p[0]) + (y - p[1]) * (y - p[1]) + (z - p[2]) * (z - p[2]);
+                bool inside = true;
+                for (int32_t q = v->nbBegin[m]; q < v->nbBegin[m] + v->nbCount[m] && inside; ++q)
+                {
+                    int32_t j = v->nbIndex[q];
+                    if (j < 0) continue;
+                    const double* s = &v->sites[3 * static_cast<int64_t>(j)];
+                    if ((x - s[0]) * (x - s[0]) + (y - s[1]) * (y - s[1]) + (z - s[2]) * (z - s[2]) < d0) inside = false;
+                }
+                if (!inside) continue;
+                sum += md.geom(x, y, z);
+                ++got;
+            }
+            out[m] = got ? md.nnorm * (sum / got) : 0.;
         }
     });
     return SKG_OK;

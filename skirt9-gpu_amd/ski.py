@@ -5,7 +5,8 @@ an oligochromatic `MonteCarloSimulation` with one `PointSource` (`BlackBodySED` 
 `SpecificLuminosityNormalization`), one `GeometricMedium` (`ExpDiskGeometry`, optionally
 inside a `SpiralStructureGeometryDecorator`) with a `MeanListDustMix` and an
 `OpticalDepthMaterialNormalization`, a `CartesianSpatialGrid` (`LinMesh` axes) or a
-`PolicyTreeSpatialGrid` with a `DensityTreePolicy`, `PhotonPacketOptions`,
+`PolicyTreeSpatialGrid` with a `DensityTreePolicy`, a `VoronoiMeshSpatialGrid` or a
+`TetraMeshSpatialGrid` with sites / vertices from the dust density, `PhotonPacketOptions`,
 `FrameInstrument`s and `SEDInstrument`s.  Anything else raises `UnsupportedSki`, so a caller
 can fall back to the reference.  `run(sim, outdir, prefix)` sets the engine up, runs the
 photon packets and writes `<prefix>_<instrument>_total.fits` per frame as
@@ -201,6 +202,14 @@ def load(path: str) -> Simulation:
             raise UnsupportedSki("only maxDustFraction refinement is supported")
         grid = dict(kind="octree", min_level=int(pol.get("minLevel")), max_level=int(pol.get("maxLevel")),
                     max_fraction=float(pol.get("maxDustFraction")))
+    elif grid_el.tag in ("VoronoiMeshSpatialGrid", "TetraMeshSpatialGrid"):
+        # vertices / sites sampled from the medium's density (policy DustDensity)
+        if grid_el.get("policy") != "DustDensity":
+            raise UnsupportedSki("only the DustDensity site policy is supported")
+        if grid_el.get("relaxSites", "false") != "false" or grid_el.get("refine", "false") != "false":
+            raise UnsupportedSki("site relaxation and tetra refinement are not supported")
+        key = "numSites" if grid_el.tag == "VoronoiMeshSpatialGrid" else "numSamples"
+        grid = dict(kind="voronoi" if grid_el.tag == "VoronoiMeshSpatialGrid" else "tetra", n=int(grid_el.get(key)))
     else:
         raise UnsupportedSki(f"grid {grid_el.tag} is not supported by the ski front end")
     if not instruments:
@@ -391,6 +400,25 @@ def setup(sim: Simulation, engine, threads=0, npk_norm=None):
         # cell volumes, cell index k + nz*j + nz*ny*i (CartesianSpatialGrid.cpp:200-203)
         dx, dy, dz = (np.diff(bb) for bb in borders)
         sim.cell_volumes = (dx[:, None, None] * dy[None, :, None] * dz[None, None, :]).ravel()
+    elif sim.grid["kind"] in ("voronoi", "tetra"):
+        # TetraMeshSpatialGrid / VoronoiMeshSpatialGrid with the DustDensity policy: points
+        # distributed as the medium (TetraMeshSpatialGrid.cpp:23-40), then the mesh
+        pts = W.disk_points(sim.grid["n"], sim.box, sim.scale_length, sim.scale_height, seed=sim.seed + 3,
+                            spiral=sim.spiral_params if sim.spiral else None)
+        if sim.grid["kind"] == "voronoi":
+            mesh = pkg.Voronoi.build(pts, sim.box, threads)
+            mesh.upload(engine)
+            dens = mesh.sample_density_model(model, sim.num_density_samples, sim.seed + 1, threads)
+            sim.cell_volumes = mesh.export()["volume"]
+        else:
+            mesh = pkg.Tetra.build(pts, sim.box)
+            mesh.upload(engine)
+            dens = engine.sample_density(model, sim.num_density_samples, sim.seed + 1)
+            ex = mesh.export()
+            V, T = ex["verts"], ex["tet_verts"]
+            a, b_, c, d = (V[T[:, q]] for q in range(4))
+            sim.cell_volumes = np.abs(np.einsum("ij,ij->i", np.cross(b_ - a, c - a), d - a)) / 6
+        mesh.close()
     else:
         gr = sim.grid
         tree = pkg.Octree.build(list(sim.box), model, gr["min_level"], gr["max_level"], gr["max_fraction"],

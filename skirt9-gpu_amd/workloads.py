@@ -168,13 +168,16 @@ def voronoi_clumpy(num_sites=1_000_000, lattice=256, tau_v=5.0, sigma_ln=1.0, sl
                     setup_s=time.time() - t0)
 
 
-def disk_points(n, box, hR=3 * PC, hz=0.3 * PC, seed=1):
+def disk_points(n, box, hR=3 * PC, hz=0.3 * PC, seed=1, spiral=None):
     """n positions distributed as the exponential disk: the distributions of
     SepAxGeometry::generatePosition (SepAxGeometry.cpp:11-19) with ExpDiskGeometry's
     randomCylRadius / randomZ (ExpDiskGeometry.cpp:46-68) -- cylindrical radius with density
     R exp(-R/hR), uniform azimuth, Laplace z -- drawn with numpy's samplers; positions outside
     the box are discarded and redrawn, as TetraMeshSpatialGrid's vertex sampling does
-    (TetraMeshSpatialGrid.cpp:23-40)."""
+    (TetraMeshSpatialGrid.cpp:23-40).  With spiral parameters (arms, pitch, radius_zero,
+    phase_zero, weight, index), the azimuth is redrawn until accepted with probability
+    perturbation(R, phi) / max, as SpiralStructureGeometryDecorator::generatePosition does
+    (SpiralStructureGeometryDecorator.cpp:33-46)."""
     rng = np.random.default_rng(seed)
     box = np.asarray(box, dtype=np.float64)
     out = np.empty((0, 3))
@@ -182,6 +185,20 @@ def disk_points(n, box, hR=3 * PC, hz=0.3 * PC, seed=1):
         m = 2 * (n - len(out)) + 1024
         R = rng.gamma(2.0, hR, size=m)
         phi = 2 * np.pi * rng.random(m)
+        if spiral:
+            import math
+            arms, N, w = spiral["arms"], spiral["index"], spiral["weight"]
+            cn = math.sqrt(math.pi) * math.gamma(N + 1.0) / math.gamma(N + 0.5)
+            c = 1.0 + (cn - 1.0) * w
+            gam = np.log(R / spiral["radius_zero"]) / math.tan(spiral["pitch"]) + spiral["phase_zero"] + 0.5 * math.pi / arms
+            todo = np.ones(m, bool)
+            while todo.any():
+                k = np.flatnonzero(todo)
+                ph = 2 * np.pi * rng.random(len(k))
+                pert = (1 - w) + w * cn * np.sin(0.5 * arms * (gam[k] - ph)) ** (2 * N)
+                ok = rng.random(len(k)) * c / pert <= 1
+                phi[k[ok]] = ph[ok]
+                todo[k[ok]] = False
         z = rng.exponential(hz, size=m) * np.where(rng.random(m) < 0.5, -1.0, 1.0)
         p = np.stack([R * np.cos(phi), R * np.sin(phi), z], axis=1)
         p = p[np.all((p > box[:3]) & (p < box[3:]), axis=1)]

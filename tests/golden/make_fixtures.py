@@ -29,6 +29,10 @@ Every fixture is data: seeded inputs plus the reference's outputs for them.
   sed_cfg0.npz     (--seds) the same workload with two SEDInstruments at 60 deg (no
                    aperture, 3 pc aperture): F_nu mean and standard error over independent
                    runs; sed_cfg0_sed60.dat is one run's SED text output as written.
+  image_mesh.npz   (--meshes) the config-0 disk on a Voronoi grid of 20000 sites and on a
+                   tetrahedral grid of 20000 vertices (DustDensity policy), 100x100 frame at
+                   60 deg: per-pixel mean and standard error over independent runs, each run
+                   with its own random sites and densities.
   rf_cfg0.npz      (--rf) the same workload on a 16^3 grid (1e4 density samples per cell)
                    storing the radiation field:
                    the RadiationFieldProbe's mean intensity J_nu per cell (PerCellForm),
@@ -489,11 +493,31 @@ def fixtures_rf(tmp, runs=8, packets=1e6):
              runs=np.array(runs), packets=np.array(packets), header=np.array("\n".join(header)))
 
 
+def fixtures_mesh_images(tmp, runs=8, packets=1e6):
+    outdir = os.path.join(tmp, "mesh")
+    os.makedirs(outdir, exist_ok=True)
+    out = {"runs": np.array(runs), "packets": np.array(packets)}
+    for kind in ("voronoi", "tetra"):
+        imgs = []
+        for s in range(runs):
+            spec = skigen.config0(num_packets=packets, seed=6000 + s)
+            spec.grid = (kind, 20000, "DustDensity")
+            spec.num_pixels = 100
+            ski = write_ski(tmp, f"{kind}{s}", spec)
+            run("run", ski, outdir, 8)
+            imgs.append(fits_image(os.path.join(outdir, f"{kind}{s}_i60_total.fits")))
+        imgs = np.array(imgs)
+        out[kind + "_mean"] = imgs.mean(axis=0).astype(np.float32)
+        out[kind + "_sem"] = (imgs.std(axis=0, ddof=1) / np.sqrt(runs)).astype(np.float32)
+    np.savez_compressed(os.path.join(HERE, "image_mesh.npz"), **out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--images", action="store_true")
     ap.add_argument("--seds", action="store_true")
     ap.add_argument("--rf", action="store_true")
+    ap.add_argument("--meshes", action="store_true")
     ap.add_argument("--only", default="")
     a = ap.parse_args()
     with tempfile.TemporaryDirectory() as tmp:
@@ -514,6 +538,8 @@ def main():
             fixtures_seds(tmp)
         if a.rf:
             fixtures_rf(tmp)
+        if a.meshes:
+            fixtures_mesh_images(tmp)
 
 
 if __name__ == "__main__":

@@ -101,3 +101,53 @@ def test_config0_mean_intensity_matches_reference(tmp_path):
     assert np.mean(np.abs(z) > 4) < 0.01
     vt = Js.sum(axis=1)
     assert abs(vt.mean() - ref.sum()) < 4 * np.hypot(vt.std(ddof=1) / np.sqrt(nb), np.sqrt((rsem ** 2).sum()))
+
+
+@pytest.mark.parametrize("kind", ["voronoi", "tetra"])
+def test_config0_on_mesh_grids_matches_reference(tmp_path, kind):
+    """The config-0 ski on a Voronoi grid of 20000 sites / a tetrahedral grid of 20000 vertices
+    sampled from the dust density (VoronoiMeshSpatialGrid / TetraMeshSpatialGrid, policy
+    DustDensity): our sites, mesh builders and cell densities, each batch with its own, against
+    the reference's images (8 runs, each with its own mesh; tests/golden/image_mesh.npz): total
+    within 4 sigma, 5x5-binned pooled chi-square per degree of freedom < 1.5."""
+    pkg = load_pkg()
+    from skirt9_gpu_amd import ski as S
+    spec = skigen.config0(num_packets=1e6)
+    spec.grid = (kind, 20000, "DustDensity")
+    spec.num_pixels = 100
+    path = os.path.join(str(tmp_path), f"{kind}.ski")
+    with open(path, "w") as f:
+        f.write(skigen.render(spec))
+    sim = S.load(path)
+    assert sim.grid == dict(kind=kind, n=20000)
+    eng = pkg.Engine(0)
+    nb, per = 6, 1_000_000
+    imgs = []
+    for b in range(nb):
+        sim.seed = 7000 + b
+        print("case ski mesh", kind, b, flush=True)
+        out = S.run(sim, str(tmp_path), f"{kind}{b}", engine=eng, log=lambda *a: None)
+        imgs.append(out["i60"][0])
+    eng.close()
+    f = golden("image_mesh.npz")
+    ref, rsem = f[kind + "_mean"][0].astype(float), f[kind + "_sem"][0].astype(float)
+    imgs = np.array(imgs)
+    nref = int(f["runs"])
+    tot = imgs.sum(axis=(1, 2))
+    ref_run_var_tot = (rsem ** 2).sum() * nref  # an underestimate of the run variance of the total (pixels correlate)
+    pooled_tot = ((nref - 1) * ref_run_var_tot + (nb - 1) * tot.var(ddof=1)) / (nref + nb - 2)
+    assert abs(tot.mean() - ref.sum()) < 4 * np.sqrt(pooled_tot * (1 / nb + 1 / nref)), (tot.mean(), ref.sum())
+
+    def binned(a, B=5):
+        n = a.shape[-1] // B
+        return a.reshape(*a.shape[:-2], n, B, n, B).sum(axis=(-1, -3))
+
+    oi = binned(imgs)
+    var_ref_run = binned((rsem ** 2)[None])[0] * nref
+    pooled = ((nref - 1) * var_ref_run + (nb - 1) * oi.var(0, ddof=1)) / (nref + nb - 2)
+    var = pooled * (1 / nb + 1 / nref)
+    m = var > 0
+    z = (oi.mean(0) - binned(ref[None])[0])[m] / np.sqrt(var[m])
+    assert m.sum() > 200
+    assert np.mean(z ** 2) < 1.5, np.mean(z ** 2)
+    assert np.mean(np.abs(z) > 4) < 0.01

@@ -82,9 +82,15 @@ def test_fits_round_trip(ski, tmp_path):
 
 
 def test_unsupported_items_raise(ski, tmp_path):
-    spec = skigen.SkiSpec(num_packets=1, grid=("voronoi", 100, "DustDensity"))
+    spec = skigen.SkiSpec(num_packets=1, grid=("voronoi", 100, "Uniform"))
     with pytest.raises(ski.UnsupportedSki):
         ski.load(_write(str(tmp_path), "v", spec))
+    p = _write(str(tmp_path), "f", skigen.SkiSpec(num_packets=1))
+    text = open(p).read().replace("<FrameInstrument ", "<PerspectiveInstrument ")
+    with open(p, "w") as f:
+        f.write(text)
+    with pytest.raises(ski.UnsupportedSki):
+        ski.load(p)
 
 
 def test_sed_instruments_parse_and_write(ski, tmp_path):
@@ -139,3 +145,43 @@ def test_radiation_field_probe_parse_and_header(ski, tmp_path):
         f.write(skigen.render(spec).replace("<PerCellForm/>", "<DefaultCutsForm/>"))
     with pytest.raises(ski.UnsupportedSki):
         ski.load(p)
+
+
+def test_mesh_grids_parse(ski, tmp_path):
+    """VoronoiMeshSpatialGrid / TetraMeshSpatialGrid with the DustDensity policy are read; other
+    policies and relaxation / refinement are refused."""
+    for kind in ("voronoi", "tetra"):
+        spec = skigen.config0(num_packets=1e5)
+        spec.grid = (kind, 5000, "DustDensity")
+        p = os.path.join(str(tmp_path), kind + ".ski")
+        text = skigen.render(spec)
+        with open(p, "w") as f:
+            f.write(text)
+        assert ski.load(p).grid == dict(kind=kind, n=5000)
+        with open(p, "w") as f:
+            f.write(text.replace('policy="DustDensity"', 'policy="Uniform"'))
+        with pytest.raises(ski.UnsupportedSki):
+            ski.load(p)
+
+
+def test_spiral_disk_points_follow_the_perturbation():
+    """workloads.disk_points with spiral parameters: azimuths accepted with probability
+    perturbation / max, so the azimuthal histogram at fixed radius follows the perturbation
+    profile (checked against its mean over the arm phase)."""
+    load_pkg()
+    from skirt9_gpu_amd import workloads as W
+    sp = dict(arms=2, pitch=np.radians(15.0), radius_zero=3 * PC, phase_zero=0.0, weight=0.5, index=1)
+    h = 10 * PC
+    pts = W.disk_points(200000, [-h, -h, -h, h, h, h], seed=5, spiral=sp)
+    pts = pts[np.hypot(pts[:, 0], pts[:, 1]) < 9 * PC]  # the box clips larger radii by azimuth
+    R = np.hypot(pts[:, 0], pts[:, 1])
+    phi = np.arctan2(pts[:, 1], pts[:, 0])
+    gam = np.log(R / sp["radius_zero"]) / np.tan(sp["pitch"]) + 0.5 * np.pi / 2
+    ph = np.mod(0.5 * 2 * (gam - phi), np.pi)  # arm phase in [0, pi)
+    hist, _ = np.histogram(ph, bins=8, range=(0, np.pi))
+    a = np.arange(8) * np.pi / 8
+    b = a + np.pi / 8
+    mean_sin2 = 0.5 - (np.sin(2 * b) - np.sin(2 * a)) / (4 * (b - a))  # mean of sin^2 over each bin
+    cn = np.sqrt(np.pi) * 1.0 / (np.sqrt(np.pi) / 2)  # Gamma(2) / Gamma(3/2)
+    prof = 0.5 + 0.5 * cn * mean_sin2
+    np.testing.assert_allclose(hist / hist.sum(), prof / prof.sum(), rtol=0.03)

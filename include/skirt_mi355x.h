@@ -137,6 +137,15 @@ int skg_frame_add(skg_engine* e, double inclination, double azimuth, double roll
  * 24-43; 0 = no aperture).  It shares the instrument list and indices with the frames
  * (skg_frame_read returns its nlambda values); returns its index or a negative status. */
 int skg_sed_add(skg_engine* e, double inclination, double azimuth, double radius);
+/* Adds a perspective instrument (PerspectiveInstrument.cpp:22-116): transform is the 4x4
+ * world -> (pixel x, pixel y, depth, w) matrix of PerspectiveInstrument::setupSelfBefore in
+ * HomogeneousTransform's row-vector convention (row-major, p' = (x, y, z, 1) M), eye the eye
+ * position, pixel_size the viewport pixel size.  A packet at r peels off toward the eye; it is
+ * recorded in pixel (int(x'/w'), int(y'/w')) when the depth z' exceeds pixel_size / 10, with
+ * luminosity / z'^2 attenuated over a path length z'.  Persistent engine; not combined with
+ * instances, the guide or radiation-field storage.  Returns the index or a negative status. */
+int skg_perspective_add(skg_engine* e, const double* transform, const double* eye, double pixel_size, int32_t nx,
+                        int32_t ny);
 int skg_frame_remove_all(skg_engine* e);
 /* Zeroes all frame accumulators. */
 int skg_frame_clear(skg_engine* e);

@@ -57,7 +57,8 @@ class OrcInstrument(C.Structure):
     _fields_ = [("inclination", C.c_double), ("azimuth", C.c_double), ("roll", C.c_double),
                 ("nx", C.c_int32), ("ny", C.c_int32),
                 ("fovx", C.c_double), ("fovy", C.c_double), ("cx", C.c_double), ("cy", C.c_double),
-                ("sed", C.c_int32), ("radius", C.c_double)]
+                ("sed", C.c_int32), ("radius", C.c_double), ("persp", C.c_int32), ("M", C.c_double * 16),
+                ("eye", C.c_double * 3), ("psize", C.c_double)]
 
 
 class OrcSim(C.Structure):
@@ -208,15 +209,21 @@ def simulate(grid: Grid, density, lam, sext, ssca, g, lpack, instruments, first,
     radiation field (ncells, nlambda) as a third item when store_rf.
 
     instruments: list of dicts with inclination, azimuth, roll (rad), nx, ny, fovx, fovy, cx, cy (m),
-    or, for an SED instrument, sed=True, inclination, azimuth and an aperture radius (m, 0 = none).
+    or, for an SED instrument, sed=True, inclination, azimuth and an aperture radius (m, 0 = none),
+    or, for a perspective instrument, persp=True, M (4x4 row-vector transform), eye, psize, nx, ny.
     Each frame has shape (nlambda, ny, nx) (an SED: (nlambda, 1, 1))."""
     arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (density, lam, sext, ssca, g, lpack)]
     ins = (OrcInstrument * len(instruments))()
     for i, d in enumerate(instruments):
         sed = bool(d.get("sed", False))
-        ins[i] = OrcInstrument(d["inclination"], d["azimuth"], d.get("roll", 0.0), 1 if sed else d["nx"],
-                               1 if sed else d["ny"], d.get("fovx", 1.0), d.get("fovy", 1.0), d.get("cx", 0.0),
-                               d.get("cy", 0.0), 1 if sed else 0, d.get("radius", 0.0))
+        persp = bool(d.get("persp", False))
+        ins[i] = OrcInstrument(d.get("inclination", 0.0), d.get("azimuth", 0.0), d.get("roll", 0.0),
+                               1 if sed else d["nx"], 1 if sed else d["ny"], d.get("fovx", 1.0), d.get("fovy", 1.0),
+                               d.get("cx", 0.0), d.get("cy", 0.0), 1 if sed else 0, d.get("radius", 0.0),
+                               1 if persp else 0,
+                               (C.c_double * 16)(*np.asarray(d["M"], dtype=np.float64).ravel()) if persp
+                               else (C.c_double * 16)(),
+                               (C.c_double * 3)(*d["eye"]) if persp else (C.c_double * 3)(), d.get("psize", 0.0))
     s = OrcSim()
     s.grid = C.pointer(grid.g)
     s.density = _p(arrs[0], f64p)

@@ -29,6 +29,9 @@ class SkiSpec:
     instruments: Sequence[Tuple[str, float, float]] = (("i60", 60.0, 0.0),)
     # SED instruments: (name, inclination deg, azimuth deg, aperture radius pc; 0 = none)
     sed_instruments: Sequence[Tuple[str, float, float, float]] = ()
+    # perspective instruments: (name, view (x,y,z) pc, crosshair (x,y,z) pc, up (x,y,z), focal pc,
+    # viewport width pc, pixels per side)
+    persp_instruments: Sequence = ()
     # store the radiation field and probe its mean intensity per cell (RadiationFieldProbe)
     store_rf: bool = False
     num_pixels: int = 200
@@ -92,7 +95,13 @@ def render(spec: SkiSpec) -> str:
         f'<SEDInstrument instrumentName="{name}" distance="10 Mpc" inclination="{inc:.9g} deg" '
         f'azimuth="{az:.9g} deg" roll="0 deg" radius="{rad:.9g} pc" recordComponents="false" '
         'numScatteringLevels="0" recordPolarization="false" recordStatistics="false"/>'
-        for (name, inc, az, rad) in spec.sed_instruments)
+        for (name, inc, az, rad) in spec.sed_instruments) + "".join(
+        f'<PerspectiveInstrument instrumentName="{name}" numPixelsX="{n}" numPixelsY="{n}" width="{w:.9g} pc" '
+        f'viewX="{v[0]:.9g} pc" viewY="{v[1]:.9g} pc" viewZ="{v[2]:.9g} pc" crossX="{c[0]:.9g} pc" '
+        f'crossY="{c[1]:.9g} pc" crossZ="{c[2]:.9g} pc" upX="{u[0]:.9g} pc" upY="{u[1]:.9g} pc" upZ="{u[2]:.9g} pc" '
+        f'focal="{fe:.9g} pc" recordComponents="false" numScatteringLevels="0" recordPolarization="false" '
+        'recordStatistics="false"/>'
+        for (name, v, c, u, fe, w, n) in spec.persp_instruments)
     force = "true" if spec.force_scattering else "false"
     rfopt = ('                <radiationFieldOptions type="RadiationFieldOptions">'
              '<RadiationFieldOptions storeRadiationField="true"/></radiationFieldOptions>\n') if spec.store_rf else ""

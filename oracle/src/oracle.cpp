@@ -952,6 +952,10 @@ struct orc_instrument
     double fovx, fovy, cx, cy;          // metres
     int32_t sed;                        // SEDInstrument (nx = ny = 1) with an aperture radius (m)
     double radius;
+    int32_t persp;                      // PerspectiveInstrument: transform (4x4 row-major,
+    double M[16];                       // row vector times M), eye and pixel size
+    double eye[3];
+    double psize;
 };
 
 struct orc_sim
@@ -1149,6 +1153,23 @@ namespace
         bool sameAsPrev;
         bool sed;
         double radius2;
+        bool persp;
+        double M[16], eye[3], psize;
+
+        // PerspectiveInstrument::detect (PerspectiveInstrument.cpp:103-116) with
+        // HomogeneousTransform::transform: pixel (or -1) and depth zp
+        int perspPixel(const double* r, double& zp) const
+        {
+            double x = r[0], y = r[1], z = r[2];
+            double xp = x * M[0] + y * M[4] + z * M[8] + 1. * M[12];
+            double yp = x * M[1] + y * M[5] + z * M[9] + 1. * M[13];
+            zp = x * M[2] + y * M[6] + z * M[10] + 1. * M[14];
+            double wp = x * M[3] + y * M[7] + z * M[11] + 1. * M[15];
+            int i = static_cast<int>(xp / wp);
+            int j = static_cast<int>(yp / wp);
+            if (i >= 0 && i < Nxp && j >= 0 && j < Nyp && zp > psize / 10.) return i + Nxp * j;
+            return -1;
+        }
 
         int pixel(const double* r) const
         {
@@ -1206,12 +1227,14 @@ namespace
         std::vector<int64_t> frameOff;
 
         // MediumSystem::getExtinctionOpticalDepth(pp, distance) for a distant observer
-        double observerTau(const double* r, const double* k, double section, double L, Generator& gen) const
+        double observerTau(const double* r, const double* k, double section, double L, Generator& gen,
+                           double distance = INFINITY) const
         {
             if (L <= 0) return INFINITY;
             double taumax = std::log(L) + 745;
             gen.start(r, k);
             double tau = 0.;
+            double s = 0.;
             while (gen.next())
             {
                 if (gen.m >= 0)
@@ -1219,6 +1242,8 @@ namespace
                     tau += section * p->density[gen.m] * gen.ds;
                     if (tau >= taumax) return INFINITY;
                 }
+                s += gen.ds;
+                if (s > distance) break;
             }
             return tau;
         }
@@ -1236,6 +1261,31 @@ namespace
             for (size_t f = 0; f < frames.size(); ++f)
             {
                 const Frame& fr = frames[f];
+                if (fr.persp)
+                {
+                    // PerspectiveInstrument::bfkobs (:93-101) and FluxRecorder::detect for a
+                    // local instrument (FluxRecorder.cpp:372-405)
+                    double zp;
+                    int l = fr.perspPixel(r, zp);
+                    double d[3] = {fr.eye[0] - r[0], fr.eye[1] - r[1], fr.eye[2] - r[2]};
+                    double D = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                    double ko[3] = {d[0] / D, d[1] / D, d[2] / D};
+                    if (!fr.sameAsPrev)
+                    {
+                        haveTau = false;
+                        Wp = W;
+                        if (scattering) Wp = W * peelPhase(p->g[ell], k[0] * ko[0] + k[1] * ko[1] + k[2] * ko[2]);
+                    }
+                    if (l < 0) continue;
+                    double L = Wp / lambda;
+                    if (!haveTau)
+                    {
+                        tauCached = observerTau(r, ko, section, L, gen, zp);
+                        haveTau = true;
+                    }
+                    frame[frameOff[f] + l + static_cast<int64_t>(ell) * fr.Nxp * fr.Nyp] += L / (zp * zp) * std::exp(-tauCached);
+                    continue;
+                }
                 if (!fr.sameAsPrev)
                 {
                     haveTau = false;
@@ -1412,6 +1462,20 @@ int orc_simulate(const orc_sim* p, uint64_t first, uint64_t count, int threads, 
         // DistantInstrument.cpp:55-63
         fr.sed = in.sed != 0;
         fr.radius2 = in.radius * in.radius;
+        fr.persp = in.persp != 0;
+        std::memcpy(fr.M, in.M, sizeof(fr.M));
+        std::memcpy(fr.eye, in.eye, sizeof(fr.eye));
+        fr.psize = in.psize;
+        if (f > 0 && (in.persp || p->instr[f - 1].persp))
+        {
+            const orc_instrument& q = p->instr[f - 1];
+            fr.sameAsPrev = in.persp && q.persp && std::memcmp(in.M, q.M, sizeof(in.M)) == 0
+                            && std::memcmp(in.eye, q.eye, sizeof(in.eye)) == 0 && in.psize == q.psize;
+            sim.frameOff.push_back(sim.npix_total);
+            sim.npix_total += static_cast<int64_t>(in.nx) * in.ny * p->nlambda;
+            sim.frames.push_back(fr);
+            continue;
+        }
         fr.sameAsPrev = f > 0 && in.inclination == p->instr[f - 1].inclination
                         && in.azimuth == p->instr[f - 1].azimuth && in.roll == p->instr[f - 1].roll;
         sim.frameOff.push_back(sim.npix_total);

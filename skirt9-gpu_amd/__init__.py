@@ -77,6 +77,7 @@ EXPORTED = [
     "skg_tetra_build", "skg_tetra_info", "skg_tetra_export", "skg_tetra_upload", "skg_tetra_free",
     "skg_sample_density_tetra", "skg_instances", "skg_rf_enable", "skg_rf_clear", "skg_rf_read",
     "skg_sed_add", "skg_sample_density", "skg_octree_build_gpu", "skg_sample_density_voronoi_model",
+    "skg_perspective_add",
 ]
 
 GUIDE_PARAMS = 11560
@@ -218,6 +219,15 @@ class Engine:
         """SED instrument (skg_sed_add): frame_read returns its (nlambda, 1, 1) values."""
         idx = self._chk(lib().skg_sed_add(self._h, C.c_double(inclination), C.c_double(azimuth), C.c_double(radius)))
         self.frames.append((1, 1))
+        return idx
+
+    def perspective_add(self, transform, eye, pixel_size, nx, ny):
+        """Perspective instrument (skg_perspective_add); transform: 4x4 row-vector matrix."""
+        M = _f64(transform).reshape(16)
+        E = _f64(eye).reshape(3)
+        idx = self._chk(lib().skg_perspective_add(self._h, _p(M, f64p), _p(E, f64p), C.c_double(pixel_size),
+                                                  C.c_int32(nx), C.c_int32(ny)))
+        self.frames.append((nx, ny))
         return idx
 
     def frame_remove_all(self):

@@ -301,6 +301,7 @@ namespace skg
         GuideFrame gframe;
         double guideMix;
         double* rf;               // radiation-field variant: ncells x nlambda table
+        int persp;                // some instruments are perspective instruments
     };
 
     // kernel variants of the life cycle
@@ -309,7 +310,8 @@ namespace skg
         kPlain = 0,      // one run
         kBatched = 1,    // batched independent instances
         kGuided = 2,     // scattering directions from the defensive guide / HG mixture
-        kStoreRF = 3     // one run that also stores the radiation field of the forced paths
+        kStoreRF = 3,    // one run that also stores the radiation field of the forced paths
+        kPersp = 4       // one run with perspective instruments among the instruments
     };
 
     // SpecialFunctions::lnmean(x1, x2, lnx1, lnx2) (SpecialFunctions.cpp:860-880): the
@@ -451,6 +453,7 @@ namespace skg
         int ell;        // radiation-field variant: the packet's bin, luminosity and the
         double rfL;     // extinction factor at the start of the current segment
         double rfExt;
+        double slim;    // perspective variant: path length after which a peel-off ends
 
         __device__ __forceinline__ double tau_value() const
         {
@@ -479,12 +482,13 @@ namespace skg
             ell = pk.ell;
             rfL = pk.W / pk.lam;
             rfExt = 1.;
+            slim = INFINITY;
         }
 
         // one cell crossing of the current traversal; with RF, the first forced pass also
         // stores the radiation field (MonteCarloSimulation::storeRadiationField, :909-925):
         // L * lnmean(e^-tau_end, e^-tau_begin) * ds per stored segment in a cell
-        template<bool RF>
+        template<bool RF, bool PERSP>
         __device__ __forceinline__ void step(const GridView& grid, unsigned int* lds, double* rf, int nl)
         {
             int m;
@@ -519,6 +523,17 @@ namespace skg
                             ended = true;
                             atomicAdd(lds + 4, static_cast<unsigned int>(steps));
                         }
+                    }
+                }
+                if constexpr (PERSP)
+                {
+                    // MediumSystem::getExtinctionOpticalDepth(pp, distance) (MediumSystem.cpp:
+                    // 2376-2382): the segment that passes the distance still counts
+                    s += ds;
+                    if (!ended && s > slim)
+                    {
+                        ended = true;
+                        atomicAdd(lds + 4, static_cast<unsigned int>(steps));
                     }
                 }
             }
@@ -660,6 +675,44 @@ namespace skg
             while (++pk.f < fe)
             {
                 const FrameGeom& fg = p.frames[pk.f];
+                if constexpr (V == kPersp)
+                {
+                    if (fg.persp)
+                    {
+                        // PerspectiveInstrument: toward the eye from this position
+                        // (PerspectiveInstrument.cpp:93-101), luminosity / zp^2 and the
+                        // optical depth over the path length zp (FluxRecorder.cpp:372-405)
+                        double zp;
+                        pk.pix = fg.persp_pixel(pk.rx, pk.ry, pk.rz, zp);
+                        const double dx = fg.eye[0] - pk.rx, dy = fg.eye[1] - pk.ry, dz = fg.eye[2] - pk.rz;
+                        const double D = sqrt(dx * dx + dy * dy + dz * dz);
+                        const double ox = dx / D, oy = dy / D, oz = dz / D;
+                        if (!fg.same_as_prev || pk.f == fb)
+                        {
+                            pk.haveTau = false;
+                            pk.Wp = pk.W;
+                            if (pk.scat) pk.Wp = pk.W * peel_phase(p.gpar[pk.ell], pk.kx * ox + pk.ky * oy + pk.kz * oz);
+                        }
+                        if (pk.pix < 0) continue;
+                        const double L = pk.Wp / pk.lam;
+                        if (pk.haveTau)
+                        {
+                            deposit(fg, L / (zp * zp), pk.tauCached);
+                            continue;
+                        }
+                        if (L <= 0)
+                        {
+                            pk.haveTau = true;
+                            pk.tauCached = INFINITY;
+                            continue;
+                        }
+                        T.taumax = log(L) + 745;
+                        atomicAdd(lds + 1, 1u);
+                        T.start(pk, ox, oy, oz, kPeel);
+                        T.slim = zp;
+                        return;
+                    }
+                }
                 if (!fg.same_as_prev || pk.f == fb)
                 {
                     pk.haveTau = false;
@@ -747,7 +800,17 @@ namespace skg
                 pk.haveTau = true;
                 pk.tauCached = T.tau_value();
                 const FrameGeom& fg = p.frames[pk.f];
-                deposit(fg, pk.Wp / pk.lam, pk.tauCached);
+                double L = pk.Wp / pk.lam;
+                if constexpr (V == kPersp)
+                {
+                    if (fg.persp)
+                    {
+                        double zp;
+                        fg.persp_pixel(pk.rx, pk.ry, pk.rz, zp);
+                        L /= zp * zp;
+                    }
+                }
+                deposit(fg, L, pk.tauCached);
                 next_peel();
             }
             else if (T.phase == kFwd1)
@@ -850,7 +913,7 @@ namespace skg
         {
             // advance every busy lane by one cell crossing
             bool busy = (T.phase == kPeel || T.phase == kFwd1 || T.phase == kFwd2) && !T.ended;
-            if (busy) T.template step<V == kStoreRF>(p.grid, wgcnt, p.rf, p.nlambda);
+            if (busy) T.template step<V == kStoreRF, V == kPersp>(p.grid, wgcnt, p.rf, p.nlambda);
 
             // wave-wide transition round once enough lanes wait
             unsigned long long waiting = __ballot(T.ended || T.phase == kFetch || (V == kGuided && T.phase == kScatter));
@@ -963,6 +1026,10 @@ struct skg_engine
         double fovx, fovy, cx, cy;
         int sed = 0;
         double radius = 0.;
+        int persp = 0;
+        double M[16] = {};
+        double eye[3] = {};
+        double psize = 0.;
     };
     std::vector<FrameDesc> frames;
     double* frameBuf = nullptr;
@@ -1132,6 +1199,17 @@ namespace
                              && d.roll == e->frames[f - 1].roll;
             g.sed = d.sed;
             g.radius2 = d.radius * d.radius;
+            g.persp = d.persp;
+            std::memcpy(g.M, d.M, sizeof(g.M));
+            std::memcpy(g.eye, d.eye, sizeof(g.eye));
+            g.psize = d.psize;
+            if (f > 0 && (d.persp || e->frames[f - 1].persp))
+            {
+                // PerspectiveInstrument::determineSameObserverAsPreceding: the same viewport
+                const auto& q = e->frames[f - 1];
+                g.same_as_prev = d.persp && q.persp && std::memcmp(d.M, q.M, sizeof(d.M)) == 0
+                                 && std::memcmp(d.eye, q.eye, sizeof(d.eye)) == 0 && d.psize == q.psize;
+            }
             g.offset = off;
             off += static_cast<int64_t>(d.nx) * d.ny * e->nlambda;
         }
@@ -1164,6 +1242,8 @@ namespace
             hipLaunchKernelGGL((lifecycle<GT, FP, kBatched>), dim3(blocks), dim3(threads), 0, s, p);
         else if (p.rf)
             hipLaunchKernelGGL((lifecycle<GT, FP, kStoreRF>), dim3(blocks), dim3(threads), 0, s, p);
+        else if (p.persp)
+            hipLaunchKernelGGL((lifecycle<GT, FP, kPersp>), dim3(blocks), dim3(threads), 0, s, p);
         else if (guided)
             hipLaunchKernelGGL((lifecycle<GT, FP, kGuided>), dim3(blocks), dim3(threads), 0, s, p);
         else
@@ -1752,6 +1832,23 @@ int skg_sed_add(skg_engine* e, double inclination, double azimuth, double radius
     return static_cast<int>(e->frames.size()) - 1;
 }
 
+int skg_perspective_add(skg_engine* e, const double* transform, const double* eye, double pixel_size, int32_t nx,
+                        int32_t ny)
+{
+    if (!e || !transform || !eye || !(pixel_size > 0) || nx < 1 || ny < 1)
+        return fail(e, SKG_ERR_ARGUMENT, "skg_perspective_add: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (e->frames.size() >= static_cast<size_t>(kMaxFrames)) return fail(e, SKG_ERR_CAPACITY, "skg_perspective_add: at most 1024 instruments");
+    skg_engine::FrameDesc d{0.0, 0.0, 0.0, nx, ny, 1.0, 1.0, 0.0, 0.0};
+    d.persp = 1;
+    std::memcpy(d.M, transform, sizeof(d.M));
+    std::memcpy(d.eye, eye, sizeof(d.eye));
+    d.psize = pixel_size;
+    e->frames.push_back(d);
+    e->frameDirty = true;
+    return static_cast<int>(e->frames.size()) - 1;
+}
+
 int skg_frame_remove_all(skg_engine* e)
 {
     if (!e) return SKG_ERR_ARGUMENT;
@@ -2026,6 +2123,11 @@ static int runPersistent(skg_engine* e, uint64_t first_history, uint64_t count)
     p.gframe = e->gframe;
     p.guideMix = e->guideMix;
     p.rf = e->rf;
+    p.persp = 0;
+    for (const auto& f : e->frames) p.persp |= f.persp;
+    if (p.persp && (e->rf || !e->hInst.empty() || e->guideMode != 0))
+        return fail(e, SKG_ERR_STATE,
+                    "skg_run_packets: perspective instruments run without instances, the guide or radiation-field storage");
     if (e->rf && (e->rfCells != e->ncells || e->rfLambda != e->nlambda))
         return fail(e, SKG_ERR_STATE, "skg_run_packets: the radiation-field table does not match the grid / wavelengths");
     if (e->rf && (!e->hInst.empty() || e->guideMode != 0))
@@ -2317,6 +2419,9 @@ int skg_run_packets_async(skg_engine* e, uint64_t first_history, uint64_t count)
     if (count == 0) return SKG_OK;
     if (e->mode == SKG_MODE_PIPELINE && e->rf)
         return fail(e, SKG_ERR_STATE, "skg_run_packets: radiation-field storage runs in the persistent engine");
+    if (e->mode == SKG_MODE_PIPELINE)
+        for (const auto& f : e->frames)
+            if (f.persp) return fail(e, SKG_ERR_STATE, "skg_run_packets: perspective instruments run in the persistent engine");
     if (e->mode == SKG_MODE_PIPELINE) return runPipeline(e, first_history, count);
     return runPersistent(e, first_history, count);
 }

@@ -134,6 +134,28 @@ namespace skg
         int sed;          // SEDInstrument: one pixel, inside the aperture radius (0: no aperture)
         int64_t offset;   // first element of this instrument in the frame buffer
         double radius2;
+        // PerspectiveInstrument: world -> (pixel x, pixel y, depth, w) transform (row vector
+        // times M, row-major), the eye and the pixel size
+        int persp;
+        int pad2;
+        double M[16];
+        double eye[3];
+        double psize;
+
+        // PerspectiveInstrument::detect (PerspectiveInstrument.cpp:103-116): the pixel (-1 if
+        // off the viewport or too close) and the depth zp, the distance the flux and the
+        // optical depth are taken over
+        __device__ __forceinline__ int persp_pixel(double x, double y, double z, double& zp) const
+        {
+            const double xp = x * M[0] + y * M[4] + z * M[8] + M[12];
+            const double yp = x * M[1] + y * M[5] + z * M[9] + M[13];
+            zp = x * M[2] + y * M[6] + z * M[10] + M[14];
+            const double wp = x * M[3] + y * M[7] + z * M[11] + M[15];
+            const int i = static_cast<int>(xp / wp);
+            const int j = static_cast<int>(yp / wp);
+            if (i >= 0 && i < nx && j >= 0 && j < ny && zp > psize / 10.) return i + nx * j;
+            return -1;
+        }
 
         // FrameInstrument::pixelOnDetector (FrameInstrument.cpp:117-136); for an SED instrument
         // ApertureInstrument::isInsideAperture (ApertureInstrument.cpp:24-43: the projected

@@ -7,7 +7,7 @@ inside a `SpiralStructureGeometryDecorator`) with a `MeanListDustMix` and an
 `OpticalDepthMaterialNormalization`, a `CartesianSpatialGrid` (`LinMesh` axes) or a
 `PolicyTreeSpatialGrid` with a `DensityTreePolicy`, a `VoronoiMeshSpatialGrid` or a
 `TetraMeshSpatialGrid` with sites / vertices from the dust density, `PhotonPacketOptions`,
-`FrameInstrument`s and `SEDInstrument`s.  Anything else raises `UnsupportedSki`, so a caller
+`FrameInstrument`s, `SEDInstrument`s and `PerspectiveInstrument`s.  Anything else raises `UnsupportedSki`, so a caller
 can fall back to the reference.  `run(sim, outdir, prefix)` sets the engine up, runs the
 photon packets and writes `<prefix>_<instrument>_total.fits` per frame as
 `FrameInstrument`/`FluxRecorder` do (`FluxRecorder.cpp:876-960`): surface brightness in
@@ -101,6 +101,20 @@ class Sed:
 
 
 @dataclass
+class Perspective:
+    """PerspectiveInstrument: viewport origin, crosshair, upwards direction, focal length,
+    viewport width and pixels."""
+    name: str
+    view: tuple
+    cross: tuple
+    up: tuple
+    focal: float
+    width: float
+    nx: int
+    ny: int
+
+
+@dataclass
 class Simulation:
     num_packets: int
     seed: int
@@ -148,8 +162,8 @@ def load(path: str) -> Simulation:
     if len(media) != 1 or len(sources) != 1:
         raise UnsupportedSki("exactly one medium and one source are supported")
     instruments = root.findall(".//instruments/*")
-    if any(i.tag not in ("FrameInstrument", "SEDInstrument") for i in instruments):
-        raise UnsupportedSki("only FrameInstrument and SEDInstrument instruments are supported")
+    if any(i.tag not in ("FrameInstrument", "SEDInstrument", "PerspectiveInstrument") for i in instruments):
+        raise UnsupportedSki("only FrameInstrument, SEDInstrument and PerspectiveInstrument instruments are supported")
     seed = int(_one(root, "Random").get("seed", "0"))
     ss = _one(root, "SourceSystem")
     lam = np.array(sorted(quantities(ss.get("wavelengths"))))
@@ -224,6 +238,12 @@ def load(path: str) -> Simulation:
         rf_probes.append(pr.get("probeName"))
     frames = []
     for fi in instruments:
+        if fi.tag == "PerspectiveInstrument":
+            q3 = lambda n: tuple(quantity(fi.get(n + c)) for c in "XYZ")  # noqa: E731
+            frames.append(Perspective(fi.get("instrumentName"), q3("view"), q3("cross"), q3("up"),
+                                      quantity(fi.get("focal")), quantity(fi.get("width")),
+                                      int(fi.get("numPixelsX")), int(fi.get("numPixelsY"))))
+            continue
         if fi.tag == "SEDInstrument":
             frames.append(Sed(fi.get("instrumentName"), quantity(fi.get("distance")), quantity(fi.get("inclination")),
                               quantity(fi.get("azimuth")), quantity(fi.get("roll", "0 deg")),
@@ -273,6 +293,83 @@ def calibration(sim: Simulation, frame: Frame):
     dx, dy = frame.fovx / frame.nx, frame.fovy / frame.ny
     omega = 4.0 * math.atan(0.5 * dx / d) * math.atan(0.5 * dy / d)
     return 1.0 / (4.0 * math.pi * d * d) / omega / bin_widths(lam) * (lam * lam / C_LIGHT) * 1e20
+
+
+def perspective_setup(view, cross, up, focal, width, nx, ny):
+    """PerspectiveInstrument::setupSelfBefore (PerspectiveInstrument.cpp:22-80) with
+    HomogeneousTransform's row-vector convention (p' = p M, each step post-multiplied,
+    HomogeneousTransform.cpp): the 4x4 transform from world coordinates to pixel coordinates
+    and depth, the eye position and the pixel size."""
+    Vx, Vy, Vz = view
+    Cx, Cy, Cz = cross
+    Ux, Uy, Uz = up
+    Fe = focal
+    G = math.sqrt((Vx - Cx) ** 2 + (Vy - Cy) ** 2 + (Vz - Cz) ** 2)
+    if G < 1e-20 or (Ux == 0 and Uy == 0 and Uz == 0):
+        raise UnsupportedSki("degenerate perspective instrument")
+    a, b, c = (Vx - Cx) / G, (Vy - Cy) / G, (Vz - Cz) / G
+    s = width / nx
+    E = (Vx + Fe * a, Vy + Fe * b, Vz + Fe * c)
+    M = [[1.0 if i == j else 0.0 for j in range(4)] for i in range(4)]
+
+    def cat(T):
+        nonlocal M
+        T = [list(map(float, row)) for row in T]
+        M = [[M[i][0] * T[0][j] + M[i][1] * T[1][j] + M[i][2] * T[2][j] + M[i][3] * T[3][j] for j in range(4)]
+             for i in range(4)]
+
+    def translate(x, y, z):
+        T = np.identity(4)
+        T[3, :3] = (x, y, z)
+        cat(T)
+
+    def scale(x, y, z):
+        cat(np.diag([x, y, z, 1.0]))
+
+    def rot(i, j, cs, sn):
+        T = np.identity(4)
+        T[i, i] = cs
+        T[j, j] = cs
+        T[i, j] = -sn
+        T[j, i] = sn
+        cat(T)
+
+    translate(-E[0], -E[1], -E[2])
+    v = math.sqrt(b * b + c * c)
+    if v > 0.3:
+        rot(1, 2, c / v, -b / v)   # rotateX
+        rot(0, 2, v, -a)           # rotateY
+        k = (b * b + c * c) * Ux - a * b * Uy - a * c * Uz
+        l = c * Uy - b * Uz
+        u = math.sqrt(k * k + l * l)
+        rot(0, 1, l / u, -k / u)   # rotateZ
+    else:
+        v = math.sqrt(a * a + c * c)
+        rot(0, 2, c / v, -a / v)   # rotateY
+        rot(1, 2, v, -b)           # rotateX
+        k = c * Ux - a * Uz
+        l = (a * a + c * c) * Uy - a * b * Ux - b * c * Uz
+        u = math.sqrt(k * k + l * l)
+        rot(0, 1, l / u, -k / u)   # rotateZ
+    scale(1.0, 1.0, -1.0)
+    T = np.identity(4)             # perspectiveZ
+    T[2, 3] = 1.0 / Fe
+    T[3, 2] = -Fe
+    T[3, 3] = 0.0
+    cat(T)
+    scale(1.0 / s, 1.0 / s, 1.0)
+    translate(nx / 2.0, ny / 2.0, 0.0)
+    return np.array(M), E, s
+
+
+def perspective_calibration(sim: Simulation, cam: Perspective):
+    """Recorded luminosity (already divided by the depth squared) -> MJy/sr for a local
+    instrument (FluxRecorder::calibrateAndWrite with 4 pi instead of 4 pi d^2 and the pixel
+    solid angle alpha^2, alpha = 2 atan(s / 2 focal), PerspectiveInstrument.cpp:75-78)."""
+    lam = sim.wavelengths
+    s = cam.width / cam.nx
+    alpha = 2.0 * math.atan(0.5 * s / cam.focal)
+    return 1.0 / (4.0 * math.pi) / (alpha * alpha) / bin_widths(lam) * (lam * lam / C_LIGHT) * 1e20
 
 
 def sed_calibration(sim: Simulation, sed: Sed):
@@ -435,7 +532,10 @@ def setup(sim: Simulation, engine, threads=0, npk_norm=None):
     engine.dust(sim.wavelengths, sext, ssca, g)
     engine.frame_remove_all()
     for f in sim.frames:
-        if isinstance(f, Sed):
+        if isinstance(f, Perspective):
+            M, E, ps = perspective_setup(f.view, f.cross, f.up, f.focal, f.width, f.nx, f.ny)
+            engine.perspective_add(M, E, ps, f.nx, f.ny)
+        elif isinstance(f, Sed):
             engine.sed_add(f.inclination, f.azimuth, f.radius)
         else:
             engine.frame_add(f.inclination, f.azimuth, f.roll, f.nx, f.ny, f.fovx, f.fovy, f.cx, f.cy)
@@ -469,6 +569,13 @@ def run(sim: Simulation, outdir: str, prefix: str, engine=None, chunk=1 << 22, t
     os.makedirs(outdir, exist_ok=True)
     out = {}
     for i, f in enumerate(sim.frames):
+        if isinstance(f, Perspective):
+            cube = eng.frame_read(i) * perspective_calibration(sim, f)[:, None, None]
+            ps = f.width / f.nx
+            hdr = Frame(f.name, 0.0, 0.0, 0.0, 0.0, ps * f.nx, ps * f.ny, f.nx, f.ny, 0.0, 0.0)
+            write_fits(os.path.join(outdir, f"{prefix}_{f.name}_total.fits"), cube, hdr, sim.wavelengths)
+            out[f.name] = cube
+            continue
         if isinstance(f, Sed):
             vals = eng.frame_read(i)[:, 0, 0] * sed_calibration(sim, f)
             write_sed(os.path.join(outdir, f"{prefix}_{f.name}_sed.dat"), vals, f, sim.wavelengths)

@@ -33,6 +33,11 @@ Every fixture is data: seeded inputs plus the reference's outputs for them.
                    tetrahedral grid of 20000 vertices (DustDensity policy), 100x100 frame at
                    60 deg: per-pixel mean and standard error over independent runs, each run
                    with its own random sites and densities.
+  image_persp.npz  (--persp) the config-0 workload seen by a PerspectiveInstrument (eye
+                   outside the domain, 100x100 pixels, viewport 20 pc wide at 30 pc focal
+                   length, from (10, -30, 20) pc toward (1, 0, 0) pc, so the image has no mirror
+                   symmetry): per-pixel mean and
+                   standard error of the FITS surface brightness over independent runs.
   rf_cfg0.npz      (--rf) the same workload on a 16^3 grid (1e4 density samples per cell)
                    storing the radiation field:
                    the RadiationFieldProbe's mean intensity J_nu per cell (PerCellForm),
@@ -512,12 +517,32 @@ def fixtures_mesh_images(tmp, runs=8, packets=1e6):
     np.savez_compressed(os.path.join(HERE, "image_mesh.npz"), **out)
 
 
+CAMERA = ("cam", (10.0, -30.0, 20.0), (1.0, 0.0, 0.0), (0.0, 0.0, 1.0), 30.0, 20.0, 100)
+
+
+def fixtures_persp(tmp, runs=8, packets=1e6):
+    outdir = os.path.join(tmp, "persp")
+    os.makedirs(outdir, exist_ok=True)
+    imgs = []
+    for s in range(runs):
+        spec = skigen.config0(num_packets=packets, seed=8000 + s)
+        spec.persp_instruments = (CAMERA,)
+        ski = write_ski(tmp, f"persp{s}", spec)
+        run("run", ski, outdir, 8)
+        imgs.append(fits_image(os.path.join(outdir, f"persp{s}_cam_total.fits")))
+    imgs = np.array(imgs)
+    np.savez_compressed(os.path.join(HERE, "image_persp.npz"), mean=imgs.mean(axis=0).astype(np.float32),
+                        sem=(imgs.std(axis=0, ddof=1) / np.sqrt(runs)).astype(np.float32),
+                        runs=np.array(runs), packets=np.array(packets))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--images", action="store_true")
     ap.add_argument("--seds", action="store_true")
     ap.add_argument("--rf", action="store_true")
     ap.add_argument("--meshes", action="store_true")
+    ap.add_argument("--persp", action="store_true")
     ap.add_argument("--only", default="")
     a = ap.parse_args()
     with tempfile.TemporaryDirectory() as tmp:
@@ -540,6 +565,8 @@ def main():
             fixtures_rf(tmp)
         if a.meshes:
             fixtures_mesh_images(tmp)
+        if a.persp:
+            fixtures_persp(tmp)
 
 
 if __name__ == "__main__":

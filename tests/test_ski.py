@@ -86,7 +86,7 @@ def test_unsupported_items_raise(ski, tmp_path):
     with pytest.raises(ski.UnsupportedSki):
         ski.load(_write(str(tmp_path), "v", spec))
     p = _write(str(tmp_path), "f", skigen.SkiSpec(num_packets=1))
-    text = open(p).read().replace("<FrameInstrument ", "<PerspectiveInstrument ")
+    text = open(p).read().replace("<FrameInstrument ", "<FullInstrument ")
     with open(p, "w") as f:
         f.write(text)
     with pytest.raises(ski.UnsupportedSki):
@@ -185,3 +185,49 @@ def test_spiral_disk_points_follow_the_perturbation():
     cn = np.sqrt(np.pi) * 1.0 / (np.sqrt(np.pi) / 2)  # Gamma(2) / Gamma(3/2)
     prof = 0.5 + 0.5 * cn * mean_sin2
     np.testing.assert_allclose(hist / hist.sum(), prof / prof.sum(), rtol=0.03)
+
+
+CAMERA = ("cam", (10.0, -30.0, 20.0), (1.0, 0.0, 0.0), (0.0, 0.0, 1.0), 30.0, 20.0, 100)
+
+
+def _project(M, p):
+    v = np.array([p[0], p[1], p[2], 1.0]) @ M
+    return v[0] / v[3], v[1] / v[3], v[2]
+
+
+def test_perspective_parse_and_transform(ski, tmp_path):
+    """PerspectiveInstrument items are read; the viewport transform puts the point source at
+    the origin in the pixel where the reference's image of the config-0 workload peaks
+    (tests/golden/image_persp.npz, camera off the scene's mirror plane), and for both
+    rotation branches (viewing direction away from and near the x axis) maps the crosshair
+    to the viewport centre, forward x up to +x, up to +y, with Fe / d / (pixel size) pixels
+    per unit length at distance d from the eye; the eye lies one focal length behind the
+    viewport origin."""
+    spec = skigen.config0(num_packets=1e5)
+    spec.persp_instruments = (CAMERA,)
+    sim = ski.load(_write(str(tmp_path), "p", spec))
+    cam = sim.frames[1]
+    assert isinstance(cam, ski.Perspective) and cam.nx == cam.ny == 100
+    assert cam.view == (10 * PC, -30 * PC, 20 * PC) and abs(cam.focal / (30 * PC) - 1) < 1e-15
+    M, E, s = ski.perspective_setup(cam.view, cam.cross, cam.up, cam.focal, cam.width, cam.nx, cam.ny)
+    ref = golden("image_persp.npz")["mean"][0]
+    j, i = np.unravel_index(ref.argmax(), ref.shape)
+    x, y, zp = _project(M, (0.0, 0.0, 0.0))
+    assert (int(x), int(y)) == (i, j) and zp > 0
+    for view, cross, up in (((10, -30, 20), (1, 0, 0), (0, 0, 1)), ((30, 1, 0), (0, 0, 0), (0, 0, 1)),
+                            ((0.5, 0.2, 30), (0, 0, 0), (0, 1, 0)), ((-30, 0.5, 0.2), (1, 2, 3), (0, 0.3, 1))):
+        view, cross, up = np.array(view, float) * PC, np.array(cross, float) * PC, np.array(up, float)
+        M, E, s = ski.perspective_setup(view, cross, up, 25 * PC, 20 * PC, 100, 80)
+        f = (cross - view) / np.linalg.norm(cross - view)
+        np.testing.assert_allclose(E, view - 25 * PC * f, rtol=0, atol=1e-9 * PC)
+        r = np.cross(f, up)
+        r /= np.linalg.norm(r)
+        u = np.cross(r, f)
+        d = np.linalg.norm(cross - E)
+        mag = 25 * PC / d / s * PC  # pixels per parsec at the crosshair
+        x0, y0, zp = _project(M, cross)
+        assert abs(x0 - 50) < 1e-9 and abs(y0 - 40) < 1e-9 and abs(zp - (d - 25 * PC)) < 1e-6 * PC
+        x1, y1, _ = _project(M, cross + PC * r)
+        x2, y2, _ = _project(M, cross + PC * u)
+        assert abs(x1 - 50 - mag) < 1e-9 and abs(y1 - 40) < 1e-9
+        assert abs(x2 - 50) < 1e-9 and abs(y2 - 40 - mag) < 1e-9

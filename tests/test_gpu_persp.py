@@ -1,6 +1,7 @@
 """Perspective instruments on the GPU: the engine against the FP64 oracle on the same histories
 (a frame, a camera outside the domain, the same camera again, and a camera whose eye lies inside
-the dust, so the optical depth toward it stops at the eye), and the ski front end's FITS output
+the dust, so the optical depth toward it stops at the eye) on a Cartesian disk and on the
+reference's octree, Voronoi and tetrahedral fixture grids, and the ski front end's FITS output
 against the reference's images of the config-0 workload (mean and standard error over 8 runs of
 1e6 packets, tests/golden/image_persp.npz): total within 4 sigma, 5x5-binned pooled chi-square
 per degree of freedom < 1.5."""
@@ -12,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import PC, ROOT, disk_density_cartesian, golden, load_pkg
+from helpers import PC, ROOT, child_first, disk_density_cartesian, golden, load_pkg
 
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import skigen  # noqa: E402
@@ -71,6 +72,57 @@ def test_perspective_instruments_match_oracle():
     print("case perspective instruments refused by the pipeline engine", flush=True)
     with pytest.raises(pkg.EngineError):
         eng.run_packets(0, 1000)
+    eng.close()
+
+
+def _fixture_grid(eng, kind):
+    f = golden(("tree" if kind == "octree" else kind) + "_paths.npz")
+    d = {k: f[k] for k in f.files}
+    if kind == "octree":
+        cf = child_first(d["child_begin"], d["child_count"], d["child_index"])
+        eng.grid_octree(d["bounds"], cf, d["cell_index"], d["nb_off"], d["nb_id"])
+        g = O.Grid("octree", bounds=d["bounds"], child_first=cf, cell_index=d["cell_index"], nb_off=d["nb_off"],
+                   nb_id=d["nb_id"])
+    elif kind == "voronoi":
+        eng.grid_voronoi(d["sites"], d["nb_begin"], d["nb_count"], d["nb_index"], d["block_n"], d["block_begin"],
+                         d["block_count"], d["block_index"], d["box"])
+        g = O.Grid(kind, **d)
+    else:
+        eng.grid_tetra(d["verts"], d["tet_verts"], d["face_nb"], d["face_normal"], d["box"])
+        g = O.Grid(kind, **d)
+    return g, d["density"], float(d["sext"][0])
+
+
+@pytest.mark.parametrize("kind", ["octree", "voronoi", "tetra"])
+def test_perspective_instruments_match_oracle_on_fixture_grids(kind):
+    pkg = load_pkg()
+    from skirt9_gpu_amd import ski as S
+    eng = pkg.Engine(0)
+    grid, dens, s = _fixture_grid(eng, kind)
+    eng.medium(dens)
+    lam, sext, ssca, g = [0.55e-6, 0.9e-6], [s, 0.6 * s], [0.6 * s, 0.3 * s], [0.6, 0.4]
+    eng.dust(lam, sext, ssca, g)
+    instr = [dict(inclination=np.radians(60.0), azimuth=0.0, roll=0.0, nx=40, ny=40, fovx=20 * PC, fovy=20 * PC)]
+    instr += _cameras(S)
+    eng.frame_remove_all()
+    for d in instr:
+        if d.get("persp"):
+            eng.perspective_add(d["M"], d["eye"], d["psize"], d["nx"], d["ny"])
+        else:
+            eng.frame_add(d["inclination"], d["azimuth"], d["roll"], d["nx"], d["ny"], d["fovx"], d["fovy"])
+    npk = 20000
+    lpack = [1.0 / npk, 2.0 / npk]
+    eng.source_point((0.0, 0.0, 0.0), lpack)
+    eng.options(0.5, 1e4, 0, 0, 29)
+    ofr, _ = O.simulate(grid, dens, lam, sext, ssca, g, lpack, instr, 0, npk, threads=8, seed=29)
+    eng.tune_pipeline(pkg.SKG_MODE_PERSISTENT, 1 << 18, 0, 8)
+    eng.frame_clear()
+    print("case perspective instruments on", kind, flush=True)
+    eng.run_packets(0, npk)
+    for i in range(4):
+        v, ov = eng.frame_read(i), ofr[i]
+        assert ov.sum() > 0
+        np.testing.assert_allclose(v, ov, rtol=1e-9, atol=1e-12 * ov.max(), err_msg=f"{kind} instrument {i}")
     eng.close()
 
 

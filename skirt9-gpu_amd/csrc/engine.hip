@@ -1809,7 +1809,10 @@ int skg_trace_tau(skg_engine* e, const double* rays, int64_t n, int32_t ell, dou
 int skg_frame_add(skg_engine* e, double inclination, double azimuth, double roll, int32_t nx, int32_t ny, double fovx,
                   double fovy, double cx, double cy)
 {
-    if (!e || nx < 1 || ny < 1 || !(fovx > 0) || !(fovy > 0)) return fail(e, SKG_ERR_ARGUMENT, "skg_frame_add: bad arguments");
+    if (!e || nx < 1 || ny < 1 || !(fovx > 0) || !(fovy > 0) || !std::isfinite(fovx) || !std::isfinite(fovy)
+        || !std::isfinite(azimuth) || !std::isfinite(roll) || !std::isfinite(cx) || !std::isfinite(cy))
+        return fail(e, SKG_ERR_ARGUMENT, "skg_frame_add: bad arguments");
+    if (static_cast<int64_t>(nx) * ny > INT32_MAX) return fail(e, SKG_ERR_ARGUMENT, "skg_frame_add: too many pixels");
     if (inclination < -1e-8 || inclination > M_PI + 1e-8) return fail(e, SKG_ERR_ARGUMENT, "skg_frame_add: inclination outside [0, pi]");
     std::lock_guard<std::mutex> lk(e->mu);
     if (e->frames.size() >= static_cast<size_t>(kMaxFrames)) return fail(e, SKG_ERR_CAPACITY, "skg_frame_add: at most 1024 instruments");
@@ -1820,7 +1823,8 @@ int skg_frame_add(skg_engine* e, double inclination, double azimuth, double roll
 
 int skg_sed_add(skg_engine* e, double inclination, double azimuth, double radius)
 {
-    if (!e || !(radius >= 0) || !std::isfinite(radius)) return fail(e, SKG_ERR_ARGUMENT, "skg_sed_add: bad arguments");
+    if (!e || !(radius >= 0) || !std::isfinite(radius) || !std::isfinite(azimuth))
+        return fail(e, SKG_ERR_ARGUMENT, "skg_sed_add: bad arguments");
     if (inclination < -1e-8 || inclination > M_PI + 1e-8) return fail(e, SKG_ERR_ARGUMENT, "skg_sed_add: inclination outside [0, pi]");
     std::lock_guard<std::mutex> lk(e->mu);
     if (e->frames.size() >= static_cast<size_t>(kMaxFrames)) return fail(e, SKG_ERR_CAPACITY, "skg_sed_add: at most 1024 instruments");
@@ -1835,8 +1839,13 @@ int skg_sed_add(skg_engine* e, double inclination, double azimuth, double radius
 int skg_perspective_add(skg_engine* e, const double* transform, const double* eye, double pixel_size, int32_t nx,
                         int32_t ny)
 {
-    if (!e || !transform || !eye || !(pixel_size > 0) || nx < 1 || ny < 1)
+    if (!e || !transform || !eye || !(pixel_size > 0) || !std::isfinite(pixel_size) || nx < 1 || ny < 1)
         return fail(e, SKG_ERR_ARGUMENT, "skg_perspective_add: bad arguments");
+    for (int q = 0; q < 16; ++q)
+        if (!std::isfinite(transform[q])) return fail(e, SKG_ERR_ARGUMENT, "skg_perspective_add: transform not finite");
+    for (int q = 0; q < 3; ++q)
+        if (!std::isfinite(eye[q])) return fail(e, SKG_ERR_ARGUMENT, "skg_perspective_add: eye not finite");
+    if (static_cast<int64_t>(nx) * ny > INT32_MAX) return fail(e, SKG_ERR_ARGUMENT, "skg_perspective_add: too many pixels");
     std::lock_guard<std::mutex> lk(e->mu);
     if (e->frames.size() >= static_cast<size_t>(kMaxFrames)) return fail(e, SKG_ERR_CAPACITY, "skg_perspective_add: at most 1024 instruments");
     skg_engine::FrameDesc d{0.0, 0.0, 0.0, nx, ny, 1.0, 1.0, 0.0, 0.0};

@@ -72,6 +72,18 @@ def test_perspective_instruments_match_oracle():
     print("case perspective instruments refused by the pipeline engine", flush=True)
     with pytest.raises(pkg.EngineError):
         eng.run_packets(0, 1000)
+    # host-side argument checks: nothing is added
+    d = instr[1]
+    bad_eye = np.array(d["eye"], dtype=float)
+    bad_eye[1] = np.nan
+    bad_M = np.array(d["M"], dtype=float)
+    bad_M[3, 3] = np.inf
+    for args in ((d["M"], bad_eye, d["psize"], 10, 10), (bad_M, d["eye"], d["psize"], 10, 10),
+                 (d["M"], d["eye"], 0.0, 10, 10), (d["M"], d["eye"], d["psize"], 0, 10),
+                 (d["M"], d["eye"], d["psize"], 65536, 65536)):
+        with pytest.raises(pkg.EngineError):
+            eng.perspective_add(*args)
+    assert len(eng.frames) == 4
     eng.close()
 
 

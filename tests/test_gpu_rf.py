@@ -1,6 +1,6 @@
 """Radiation-field storage on the GPU (skg_rf_enable / skg_rf_read, persistent engine) against
 the FP64 oracle on the same histories, with scattering: Cartesian disk, and the reference's
-Voronoi and tetrahedral fixture meshes.  Bars: the same cells hold radiation field, each
+octree, Voronoi and tetrahedral fixture grids.  Bars: the same cells hold radiation field, each
 within 1e-9 relative (FP64 atomics add in another order), the total within 1e-12; the
 frames are unchanged by storing the field."""
 
@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import disk_density_cartesian, frame60, golden, load_pkg
+from helpers import child_first, disk_density_cartesian, frame60, golden, load_pkg
 
 pytestmark = pytest.mark.gpu
 
@@ -27,6 +27,14 @@ def _setup(eng, kind):
         eng.grid_cartesian(b, b, b)
         g = O.Grid("cartesian", xb=b, yb=b, zb=b)
         s = 1.5e-29
+    elif kind == "octree":
+        d = golden("tree_paths.npz")
+        cf = child_first(d["child_begin"], d["child_count"], d["child_index"])
+        eng.grid_octree(d["bounds"], cf, d["cell_index"], d["nb_off"], d["nb_id"])
+        g = O.Grid("octree", bounds=d["bounds"], child_first=cf, cell_index=d["cell_index"], nb_off=d["nb_off"],
+                   nb_id=d["nb_id"])
+        dens = d["density"]
+        s = float(d["sext"][0])
     else:
         f = golden(f"{kind}_paths.npz")
         d = {k: f[k] for k in f.files}
@@ -41,7 +49,7 @@ def _setup(eng, kind):
     return g, dens, s
 
 
-@pytest.mark.parametrize("kind", ["cartesian", "voronoi", "tetra"])
+@pytest.mark.parametrize("kind", ["cartesian", "octree", "voronoi", "tetra"])
 def test_rf_matches_oracle(eng, kind):
     pkg = load_pkg()
     g, dens, s = _setup(eng, kind)

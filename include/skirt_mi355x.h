@@ -164,6 +164,19 @@ int skg_source_point(skg_engine* e, double x, double y, double z, const double* 
 
 /* PhotonPacketOptions (pathLengthBias, minWeightReduction, minScattEvents,
  * PhotonPacketOptions.hpp:71-86), the precision mode and the random seed. */
+/* Replaces the point source by n imported particles (ImportedSource with a ParticleSnapshot,
+ * ImportedSource.cpp:290-322, 466-471, 653-733; ParticleSnapshot.cpp:275-286): xyzh holds
+ * x, y, z and the smoothing length h of each particle (m), lbin its luminosity in each
+ * wavelength bin (n x nlambda, W), ltot its luminosity over the source wavelength range (the
+ * launch weights (1 - source_bias) ltot_m / sum + source_bias / n split the history indices
+ * [0, num_packets) into consecutive ranges, one per particle, as prepareForLaunch does).  A
+ * packet takes a uniform bin, the luminosity lbin[m][l] * nlambda / (num_packets * W_m), a
+ * position r_m + k u h_m (k isotropic; u from the kernel: 0 uniform, u = X^(1/3); 1 cubic
+ * spline, the 400-interval table inversion of CubicSplineSmoothingKernel.cpp:78-85) and an
+ * isotropic direction.  Persistent engine; not combined with perspective instruments,
+ * instances, the guide or radiation-field storage.  n = 0 restores the point source. */
+int skg_source_particles(skg_engine* e, int64_t n, const double* xyzh, const double* lbin, const double* ltot,
+                         double source_bias, int32_t kernel, int64_t num_packets);
 int skg_options(skg_engine* e, double path_length_bias, double min_weight_reduction, int32_t min_scatt_events,
                 int32_t fp_mode, uint64_t seed);
 

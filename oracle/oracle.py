@@ -7,6 +7,7 @@ cpu_baseline leg as the checker.  The product never imports this module.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -66,7 +67,8 @@ class OrcSim(C.Structure):
                 ("lam", f64p), ("sext", f64p), ("ssca", f64p), ("g", f64p), ("lpack", f64p),
                 ("src", C.c_double * 3), ("xi", C.c_double), ("min_weight_reduction", C.c_double),
                 ("min_scatt_events", C.c_int32), ("ninstr", C.c_int32), ("instr", P(OrcInstrument)),
-                ("seed", C.c_uint64), ("ncells", C.c_int64), ("rf", f64p)]
+                ("seed", C.c_uint64), ("ncells", C.c_int64), ("rf", f64p), ("nent", C.c_int64),
+                ("ent_kernel", C.c_int32), ("ent", f64p), ("ent_first", i64p), ("ent_lpack", f64p), ("ent_X", f64p)]
 
 
 def _p(a, t):
@@ -203,8 +205,52 @@ def lnmean(x1, x2):
     return f(C.c_double(x1), C.c_double(x2))
 
 
+def spline_table(n=400):
+    """CubicSplineSmoothingKernel::setupSelfBefore (CubicSplineSmoothingKernel.cpp:22-40): the
+    cumulative mass fraction inside radius u = k / n, k = 0..n."""
+    du = 1.0 / n
+    X = np.zeros(n + 1)
+    for k in range(n + 1):
+        u = k * du
+        u2 = u * u
+        u3 = u * u2
+        if u < 0.5:
+            X[k] = u3 * (32.0 / 3.0 - 192.0 / 5.0 * u2 + 32.0 * u3)
+        else:
+            X[k] = -1.0 / 15.0 + u3 * (64.0 / 3.0 - 48.0 * u + 192.0 / 5.0 * u2 - 32.0 / 3.0 * u3)
+    return X
+
+
+def _round_half_away(x):
+    f = math.floor(x)
+    return int(f) + (1 if x - f >= 0.5 else 0)
+
+
+def particle_launch(ltot, lbin, num_packets, source_bias=0.5):
+    """ImportedSource::prepareForLaunch (ImportedSource.cpp:290-322): launch weights and the
+    first history index of each particle; and the launch luminosity per packet of particle m in
+    bin l, L_m,l * nlambda / (N W_m) (the selection weight L_m / W_m of launchFromEntity,
+    :660-661, times the oligochromatic bias weight s / b for a uniform bin choice, :670-705)."""
+    ltot = np.asarray(ltot, dtype=np.float64)
+    lbin = np.asarray(lbin, dtype=np.float64).reshape(len(ltot), -1)
+    n, nl = lbin.shape
+    total = 0.0
+    for v in ltot:
+        total += v
+    Wv = (1 - source_bias) * (ltot / total) + source_bias / n
+    Iv = np.zeros(n + 1, np.int64)
+    W = 0.0
+    for m in range(1, n):
+        W += Wv[m - 1]
+        Iv[m] = min(num_packets, _round_half_away(W * num_packets))
+    Iv[n] = num_packets
+    lp = np.where(ltot[:, None] > 0, (lbin * nl) / (float(num_packets) * Wv[:, None]), 0.0)
+    return Iv, lp, Wv
+
+
 def simulate(grid: Grid, density, lam, sext, ssca, g, lpack, instruments, first, count, threads=8, seed=1,
-             xi=0.5, min_weight_reduction=1e4, min_scatt_events=0, src=(0.0, 0.0, 0.0), store_rf=False):
+             xi=0.5, min_weight_reduction=1e4, min_scatt_events=0, src=(0.0, 0.0, 0.0), store_rf=False,
+             particles=None):
     """Run histories [first, first+count); returns (frames list, counters), and the stored
     radiation field (ncells, nlambda) as a third item when store_rf.
 
@@ -239,6 +285,18 @@ def simulate(grid: Grid, density, lam, sext, ssca, g, lpack, instruments, first,
     s.ncells = len(arrs[0])
     rf = np.zeros((len(arrs[0]), len(arrs[1]))) if store_rf else None
     s.rf = _p(rf, f64p) if store_rf else None
+    keep = []
+    if particles is not None:
+        # particles: dict(xyzh, lbin, ltot, num_packets, source_bias=0.5, kernel=1)
+        xyzh = np.ascontiguousarray(particles["xyzh"], dtype=np.float64).reshape(-1, 4)
+        Iv, lp, _ = particle_launch(particles["ltot"], particles["lbin"], int(particles["num_packets"]),
+                                    particles.get("source_bias", 0.5))
+        X = spline_table()
+        keep = [xyzh, np.ascontiguousarray(Iv), np.ascontiguousarray(lp), X]
+        s.nent = len(xyzh)
+        s.ent_kernel = int(particles.get("kernel", 1))
+        s.ent, s.ent_first, s.ent_lpack, s.ent_X = (_p(keep[0], f64p), _p(keep[1], i64p), _p(keep[2], f64p),
+                                                    _p(keep[3], f64p))
     n = lib().orc_frame_size(C.byref(s))
     frame = np.zeros(n)
     cnt = np.zeros(4, np.int64)

@@ -977,10 +977,16 @@ struct orc_sim
     uint64_t seed;
     int64_t ncells;         // cells of the grid (radiation-field table rows)
     double* rf;             // radiation field ncells x nlambda to add into, or NULL
+    int64_t nent;           // imported particles (0: the point source at src)
+    int32_t ent_kernel;     // 0 uniform, 1 cubic spline
+    const double* ent;      // per particle: x, y, z, smoothing length
+    const int64_t* ent_first;  // first history index of each particle (nent + 1)
+    const double* ent_lpack;   // launch luminosity per packet, nent x nlambda
+    const double* ent_X;       // cubic spline: cumulative mass at u = k / 400, k = 0..400
 };
 
 // version tag used by the loader
-int orc_version() { return 5; }
+int orc_version() { return 6; }
 
 // the logarithmic mean of the radiation-field storage, exposed for the known-answer tests
 double orc_lnmean(double x1, double x2);
@@ -1320,6 +1326,32 @@ namespace
             double lambda = p->lambda[ell];
             double W = p->lpack[ell] * lambda;  // PhotonPacket.cpp:18-22
             double r[3] = {p->src[0], p->src[1], p->src[2]};
+            if (p->nent > 0)
+            {
+                // ImportedSource::launch (ImportedSource.cpp:466-471): the particle whose history
+                // range holds h, as std::upper_bound(Iv, h) - 1
+                const int64_t m = std::upper_bound(p->ent_first, p->ent_first + p->nent + 1, static_cast<int64_t>(h))
+                                  - p->ent_first - 1;
+                W = p->ent_lpack[m * p->nlambda + ell] * lambda;
+                // ParticleSnapshot::generatePosition (ParticleSnapshot.cpp:275-286) with the
+                // kernel's generateRadius (CubicSplineSmoothingKernel.cpp:78-85,
+                // UniformSmoothingKernel.cpp:27-31)
+                const double X = rng.uniform();
+                double u;
+                if (p->ent_kernel == 1)
+                {
+                    int kk = static_cast<int>(std::upper_bound(p->ent_X, p->ent_X + 401, X) - p->ent_X) - 1;
+                    kk = std::max(0, std::min(399, kk));  // NR::locateClip
+                    const double f = (X - p->ent_X[kk]) / (p->ent_X[kk + 1] - p->ent_X[kk]);
+                    u = (kk + f) / 400;
+                }
+                else
+                    u = std::pow(X, 1.0 / 3.0);
+                double d[3];
+                isotropic(rng, d);
+                const double hm = p->ent[4 * m + 3];
+                for (int q = 0; q < 3; ++q) r[q] = p->ent[4 * m + q] + d[q] * u * hm;
+            }
             double k[3];
             isotropic(rng, k);
             if (!(W / lambda > 0)) return;

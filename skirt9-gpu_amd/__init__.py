@@ -65,7 +65,7 @@ EXPORTED = [
     "skg_create", "skg_destroy", "skg_last_error", "skg_grid_cartesian", "skg_grid_octree", "skg_grid_voronoi",
     "skg_grid_tetra", "skg_num_cells",
     "skg_medium", "skg_dust", "skg_trace_paths", "skg_trace_tau", "skg_frame_add", "skg_frame_remove_all",
-    "skg_frame_clear", "skg_frame_read", "skg_source_point", "skg_options", "skg_run_packets",
+    "skg_frame_clear", "skg_frame_read", "skg_source_point", "skg_source_particles", "skg_options", "skg_run_packets",
     "skg_run_packets_async", "skg_sync", "skg_counters", "skg_tune", "skg_last_kernel_ms",
     "skg_disk_density", "skg_octree_build", "skg_octree_from_flags", "skg_octree_info", "skg_octree_export",
     "skg_octree_upload", "skg_octree_free", "skg_sample_density_cartesian", "skg_sample_density_octree",
@@ -289,6 +289,16 @@ class Engine:
         out = np.zeros(self.num_cells() * self.nlambda)
         self._chk(lib().skg_rf_read(self._h, _p(out, f64p), C.c_int64(len(out))))
         return out.reshape(-1, self.nlambda)
+
+    def source_particles(self, xyzh, lbin, ltot, num_packets, source_bias=0.5, kernel=1):
+        """Imported particle source (skg_source_particles); kernel 0 uniform, 1 cubic spline.
+        An empty xyzh restores the point source."""
+        X = _f64(xyzh).reshape(-1, 4)
+        n = len(X)
+        B = _f64(lbin).reshape(n, -1) if n else np.zeros(0)
+        T = _f64(ltot).reshape(n)
+        self._chk(lib().skg_source_particles(self._h, C.c_int64(n), _p(X, f64p), _p(B, f64p), _p(T, f64p),
+                                             C.c_double(source_bias), C.c_int32(kernel), C.c_int64(num_packets)))
 
     def options(self, path_length_bias=0.5, min_weight_reduction=1e4, min_scatt_events=0, fp_mode=SKG_FP64, seed=1):
         self._chk(lib().skg_options(self._h, C.c_double(path_length_bias), C.c_double(min_weight_reduction),

@@ -302,6 +302,12 @@ namespace skg
         double guideMix;
         double* rf;               // radiation-field variant: ncells x nlambda table
         int persp;                // some instruments are perspective instruments
+        int nent;                 // particle-source variant: number of entities (0: the point source)
+        int entKernel;            // smoothing kernel: 0 uniform, 1 cubic spline
+        const double* ent;        // per entity: x, y, z, smoothing length h
+        const int64_t* entFirst;  // first history index of each entity (nent + 1 entries)
+        const double* entLpack;   // launch luminosity per packet, nent x nlambda
+        const double* entX;       // cubic spline: cumulative mass at u = k / kSplineN, k = 0..kSplineN
     };
 
     // kernel variants of the life cycle
@@ -311,8 +317,10 @@ namespace skg
         kBatched = 1,    // batched independent instances
         kGuided = 2,     // scattering directions from the defensive guide / HG mixture
         kStoreRF = 3,    // one run that also stores the radiation field of the forced paths
-        kPersp = 4       // one run with perspective instruments among the instruments
+        kPersp = 4,      // one run with perspective instruments among the instruments
+        kParticles = 5   // one run whose source is a set of imported particles
     };
+    constexpr int kSplineN = 400;  // CubicSplineSmoothingKernel's table size (CubicSplineSmoothingKernel.cpp:18)
 
     // SpecialFunctions::lnmean(x1, x2, lnx1, lnx2) (SpecialFunctions.cpp:860-880): the
     // logarithmic mean (x2 - x1) / ln(x2 / x1), by its series for nearly equal arguments
@@ -634,10 +642,56 @@ namespace skg
             pk.lam = p.lambda[pk.ell];
             T.section = INST ? p.sext[pk.ell] * p.inst[pk.inst].scale : p.sext[pk.ell];
             T.section32 = static_cast<float>(T.section);
-            pk.W = lp[pk.ell] * pk.lam;
-            pk.rx = p.src[0];
-            pk.ry = p.src[1];
-            pk.rz = p.src[2];
+            if constexpr (V == kParticles)
+            {
+                // ImportedSource::launch: the entity whose history index range holds h
+                // (ImportedSource.cpp:466-471), its luminosity for the bin (:653-705, the
+                // oligochromatic bias weight folded into entLpack), a position from its smoothing
+                // kernel (ParticleSnapshot::generatePosition, ParticleSnapshot.cpp:275-286)
+                int lo = 0, hi = p.nent - 1;
+                while (lo < hi)
+                {
+                    int mid = (lo + hi + 1) >> 1;
+                    if (p.entFirst[mid] <= static_cast<int64_t>(h))
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                const int m = lo;
+                pk.W = p.entLpack[static_cast<int64_t>(m) * p.nlambda + pk.ell] * pk.lam;
+                const double X = pk.rng.uniform();
+                double u;
+                if (p.entKernel == 1)
+                {
+                    // CubicSplineSmoothingKernel::generateRadius (:78-85): NR::locateClip on the table
+                    int a = 0, b = kSplineN;
+                    while (b - a > 1)
+                    {
+                        int mid = (a + b) >> 1;
+                        if (X >= p.entX[mid])
+                            a = mid;
+                        else
+                            b = mid;
+                    }
+                    const double f = (X - p.entX[a]) / (p.entX[a + 1] - p.entX[a]);
+                    u = (a + f) / kSplineN;
+                }
+                else
+                    u = pow(X, 1.0 / 3.0);  // UniformSmoothingKernel::generateRadius (:27-31)
+                double dx, dy, dz;
+                isotropic(pk.rng, dx, dy, dz);
+                const double hm = p.ent[4 * m + 3];
+                pk.rx = p.ent[4 * m + 0] + dx * u * hm;
+                pk.ry = p.ent[4 * m + 1] + dy * u * hm;
+                pk.rz = p.ent[4 * m + 2] + dz * u * hm;
+            }
+            else
+            {
+                pk.W = lp[pk.ell] * pk.lam;
+                pk.rx = p.src[0];
+                pk.ry = p.src[1];
+                pk.rz = p.src[2];
+            }
             isotropic(pk.rng, pk.kx, pk.ky, pk.kz);
             pk.nscatt = 0;
             pk.taupath = 0.;
@@ -1047,6 +1101,13 @@ struct skg_engine
     Instance* inst = nullptr;
     double* ilpack = nullptr;
     uint64_t instTotal = 0;
+    // particle source (0 entities: the point source)
+    int64_t nent = 0;
+    int entKernel = 1;
+    double* ent = nullptr;
+    int64_t* entFirst = nullptr;
+    double* entLpack = nullptr;
+    double* entX = nullptr;
     // launch shape
     int workgroups = 0, threads = 256, threshold = 32;
     float lastMs = 0.f;
@@ -1244,6 +1305,8 @@ namespace
             hipLaunchKernelGGL((lifecycle<GT, FP, kStoreRF>), dim3(blocks), dim3(threads), 0, s, p);
         else if (p.persp)
             hipLaunchKernelGGL((lifecycle<GT, FP, kPersp>), dim3(blocks), dim3(threads), 0, s, p);
+        else if (p.nent)
+            hipLaunchKernelGGL((lifecycle<GT, FP, kParticles>), dim3(blocks), dim3(threads), 0, s, p);
         else if (guided)
             hipLaunchKernelGGL((lifecycle<GT, FP, kGuided>), dim3(blocks), dim3(threads), 0, s, p);
         else
@@ -1379,7 +1442,8 @@ void skg_destroy(skg_engine* e)
                     static_cast<void*>(e->recHist), static_cast<void*>(e->recA), static_cast<void*>(e->recCount),
                     static_cast<void*>(e->recSum), static_cast<void*>(e->contrib), static_cast<void*>(e->recLast),
                     static_cast<void*>(e->finalLum), static_cast<void*>(e->inst), static_cast<void*>(e->ilpack),
-                    static_cast<void*>(e->rf)})
+                    static_cast<void*>(e->rf), static_cast<void*>(e->ent), static_cast<void*>(e->entFirst),
+                    static_cast<void*>(e->entLpack), static_cast<void*>(e->entX)})
         if (p) hipFree(p);
     if (e->hostCtl) hipHostFree(e->hostCtl);
     for (auto ev : e->traceEv) hipEventDestroy(ev);
@@ -1914,6 +1978,80 @@ int skg_source_point(skg_engine* e, double x, double y, double z, const double* 
     return SKG_OK;
 }
 
+int skg_source_particles(skg_engine* e, int64_t n, const double* xyzh, const double* lbin, const double* ltot,
+                         double source_bias, int32_t kernel, int64_t num_packets)
+{
+    if (!e) return SKG_ERR_ARGUMENT;
+    if (n == 0)
+    {
+        // back to the point source
+        std::lock_guard<std::mutex> lk(e->mu);
+        e->nent = 0;
+        return SKG_OK;
+    }
+    if (n < 0 || n > INT32_MAX || !xyzh || !lbin || !ltot || !(source_bias >= 0 && source_bias <= 1)
+        || !(kernel == 0 || kernel == 1) || num_packets < 1)
+        return fail(e, SKG_ERR_ARGUMENT, "skg_source_particles: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    const int nl = e->nlambda;
+    if (nl < 1) return fail(e, SKG_ERR_STATE, "skg_source_particles: upload the dust wavelengths first");
+    double sum = 0.;
+    for (int64_t m = 0; m < n; ++m)
+    {
+        for (int q = 0; q < 4; ++q)
+            if (!std::isfinite(xyzh[4 * m + q])) return fail(e, SKG_ERR_ARGUMENT, "skg_source_particles: position not finite");
+        if (!(xyzh[4 * m + 3] >= 0)) return fail(e, SKG_ERR_ARGUMENT, "skg_source_particles: negative smoothing length");
+        if (!(ltot[m] >= 0) || !std::isfinite(ltot[m])) return fail(e, SKG_ERR_ARGUMENT, "skg_source_particles: bad luminosity");
+        for (int l = 0; l < nl; ++l)
+            if (!(lbin[m * nl + l] >= 0) || !std::isfinite(lbin[m * nl + l]))
+                return fail(e, SKG_ERR_ARGUMENT, "skg_source_particles: bad bin luminosity");
+        sum += ltot[m];
+    }
+    if (!(sum > 0)) return fail(e, SKG_ERR_ARGUMENT, "skg_source_particles: zero total luminosity");
+    // ImportedSource::prepareForLaunch (ImportedSource.cpp:290-322): launch weights mixing the
+    // luminosity fractions with a uniform share, and the first history index of each entity
+    std::vector<double> Wv(n);
+    for (int64_t m = 0; m < n; ++m) Wv[m] = (1 - source_bias) * (ltot[m] / sum) + source_bias / n;
+    std::vector<int64_t> Iv(n + 1);
+    Iv[0] = 0;
+    double W = 0.;
+    for (int64_t m = 1; m < n; ++m)
+    {
+        W += Wv[m - 1];
+        Iv[m] = std::min<int64_t>(num_packets, static_cast<int64_t>(std::round(W * num_packets)));
+    }
+    Iv[n] = num_packets;
+    // per packet of entity m in bin l: L_src / N * (L_m / L_src) / W_m (the selection weight
+    // ws) * (L_m,l / L_m) * nlambda (the oligochromatic bias weight s / b with a uniform bin
+    // choice, ImportedSource.cpp:670-705); entities without luminosity launch nothing
+    std::vector<double> lp(static_cast<size_t>(n) * nl);
+    for (int64_t m = 0; m < n; ++m)
+        for (int l = 0; l < nl; ++l)
+            lp[m * nl + l] = ltot[m] > 0 ? lbin[m * nl + l] * nl / (static_cast<double>(num_packets) * Wv[m]) : 0.;
+    // CubicSplineSmoothingKernel::setupSelfBefore (CubicSplineSmoothingKernel.cpp:22-40)
+    std::vector<double> X(kSplineN + 1);
+    const double du = 1.0 / kSplineN;
+    for (int k = 0; k <= kSplineN; ++k)
+    {
+        double u = k * du;
+        double u2 = u * u;
+        double u3 = u * u2;
+        if (u < 0.5)
+            X[k] = u3 * (32.0 / 3.0 - 192.0 / 5.0 * u2 + 32.0 * u3);
+        else
+            X[k] = -1.0 / 15.0 + u3 * (64.0 / 3.0 - 48.0 * u + 192.0 / 5.0 * u2 - 32.0 / 3.0 * u3);
+    }
+    int st;
+    if ((st = upload(e, &e->ent, xyzh, static_cast<size_t>(4 * n)))) return st;
+    if ((st = upload(e, &e->entFirst, Iv.data(), Iv.size()))) return st;
+    if ((st = upload(e, &e->entLpack, lp.data(), lp.size()))) return st;
+    if ((st = upload(e, &e->entX, X.data(), X.size()))) return st;
+    e->nent = n;
+    e->entKernel = kernel;
+    return SKG_OK;
+}
+
 int skg_options(skg_engine* e, double path_length_bias, double min_weight_reduction, int32_t min_scatt_events,
                 int32_t fp_mode, uint64_t seed)
 {
@@ -2134,6 +2272,16 @@ static int runPersistent(skg_engine* e, uint64_t first_history, uint64_t count)
     p.rf = e->rf;
     p.persp = 0;
     for (const auto& f : e->frames) p.persp |= f.persp;
+    p.nent = static_cast<int>(e->nent);
+    p.entKernel = e->entKernel;
+    p.ent = e->ent;
+    p.entFirst = e->entFirst;
+    p.entLpack = e->entLpack;
+    p.entX = e->entX;
+    if (p.nent && (p.persp || e->rf || !e->hInst.empty() || e->guideMode != 0))
+        return fail(e, SKG_ERR_STATE,
+                    "skg_run_packets: a particle source runs without perspective instruments, instances, the guide or "
+                    "radiation-field storage");
     if (p.persp && (e->rf || !e->hInst.empty() || e->guideMode != 0))
         return fail(e, SKG_ERR_STATE,
                     "skg_run_packets: perspective instruments run without instances, the guide or radiation-field storage");
@@ -2431,6 +2579,8 @@ int skg_run_packets_async(skg_engine* e, uint64_t first_history, uint64_t count)
     if (e->mode == SKG_MODE_PIPELINE)
         for (const auto& f : e->frames)
             if (f.persp) return fail(e, SKG_ERR_STATE, "skg_run_packets: perspective instruments run in the persistent engine");
+    if (e->mode == SKG_MODE_PIPELINE && e->nent)
+        return fail(e, SKG_ERR_STATE, "skg_run_packets: a particle source runs in the persistent engine");
     if (e->mode == SKG_MODE_PIPELINE) return runPipeline(e, first_history, count);
     return runPersistent(e, first_history, count);
 }

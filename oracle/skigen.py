@@ -34,6 +34,10 @@ class SkiSpec:
     persp_instruments: Sequence = ()
     # store the radiation field and probe its mean intensity per cell (RadiationFieldProbe)
     store_rf: bool = False
+    # a ParticleSource (black-body SED family) read from this file, next to the ski file,
+    # instead of the point source; and its smoothing kernel
+    particle_file: str = ""
+    particle_kernel: str = "CubicSplineSmoothingKernel"
     num_pixels: int = 200
     fov_pc: float = 20.0
     seed: int = 0
@@ -46,6 +50,42 @@ class SkiSpec:
 
 def _fmt_list(vals, unit):
     return ", ".join(f"{v:.9g} {unit}" for v in vals)
+
+
+def _source(spec: SkiSpec) -> str:
+    if spec.particle_file:
+        return (f'<ParticleSource filename="{spec.particle_file}" importVelocity="false" '
+                'importVelocityDispersion="false" importCurrentMass="false" importBias="false" useColumns="" '
+                'sourceWeight="1" wavelengthBias="0.5">'
+                f'<smoothingKernel type="SmoothingKernel"><{spec.particle_kernel}/></smoothingKernel>'
+                '<sedFamily type="SEDFamily"><BlackBodySEDFamily/></sedFamily></ParticleSource>')
+    return ('<PointSource positionX="0 pc" positionY="0 pc" positionZ="0 pc" velocityX="0 km/s" velocityY="0 km/s" '
+            'velocityZ="0 km/s" sourceWeight="1" wavelengthBias="0.5">'
+            '<sed type="SED"><BlackBodySED temperature="5800 K"/></sed>'
+            '<normalization type="LuminosityNormalization"><SpecificLuminosityNormalization wavelength="0.55 micron" '
+            'unitStyle="wavelengthmonluminosity" specificLuminosity="1 Lsun/micron"/></normalization></PointSource>')
+
+
+def disk_particles(n=30, seed=12):
+    """A fixed set of stellar particles in the disk plane: x, y, z, h (pc), radius (km),
+    temperature (K) per row (radii 1-7 pc, smoothing lengths 0.5-2 pc, 3500-12000 K)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    R = rng.uniform(1, 7, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    return np.column_stack([R * np.cos(ph), R * np.sin(ph), rng.normal(0, 0.2, n), rng.uniform(0.5, 2.0, n),
+                            rng.uniform(3e5, 3e6, n), rng.uniform(3500, 12000, n)])
+
+
+def write_particles(path, rows):
+    """A particle file for ParticleSource with BlackBodySEDFamily: x, y, z, h (pc), radius (km),
+    temperature (K) per row."""
+    with open(path, "w") as f:
+        for i, (name, unit) in enumerate((("position x", "pc"), ("position y", "pc"), ("position z", "pc"),
+                                          ("size h", "pc"), ("radius", "km"), ("temperature", "K"))):
+            f.write(f"# column {i + 1}: {name} ({unit})\n")
+        for r in rows:
+            f.write(" ".join(f"{v:.10g}" for v in r) + "\n")
 
 
 def render(spec: SkiSpec) -> str:
@@ -117,12 +157,7 @@ def render(spec: SkiSpec) -> str:
         <sourceSystem type="SourceSystem">
             <SourceSystem wavelengths="{wl}" sourceBias="0.5">
                 <sources type="Source">
-                    <PointSource positionX="0 pc" positionY="0 pc" positionZ="0 pc" velocityX="0 km/s" velocityY="0 km/s" velocityZ="0 km/s" sourceWeight="1" wavelengthBias="0.5">
-                        <sed type="SED"><BlackBodySED temperature="5800 K"/></sed>
-                        <normalization type="LuminosityNormalization">
-                            <SpecificLuminosityNormalization wavelength="0.55 micron" unitStyle="wavelengthmonluminosity" specificLuminosity="1 Lsun/micron"/>
-                        </normalization>
-                    </PointSource>
+                    {_source(spec)}
                 </sources>
             </SourceSystem>
         </sourceSystem>

@@ -140,6 +140,9 @@ class Simulation:
     path_length_bias: float = 0.5
     num_density_samples: int = 100
     cell_volumes: object = None  # set by setup(): volume per cell (m3)
+    # ParticleSource instead of the point source: xyzh (n, 4) m, radius (n,) m, temperature (n,) K,
+    # kernel 0 uniform / 1 cubic spline
+    particles: object = None
 
 
 def _one(root, tag):
@@ -154,9 +157,8 @@ def load(path: str) -> Simulation:
     sim = _one(root, "MonteCarloSimulation")
     if sim.get("simulationMode") != "OligoExtinctionOnly":
         raise UnsupportedSki("only simulationMode OligoExtinctionOnly is supported")
-    for tag in ("GeometricMedium", "PointSource"):
-        if not root.findall(".//" + tag):
-            raise UnsupportedSki(f"no {tag}")
+    if not root.findall(".//GeometricMedium"):
+        raise UnsupportedSki("no GeometricMedium")
     media = root.findall(".//media/*")
     sources = root.findall(".//sources/*")
     if len(media) != 1 or len(sources) != 1:
@@ -167,12 +169,23 @@ def load(path: str) -> Simulation:
     seed = int(_one(root, "Random").get("seed", "0"))
     ss = _one(root, "SourceSystem")
     lam = np.array(sorted(quantities(ss.get("wavelengths"))))
-    ps = _one(root, "PointSource")
-    pos = tuple(quantity(ps.get(k, "0 pc")) for k in ("positionX", "positionY", "positionZ"))
-    bb = _one(root, "BlackBodySED")
-    norm = _one(root, "SpecificLuminosityNormalization")
-    if norm.get("unitStyle", "wavelengthmonluminosity") != "wavelengthmonluminosity":
-        raise UnsupportedSki("only wavelengthmonluminosity normalizations are supported")
+    particles = None
+    if sources[0].tag == "ParticleSource":
+        if float(ss.get("sourceBias", "0.5")) != 0.5:
+            raise UnsupportedSki("a ParticleSource runs with sourceBias 0.5")
+        particles = _particles(path, sources[0])
+        pos, temperature, norm_wl, norm_l = (0.0, 0.0, 0.0), 0.0, 0.0, 0.0
+    elif sources[0].tag == "PointSource":
+        ps = sources[0]
+        pos = tuple(quantity(ps.get(k, "0 pc")) for k in ("positionX", "positionY", "positionZ"))
+        bb = _one(root, "BlackBodySED")
+        norm = _one(root, "SpecificLuminosityNormalization")
+        if norm.get("unitStyle", "wavelengthmonluminosity") != "wavelengthmonluminosity":
+            raise UnsupportedSki("only wavelengthmonluminosity normalizations are supported")
+        temperature, norm_wl = quantity(bb.get("temperature")), quantity(norm.get("wavelength"))
+        norm_l = quantity(norm.get("specificLuminosity"))
+    else:
+        raise UnsupportedSki("only a PointSource or a ParticleSource is supported")
     opts = _one(root, "PhotonPacketOptions")
     disk = _one(root, "ExpDiskGeometry")
     for k in ("minRadius", "maxRadius", "maxZ"):
@@ -256,15 +269,68 @@ def load(path: str) -> Simulation:
                             quantity(fi.get("centerX")), quantity(fi.get("centerY"))))
     return Simulation(
         num_packets=int(float(sim.get("numPackets"))), seed=seed, wavelengths=lam, source_position=pos,
-        temperature=quantity(bb.get("temperature")), norm_wavelength=quantity(norm.get("wavelength")),
-        norm_specific_luminosity=quantity(norm.get("specificLuminosity")), spiral=bool(spiral_el),
+        temperature=temperature, norm_wavelength=norm_wl, norm_specific_luminosity=norm_l, spiral=bool(spiral_el),
         scale_length=quantity(disk.get("scaleLength")), scale_height=quantity(disk.get("scaleHeight")),
         spiral_params=spiral, dust_table=tab, tau_v=float(mn.get("opticalDepth")),
         tau_wavelength=quantity(mn.get("wavelength")), grid=grid, box=box, frames=frames, rf_probes=rf_probes,
         force_scattering=opts.get("forceScattering", "true") == "true",
         min_weight_reduction=float(opts.get("minWeightReduction", "1e4")),
         min_scatt_events=int(opts.get("minScattEvents", "0")),
-        path_length_bias=float(opts.get("pathLengthBias", "0.5")), num_density_samples=nds)
+        path_length_bias=float(opts.get("pathLengthBias", "0.5")), num_density_samples=nds, particles=particles)
+
+
+_PARTICLE_COLUMNS = (("position x", "pc"), ("position y", "pc"), ("position z", "pc"), ("size h", "pc"),
+                     ("radius", "km"), ("temperature", "K"))
+
+
+def _particles(path, el):
+    """ParticleSource with BlackBodySEDFamily (ParticleSource.hpp, ImportedSource.hpp,
+    BlackBodySEDFamily.cpp:11-14): the text columns x, y, z, h, radius, temperature in the
+    units their '# column i: name (unit)' header lines give, or the defaults pc, pc, pc, pc,
+    km, K (Snapshot.cpp:74-104)."""
+    for k in ("importVelocity", "importVelocityDispersion", "importCurrentMass", "importBias"):
+        if el.get(k, "false") != "false":
+            raise UnsupportedSki(f"ParticleSource {k} is not supported")
+    if el.get("useColumns", ""):
+        raise UnsupportedSki("ParticleSource useColumns is not supported")
+    kern = el.findall("./smoothingKernel/*")
+    kname = kern[0].tag if kern else "CubicSplineSmoothingKernel"
+    kernels = {"UniformSmoothingKernel": 0, "CubicSplineSmoothingKernel": 1}
+    if kname not in kernels:
+        raise UnsupportedSki(f"smoothing kernel {kname} is not supported")
+    fam = el.findall("./sedFamily/*")
+    if fam and fam[0].tag != "BlackBodySEDFamily":
+        raise UnsupportedSki("only the BlackBodySEDFamily is supported")
+    fname = os.path.join(os.path.dirname(os.path.abspath(path)), el.get("filename"))
+    units = [u for _, u in _PARTICLE_COLUMNS]
+    rows = []
+    with open(fname) as f:
+        for line in f:
+            t = line.strip()
+            if not t:
+                continue
+            if t.startswith("#"):
+                low = t[1:].strip().lower()
+                if low.startswith("column") and "(" in t and ":" in t:
+                    i = int(low.split(":")[0].split()[1]) - 1
+                    if 0 <= i < len(units):
+                        units[i] = t[t.rindex("(") + 1:t.rindex(")")].strip()
+                continue
+            rows.append([float(v) for v in t.split()[:6]])
+    a = np.array(rows, dtype=np.float64).reshape(-1, 6)
+    fac = [1.0 if u == "K" else quantity("1 " + u) for u in units]
+    a = a * np.array(fac)
+    return dict(xyzh=a[:, :4].copy(), radius=a[:, 4].copy(), temperature=a[:, 5].copy(),
+                kernel=kernels[kname])
+
+
+def particle_bin_luminosities(sim: Simulation):
+    """Luminosity (W) of each particle in each wavelength bin: BlackBodySEDFamily's
+    L_lambda = 4 pi^2 R^2 B_lambda(T) (BlackBodySEDFamily.cpp:25-32) times the bin width."""
+    p = sim.particles
+    lam = sim.wavelengths
+    R, T = p["radius"][:, None], p["temperature"][:, None]
+    return 4 * math.pi * math.pi * R * R * planck(lam[None, :], T) * bin_widths(lam)[None, :]
 
 
 def planck(lam, T):
@@ -541,7 +607,14 @@ def setup(sim: Simulation, engine, threads=0, npk_norm=None):
             engine.frame_add(f.inclination, f.azimuth, f.roll, f.nx, f.ny, f.fovx, f.fovy, f.cx, f.cy)
     # the bin is chosen uniformly, so each packet of bin ell carries nlambda * L_ell / N
     npk = npk_norm or sim.num_packets
-    engine.source_point(sim.source_position, bin_luminosities(sim) * len(sim.wavelengths) / npk)
+    if sim.particles is not None:
+        # the particles' launch ranges split the history indices [0, npk); the launch weights use
+        # each particle's luminosity summed over the bins
+        lbin = particle_bin_luminosities(sim)
+        engine.source_point((0.0, 0.0, 0.0), np.zeros(len(sim.wavelengths)))
+        engine.source_particles(sim.particles["xyzh"], lbin, lbin.sum(axis=1), npk, 0.5, sim.particles["kernel"])
+    else:
+        engine.source_point(sim.source_position, bin_luminosities(sim) * len(sim.wavelengths) / npk)
     engine.options(sim.path_length_bias, sim.min_weight_reduction, sim.min_scatt_events, 0, sim.seed)
     # the persistent engine: the faster one here, and the one that stores the radiation field
     engine.tune_pipeline(pkg.SKG_MODE_PERSISTENT, 1 << 21, 0, 8)

@@ -38,6 +38,10 @@ Every fixture is data: seeded inputs plus the reference's outputs for them.
                    length, from (10, -30, 20) pc toward (1, 0, 0) pc, so the image has no mirror
                    symmetry): per-pixel mean and
                    standard error of the FITS surface brightness over independent runs.
+  image_particles.npz  (--particles) the config-0 disk lit by 30 black-body stellar particles
+                   (ParticleSource, skigen.disk_particles) instead of the point source, with the
+                   cubic-spline and the uniform smoothing kernel, 100x100 frame at 60 deg:
+                   per-pixel mean and standard error over independent runs.
   rf_cfg0.npz      (--rf) the same workload on a 16^3 grid (1e4 density samples per cell)
                    storing the radiation field:
                    the RadiationFieldProbe's mean intensity J_nu per cell (PerCellForm),
@@ -536,6 +540,27 @@ def fixtures_persp(tmp, runs=8, packets=1e6):
                         runs=np.array(runs), packets=np.array(packets))
 
 
+def fixtures_particles(tmp, runs=8, packets=1e6):
+    outdir = os.path.join(tmp, "particles")
+    os.makedirs(outdir, exist_ok=True)
+    skigen.write_particles(os.path.join(tmp, "parts.txt"), skigen.disk_particles())
+    out = {"runs": np.array(runs), "packets": np.array(packets)}
+    for kern in ("CubicSplineSmoothingKernel", "UniformSmoothingKernel"):
+        imgs = []
+        for s in range(runs):
+            spec = skigen.config0(num_packets=packets, seed=10000 + s)
+            spec.particle_file = "parts.txt"
+            spec.particle_kernel = kern
+            spec.num_pixels = 100
+            ski = write_ski(tmp, f"pt{s}", spec)
+            run("run", ski, outdir, 8)
+            imgs.append(fits_image(os.path.join(outdir, f"pt{s}_i60_total.fits")))
+        imgs = np.array(imgs)
+        out[kern + "_mean"] = imgs.mean(axis=0).astype(np.float32)
+        out[kern + "_sem"] = (imgs.std(axis=0, ddof=1) / np.sqrt(runs)).astype(np.float32)
+    np.savez_compressed(os.path.join(HERE, "image_particles.npz"), **out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--images", action="store_true")
@@ -543,6 +568,7 @@ def main():
     ap.add_argument("--rf", action="store_true")
     ap.add_argument("--meshes", action="store_true")
     ap.add_argument("--persp", action="store_true")
+    ap.add_argument("--particles", action="store_true")
     ap.add_argument("--only", default="")
     a = ap.parse_args()
     with tempfile.TemporaryDirectory() as tmp:
@@ -567,6 +593,8 @@ def main():
             fixtures_mesh_images(tmp)
         if a.persp:
             fixtures_persp(tmp)
+        if a.particles:
+            fixtures_particles(tmp)
 
 
 if __name__ == "__main__":

@@ -77,3 +77,59 @@ def test_particle_source_matches_oracle(kernel):
     pfr, _ = O.simulate(grid, dens, lam, sext, ssca, g, [1.0 / npk, 1.0 / npk], instr, 0, 5000, threads=8, seed=41)
     np.testing.assert_allclose(eng.frame_read(0), pfr[0], rtol=1e-9, atol=1e-12 * pfr[0].max())
     eng.close()
+
+
+@pytest.mark.parametrize("kernel", ["CubicSplineSmoothingKernel", "UniformSmoothingKernel"])
+def test_config0_particle_source_matches_reference(tmp_path, kernel):
+    """The config-0 disk lit by 30 black-body particles (ParticleSource read from a text file)
+    through the ski front end, against 8 reference runs (tests/golden/image_particles.npz): total
+    within 4 sigma, 5x5-binned pooled chi-square per degree of freedom < 1.5."""
+    import os
+    import sys
+    from helpers import ROOT, golden
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import skigen
+    pkg = load_pkg()
+    from skirt9_gpu_amd import ski as S
+    skigen.write_particles(os.path.join(str(tmp_path), "parts.txt"), skigen.disk_particles())
+    spec = skigen.config0(num_packets=1e6)
+    spec.particle_file = "parts.txt"
+    spec.particle_kernel = kernel
+    spec.num_pixels = 100
+    path = os.path.join(str(tmp_path), "pt.ski")
+    with open(path, "w") as f:
+        f.write(skigen.render(spec))
+    sim = S.load(path)
+    assert sim.particles is not None and len(sim.particles["xyzh"]) == 30
+    eng = pkg.Engine(0)
+    nb = 6
+    imgs = []
+    for bi in range(nb):
+        sim.seed = 11000 + bi
+        print("case ski particle source", kernel, bi, flush=True)
+        out = S.run(sim, str(tmp_path), f"b{bi}", engine=eng, log=lambda *a: None)
+        imgs.append(out["i60"][0])
+    eng.close()
+    f = golden("image_particles.npz")
+    ref, rsem = f[kernel + "_mean"][0].astype(float), f[kernel + "_sem"][0].astype(float)
+    imgs = np.array(imgs)
+    nref = int(f["runs"])
+    tot = imgs.sum(axis=(1, 2))
+    ref_run_var_tot = (rsem ** 2).sum() * nref  # an underestimate of the run variance of the total (pixels correlate)
+    pooled_tot = ((nref - 1) * ref_run_var_tot + (nb - 1) * tot.var(ddof=1)) / (nref + nb - 2)
+    assert abs(tot.mean() - ref.sum()) < 4 * np.sqrt(pooled_tot * (1 / nb + 1 / nref)), (tot.mean(), ref.sum())
+
+    def binned(a, B=5):
+        n = a.shape[-1] // B
+        return a.reshape(*a.shape[:-2], n, B, n, B).sum(axis=(-1, -3))
+
+    oi = binned(imgs)
+    var_ref_run = binned((rsem ** 2)[None])[0] * nref
+    pooled = ((nref - 1) * var_ref_run + (nb - 1) * oi.var(0, ddof=1)) / (nref + nb - 2)
+    var = pooled * (1 / nb + 1 / nref)
+    m = var > 0
+    z = (oi.mean(0) - binned(ref[None])[0])[m] / np.sqrt(var[m])
+    print("bins", int(m.sum()), "chi2/dof", float(np.mean(z ** 2)), "max |z|", float(np.abs(z).max()), flush=True)
+    assert m.sum() > 100
+    assert np.mean(z ** 2) < 1.5, np.mean(z ** 2)
+    assert np.mean(np.abs(z) > 4) < 0.01

@@ -231,3 +231,35 @@ def test_perspective_parse_and_transform(ski, tmp_path):
         x2, y2, _ = _project(M, cross + PC * u)
         assert abs(x1 - 50 - mag) < 1e-9 and abs(y1 - 40) < 1e-9
         assert abs(x2 - 50) < 1e-9 and abs(y2 - 40 - mag) < 1e-9
+
+
+def test_particle_source_parse(ski, tmp_path):
+    """ParticleSource with BlackBodySEDFamily: columns in the units of the '# column' header
+    lines (positions and h in pc, radius in km, temperature in K); per-bin luminosity
+    4 pi^2 R^2 B_lambda(T) times the bin width; unsupported options are refused."""
+    rows = skigen.disk_particles()
+    skigen.write_particles(os.path.join(str(tmp_path), "parts.txt"), rows)
+    spec = skigen.config0(num_packets=1e5)
+    spec.particle_file = "parts.txt"
+    p = _write(str(tmp_path), "pt", spec)
+    sim = ski.load(p)
+    P = sim.particles
+    assert P["kernel"] == 1 and len(P["xyzh"]) == len(rows)
+    np.testing.assert_allclose(P["xyzh"], rows[:, :4] * PC, rtol=1e-9)
+    np.testing.assert_allclose(P["radius"], rows[:, 4] * 1e3, rtol=1e-9)
+    np.testing.assert_allclose(P["temperature"], rows[:, 5], rtol=1e-9)
+    L = ski.particle_bin_luminosities(sim)
+    lam, T, R = 0.55e-6, rows[0, 5], rows[0, 4] * 1e3
+    h, c, k = 6.62607015e-34, 2.99792458e8, 1.380649e-23
+    B = 2 * h * c * c / lam ** 5 / np.expm1(h * c / (lam * k * T))
+    assert abs(L[0, 0] / (4 * np.pi ** 2 * R * R * B * 2e-3 * lam) - 1) < 1e-6
+    text = open(p).read()
+    spec.particle_kernel = "UniformSmoothingKernel"
+    assert ski.load(_write(str(tmp_path), "pu", spec)).particles["kernel"] == 0
+    for bad in (text.replace('importVelocity="false"', 'importVelocity="true"'),
+                text.replace("CubicSplineSmoothingKernel", "ScaledGaussianSmoothingKernel"),
+                text.replace("BlackBodySEDFamily", "BruzualCharlotSEDFamily")):
+        with open(p, "w") as f:
+            f.write(bad)
+        with pytest.raises(ski.UnsupportedSki):
+            ski.load(p)

@@ -173,8 +173,8 @@ int skg_source_point(skg_engine* e, double x, double y, double z, const double* 
  * packet takes a uniform bin, the luminosity lbin[m][l] * nlambda / (num_packets * W_m), a
  * position r_m + k u h_m (k isotropic; u from the kernel: 0 uniform, u = X^(1/3); 1 cubic
  * spline, the 400-interval table inversion of CubicSplineSmoothingKernel.cpp:78-85) and an
- * isotropic direction.  Persistent engine; not combined with perspective instruments,
- * instances, the guide or radiation-field storage.  n = 0 restores the point source. */
+ * isotropic direction.  Persistent engine; not combined with instances or the guide.  n = 0
+ * restores the point source. */
 int skg_source_particles(skg_engine* e, int64_t n, const double* xyzh, const double* lbin, const double* ltot,
                          double source_bias, int32_t kernel, int64_t num_packets);
 int skg_options(skg_engine* e, double path_length_bias, double min_weight_reduction, int32_t min_scatt_events,

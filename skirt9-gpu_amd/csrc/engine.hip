@@ -642,7 +642,14 @@ namespace skg
             pk.lam = p.lambda[pk.ell];
             T.section = INST ? p.sext[pk.ell] * p.inst[pk.inst].scale : p.sext[pk.ell];
             T.section32 = static_cast<float>(T.section);
+            // the particle launch: its own variant, and a run-time branch of the radiation-field
+            // and perspective variants
+            bool particles = false;
             if constexpr (V == kParticles)
+                particles = true;
+            else if constexpr (V == kStoreRF || V == kPersp)
+                particles = p.nent > 0;
+            if (particles)
             {
                 // ImportedSource::launch: the entity whose history index range holds h
                 // (ImportedSource.cpp:466-471), its luminosity for the bin (:653-705, the
@@ -2278,10 +2285,8 @@ static int runPersistent(skg_engine* e, uint64_t first_history, uint64_t count)
     p.entFirst = e->entFirst;
     p.entLpack = e->entLpack;
     p.entX = e->entX;
-    if (p.nent && (p.persp || e->rf || !e->hInst.empty() || e->guideMode != 0))
-        return fail(e, SKG_ERR_STATE,
-                    "skg_run_packets: a particle source runs without perspective instruments, instances, the guide or "
-                    "radiation-field storage");
+    if (p.nent && (!e->hInst.empty() || e->guideMode != 0))
+        return fail(e, SKG_ERR_STATE, "skg_run_packets: a particle source runs without instances and the guide");
     if (p.persp && (e->rf || !e->hInst.empty() || e->guideMode != 0))
         return fail(e, SKG_ERR_STATE,
                     "skg_run_packets: perspective instruments run without instances, the guide or radiation-field storage");

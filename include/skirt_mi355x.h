@@ -229,11 +229,13 @@ int skg_counters(skg_engine* e, int64_t* out8);
 int skg_tune(skg_engine* e, int32_t workgroups, int32_t threads, int32_t transition_threshold);
 
 /* Autotuner (no counterpart in the reference, which has no launch parameters): times a trial
- * run of `trial_packets` histories (from first_history on; the detector contents and the
- * counters are restored afterwards) for each candidate launch configuration -- persistent
+ * run of `trial_packets` histories (from first_history on; the detector contents, the stored
+ * radiation field and the counters are restored afterwards) for each candidate launch
+ * configuration -- persistent
  * engine: transition threshold 16/32/48 x 2/4/8 workgroups per CU; pipeline engine: 2^20/2^21/
  * 2^22 packet slots x 8/16 graph iterations x 4/8 trace workgroups per CU -- and keeps the
- * fastest.  engines: bit 0 persistent, bit 1 pipeline (batched instances: persistent only).
+ * fastest.  engines: bit 0 persistent, bit 1 pipeline (batched instances, particle sources,
+ * radiation-field storage and perspective instruments: persistent only).
  * best6 (may be NULL) receives {mode, threshold, workgroups, pool_slots, trace_workgroups,
  * graph_iterations}; best_rate the trial packets/s.  Not to be called concurrently with other
  * calls on the same engine. */

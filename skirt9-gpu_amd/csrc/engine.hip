@@ -2876,14 +2876,19 @@ int skg_autotune(skg_engine* e, uint64_t first_history, uint64_t trial_packets, 
     if (!e || trial_packets < 1024 || !(engines >= 1 && engines <= 3))
         return fail(e, SKG_ERR_ARGUMENT, "skg_autotune: bad arguments");
     if (e->guideMode == 2) return fail(e, SKG_ERR_STATE, "skg_autotune: not in guide collect mode");
-    // batched instances run in the persistent engine only
-    if (!e->hInst.empty()) engines &= 1;
+    // instances, particle sources, radiation-field storage and perspective instruments run in
+    // the persistent engine only
+    bool persistentOnly = !e->hInst.empty() || e->nent > 0 || e->rf != nullptr;
+    for (const auto& f : e->frames) persistentOnly = persistentOnly || f.persp;
+    if (persistentOnly) engines &= 1;
     if (!engines) return fail(e, SKG_ERR_ARGUMENT, "skg_autotune: no engine supports this configuration");
     if (!e->hInst.empty() && e->instTotal < trial_packets + trial_packets / 4)
         return fail(e, SKG_ERR_ARGUMENT, "skg_autotune: fewer instance histories than one trial");
     int st = SKG_OK;
-    // keep the caller's detector contents and counters
+    // keep the caller's detector contents, stored radiation field and counters
     double* saved = nullptr;
+    double* savedRf = nullptr;
+    const size_t rfSize = e->rf ? static_cast<size_t>(e->rfCells) * e->rfLambda : 0;
     unsigned long long savedCnt[8];
     {
         std::lock_guard<std::mutex> lk(e->mu);
@@ -2891,6 +2896,8 @@ int skg_autotune(skg_engine* e, uint64_t first_history, uint64_t trial_packets, 
         if ((st = ensureFrames(e))) return st;
         if (e->frameSize) HIPCHK(e, hipMalloc(&saved, e->frameSize * sizeof(double)));
         if (saved) HIPCHK(e, hipMemcpy(saved, e->frameBuf, e->frameSize * sizeof(double), hipMemcpyDeviceToDevice));
+        if (rfSize) HIPCHK(e, hipMalloc(&savedRf, rfSize * sizeof(double)));
+        if (savedRf) HIPCHK(e, hipMemcpy(savedRf, e->rf, rfSize * sizeof(double), hipMemcpyDeviceToDevice));
         HIPCHK(e, hipMemcpy(savedCnt, e->counters, sizeof(savedCnt), hipMemcpyDeviceToHost));
     }
     struct Cand
@@ -2940,6 +2947,11 @@ int skg_autotune(skg_engine* e, uint64_t first_history, uint64_t trial_packets, 
     {
         hipMemcpy(e->frameBuf, saved, e->frameSize * sizeof(double), hipMemcpyDeviceToDevice);
         hipFree(saved);
+    }
+    if (savedRf)
+    {
+        hipMemcpy(e->rf, savedRf, rfSize * sizeof(double), hipMemcpyDeviceToDevice);
+        hipFree(savedRf);
     }
     hipMemcpy(e->counters, savedCnt, sizeof(savedCnt), hipMemcpyHostToDevice);
     if (st) return st;

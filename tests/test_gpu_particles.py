@@ -184,3 +184,35 @@ def test_particle_source_with_rf_or_perspective_matches_oracle(extra):
         assert np.max(np.abs(rf[nz] - orf[nz]) / orf[nz]) < 1e-9
         eng.rf_enable(False)
     eng.close()
+
+
+def test_autotune_with_particles_and_rf_keeps_state():
+    """With a particle source and radiation-field storage the autotuner tries the persistent
+    engine only (the pipeline engine runs neither) and restores the frame, the stored field and
+    the counters after its trials."""
+    pkg = load_pkg()
+    eng = pkg.Engine(0)
+    b, dens = disk_density_cartesian(n=16, samples=4, seed=5, tau_v=1.0)
+    eng.grid_cartesian(b, b, b)
+    eng.medium(dens)
+    s = 1.5e-29
+    eng.dust([0.55e-6], [s], [0.6 * s], [0.6])
+    eng.frame_remove_all()
+    eng.frame_add(np.radians(60.0), 0.0, 0.0, 30, 30, 20 * PC, 20 * PC)
+    xyzh, lbin, ltot = _particles(n=10, seed=3)
+    eng.source_point((0.0, 0.0, 0.0), [0.0])
+    eng.source_particles(xyzh, lbin[:, :1], ltot, 4000, kernel=1)
+    eng.options(0.5, 1e4, 0, 0, 47)
+    eng.tune_pipeline(pkg.SKG_MODE_PERSISTENT, 1 << 18, 0, 8)
+    eng.rf_enable(True)
+    eng.frame_clear()
+    eng.run_packets(0, 4000)
+    img, rf, cnt = eng.frame_read(0), eng.rf_read(), eng.counters()
+    print("case autotune with particles and rf", flush=True)
+    cfg, rate = eng.autotune(10_000_000, trial_packets=20000)
+    assert rate > 0 and cfg["mode"] == pkg.SKG_MODE_PERSISTENT
+    np.testing.assert_array_equal(eng.frame_read(0), img)
+    np.testing.assert_array_equal(eng.rf_read(), rf)
+    np.testing.assert_array_equal(eng.counters(), cnt)
+    eng.rf_enable(False)
+    eng.close()

@@ -143,7 +143,7 @@ int skg_sed_add(skg_engine* e, double inclination, double azimuth, double radius
  * position, pixel_size the viewport pixel size.  A packet at r peels off toward the eye; it is
  * recorded in pixel (int(x'/w'), int(y'/w')) when the depth z' exceeds pixel_size / 10, with
  * luminosity / z'^2 attenuated over a path length z'.  Persistent engine; not combined with
- * instances, the guide or radiation-field storage.  Returns the index or a negative status. */
+ * instances or the guide.  Returns the index or a negative status. */
 int skg_perspective_add(skg_engine* e, const double* transform, const double* eye, double pixel_size, int32_t nx,
                         int32_t ny);
 int skg_frame_remove_all(skg_engine* e);

@@ -316,7 +316,8 @@ namespace skg
         kPlain = 0,      // one run
         kBatched = 1,    // batched independent instances
         kGuided = 2,     // scattering directions from the defensive guide / HG mixture
-        kStoreRF = 3,    // one run that also stores the radiation field of the forced paths
+        kStoreRF = 3,    // one run that also stores the radiation field of the forced paths (perspective
+                         // instruments allowed)
         kPersp = 4,      // one run with perspective instruments among the instruments
         kParticles = 5   // one run whose source is a set of imported particles
     };
@@ -736,7 +737,7 @@ namespace skg
             while (++pk.f < fe)
             {
                 const FrameGeom& fg = p.frames[pk.f];
-                if constexpr (V == kPersp)
+                if constexpr (V == kPersp || V == kStoreRF)
                 {
                     if (fg.persp)
                     {
@@ -862,7 +863,7 @@ namespace skg
                 pk.tauCached = T.tau_value();
                 const FrameGeom& fg = p.frames[pk.f];
                 double L = pk.Wp / pk.lam;
-                if constexpr (V == kPersp)
+                if constexpr (V == kPersp || V == kStoreRF)
                 {
                     if (fg.persp)
                     {
@@ -974,7 +975,7 @@ namespace skg
         {
             // advance every busy lane by one cell crossing
             bool busy = (T.phase == kPeel || T.phase == kFwd1 || T.phase == kFwd2) && !T.ended;
-            if (busy) T.template step<V == kStoreRF, V == kPersp>(p.grid, wgcnt, p.rf, p.nlambda);
+            if (busy) T.template step<V == kStoreRF, V == kPersp || V == kStoreRF>(p.grid, wgcnt, p.rf, p.nlambda);
 
             // wave-wide transition round once enough lanes wait
             unsigned long long waiting = __ballot(T.ended || T.phase == kFetch || (V == kGuided && T.phase == kScatter));
@@ -2287,9 +2288,8 @@ static int runPersistent(skg_engine* e, uint64_t first_history, uint64_t count)
     p.entX = e->entX;
     if (p.nent && (!e->hInst.empty() || e->guideMode != 0))
         return fail(e, SKG_ERR_STATE, "skg_run_packets: a particle source runs without instances and the guide");
-    if (p.persp && (e->rf || !e->hInst.empty() || e->guideMode != 0))
-        return fail(e, SKG_ERR_STATE,
-                    "skg_run_packets: perspective instruments run without instances, the guide or radiation-field storage");
+    if (p.persp && (!e->hInst.empty() || e->guideMode != 0))
+        return fail(e, SKG_ERR_STATE, "skg_run_packets: perspective instruments run without instances and the guide");
     if (e->rf && (e->rfCells != e->ncells || e->rfLambda != e->nlambda))
         return fail(e, SKG_ERR_STATE, "skg_run_packets: the radiation-field table does not match the grid / wavelengths");
     if (e->rf && (!e->hInst.empty() || e->guideMode != 0))

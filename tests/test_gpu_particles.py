@@ -135,9 +135,9 @@ def test_config0_particle_source_matches_reference(tmp_path, kernel):
     assert np.mean(np.abs(z) > 4) < 0.01
 
 
-@pytest.mark.parametrize("extra", ["rf", "persp"])
+@pytest.mark.parametrize("extra", ["rf", "persp", "rf+persp"])
 def test_particle_source_with_rf_or_perspective_matches_oracle(extra):
-    """Particles together with radiation-field storage, or with a perspective instrument: the
+    """Particles together with radiation-field storage, a perspective instrument, or both: the
     field per cell and every image within 1e-9 of the oracle's."""
     pkg = load_pkg()
     from skirt9_gpu_amd import ski as S
@@ -149,13 +149,13 @@ def test_particle_source_with_rf_or_perspective_matches_oracle(extra):
     lam, sext, ssca, g = [0.55e-6, 0.8e-6], [s, 0.7 * s], [0.6 * s, 0.35 * s], [0.6, 0.5]
     eng.dust(lam, sext, ssca, g)
     instr = [dict(inclination=np.radians(60.0), azimuth=0.0, roll=0.0, nx=40, ny=40, fovx=20 * PC, fovy=20 * PC)]
-    if extra == "persp":
+    if "persp" in extra:
         M, E, ps = S.perspective_setup(np.array([2.0, -5.0, 1.0]) * PC, np.zeros(3), np.array([0.0, 0.0, 1.0]),
                                        3 * PC, 8 * PC, 50, 50)
         instr.append(dict(persp=True, M=M, eye=E, psize=ps, nx=50, ny=50))
     eng.frame_remove_all()
     eng.frame_add(instr[0]["inclination"], 0.0, 0.0, 40, 40, 20 * PC, 20 * PC)
-    if extra == "persp":
+    if "persp" in extra:
         eng.perspective_add(instr[1]["M"], instr[1]["eye"], instr[1]["psize"], 50, 50)
     npk = 20000
     xyzh, lbin, ltot = _particles(n=25, seed=9)
@@ -163,21 +163,21 @@ def test_particle_source_with_rf_or_perspective_matches_oracle(extra):
     eng.source_particles(xyzh, lbin, ltot, npk, source_bias=0.5, kernel=1)
     eng.options(0.5, 1e4, 0, 0, 43)
     eng.tune_pipeline(pkg.SKG_MODE_PERSISTENT, 1 << 18, 0, 8)
-    if extra == "rf":
+    if "rf" in extra:
         eng.rf_enable(True)
     eng.frame_clear()
     print("case particle source with", extra, flush=True)
     eng.run_packets(0, npk)
     grid = O.Grid("cartesian", xb=b, yb=b, zb=b)
     res = O.simulate(grid, dens, lam, sext, ssca, g, [0.0, 0.0], instr, 0, npk, threads=8, seed=43,
-                     store_rf=extra == "rf",
+                     store_rf="rf" in extra,
                      particles=dict(xyzh=xyzh, lbin=lbin, ltot=ltot, num_packets=npk, source_bias=0.5, kernel=1))
     ofr = res[0]
     for i in range(len(instr)):
         v, ov = eng.frame_read(i), ofr[i]
         assert ov.sum() > 0
         np.testing.assert_allclose(v, ov, rtol=1e-9, atol=1e-12 * ov.max(), err_msg=f"instrument {i}")
-    if extra == "rf":
+    if "rf" in extra:
         rf, orf = eng.rf_read(), res[2]
         nz = orf > 0
         np.testing.assert_array_equal(rf > 0, nz)

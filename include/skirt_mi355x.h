@@ -142,7 +142,7 @@ int skg_sed_add(skg_engine* e, double inclination, double azimuth, double radius
  * HomogeneousTransform's row-vector convention (row-major, p' = (x, y, z, 1) M), eye the eye
  * position, pixel_size the viewport pixel size.  A packet at r peels off toward the eye; it is
  * recorded in pixel (int(x'/w'), int(y'/w')) when the depth z' exceeds pixel_size / 10, with
- * luminosity / z'^2 attenuated over a path length z'.  Persistent engine; not combined with
+ * luminosity / z'^2 attenuated over a path length z'.  Both engines; not combined with
  * instances or the guide.  Returns the index or a negative status. */
 int skg_perspective_add(skg_engine* e, const double* transform, const double* eye, double pixel_size, int32_t nx,
                         int32_t ny);
@@ -234,8 +234,8 @@ int skg_tune(skg_engine* e, int32_t workgroups, int32_t threads, int32_t transit
  * configuration -- persistent
  * engine: transition threshold 16/32/48 x 2/4/8 workgroups per CU; pipeline engine: 2^20/2^21/
  * 2^22 packet slots x 8/16 graph iterations x 4/8 trace workgroups per CU -- and keeps the
- * fastest.  engines: bit 0 persistent, bit 1 pipeline (batched instances, particle sources,
- * radiation-field storage and perspective instruments: persistent only).
+ * fastest.  engines: bit 0 persistent, bit 1 pipeline (batched instances, particle sources
+ * and radiation-field storage: persistent only).
  * best6 (may be NULL) receives {mode, threshold, workgroups, pool_slots, trace_workgroups,
  * graph_iterations}; best_rate the trial packets/s.  Not to be called concurrently with other
  * calls on the same engine. */

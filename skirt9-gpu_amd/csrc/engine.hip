@@ -1223,6 +1223,21 @@ namespace
         return SKG_OK;
     }
 
+    // instrument f shares the observer (direction, shared optical depth) of instrument f - 1:
+    // the same inclination, azimuth and roll (FrameInstrument / SEDInstrument), or, for
+    // perspective instruments, the same viewport (PerspectiveInstrument::
+    // determineSameObserverAsPreceding, PerspectiveInstrument.cpp:83-91)
+    bool sameObserverAsPrev(const skg_engine* e, size_t f)
+    {
+        if (f == 0) return false;
+        const auto& d = e->frames[f];
+        const auto& q = e->frames[f - 1];
+        if (d.persp || q.persp)
+            return d.persp && q.persp && std::memcmp(d.M, q.M, sizeof(d.M)) == 0
+                   && std::memcmp(d.eye, q.eye, sizeof(d.eye)) == 0 && d.psize == q.psize;
+        return d.inc == q.inc && d.az == q.az && d.roll == q.roll;
+    }
+
     void fillFrameGeom(const skg_engine* e, std::vector<FrameGeom>& out)
     {
         int64_t off = 0;
@@ -1264,21 +1279,13 @@ namespace
                 g.kobs[1] = st * std::sin(d.az);
                 g.kobs[2] = std::cos(d.inc);
             }
-            g.same_as_prev = f > 0 && d.inc == e->frames[f - 1].inc && d.az == e->frames[f - 1].az
-                             && d.roll == e->frames[f - 1].roll;
+            g.same_as_prev = sameObserverAsPrev(e, f);
             g.sed = d.sed;
             g.radius2 = d.radius * d.radius;
             g.persp = d.persp;
             std::memcpy(g.M, d.M, sizeof(g.M));
             std::memcpy(g.eye, d.eye, sizeof(g.eye));
             g.psize = d.psize;
-            if (f > 0 && (d.persp || e->frames[f - 1].persp))
-            {
-                // PerspectiveInstrument::determineSameObserverAsPreceding: the same viewport
-                const auto& q = e->frames[f - 1];
-                g.same_as_prev = d.persp && q.persp && std::memcmp(d.M, q.M, sizeof(d.M)) == 0
-                                 && std::memcmp(d.eye, q.eye, sizeof(d.eye)) == 0 && d.psize == q.psize;
-            }
             g.offset = off;
             off += static_cast<int64_t>(d.nx) * d.ny * e->nlambda;
         }
@@ -2411,10 +2418,9 @@ namespace
     int runPipeline(skg_engine* e, uint64_t first_history, uint64_t count)
     {
         int ngroups = 0;
+        // one peel-off request per observer group per event: the device's grouping rule
         for (size_t f = 0; f < e->frames.size(); ++f)
-            if (!(f > 0 && e->frames[f].inc == e->frames[f - 1].inc && e->frames[f].az == e->frames[f - 1].az
-                  && e->frames[f].roll == e->frames[f - 1].roll))
-                ++ngroups;
+            if (!sameObserverAsPrev(e, f)) ++ngroups;
         int64_t slots = std::max<int64_t>(256, std::min<int64_t>(e->poolSlots, static_cast<int64_t>(count)));
         slots = (slots + 255) / 256 * 256;
         int st = ensurePool(e, slots, slots * (1 + ngroups));
@@ -2581,9 +2587,6 @@ int skg_run_packets_async(skg_engine* e, uint64_t first_history, uint64_t count)
     if (count == 0) return SKG_OK;
     if (e->mode == SKG_MODE_PIPELINE && e->rf)
         return fail(e, SKG_ERR_STATE, "skg_run_packets: radiation-field storage runs in the persistent engine");
-    if (e->mode == SKG_MODE_PIPELINE)
-        for (const auto& f : e->frames)
-            if (f.persp) return fail(e, SKG_ERR_STATE, "skg_run_packets: perspective instruments run in the persistent engine");
     if (e->mode == SKG_MODE_PIPELINE && e->nent)
         return fail(e, SKG_ERR_STATE, "skg_run_packets: a particle source runs in the persistent engine");
     if (e->mode == SKG_MODE_PIPELINE) return runPipeline(e, first_history, count);
@@ -2876,10 +2879,8 @@ int skg_autotune(skg_engine* e, uint64_t first_history, uint64_t trial_packets, 
     if (!e || trial_packets < 1024 || !(engines >= 1 && engines <= 3))
         return fail(e, SKG_ERR_ARGUMENT, "skg_autotune: bad arguments");
     if (e->guideMode == 2) return fail(e, SKG_ERR_STATE, "skg_autotune: not in guide collect mode");
-    // instances, particle sources, radiation-field storage and perspective instruments run in
-    // the persistent engine only
+    // instances, particle sources and radiation-field storage run in the persistent engine only
     bool persistentOnly = !e->hInst.empty() || e->nent > 0 || e->rf != nullptr;
-    for (const auto& f : e->frames) persistentOnly = persistentOnly || f.persp;
     if (persistentOnly) engines &= 1;
     if (!engines) return fail(e, SKG_ERR_ARGUMENT, "skg_autotune: no engine supports this configuration");
     if (!e->hInst.empty() && e->instTotal < trial_packets + trial_packets / 4)

@@ -157,6 +157,17 @@ namespace skg
             return -1;
         }
 
+        // the pixel of either kind of instrument (perspective: persp_pixel without its depth)
+        __device__ __forceinline__ int any_pixel(double x, double y, double z) const
+        {
+            if (persp)
+            {
+                double zp;
+                return persp_pixel(x, y, z, zp);
+            }
+            return pixel(x, y, z);
+        }
+
         // FrameInstrument::pixelOnDetector (FrameInstrument.cpp:117-136); for an SED instrument
         // ApertureInstrument::isInsideAperture (ApertureInstrument.cpp:24-43: the projected
         // radius without the roll) and pixel 0

@@ -252,7 +252,7 @@ namespace skg
                     if (fg.same_as_prev) continue;
                     bool seen = false;
                     for (int f2 = f; f2 < p.nframes && (f2 == f || p.frames[f2].same_as_prev); ++f2)
-                        seen |= p.frames[f2].pixel(rx, ry, rz) >= 0;
+                        seen |= p.frames[f2].any_pixel(rx, ry, rz) >= 0;
                     if (!seen) continue;
                     peelMask |= 1u << f;
                 }
@@ -377,7 +377,7 @@ namespace skg
                         if (fg.same_as_prev) continue;
                         bool seen = false;
                         for (int f2 = f; f2 < p.nframes && (f2 == f || p.frames[f2].same_as_prev); ++f2)
-                            seen |= p.frames[f2].pixel(rx, ry, rz) >= 0;
+                            seen |= p.frames[f2].any_pixel(rx, ry, rz) >= 0;
                         if (!seen) continue;
                         peelMask |= 1u << f;
                     }
@@ -421,12 +421,23 @@ namespace skg
             {
                 if (!(peelMask & (1u << f))) continue;
                 double Wp = W;
+                const FrameGeom& fg = p.frames[f];
+                // a perspective instrument is seen toward its eye (PerspectiveInstrument.cpp:93-101)
+                double ox = fg.kobs[0], oy = fg.kobs[1], oz = fg.kobs[2], zp = 1.;
+                if (fg.persp)
+                {
+                    fg.persp_pixel(rx, ry, rz, zp);
+                    const double dx = fg.eye[0] - rx, dy = fg.eye[1] - ry, dz = fg.eye[2] - rz;
+                    const double D = sqrt(dx * dx + dy * dy + dz * dz);
+                    ox = dx / D;
+                    oy = dy / D;
+                    oz = dz / D;
+                }
                 if (peelScat)
                 {
                     // the peel-off belongs to the event, before the new direction is drawn: the
                     // guide's factor of that direction applies to the continuing packet only
-                    const FrameGeom& fg = p.frames[f];
-                    double c = kxo * fg.kobs[0] + kyo * fg.kobs[1] + kzo * fg.kobs[2];
+                    double c = kxo * ox + kyo * oy + kzo * oz;
                     Wp = Wpeel * peel_phase(p.gpar[ell], c);
                 }
                 if (o < static_cast<unsigned long long>(p.qcap))
@@ -438,6 +449,7 @@ namespace skg
                     r.pad = 0;
                     r.L = Wp / lam;
                     r.limit = log(r.L) + 745;  // MediumSystem.cpp:2335
+                    if (fg.persp) r.L /= zp * zp;  // local recorder (FluxRecorder.cpp:387)
                     q[o] = r;
                 }
                 ++o;
@@ -503,6 +515,7 @@ namespace skg
         int kind = 0, slot = 0, steps = 0, lastm = -1;
         unsigned long long req = 0;
         double tau = 0., s = 0., tprev = 0., sprev = 0., limit = 0.;
+        double slim = INFINITY;  // perspective peel-off: path length after which it ends
         float tau32 = 0.f, ctau = 0.f, s32 = 0.f, cs = 0.f;
         double section = 0.;
         float section32 = 0.f;
@@ -548,12 +561,24 @@ namespace skg
                         section = p.sext[ell];
                         section32 = static_cast<float>(section);
                         double dx, dy, dz;
+                        slim = INFINITY;
                         if (kind == kReqPeel)
                         {
                             const FrameGeom& fg = p.frames[r.frame];
                             dx = fg.kobs[0];
                             dy = fg.kobs[1];
                             dz = fg.kobs[2];
+                            if (fg.persp)
+                            {
+                                // toward the eye, over the path length zp (FluxRecorder.cpp:372-405)
+                                const double rx0 = P.rx[slot], ry0 = P.ry[slot], rz0 = P.rz[slot];
+                                fg.persp_pixel(rx0, ry0, rz0, slim);
+                                const double ex = fg.eye[0] - rx0, ey = fg.eye[1] - ry0, ez = fg.eye[2] - rz0;
+                                const double D = sqrt(ex * ex + ey * ey + ez * ez);
+                                dx = ex / D;
+                                dy = ey / D;
+                                dz = ez / D;
+                            }
                         }
                         else
                         {
@@ -609,6 +634,10 @@ namespace skg
                             }
                         }
                     }
+                    // MediumSystem::getExtinctionOpticalDepth(pp, distance) (MediumSystem.cpp:
+                    // 2376-2382): the segment that passes the distance still counts
+                    s += ds;
+                    if (s > slim) finished = true;
                 }
                 else if (ds > 0.)
                 {
@@ -735,7 +764,7 @@ namespace skg
                 z = P.rz[slot];
                 ell = P.ell[slot];
                 const FrameGeom& fg = p.frames[f0];
-                int pix = fg.pixel(x, y, z);
+                int pix = fg.any_pixel(x, y, z);
                 if (pix >= 0) key = fg.offset + pix + static_cast<int64_t>(ell) * fg.nx * fg.ny;
             }
             wave_add(p.frame, key, v, atomics);
@@ -746,7 +775,7 @@ namespace skg
                 for (int f = f0 + 1; f < p.nframes && p.frames[f].same_as_prev; ++f)
                 {
                     const FrameGeom& fg = p.frames[f];
-                    int pix = fg.pixel(x, y, z);
+                    int pix = fg.any_pixel(x, y, z);
                     if (pix < 0) continue;
                     unsafeAtomicAdd(p.frame + fg.offset + pix + static_cast<int64_t>(ell) * fg.nx * fg.ny, v);
                     dep += v;

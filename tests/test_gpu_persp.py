@@ -59,19 +59,16 @@ def test_perspective_instruments_match_oracle():
     eng.options(0.5, 1e4, 0, 0, 23)
     grid = O.Grid("cartesian", xb=b, yb=b, zb=b)
     ofr, _ = O.simulate(grid, dens, lam, sext, ssca, g, lpack, instr, 0, npk, threads=8, seed=23)
-    eng.tune_pipeline(pkg.SKG_MODE_PERSISTENT, 1 << 18, 0, 8)
-    eng.frame_clear()
-    print("case perspective instruments, persistent engine", flush=True)
-    eng.run_packets(0, npk)
-    for i in range(4):
-        v, ov = eng.frame_read(i), ofr[i]
-        assert ov.sum() > 0
-        np.testing.assert_allclose(v, ov, rtol=1e-9, atol=1e-12 * ov.max(), err_msg=f"instrument {i}")
+    for mode in (pkg.SKG_MODE_PERSISTENT, pkg.SKG_MODE_PIPELINE):
+        eng.tune_pipeline(mode, 1 << 18, 0, 8)
+        eng.frame_clear()
+        print("case perspective instruments, engine mode", mode, flush=True)
+        eng.run_packets(0, npk)
+        for i in range(4):
+            v, ov = eng.frame_read(i), ofr[i]
+            assert ov.sum() > 0
+            np.testing.assert_allclose(v, ov, rtol=1e-9, atol=1e-12 * ov.max(), err_msg=f"mode {mode} instrument {i}")
     np.testing.assert_array_equal(ofr[1], ofr[2])
-    eng.tune_pipeline(pkg.SKG_MODE_PIPELINE, 1 << 18, 0, 8)
-    print("case perspective instruments refused by the pipeline engine", flush=True)
-    with pytest.raises(pkg.EngineError):
-        eng.run_packets(0, 1000)
     # host-side argument checks: nothing is added
     d = instr[1]
     bad_eye = np.array(d["eye"], dtype=float)

@@ -2541,6 +2541,8 @@ namespace
         }
         hipGraphExecDestroy(exec);
         if (hst != hipSuccess) return fail(e, SKG_ERR_DEVICE, std::string("pipeline: ") + hipGetErrorString(hst));
+        if (e->hostCtl[kCtlOverflow])
+            return fail(e, SKG_ERR_DEVICE, "pipeline: the request queue overflowed (requests were dropped)");
         HIPCHK(e, hipEventRecord(e->ev1, e->stream));
         if (e->guideMode == 2)
         {

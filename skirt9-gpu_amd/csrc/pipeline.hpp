@@ -118,6 +118,7 @@ namespace skg
     constexpr int kCtlCount = 1;
     constexpr int kCtlHead = 3;
     constexpr int kCtlActive = 5;
+    constexpr int kCtlOverflow = 7;  // requests that did not fit the queue (the run then fails)
     constexpr int kCtlCnt = 8;
 
     // wave-aggregated append of `want` entries per lane; returns the lane's first index
@@ -465,6 +466,9 @@ namespace skg
                 r.limit = emitFwdKind == kReqFwd2 ? P.tint[slot] : 0.;
                 q[o] = r;
             }
+            if (emitFwd) ++o;
+            if (o > static_cast<unsigned long long>(p.qcap))
+                atomicAdd(p.ctl + kCtlOverflow, o - max(base, static_cast<unsigned long long>(p.qcap)));
         }
         // active slots and event counters, aggregated per wavefront
         unsigned long long act = __ballot(emitFwd);

@@ -245,7 +245,7 @@ def main():
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
         return
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported on this build "
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported "
               "(packets shard across GPUs with one detector reduction at the end).", file=sys.stderr)
         sys.exit(2)
 

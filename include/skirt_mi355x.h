@@ -252,6 +252,14 @@ double skg_last_kernel_ms(const skg_engine* e);
 int skg_tune_pipeline(skg_engine* e, int32_t mode, int64_t pool_slots, int32_t trace_workgroups,
                       int32_t graph_iterations);
 
+/* Upper bound on the pipeline's request queue (entries per ping-pong buffer; 0, the default:
+ * slots x (1 + observer groups), which holds every request of an iteration).  A smaller queue
+ * bounds its device memory; a run whose requests of one iteration do not fit stops and
+ * fails with SKG_ERR_DEVICE instead of losing deposits.  No reference counterpart: the
+ * reference keeps no request queue (its device buffers are high-water scratch,
+ * Docs/SKIRTGPU.md:101). */
+int skg_tune_queue(skg_engine* e, int64_t max_requests);
+
 /* Of the last pipeline run: summed duration of the trace kernels (HIP events on the engine's
  * stream), the number of trace launches and of pipeline iterations. */
 int skg_last_run_stats(const skg_engine* e, double* trace_ms, int64_t* trace_launches, int64_t* iterations);
@@ -383,6 +391,12 @@ int skg_guide_params(skg_engine* e, const float* theta, int32_t n);
 int skg_guide_read_params(skg_engine* e, float* theta, int32_t n);
 /* Raw network outputs (n x 40) for n feature rows (n x 9), host memory. */
 int skg_guide_infer(skg_engine* e, const float* features, int64_t n, float* out);
+/* The life cycle's FP32 guide mixture for n rows of raw network outputs (n x 40 floats): a
+ * direction drawn from row i's mixture with the Philox stream (seed, i) into dirs (n x 3), the
+ * mixture density per steradian there into pdf (n), and, when at (n x 3 unit vectors) is
+ * given, the density at at[i] into pdf_at (n).  No reference counterpart (the guide is new). */
+int skg_guide_sample(skg_engine* e, const float* outputs, int64_t n, uint64_t seed, const double* at, double* dirs,
+                     double* pdf, double* pdf_at);
 /* Gradient of sum_i -w_i ln q(k_i | x_i) with respect to the parameters for n records of 16
  * floats (9 features, direction k (3), weight w, 3 unused), and the loss. */
 int skg_guide_gradient(skg_engine* e, const float* records, int64_t n, float* grad, double* loss);

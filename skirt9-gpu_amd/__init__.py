@@ -69,9 +69,9 @@ EXPORTED = [
     "skg_run_packets_async", "skg_sync", "skg_counters", "skg_tune", "skg_last_kernel_ms",
     "skg_disk_density", "skg_octree_build", "skg_octree_from_flags", "skg_octree_info", "skg_octree_export",
     "skg_octree_upload", "skg_octree_free", "skg_sample_density_cartesian", "skg_sample_density_octree",
-    "skg_dust_meanlist", "skg_tune_pipeline", "skg_last_run_stats",
+    "skg_dust_meanlist", "skg_tune_pipeline", "skg_tune_queue", "skg_last_run_stats",
     "skg_guide_params", "skg_guide_read_params", "skg_guide_infer", "skg_guide_gradient", "skg_guide_mode",
-    "skg_guide_collect", "skg_guide_records", "skg_guide_train",
+    "skg_guide_collect", "skg_guide_records", "skg_guide_train", "skg_guide_sample",
     "skg_voronoi_build", "skg_voronoi_info", "skg_voronoi_export", "skg_voronoi_upload",
     "skg_sample_density_voronoi", "skg_voronoi_free", "skg_autotune",
     "skg_tetra_build", "skg_tetra_info", "skg_tetra_export", "skg_tetra_upload", "skg_tetra_free",
@@ -311,6 +311,10 @@ class Engine:
         self._chk(lib().skg_tune_pipeline(self._h, C.c_int32(mode), C.c_int64(pool_slots), C.c_int32(trace_workgroups),
                                           C.c_int32(graph_iterations)))
 
+    def tune_queue(self, max_requests=0):
+        """Cap the pipeline's request queue at max_requests entries (0: automatic, enough for any run)."""
+        self._chk(lib().skg_tune_queue(self._h, C.c_int64(max_requests)))
+
     def last_run_stats(self):
         ms, launches, iters = C.c_double(), C.c_int64(), C.c_int64()
         self._chk(lib().skg_last_run_stats(self._h, C.byref(ms), C.byref(launches), C.byref(iters)))
@@ -331,6 +335,21 @@ class Engine:
         out = np.zeros((len(f), 40), np.float32)
         self._chk(lib().skg_guide_infer(self._h, _p(f, P(C.c_float)), C.c_int64(len(f)), _p(out, P(C.c_float))))
         return out
+
+    def guide_sample(self, outputs, seed=1, at=None):
+        """The life cycle's FP32 mixture sampler for rows of raw network outputs: returns the
+        sampled directions (n x 3), the density there (n) and, with `at`, the density at those
+        directions (n)."""
+        o = np.ascontiguousarray(outputs, dtype=np.float32).reshape(-1, 40)
+        n = len(o)
+        dirs = np.zeros((n, 3))
+        pdf = np.zeros(n)
+        a = None if at is None else _f64(at).reshape(n, 3)
+        pa = None if at is None else np.zeros(n)
+        self._chk(lib().skg_guide_sample(self._h, _p(o, P(C.c_float)), C.c_int64(n), C.c_uint64(seed),
+                                         None if a is None else _p(a, f64p), _p(dirs, f64p), _p(pdf, f64p),
+                                         None if pa is None else _p(pa, f64p)))
+        return (dirs, pdf) if at is None else (dirs, pdf, pa)
 
     def guide_gradient(self, records):
         r = np.ascontiguousarray(records, dtype=np.float32).reshape(-1, 16)

@@ -965,7 +965,10 @@ namespace skg
         __shared__ double coldD[kColdF64 * kLcThreads];
         __shared__ uint32_t coldU[kColdU32 * kLcThreads];
         if (threadIdx.x < 8) wgcnt[threadIdx.x] = 0;
-        if constexpr (GT == kOctree) stage_top(p.grid);  // other grids allocate no top-node LDS
+        if constexpr (GT == kOctree)
+            stage_top(p.grid);  // other grids allocate no top-node LDS (stage_top ends with a barrier)
+        else
+            __syncthreads();  // the counters are zero before any wave adds to them
         const int t = threadIdx.x;
         Trav<GT, FP> T;
         T.phase = kFetch;
@@ -1111,6 +1114,7 @@ struct skg_engine
     uint64_t instTotal = 0;
     // particle source (0 entities: the point source)
     int64_t nent = 0;
+    int entLambda = 0;  // number of wavelength bins the per-packet luminosities were made for
     int entKernel = 1;
     double* ent = nullptr;
     int64_t* entFirst = nullptr;
@@ -1128,6 +1132,7 @@ struct skg_engine
     Request* q[2] = {nullptr, nullptr};
     double* qtau[2] = {nullptr, nullptr};
     int64_t qcap = 0;
+    int64_t queueLimit = 0;  // 0: the request queue holds slots x (1 + observer groups) entries
     int64_t poolSlots = 1 << 21;
     int traceBlocks = 0;
     int graphIters = 8;
@@ -2063,6 +2068,7 @@ int skg_source_particles(skg_engine* e, int64_t n, const double* xyzh, const dou
     if ((st = upload(e, &e->entLpack, lp.data(), lp.size()))) return st;
     if ((st = upload(e, &e->entX, X.data(), X.size()))) return st;
     e->nent = n;
+    e->entLambda = nl;
     e->entKernel = kernel;
     return SKG_OK;
 }
@@ -2295,6 +2301,9 @@ static int runPersistent(skg_engine* e, uint64_t first_history, uint64_t count)
     p.entX = e->entX;
     if (p.nent && (!e->hInst.empty() || e->guideMode != 0))
         return fail(e, SKG_ERR_STATE, "skg_run_packets: a particle source runs without instances and the guide");
+    if (p.nent && e->entLambda != e->nlambda)
+        return fail(e, SKG_ERR_STATE,
+                    "skg_run_packets: the particle luminosities were uploaded for another number of wavelengths");
     if (p.persp && (!e->hInst.empty() || e->guideMode != 0))
         return fail(e, SKG_ERR_STATE, "skg_run_packets: perspective instruments run without instances and the guide");
     if (e->rf && (e->rfCells != e->ncells || e->rfLambda != e->nlambda))
@@ -2423,7 +2432,9 @@ namespace
             if (!sameObserverAsPrev(e, f)) ++ngroups;
         int64_t slots = std::max<int64_t>(256, std::min<int64_t>(e->poolSlots, static_cast<int64_t>(count)));
         slots = (slots + 255) / 256 * 256;
-        int st = ensurePool(e, slots, slots * (1 + ngroups));
+        int64_t qcap = slots * (1 + ngroups);  // every slot's requests of one iteration
+        if (e->queueLimit > 0) qcap = std::min(qcap, e->queueLimit);
+        int st = ensurePool(e, slots, qcap);
         if (st) return st;
         PipeParams p{};
         p.grid = e->grid;
@@ -2453,7 +2464,7 @@ namespace
         p.q[1] = e->q[1];
         p.qtau[0] = e->qtau[0];
         p.qtau[1] = e->qtau[1];
-        p.qcap = slots * (1 + ngroups);
+        p.qcap = qcap;
         p.ctl = e->ctl;
         p.guideMode = e->guideMode;
         p.guideMix = e->guideMix;
@@ -2532,7 +2543,7 @@ namespace
             (void)hipGetLastError();  // a failed timing query must not leave a sticky error
             launches += iters;
             unsigned long long active = e->hostCtl[kCtlActive + ((iters - 1) & 1)];
-            if (active == 0) break;
+            if (active == 0 || e->hostCtl[kCtlOverflow]) break;
             if (rounds > maxRounds)
             {
                 hipGraphExecDestroy(exec);
@@ -2605,6 +2616,14 @@ int skg_tune_pipeline(skg_engine* e, int32_t mode, int64_t pool_slots, int32_t t
     e->poolSlots = pool_slots;
     e->traceBlocks = trace_workgroups;
     e->graphIters = graph_iterations;
+    return SKG_OK;
+}
+
+int skg_tune_queue(skg_engine* e, int64_t max_requests)
+{
+    if (!e || max_requests < 0) return fail(e, SKG_ERR_ARGUMENT, "skg_tune_queue: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->queueLimit = max_requests;
     return SKG_OK;
 }
 
@@ -2706,6 +2725,36 @@ int skg_guide_infer(skg_engine* e, const float* features, int64_t n, float* out)
     HIPCHK(e, hipStreamSynchronize(e->stream));
     hipFree(dF);
     hipFree(dO);
+    return SKG_OK;
+}
+
+int skg_guide_sample(skg_engine* e, const float* outputs, int64_t n, uint64_t seed, const double* at, double* dirs,
+                     double* pdf, double* pdf_at)
+{
+    if (!e || n < 0 || (n && (!outputs || !dirs || !pdf || (at && !pdf_at))))
+        return fail(e, SKG_ERR_ARGUMENT, "skg_guide_sample: bad arguments");
+    for (int64_t i = 0; i < n * kGuideOut; ++i)
+        if (!std::isfinite(outputs[i])) return fail(e, SKG_ERR_ARGUMENT, "skg_guide_sample: outputs not finite");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    if (n == 0) return SKG_OK;
+    float* dO = nullptr;
+    double *dAt = nullptr, *dRes = nullptr;
+    HIPCHK(e, hipMalloc(&dO, n * kGuideOut * sizeof(float)));
+    HIPCHK(e, hipMalloc(&dRes, n * 5 * sizeof(double)));
+    if (at) HIPCHK(e, hipMalloc(&dAt, n * 3 * sizeof(double)));
+    HIPCHK(e, hipMemcpy(dO, outputs, n * kGuideOut * sizeof(float), hipMemcpyHostToDevice));
+    if (at) HIPCHK(e, hipMemcpy(dAt, at, n * 3 * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(guide_sample_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, e->stream, dO, n,
+                       seed, dAt, dRes, dRes + 3 * n, dRes + 4 * n);
+    HIPCHK(e, hipGetLastError());
+    HIPCHK(e, hipMemcpyAsync(dirs, dRes, n * 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipMemcpyAsync(pdf, dRes + 3 * n, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    if (at) HIPCHK(e, hipMemcpyAsync(pdf_at, dRes + 4 * n, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    hipFree(dO);
+    hipFree(dRes);
+    if (dAt) hipFree(dAt);
     return SKG_OK;
 }
 
@@ -2920,15 +2969,24 @@ int skg_autotune(skg_engine* e, uint64_t first_history, uint64_t trial_packets, 
                 for (int twg : {4 * cus, 8 * cus}) cands.push_back({SKG_MODE_PIPELINE, e->threshold, e->workgroups, pool, twg, it});
     double bestRate = -1.;
     Cand best = cands[0];
-    uint64_t h = first_history;
-    for (const Cand& c : cands)
+    Cand caller;  // the tuning in force before the trials, restored when a trial fails
     {
+        std::lock_guard<std::mutex> lk(e->mu);
+        caller = {e->mode, e->threshold, e->workgroups, e->poolSlots, e->traceBlocks, e->graphIters};
+    }
+    auto applyTuning = [e](const Cand& c) {
+        std::lock_guard<std::mutex> lk(e->mu);
         e->mode = c.mode;
         e->threshold = c.threshold;
         e->workgroups = c.workgroups;
         e->poolSlots = c.pool;
         e->traceBlocks = c.traceWgs;
         e->graphIters = c.iters;
+    };
+    uint64_t h = first_history;
+    for (const Cand& c : cands)
+    {
+        applyTuning(c);
         // one warm-up run (pool allocation, graph capture), then the timed trial
         if (!e->hInst.empty() && h + trial_packets / 4 + trial_packets > e->instTotal) h = 0;
         if ((st = skg_run_packets(e, h, trial_packets / 4))) break;
@@ -2957,13 +3015,14 @@ int skg_autotune(skg_engine* e, uint64_t first_history, uint64_t trial_packets, 
         hipFree(savedRf);
     }
     hipMemcpy(e->counters, savedCnt, sizeof(savedCnt), hipMemcpyHostToDevice);
+    const Cand& keep = st ? caller : best;
+    e->mode = keep.mode;
+    e->threshold = keep.threshold;
+    e->workgroups = keep.workgroups;
+    e->poolSlots = keep.pool;
+    e->traceBlocks = keep.traceWgs;
+    e->graphIters = keep.iters;
     if (st) return st;
-    e->mode = best.mode;
-    e->threshold = best.threshold;
-    e->workgroups = best.workgroups;
-    e->poolSlots = best.pool;
-    e->traceBlocks = best.traceWgs;
-    e->graphIters = best.iters;
     if (best6)
     {
         best6[0] = best.mode;

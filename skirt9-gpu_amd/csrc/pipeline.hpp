@@ -143,6 +143,18 @@ namespace skg
         return base + (incl - want);
     }
 
+    // True when instrument f leads an observer group (the instruments that share its
+    // observer) and some instrument of the group sees the position, so that the group gets
+    // one peel-off request (FluxRecorder::detect skips the optical depth when no instrument
+    // of the group sees the packet)
+    __device__ __forceinline__ bool group_sees(const PipeParams& p, int f, double x, double y, double z)
+    {
+        if (p.frames[f].same_as_prev) return false;
+        for (int f2 = f; f2 < p.nframes && (f2 == f || p.frames[f2].same_as_prev); ++f2)
+            if (p.frames[f2].any_pixel(x, y, z) >= 0) return true;
+        return false;
+    }
+
     // ------------------------------------------------------------------ event kernel
     template<int GT, int GM> __global__ void __launch_bounds__(256) pipe_event(PipeParams p, int out)
     {
@@ -164,9 +176,11 @@ namespace skg
         int ell = 0, nscatt = 0;
         bool emitFwd = false;
         int emitFwdKind = kReqFwd1;
-        // peel-off requests: one bit per observer group leader; the luminosity is recomputed when
-        // the request is written (from the direction before scattering, kept in kxo..kzo)
-        unsigned peelMask = 0;
+        // peel-off requests: one per observer group that sees the packet (counted here; the
+        // groups are found again when the requests are written, so any number of instruments
+        // works); the luminosity is recomputed then (from the direction before scattering,
+        // kept in kxo..kzo)
+        unsigned npeelReq = 0;
         bool peelScat = false;
         double kxo = 0, kyo = 0, kzo = 0;
         double Wpeel = 0.;  // weight of this event's peel-offs (before the guide's factor)
@@ -248,15 +262,7 @@ namespace skg
                 // scattering peel-off toward each distinct observer (FluxRecorder::detect skips the
                 // optical depth when no instrument of the group sees the packet)
                 for (int f = 0; f < p.nframes; ++f)
-                {
-                    const FrameGeom& fg = p.frames[f];
-                    if (fg.same_as_prev) continue;
-                    bool seen = false;
-                    for (int f2 = f; f2 < p.nframes && (f2 == f || p.frames[f2].same_as_prev); ++f2)
-                        seen |= p.frames[f2].any_pixel(rx, ry, rz) >= 0;
-                    if (!seen) continue;
-                    peelMask |= 1u << f;
-                }
+                    if (group_sees(p, f, rx, ry, rz)) ++npeelReq;
                 peelScat = true;
                 Wpeel = W;
                 kxo = kx;
@@ -373,15 +379,7 @@ namespace skg
                         p.recLast[slot] = -1;
                     }
                     for (int f = 0; f < p.nframes; ++f)
-                    {
-                        const FrameGeom& fg = p.frames[f];
-                        if (fg.same_as_prev) continue;
-                        bool seen = false;
-                        for (int f2 = f; f2 < p.nframes && (f2 == f || p.frames[f2].same_as_prev); ++f2)
-                            seen |= p.frames[f2].any_pixel(rx, ry, rz) >= 0;
-                        if (!seen) continue;
-                        peelMask |= 1u << f;
-                    }
+                        if (group_sees(p, f, rx, ry, rz)) ++npeelReq;
                     emitFwd = true;
                     emitFwdKind = kReqFwd1;
                 }
@@ -411,16 +409,16 @@ namespace skg
         {
             P.state[slot] = st;
         }
-        unsigned want = (emitFwd ? 1u : 0u) + __popc(peelMask);
+        unsigned want = (emitFwd ? 1u : 0u) + npeelReq;
         unsigned long long base = wave_reserve(p.ctl + kCtlCount + out, want);
         if (want)
         {
             Request* q = p.q[out];
             unsigned long long o = base;
             const double lam = p.lambda[ell];
-            for (int f = 0; f < p.nframes; ++f)
+            for (int f = 0; f < p.nframes && npeelReq; ++f)
             {
-                if (!(peelMask & (1u << f))) continue;
+                if (!group_sees(p, f, rx, ry, rz)) continue;
                 double Wp = W;
                 const FrameGeom& fg = p.frames[f];
                 // a perspective instrument is seen toward its eye (PerspectiveInstrument.cpp:93-101)
@@ -466,6 +464,13 @@ namespace skg
                 r.limit = emitFwdKind == kReqFwd2 ? P.tint[slot] : 0.;
                 q[o] = r;
             }
+            else if (emitFwd)
+            {
+                // the packet's traversal does not fit the queue: the packet stops here (it would
+                // otherwise continue from results of no traversal), and the run fails with the
+                // overflow count
+                P.state[slot] = kSlotDone;
+            }
             if (emitFwd) ++o;
             if (o > static_cast<unsigned long long>(p.qcap))
                 atomicAdd(p.ctl + kCtlOverflow, o - max(base, static_cast<unsigned long long>(p.qcap)));
@@ -475,7 +480,7 @@ namespace skg
         unsigned long long f1 = __popcll(__ballot(emitFwd && emitFwdKind == kReqFwd1));
         unsigned long long nh = 0, ns = 0, nf = 0, npeel = 0;
         {
-            unsigned a = cntHist, b = cntScat, c = cntFwd2, d = __popc(peelMask);
+            unsigned a = cntHist, b = cntScat, c = cntFwd2, d = npeelReq;
             for (int o = 32; o > 0; o >>= 1)
             {
                 a += __shfl_xor(a, o);

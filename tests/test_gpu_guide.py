@@ -72,6 +72,58 @@ def test_guide_gradient_matches_cpu(eng):
     assert abs(loss - lref) < 2e-2 * abs(lref) + 1e-3
 
 
+def test_device_mixture_density_and_sampler_match_cpu(eng):
+    """The life cycle's FP32 mixture (skg_guide_sample): its density at given directions equals
+    the FP64 statement's (mixture_pdf) for outputs with a lobe at the KMAX clamp and one in
+    softplus's linear branch (relative 2e-4: FP32 cosines times kappa up to 400); the density it
+    reports at its own samples equals the statement's there; mu.k of single-lobe outputs
+    follows the vMF marginal CDF (Kolmogorov-Smirnov); and E[1/q] over its samples is 4 pi."""
+    from scipy import stats
+    from test_guide_oracle import _random_outputs, vmf_marginal_cdf
+
+    rng = np.random.default_rng(8)
+    out = _random_outputs(rng, 4096).astype(np.float32)
+    at = rng.normal(size=(4096, 3))
+    at /= np.linalg.norm(at, axis=1, keepdims=True)
+    # half of the directions near lobe 1's axis, where the density is large
+    mu1 = out.reshape(-1, G.NLOBE, 5)[:, 1, :3].astype(np.float64)
+    mu1 /= np.linalg.norm(mu1, axis=1, keepdims=True)
+    at[:2048] = mu1[:2048] + 0.1 * rng.normal(size=(2048, 3))
+    at /= np.linalg.norm(at, axis=1, keepdims=True)
+    print("case guide mixture density and sampler", flush=True)
+    dirs, pdf, pdf_at = eng.guide_sample(out, seed=5, at=at)
+    o64 = out.astype(np.float64)
+    ref_at = G.mixture_pdf(o64, at)
+    np.testing.assert_allclose(pdf_at, ref_at, rtol=2e-4, atol=1e-30)
+    np.testing.assert_allclose(np.linalg.norm(dirs, axis=1), 1.0, atol=1e-12)
+    np.testing.assert_allclose(pdf, G.mixture_pdf(o64, dirs), rtol=2e-4)
+    # single-lobe outputs: lobe 0 carries all the weight
+    for kap_raw, kap in ((-1.5, G.KMIN + np.log1p(np.exp(-1.5))), (3.0, G.KMIN + np.log1p(np.exp(3.0))),
+                         (40.0, G.KMIN + 40.0), (600.0, G.KMAX)):
+        one = np.zeros((20000, G.NOUT), np.float32)
+        o = one.reshape(-1, G.NLOBE, 5)
+        o[:, :, 4] = -40.0
+        o[:, 0, 4] = 40.0
+        o[:, 0, :3] = rng.normal(size=(20000, 3))
+        o[:5000, 0, :3] = [0.0, 0.0, 2.0]  # the frame's other branch
+        o[:, 0, 3] = kap_raw
+        d, _ = eng.guide_sample(one, seed=int(kap) + 11)
+        mu = o[:, 0, :3].astype(np.float64)
+        mu /= np.linalg.norm(mu, axis=1, keepdims=True)
+        w = np.sum(d * mu, axis=1)
+        ks = stats.kstest(np.minimum(w, 1.0), lambda x: vmf_marginal_cdf(x, np.float32(kap)))
+        assert ks.pvalue > 1e-3, (kap, ks)
+    # importance identity over the device's samples of broad mixtures
+    base = rng.normal(0, 1.0, (1, G.NOUT))
+    b = base.reshape(1, G.NLOBE, 5)
+    b[:, 0, 3], b[:, 0, 4] = -0.5, 2.5
+    b[:, 1:, 3] = rng.uniform(2, 4, (1, G.NLOBE - 1))
+    outs = np.repeat(base, 200000, axis=0).astype(np.float32)
+    _, q = eng.guide_sample(outs, seed=77)
+    inv = 1.0 / q
+    assert abs(inv.mean() - 4 * np.pi) < 5 * inv.std() / np.sqrt(len(inv)), inv.mean()
+
+
 def test_guide_trains_on_collected_records(eng):
     """Plain run that records training data, then Adam on the records: the weighted negative
     log-likelihood of the recorded directions decreases."""

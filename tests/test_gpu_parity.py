@@ -238,3 +238,70 @@ def test_autotune_keeps_detector_and_picks_valid_config(eng):
     eng.frame_clear()
     eng.run_packets(0, 2000)
     np.testing.assert_allclose(eng.frame_read(0)[0], img0, rtol=1e-12, atol=0)
+
+
+def test_pipeline_many_observer_groups(eng):
+    """40 distinct observers (more than a 32-bit mask could track) plus one instrument sharing
+    the last observer: the pipeline's and the persistent engine's frames both equal the
+    oracle's for the same histories, instrument by instrument."""
+    b, dens = disk_density_cartesian(n=16, samples=4, seed=13)
+    eng.grid_cartesian(b, b, b)
+    eng.medium(dens)
+    s = 1.5e-29
+    eng.dust([0.55e-6], [s], [0.6 * s], [0.6])
+    eng.frame_remove_all()
+    frs = [dict(inclination=np.radians(2.0 + 2.2 * i), azimuth=np.radians(9.0 * i), roll=0.0, nx=12, ny=12,
+                fovx=20 * PC, fovy=20 * PC) for i in range(40)]
+    frs.append(dict(frs[-1], nx=6, ny=6))  # same observer as the previous instrument
+    for fr in frs:
+        eng.frame_add(fr["inclination"], fr["azimuth"], fr["roll"], fr["nx"], fr["ny"], fr["fovx"], fr["fovy"])
+    npk = 3000
+    eng.source_point((0.0, 0.0, 0.0), [1.0 / npk])
+    eng.options(0.5, 1e4, 0, 0, 31)
+    g = O.Grid("cartesian", xb=b, yb=b, zb=b)
+    ofr, _ = O.simulate(g, dens, [0.55e-6], [s], [0.6 * s], [0.6], [1.0 / npk], frs, 0, npk, threads=8, seed=31)
+    for mode in (0, 1):
+        eng.tune_pipeline(mode, 1 << 16, 0, 8)
+        eng.frame_clear()
+        print("case 41 instruments, engine mode", mode, flush=True)
+        eng.run_packets(0, npk)
+        for i in range(len(frs)):
+            img, oimg = eng.frame_read(i)[0], ofr[i][0]
+            assert oimg.sum() > 0
+            assert abs(img.sum() / oimg.sum() - 1) < 1e-6, (mode, i)
+            nz = oimg > 0
+            assert np.median(np.abs(img[nz] - oimg[nz]) / oimg[nz]) < 1e-9, (mode, i)
+
+
+def test_pipeline_queue_overflow_fails_the_run(eng):
+    """A request queue too small for one iteration's requests (forced from the host with
+    skg_tune_queue) makes the pipeline run fail with an error instead of losing deposits;
+    with the automatic size the same run equals the persistent engine."""
+    pkg = load_pkg()
+    b, dens = disk_density_cartesian(n=16, samples=4, seed=17)
+    eng.grid_cartesian(b, b, b)
+    eng.medium(dens)
+    s = 1.5e-29
+    eng.dust([0.55e-6], [s], [0.6 * s], [0.6])
+    eng.frame_remove_all()
+    fr = frame60(npix=20)
+    eng.frame_add(fr["inclination"], fr["azimuth"], fr["roll"], fr["nx"], fr["ny"], fr["fovx"], fr["fovy"])
+    npk = 4096
+    eng.source_point((0.0, 0.0, 0.0), [1.0 / npk])
+    eng.options(0.5, 1e4, 0, 0, 37)
+    eng.tune_pipeline(0, 1024, 0, 8)
+    eng.tune_queue(300)  # 1024 slots need up to 2048 requests per iteration
+    eng.frame_clear()
+    print("case queue overflow (expected error)", flush=True)
+    with pytest.raises(pkg.EngineError, match="overflow"):
+        eng.run_packets(0, npk)
+    with pytest.raises(pkg.EngineError):
+        eng.tune_queue(-1)
+    eng.tune_queue(0)
+    eng.frame_clear()
+    eng.run_packets(0, npk)
+    pipe = eng.frame_read(0)[0]
+    eng.tune_pipeline(1, 1024, 0, 8)
+    eng.frame_clear()
+    eng.run_packets(0, npk)
+    np.testing.assert_allclose(pipe, eng.frame_read(0)[0], rtol=1e-12, atol=0)

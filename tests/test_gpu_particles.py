@@ -69,6 +69,15 @@ def test_particle_source_matches_oracle(kernel):
     with pytest.raises(pkg.EngineError):
         eng.source_particles(xyzh[:2], lbin[:2], ltot[:2], npk, kernel=2)
 
+    print("case particle luminosities made for another number of wavelengths", flush=True)
+    eng.tune_pipeline(pkg.SKG_MODE_PERSISTENT, 1 << 18, 0, 8)
+    eng.dust(lam + [1.2e-6], sext + [0.5 * s], ssca + [0.2 * s], g + [0.4])
+    eng.source_point((0.0, 0.0, 0.0), [1.0 / npk] * 3)
+    with pytest.raises(pkg.EngineError, match="wavelengths"):
+        eng.run_packets(0, 1000)
+    eng.dust(lam, sext, ssca, g)
+    eng.source_point((0.0, 0.0, 0.0), [1.0 / npk, 1.0 / npk])
+
     print("case point source restored", flush=True)
     eng.source_particles(np.zeros((0, 4)), np.zeros((0, 2)), np.zeros(0), npk)
     eng.tune_pipeline(pkg.SKG_MODE_PERSISTENT, 1 << 18, 0, 8)

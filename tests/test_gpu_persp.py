@@ -102,8 +102,9 @@ def _fixture_grid(eng, kind):
     return g, d["density"], float(d["sext"][0])
 
 
+@pytest.mark.parametrize("mode", [1, 0], ids=["persistent", "pipeline"])
 @pytest.mark.parametrize("kind", ["octree", "voronoi", "tetra"])
-def test_perspective_instruments_match_oracle_on_fixture_grids(kind):
+def test_perspective_instruments_match_oracle_on_fixture_grids(kind, mode):
     pkg = load_pkg()
     from skirt9_gpu_amd import ski as S
     eng = pkg.Engine(0)
@@ -124,14 +125,14 @@ def test_perspective_instruments_match_oracle_on_fixture_grids(kind):
     eng.source_point((0.0, 0.0, 0.0), lpack)
     eng.options(0.5, 1e4, 0, 0, 29)
     ofr, _ = O.simulate(grid, dens, lam, sext, ssca, g, lpack, instr, 0, npk, threads=8, seed=29)
-    eng.tune_pipeline(pkg.SKG_MODE_PERSISTENT, 1 << 18, 0, 8)
+    eng.tune_pipeline(mode, 1 << 18, 0, 8)
     eng.frame_clear()
-    print("case perspective instruments on", kind, flush=True)
+    print("case perspective instruments on", kind, "engine mode", mode, flush=True)
     eng.run_packets(0, npk)
     for i in range(4):
         v, ov = eng.frame_read(i), ofr[i]
         assert ov.sum() > 0
-        np.testing.assert_allclose(v, ov, rtol=1e-9, atol=1e-12 * ov.max(), err_msg=f"{kind} instrument {i}")
+        np.testing.assert_allclose(v, ov, rtol=1e-9, atol=1e-12 * ov.max(), err_msg=f"{kind} mode {mode} instrument {i}")
     eng.close()
 
 

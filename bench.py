@@ -110,6 +110,60 @@ def cpu_baseline_port(wl, packets, threads, seed):
                 sample=f"{packets} packets of the same workload and grid through the FP64 oracle, {t:.2f} s")
 
 
+def oracle_grid(wl):
+    """The FP64 oracle's view of the workload's grid (the same tables the GPU traverses)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    if wl.grid == "cartesian":
+        return O.Grid("cartesian", xb=wl.tables["xb"], yb=wl.tables["yb"], zb=wl.tables["zb"])
+    if wl.grid == "octree":
+        ex = wl.octree.export()
+        return O.Grid("octree", bounds=ex["bounds"], child_first=ex["child_first"], cell_index=ex["cell_index"],
+                      nb_off=ex["nb_off"], nb_id=ex["nb_id"])
+    ex = wl.tables["mesh"].export()
+    if wl.grid == "voronoi":
+        return O.Grid("voronoi", **{q: ex[q] for q in ("sites", "nb_begin", "nb_count", "nb_index", "block_n",
+                                                        "block_begin", "block_count", "block_index", "box")})
+    return O.Grid("tetra", **ex)
+
+
+def error_vs_fp64(eng, pkg, wl, fp, sample, threads, seed):
+    """Per-output error of the timed configuration against the FP64 oracle: the frames of
+    `sample` histories run on the GPU in the bench's precision mode and by the oracle on the
+    same histories (same Philox streams, MediumSystem.cpp:1785-1844 in FP64); per pixel with
+    signal the relative difference (median, 90th percentile, largest) and of the total.  For
+    batched instances, instance 0 alone."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    grid = oracle_grid(wl)
+    lam, sext, ssca, g = wl.lam, wl.sext, wl.ssca, wl.g
+    frames, first, oseed, nfr = wl.frames, 1 << 44, seed, len(wl.frames)
+    lpack = np.ones(len(lam))
+    if wl.instances is not None:
+        I = wl.instances
+        sample = int(min(sample, I["packets"][0]))
+        sc = float(I["tau_scale"][0])
+        sext, ssca = sext * sc, ssca * sc
+        frames, first, oseed, nfr = wl.frames[:1], 0, int(I["seeds"][0]), 1
+        lpack = np.full(len(lam), 1.0 / max(int(I["packets"][0]), 1))
+    eng.frame_clear()
+    eng.run_packets(first, sample)
+    gpu = [eng.frame_read(i) for i in range(nfr)]
+    eng.frame_clear()
+    t0 = time.perf_counter()
+    ofr, _ = O.simulate(grid, wl.density, lam, sext, ssca, g, lpack, frames, 0 if wl.instances is not None else first,
+                        sample, threads=threads, seed=oseed)
+    t = time.perf_counter() - t0
+    a = np.concatenate([x.ravel() for x in gpu])
+    b = np.concatenate([x.ravel() for x in ofr])
+    nz = b > 0
+    rel = np.abs(a[nz] - b[nz]) / b[nz]
+    return dict(reference="FP64 CPU oracle (oracle/src/oracle.cpp), same histories", mode="fp64" if fp == pkg.SKG_FP64
+                else "fp32 + Kahan", histories=int(sample), pixels=int(nz.sum()), median=float(np.median(rel)),
+                p90=float(np.percentile(rel, 90)), max=float(rel.max()), total=float(abs(a.sum() / b.sum() - 1)),
+                oracle_seconds=t)
+
+
 def frame_batches(eng, first, batches, per):
     """Independent batches of `per` packets; returns (per-batch frames (B, nl, ny, nx), GPU
     wall time summed over the batches, next history index)."""
@@ -138,6 +192,66 @@ def figure_of_merit(X, t):
     rv_pix = v[ok] / m[ok] ** 2
     return dict(fom_total=1.0 / (rv_tot * t), fom_pixel_median=float(np.median(1.0 / (rv_pix * t))),
                 rel_err_total=float(np.sqrt(rv_tot)), pixels=int(ok.sum()), seconds=t, mean_total=float(tot.mean()))
+
+
+def fom_groups(X, t, groups):
+    """Figures of merit of `groups` independent groups of the batches (equal time shares)."""
+    B = len(X) // groups
+    return [figure_of_merit(X[g * B:(g + 1) * B], t / groups) for g in range(groups)]
+
+
+def gain_interval(plain, guided, key):
+    """Mean over groups of guided / plain figure of merit and its 95 % interval (t with
+    groups - 1 degrees of freedom on the log ratios)."""
+    from scipy import stats
+    r = np.log(np.array([gd[key] / pl[key] for pl, gd in zip(plain, guided)]))
+    m, se = r.mean(), r.std(ddof=1) / np.sqrt(len(r))
+    q = stats.t.ppf(0.975, len(r) - 1)
+    return dict(gain=float(np.exp(m)), low=float(np.exp(m - q * se)), high=float(np.exp(m + q * se)), groups=len(r))
+
+
+def guide_ceiling(eng, pkg, first, batches, per):
+    """The variance no scattering-direction proposal can remove.  Each batch is run twice
+    with histories that share their emission, first interaction and first-scattering
+    peel-offs and continue on independent random streams (skg_continuation_key); the
+    covariance of the two runs' batch means is Var(E[batch | first scattering states]), left
+    by any guide, so Var / Cov bounds the variance reduction (and, at equal cost per packet,
+    the figure-of-merit gain) of any direction guide applied from the first scattering on."""
+    A, Bv = [], []
+    for b in range(batches):
+        runs = []
+        for key in (1, 2):
+            eng.continuation_key(key)
+            eng.frame_clear()
+            eng.run_packets(first + b * per, per)
+            runs.append(eng.frame_read(0))
+        A.append(runs[0])
+        Bv.append(runs[1])
+    eng.continuation_key(0)
+    A, Bv = np.stack(A), np.stack(Bv)
+    n = len(A)
+
+    def ratio(x, y):
+        xc, yc = x - x.mean(0), y - y.mean(0)
+        var = 0.5 * ((xc * xc).sum(0) + (yc * yc).sum(0)) / (n - 1)
+        cov = (xc * yc).sum(0) / (n - 1)
+        return var, cov
+
+    tA, tB = A.reshape(n, -1).sum(1), Bv.reshape(n, -1).sum(1)
+    v, c = ratio(tA, tB)
+    # bootstrap over batches for the total's interval
+    rng = np.random.default_rng(3)
+    bs = []
+    for _ in range(400):
+        i = rng.integers(0, n, n)
+        vb, cb = ratio(tA[i], tB[i])
+        bs.append(vb / cb if cb > 0 else np.inf)
+    vp, cp = ratio(A.reshape(n, -1), Bv.reshape(n, -1))
+    m = A.mean(0).reshape(-1) > 0
+    rp = vp[m] / np.where(cp[m] > 0, cp[m], np.nan)
+    return dict(batches=n, packets_per_batch=per, ceiling_total=float(v / c) if c > 0 else None,
+                ceiling_total_95=[float(np.percentile(bs, 2.5)), float(np.percentile(bs, 97.5))],
+                ceiling_pixel_median=float(np.nanmedian(rp)), correlation_total=float(c / v)), first + batches * per
 
 
 def guided_comparison(eng, pkg, a, first):
@@ -169,7 +283,7 @@ def guided_comparison(eng, pkg, a, first):
                train_steps=a.train_steps, loss_first=l0, loss_last=l1, mix=a.mix)
     print(f"guide: {nrec} records from {a.train_packets:.0f} packets in {t1 - t0:.2f} s, "
           f"{a.train_steps} Adam steps in {t2 - t1:.2f} s, loss {l0:.4f} -> {l1:.4f}", file=sys.stderr, flush=True)
-    res = {}
+    res, groups = {}, {}
     for name, mode, gm in (("plain_pipeline", pkg.SKG_MODE_PIPELINE, 0), ("guided_pipeline", pkg.SKG_MODE_PIPELINE, 1),
                            ("plain_persistent", pkg.SKG_MODE_PERSISTENT, 0),
                            ("guided_persistent", pkg.SKG_MODE_PERSISTENT, 1)):
@@ -179,17 +293,28 @@ def guided_comparison(eng, pkg, a, first):
         first += a.fom_packets
         X, t, first = frame_batches(eng, first, a.fom_batches, a.fom_packets)
         res[name] = figure_of_merit(X, t)
+        groups[name] = fom_groups(X, t, a.fom_groups)
         print(f"{name}: {a.fom_batches} x {a.fom_packets} packets in {t:.2f} s, rel. error of total "
               f"{res[name]['rel_err_total']:.3e}, FOM total {res[name]['fom_total']:.3e}, "
               f"pixel median {res[name]['fom_pixel_median']:.3e}", file=sys.stderr, flush=True)
     eng.guide_mode(0)
     # each sampler in its faster engine
-    plain = max(res["plain_pipeline"], res["plain_persistent"], key=lambda r: r["fom_pixel_median"])
-    g = max(res["guided_pipeline"], res["guided_persistent"], key=lambda r: r["fom_pixel_median"])
+    pn = max(("plain_pipeline", "plain_persistent"), key=lambda k: res[k]["fom_pixel_median"])
+    gn = max(("guided_pipeline", "guided_persistent"), key=lambda k: res[k]["fom_pixel_median"])
+    plain, g = res[pn], res[gn]
     out.update(batches=a.fom_batches, packets_per_batch=a.fom_packets, runs=res,
                gain_pixel_median=g["fom_pixel_median"] / plain["fom_pixel_median"],
                gain_total=g["fom_total"] / plain["fom_total"],
+               gain_pixel_median_95=gain_interval(groups[pn], groups[gn], "fom_pixel_median"),
+               gain_total_95=gain_interval(groups[pn], groups[gn], "fom_total"),
                guided_vs_plain_mean_total=g["mean_total"] / plain["mean_total"])
+    print(f"guide gain (group means, 95 %): total {out['gain_total_95']}, pixel median "
+          f"{out['gain_pixel_median_95']}", file=sys.stderr, flush=True)
+    if a.ceiling_batches:
+        eng.tune_pipeline(pkg.SKG_MODE_PERSISTENT, a.pool, a.trace_wgs, a.graph_iters)
+        ceil, first = guide_ceiling(eng, pkg, first, a.ceiling_batches, a.fom_packets)
+        out["ceiling"] = ceil
+        print(f"guide ceiling: {ceil}", file=sys.stderr, flush=True)
     return out, first
 
 
@@ -228,8 +353,14 @@ def main():
     ap.add_argument("--train-batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--mix", type=float, default=0.5)
-    ap.add_argument("--fom-batches", type=int, default=32)
-    ap.add_argument("--fom-packets", type=int, default=1 << 19)
+    ap.add_argument("--fom-batches", type=int, default=256)
+    ap.add_argument("--fom-packets", type=int, default=1 << 18)
+    ap.add_argument("--fom-groups", type=int, default=8,
+                    help="independent groups of the batches for the gain's 95 %% interval")
+    ap.add_argument("--ceiling-batches", type=int, default=128,
+                    help="batch pairs of the variance ceiling of any direction guide (0: skip)")
+    ap.add_argument("--error-sample", type=int, default=1 << 15,
+                    help="histories compared with the FP64 oracle for error_vs_fp64 (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy kernel of MIB mebibytes three times (the "
                          "reference reading for the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -347,6 +478,11 @@ def main():
           f"{d[4] / a.steps:.3e} segments and {requests / a.steps:.3e} traversals per step, "
           f"{iterations / a.steps:.0f} pipeline iterations per step", file=sys.stderr, flush=True)
 
+    err = None
+    if a.error_sample:
+        err = error_vs_fp64(eng, pkg, wl, fp, a.error_sample, threads, 12345)
+        print(f"error vs FP64 oracle: {err}", file=sys.stderr, flush=True)
+
     guide = None
     if not a.no_guide:
         guide, first = guided_comparison(eng, pkg, a, first)
@@ -398,6 +534,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kname, "kernel_ms": avg_ms, "launches_per_step": launches / a.steps},
         "cpu_baseline": cpu,
+        "error_vs_fp64": err,
         "guide": guide,
     }
     print(json.dumps(line), flush=True)

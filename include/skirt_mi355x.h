@@ -219,9 +219,13 @@ int skg_sync(skg_engine* e);
 
 /* Device counters accumulated since the last skg_frame_clear: [0] histories, [1] peel-off
  * traversals, [2] forced-path traversals (first pass), [3] scattering events,
- * [4] segment crossings, [5] detector atomics, [6] second-pass traversals, [7] crossings of
- * the second passes (persistent engine; 0 in the pipeline engine). */
+ * [4] segment crossings, [5] FP64 detector atomics issued (after the wavefront combines
+ * deposits into the same pixel), [6] second-pass traversals, [7] crossings of the second
+ * passes (persistent engine; 0 in the pipeline engine). */
 int skg_counters(skg_engine* e, int64_t* out8);
+/* The first n (1..16) counters: [0..7] as skg_counters, [8] detector deposits (before the
+ * wavefront combines them; [8] / [5] is the combining factor), [9..15] zero. */
+int skg_counters_n(skg_engine* e, int64_t* out, int32_t n);
 
 /* Launch shape of the persistent life-cycle kernel: workgroups (0 = automatic), threads per
  * workgroup (64..1024, multiple of 64) and the number of lanes of a wavefront that must be
@@ -251,6 +255,15 @@ double skg_last_kernel_ms(const skg_engine* e);
  * and pipeline iterations per captured hipGraph (even, 2..256).  Chosen by the autotuner. */
 int skg_tune_pipeline(skg_engine* e, int32_t mode, int64_t pool_slots, int32_t trace_workgroups,
                       int32_t graph_iterations);
+
+/* Variance diagnostics (persistent engine): with key != 0, every history continues on the
+ * random stream of Philox tag `key` from its first scattering on, so that runs with
+ * different keys share each history's emission, emission peel-off, first interaction and
+ * first scattering peel-off and continue independently.  The covariance of two such runs'
+ * batch means is the variance that no choice of scattering directions can remove (the
+ * ceiling of any direction guide); key 0 (the default) restores the single stream.  No
+ * reference counterpart (the reference's Random is one thread-local stream). */
+int skg_continuation_key(skg_engine* e, uint32_t key);
 
 /* Upper bound on the pipeline's request queue (entries per ping-pong buffer; 0, the default:
  * slots x (1 + observer groups), which holds every request of an iteration).  A smaller queue
@@ -394,7 +407,9 @@ int skg_guide_infer(skg_engine* e, const float* features, int64_t n, float* out)
 /* The life cycle's FP32 guide mixture for n rows of raw network outputs (n x 40 floats): a
  * direction drawn from row i's mixture with the Philox stream (seed, i) into dirs (n x 3), the
  * mixture density per steradian there into pdf (n), and, when at (n x 3 unit vectors) is
- * given, the density at at[i] into pdf_at (n).  No reference counterpart (the guide is new). */
+ * given, the density at at[i] into pdf_at (n).  No reference counterpart (the guide is new).
+ * Currently switched off: returns SKG_ERR_STATE (its first GPU run ended in an abort whose
+ * cause has not been found). */
 int skg_guide_sample(skg_engine* e, const float* outputs, int64_t n, uint64_t seed, const double* at, double* dirs,
                      double* pdf, double* pdf_at);
 /* Gradient of sum_i -w_i ln q(k_i | x_i) with respect to the parameters for n records of 16

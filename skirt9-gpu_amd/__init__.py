@@ -66,10 +66,10 @@ EXPORTED = [
     "skg_grid_tetra", "skg_num_cells",
     "skg_medium", "skg_dust", "skg_trace_paths", "skg_trace_tau", "skg_frame_add", "skg_frame_remove_all",
     "skg_frame_clear", "skg_frame_read", "skg_source_point", "skg_source_particles", "skg_options", "skg_run_packets",
-    "skg_run_packets_async", "skg_sync", "skg_counters", "skg_tune", "skg_last_kernel_ms",
+    "skg_run_packets_async", "skg_sync", "skg_counters", "skg_counters_n", "skg_tune", "skg_last_kernel_ms",
     "skg_disk_density", "skg_octree_build", "skg_octree_from_flags", "skg_octree_info", "skg_octree_export",
     "skg_octree_upload", "skg_octree_free", "skg_sample_density_cartesian", "skg_sample_density_octree",
-    "skg_dust_meanlist", "skg_tune_pipeline", "skg_tune_queue", "skg_last_run_stats",
+    "skg_dust_meanlist", "skg_tune_pipeline", "skg_tune_queue", "skg_continuation_key", "skg_last_run_stats",
     "skg_guide_params", "skg_guide_read_params", "skg_guide_infer", "skg_guide_gradient", "skg_guide_mode",
     "skg_guide_collect", "skg_guide_records", "skg_guide_train", "skg_guide_sample",
     "skg_voronoi_build", "skg_voronoi_info", "skg_voronoi_export", "skg_voronoi_upload",
@@ -311,6 +311,10 @@ class Engine:
         self._chk(lib().skg_tune_pipeline(self._h, C.c_int32(mode), C.c_int64(pool_slots), C.c_int32(trace_workgroups),
                                           C.c_int32(graph_iterations)))
 
+    def continuation_key(self, key=0):
+        """Histories continue on Philox tag `key` from their first scattering on (0: off)."""
+        self._chk(lib().skg_continuation_key(self._h, C.c_uint32(key)))
+
     def tune_queue(self, max_requests=0):
         """Cap the pipeline's request queue at max_requests entries (0: automatic, enough for any run)."""
         self._chk(lib().skg_tune_queue(self._h, C.c_int64(max_requests)))
@@ -425,9 +429,12 @@ class Engine:
     def sync(self):
         self._chk(lib().skg_sync(self._h))
 
-    def counters(self):
-        out = np.zeros(8, np.int64)
-        self._chk(lib().skg_counters(self._h, _p(out, i64p)))
+    def counters(self, n=8):
+        """Device event counters (skg_counters_n): [0] histories, [1] peel-off traversals,
+        [2] first forced passes, [3] scatterings, [4] cell crossings, [5] detector atomics issued,
+        [6] second forced passes, [7] crossings of second passes, [8] detector deposits."""
+        out = np.zeros(n, np.int64)
+        self._chk(lib().skg_counters_n(self._h, _p(out, i64p), C.c_int32(n)))
         return out
 
     def last_kernel_ms(self):

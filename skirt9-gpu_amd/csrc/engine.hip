@@ -4,16 +4,10 @@
 //   trace_count / trace_write   path service: raw (m, ds) segments per ray (SpatialGrid seam)
 //   trace_tau                   total / observer / cumulative optical depths per ray
 //   lifecycle                   persistent forced-scattering life cycle with peel-off
-//
-// The life-cycle kernel is persistent: each lane owns one photon packet at a time and runs
-// it as a small state machine (peel-off traversal, forced path first pass, second pass to the
-// interaction point, scattering).  Every loop iteration advances each busy lane by one cell
-// crossing; lanes whose traversal ended wait until enough lanes of the wavefront are waiting
-// (the transition threshold), then the wave runs one transition round together, so the
-// divergent event code is amortised over many lanes.  Lanes whose packet terminated refill
-// from a global history counter with one atomic per wavefront (ballot + mbcnt prefix), which
-// stream-compacts packets of very uneven path length without a separate compaction pass.
-// Peel-off contributions go to the frame with native FP64 atomics.
+//                               (persistent.hpp)
+//   pipe_event / pipe_trace /   wavefront pipeline of the same life cycle (pipeline.hpp)
+//   pipe_detect
+//   guide_*                     path-guide network on the matrix cores (guide.hpp)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,6 +23,7 @@
 #include "disk_model.hpp"
 #include "lifecycle.hpp"
 #include "octree_build.hpp"
+#include "persistent.hpp"
 #include "pipeline.hpp"
 #include "traverse.hpp"
 
@@ -82,24 +77,6 @@ namespace skg
         int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
         if (i < n) rec[i].dens = dens[i];
     }
-
-    // compensated (Kahan) FP32 accumulator used in SKG_FP32_KAHAN mode
-    struct Kahan32
-    {
-        float sum, comp;
-        __device__ __forceinline__ void reset()
-        {
-            sum = 0.f;
-            comp = 0.f;
-        }
-        __device__ __forceinline__ void add(float x)
-        {
-            float y = x - comp;
-            float t = sum + y;
-            comp = (t - sum) - y;
-            sum = t;
-        }
-    };
 
     template<int GT, int FP>
     __global__ void trace_tau(GridView g, const double* __restrict__ dens, const float* __restrict__ dens32,
@@ -271,784 +248,6 @@ namespace skg
         out[i] = sum / samples;
     }
 
-    // ------------------------------------------------------------------ the life cycle
-    struct SimParams
-    {
-        GridView grid;
-        const double* dens;
-        const float* dens32;
-        int nlambda;
-        int nframes;
-        const double* lambda;
-        const double* sext;
-        const double* ssca;
-        const double* gpar;
-        const double* lpack;
-        double src[3];
-        double xi, min_weight_reduction;
-        int min_scatt;
-        int threshold;
-        uint64_t seed;
-        uint64_t first, count;
-        unsigned long long* queue;     // next history offset (work queue head)
-        unsigned long long* counters;  // 8 event counters
-        double* frame;
-        const FrameGeom* frames;  // device copy of the instrument geometry
-        const Instance* inst;     // batched instances (ninst > 0) or none
-        int ninst;
-        const double* ilpack;     // launch luminosities per instance (ninst x nlambda)
-        GuideNet net;             // guided variant: the network, its input frame and the mixture
-        GuideFrame gframe;
-        double guideMix;
-        double* rf;               // radiation-field variant: ncells x nlambda table
-        int persp;                // some instruments are perspective instruments
-        int nent;                 // particle-source variant: number of entities (0: the point source)
-        int entKernel;            // smoothing kernel: 0 uniform, 1 cubic spline
-        const double* ent;        // per entity: x, y, z, smoothing length h
-        const int64_t* entFirst;  // first history index of each entity (nent + 1 entries)
-        const double* entLpack;   // launch luminosity per packet, nent x nlambda
-        const double* entX;       // cubic spline: cumulative mass at u = k / kSplineN, k = 0..kSplineN
-    };
-
-    // kernel variants of the life cycle
-    enum Variant : int
-    {
-        kPlain = 0,      // one run
-        kBatched = 1,    // batched independent instances
-        kGuided = 2,     // scattering directions from the defensive guide / HG mixture
-        kStoreRF = 3,    // one run that also stores the radiation field of the forced paths (perspective
-                         // instruments allowed)
-        kPersp = 4,      // one run with perspective instruments among the instruments
-        kParticles = 5   // one run whose source is a set of imported particles
-    };
-    constexpr int kSplineN = 400;  // CubicSplineSmoothingKernel's table size (CubicSplineSmoothingKernel.cpp:18)
-
-    // SpecialFunctions::lnmean(x1, x2, lnx1, lnx2) (SpecialFunctions.cpp:860-880): the
-    // logarithmic mean (x2 - x1) / ln(x2 / x1), by its series for nearly equal arguments
-    __device__ __forceinline__ double ln_mean(double x1, double x2, double lnx1, double lnx2)
-    {
-        if (x1 > x2)
-        {
-            const double t = x1, lt = lnx1;
-            x1 = x2;
-            x2 = t;
-            lnx1 = lnx2;
-            lnx2 = lt;
-        }
-        if (x1 <= 0) return 0.;
-        const double x = x2 / x1 - 1.;
-        if (x < 1e-3)
-        {
-            const double c3 = 1. / 3., c5 = 1. / 5., c6 = 1. / 6.;
-            return x1 / (1. - 0.5 * x + c3 * x * x - 0.25 * x * x * x + c5 * x * x * x * x - c6 * x * x * x * x * x);
-        }
-        return (x2 - x1) / (lnx2 - lnx1);
-    }
-
-    enum Phase : int
-    {
-        kFetch = 0,
-        kPeel = 1,
-        kFwd1 = 2,
-        kFwd2 = 3,
-        kDone = 4,
-        kScatter = 5  // guided variant: peel-offs done, waiting for the wave's guided scattering
-    };
-
-#ifndef SKG_GUIDE_LANES
-#define SKG_GUIDE_LANES 16
-#endif
-    constexpr int kGuideLanes = SKG_GUIDE_LANES;  // guided variant: lanes per network evaluation round
-    constexpr int kMaxScatterings = 1 << 20;  // guard against non-terminating histories (NaN weights)
-    constexpr int kMaxSteps = 1 << 26;        // guard against non-terminating traversals
-
-    // Packet state that only the event transitions read and write.  Between transitions it
-    // lives in LDS as a structure of arrays over the workgroup's lanes (conflict-free, one
-    // 8-byte or 4-byte word per lane and field), so the crossing loop keeps only the
-    // traversal state in registers.
-    enum ColdF64 : int
-    {
-        cRx, cRy, cRz, cKx, cKy, cKz, cW, cLam, cLthr, cTaupath, cTauCached, cWp, kColdF64
-    };
-    enum ColdU32 : int
-    {
-        cHist0, cHist1, cDraw, cEll, cNscatt, cF, cPix, cFlags, cInst, cHint, kColdU32
-    };
-    constexpr int kLcThreads = 256;
-    constexpr int kColdBytes = kLcThreads * (8 * kColdF64 + 4 * kColdU32);  // 34 KB per workgroup
-
-    struct Packet
-    {
-        Philox rng;
-        double rx, ry, rz, kx, ky, kz;
-        double W, lam, Lthr, taupath, tauCached, Wp;
-        int ell, nscatt, f, pix, inst;
-        int hint;  // a cell that probably holds the packet's position (-1: none)
-        bool scat, haveTau;
-
-        template<bool INST>
-        __device__ __forceinline__ void load(const double* cd, const uint32_t* cu, int t, const SimParams& p)
-        {
-            rx = cd[cRx * kLcThreads + t];
-            ry = cd[cRy * kLcThreads + t];
-            rz = cd[cRz * kLcThreads + t];
-            kx = cd[cKx * kLcThreads + t];
-            ky = cd[cKy * kLcThreads + t];
-            kz = cd[cKz * kLcThreads + t];
-            W = cd[cW * kLcThreads + t];
-            lam = cd[cLam * kLcThreads + t];
-            Lthr = cd[cLthr * kLcThreads + t];
-            taupath = cd[cTaupath * kLcThreads + t];
-            tauCached = cd[cTauCached * kLcThreads + t];
-            Wp = cd[cWp * kLcThreads + t];
-            uint64_t h = static_cast<uint64_t>(cu[cHist0 * kLcThreads + t])
-                         | (static_cast<uint64_t>(cu[cHist1 * kLcThreads + t]) << 32);
-            inst = static_cast<int>(cu[cInst * kLcThreads + t]);
-            hint = static_cast<int>(cu[cHint * kLcThreads + t]);
-            rng.init(INST ? p.inst[inst].seed : p.seed, h);
-            rng.draw = cu[cDraw * kLcThreads + t];
-            ell = static_cast<int>(cu[cEll * kLcThreads + t]);
-            nscatt = static_cast<int>(cu[cNscatt * kLcThreads + t]);
-            f = static_cast<int>(cu[cF * kLcThreads + t]);
-            pix = static_cast<int>(cu[cPix * kLcThreads + t]);
-            uint32_t fl = cu[cFlags * kLcThreads + t];
-            scat = fl & 1u;
-            haveTau = (fl >> 1) & 1u;
-        }
-
-        __device__ __forceinline__ void store(double* cd, uint32_t* cu, int t) const
-        {
-            cd[cRx * kLcThreads + t] = rx;
-            cd[cRy * kLcThreads + t] = ry;
-            cd[cRz * kLcThreads + t] = rz;
-            cd[cKx * kLcThreads + t] = kx;
-            cd[cKy * kLcThreads + t] = ky;
-            cd[cKz * kLcThreads + t] = kz;
-            cd[cW * kLcThreads + t] = W;
-            cd[cLam * kLcThreads + t] = lam;
-            cd[cLthr * kLcThreads + t] = Lthr;
-            cd[cTaupath * kLcThreads + t] = taupath;
-            cd[cTauCached * kLcThreads + t] = tauCached;
-            cd[cWp * kLcThreads + t] = Wp;
-            cu[cHist0 * kLcThreads + t] = rng.h0;
-            cu[cHist1 * kLcThreads + t] = rng.h1;
-            cu[cDraw * kLcThreads + t] = rng.draw;
-            cu[cEll * kLcThreads + t] = static_cast<uint32_t>(ell);
-            cu[cNscatt * kLcThreads + t] = static_cast<uint32_t>(nscatt);
-            cu[cF * kLcThreads + t] = static_cast<uint32_t>(f);
-            cu[cPix * kLcThreads + t] = static_cast<uint32_t>(pix);
-            cu[cFlags * kLcThreads + t] = (scat ? 1u : 0u) | (haveTau ? 2u : 0u);
-            cu[cInst * kLcThreads + t] = static_cast<uint32_t>(inst);
-            cu[cHint * kLcThreads + t] = static_cast<uint32_t>(hint);
-        }
-    };
-
-    // traversal state of one lane: the generator, the running sums and the phase
-    template<int GT, int FP> struct Trav
-    {
-        Gen<GT> gen;
-        int phase;
-        bool ended;
-        int steps;
-        double tau, s, tprev, sprev;  // FP64 sums
-        Kahan32 tau32, s32;           // FP32 sums
-        int lastm;
-        double section;
-        float section32;
-        double taumax;  // peel-off: early-out optical depth
-        double tint;    // second forced pass: interaction optical depth
-        int mint;
-        double sint;
-        int ell;        // radiation-field variant: the packet's bin, luminosity and the
-        double rfL;     // extinction factor at the start of the current segment
-        double rfExt;
-        double slim;    // perspective variant: path length after which a peel-off ends
-
-        __device__ __forceinline__ double tau_value() const
-        {
-            return FP == SKG_FP64 ? tau : static_cast<double>(tau32.sum);
-        }
-        __device__ __forceinline__ double s_value() const
-        {
-            return FP == SKG_FP64 ? s : static_cast<double>(s32.sum);
-        }
-
-        __device__ __forceinline__ void start(const Packet& pk, double dx, double dy, double dz, int ph)
-        {
-            gen.start(pk.rx, pk.ry, pk.rz, dx, dy, dz, pk.hint);
-            phase = ph;
-            ended = false;
-            steps = 0;
-            tau = 0.;
-            s = 0.;
-            tprev = 0.;
-            sprev = 0.;
-            tau32.reset();
-            s32.reset();
-            lastm = -1;
-            // radiation-field variant: the luminosity at the start of a forced path and the
-            // extinction factor before its first segment (unused, and removed, elsewhere)
-            ell = pk.ell;
-            rfL = pk.W / pk.lam;
-            rfExt = 1.;
-            slim = INFINITY;
-        }
-
-        // one cell crossing of the current traversal; with RF, the first forced pass also
-        // stores the radiation field (MonteCarloSimulation::storeRadiationField, :909-925):
-        // L * lnmean(e^-tau_end, e^-tau_begin) * ds per stored segment in a cell
-        template<bool RF, bool PERSP>
-        __device__ __forceinline__ void step(const GridView& grid, unsigned int* lds, double* rf, int nl)
-        {
-            int m;
-            double ds, nd;
-            if (!gen.next(grid, m, ds, nd) || ++steps > kMaxSteps)
-            {
-                ended = true;
-                atomicAdd(lds + 4, static_cast<unsigned int>(steps));
-                if (phase == kFwd2) atomicAdd(lds + 7, static_cast<unsigned int>(steps));
-                return;
-            }
-            if (phase == kPeel)
-            {
-                if (m >= 0)
-                {
-                    if (FP == SKG_FP64)
-                    {
-                        tau += section * nd * ds;
-                        if (tau >= taumax)
-                        {
-                            tau = INFINITY;
-                            ended = true;
-                            atomicAdd(lds + 4, static_cast<unsigned int>(steps));
-                        }
-                    }
-                    else
-                    {
-                        tau32.add(section32 * static_cast<float>(nd) * static_cast<float>(ds));
-                        if (static_cast<double>(tau32.sum) >= taumax)
-                        {
-                            tau32.sum = INFINITY;
-                            ended = true;
-                            atomicAdd(lds + 4, static_cast<unsigned int>(steps));
-                        }
-                    }
-                }
-                if constexpr (PERSP)
-                {
-                    // MediumSystem::getExtinctionOpticalDepth(pp, distance) (MediumSystem.cpp:
-                    // 2376-2382): the segment that passes the distance still counts
-                    s += ds;
-                    if (!ended && s > slim)
-                    {
-                        ended = true;
-                        atomicAdd(lds + 4, static_cast<unsigned int>(steps));
-                    }
-                }
-            }
-            else if (ds > 0.)
-            {
-                if (FP == SKG_FP64)
-                {
-                    tprev = tau;
-                    sprev = s;
-                    s += ds;
-                    if (m >= 0) tau += section * nd * ds;
-                }
-                else
-                {
-                    tprev = static_cast<double>(tau32.sum);
-                    sprev = static_cast<double>(s32.sum);
-                    s32.add(static_cast<float>(ds));
-                    if (m >= 0) tau32.add(section32 * static_cast<float>(nd) * static_cast<float>(ds));
-                }
-                lastm = m;
-                if constexpr (RF)
-                {
-                    if (phase == kFwd1)
-                    {
-                        const double lnEnd = -tau_value();
-                        const double extEnd = exp(lnEnd);
-                        if (m >= 0)
-                        {
-                            const double em = ln_mean(extEnd, rfExt, lnEnd, -tprev);
-                            unsafeAtomicAdd(rf + static_cast<int64_t>(m) * nl + ell, rfL * em * ds);
-                        }
-                        rfExt = extEnd;
-                    }
-                }
-                // second pass: stop at the first segment whose exit optical depth exceeds tau_int
-                if (phase == kFwd2 && tint < tau_value())
-                {
-                    mint = m;
-                    double tv = tau_value(), sv = s_value();
-                    sint = sprev + ((tint - tprev) / (tv - tprev)) * (sv - sprev);
-                    ended = true;
-                    atomicAdd(lds + 4, static_cast<unsigned int>(steps));
-                    atomicAdd(lds + 7, static_cast<unsigned int>(steps));
-                }
-            }
-        }
-    };
-
-    // The event transitions of the life cycle, on a packet loaded from LDS.
-    template<int GT, int FP, int V> struct Events
-    {
-        static constexpr bool INST = V == kBatched;
-        const SimParams& p;
-        Trav<GT, FP>& T;
-        Packet& pk;
-        unsigned int* lds;
-
-        __device__ __forceinline__ void finish()
-        {
-            atomicAdd(lds + 0, 1u);
-            T.phase = kFetch;
-            T.ended = false;
-        }
-
-        __device__ __forceinline__ void deposit(const FrameGeom& fg, double L, double t)
-        {
-            double v = L * exp(-t);
-            unsafeAtomicAdd(p.frame + fg.offset + pk.pix + static_cast<int64_t>(pk.ell) * fg.nx * fg.ny, v);
-            atomicAdd(lds + 5, 1u);
-        }
-
-        // MonteCarloSimulation::performLifeCycle launch + SourceSystem::launch; in a batched
-        // launch, history h is history h - offset of its instance
-        __device__ __forceinline__ void launch(uint64_t h)
-        {
-            uint64_t seed = p.seed;
-            const double* lp = p.lpack;
-            pk.inst = 0;
-            if constexpr (INST)
-            {
-                int lo = 0, hi = p.ninst - 1;  // last instance whose offset is <= h
-                while (lo < hi)
-                {
-                    int mid = (lo + hi + 1) >> 1;
-                    if (p.inst[mid].offset <= h)
-                        lo = mid;
-                    else
-                        hi = mid - 1;
-                }
-                pk.inst = lo;
-                h -= p.inst[lo].offset;
-                seed = p.inst[lo].seed;
-                lp = p.ilpack + static_cast<int64_t>(lo) * p.nlambda;
-            }
-            pk.rng.init(seed, h);
-            pk.ell = static_cast<int>(pk.rng.uniform() * p.nlambda);
-            if (pk.ell >= p.nlambda) pk.ell = p.nlambda - 1;
-            pk.lam = p.lambda[pk.ell];
-            T.section = INST ? p.sext[pk.ell] * p.inst[pk.inst].scale : p.sext[pk.ell];
-            T.section32 = static_cast<float>(T.section);
-            // the particle launch: its own variant, and a run-time branch of the radiation-field
-            // and perspective variants
-            bool particles = false;
-            if constexpr (V == kParticles)
-                particles = true;
-            else if constexpr (V == kStoreRF || V == kPersp)
-                particles = p.nent > 0;
-            if (particles)
-            {
-                // ImportedSource::launch: the entity whose history index range holds h
-                // (ImportedSource.cpp:466-471), its luminosity for the bin (:653-705, the
-                // oligochromatic bias weight folded into entLpack), a position from its smoothing
-                // kernel (ParticleSnapshot::generatePosition, ParticleSnapshot.cpp:275-286)
-                int lo = 0, hi = p.nent - 1;
-                while (lo < hi)
-                {
-                    int mid = (lo + hi + 1) >> 1;
-                    if (p.entFirst[mid] <= static_cast<int64_t>(h))
-                        lo = mid;
-                    else
-                        hi = mid - 1;
-                }
-                const int m = lo;
-                pk.W = p.entLpack[static_cast<int64_t>(m) * p.nlambda + pk.ell] * pk.lam;
-                const double X = pk.rng.uniform();
-                double u;
-                if (p.entKernel == 1)
-                {
-                    // CubicSplineSmoothingKernel::generateRadius (:78-85): NR::locateClip on the table
-                    int a = 0, b = kSplineN;
-                    while (b - a > 1)
-                    {
-                        int mid = (a + b) >> 1;
-                        if (X >= p.entX[mid])
-                            a = mid;
-                        else
-                            b = mid;
-                    }
-                    const double f = (X - p.entX[a]) / (p.entX[a + 1] - p.entX[a]);
-                    u = (a + f) / kSplineN;
-                }
-                else
-                    u = pow(X, 1.0 / 3.0);  // UniformSmoothingKernel::generateRadius (:27-31)
-                double dx, dy, dz;
-                isotropic(pk.rng, dx, dy, dz);
-                const double hm = p.ent[4 * m + 3];
-                pk.rx = p.ent[4 * m + 0] + dx * u * hm;
-                pk.ry = p.ent[4 * m + 1] + dy * u * hm;
-                pk.rz = p.ent[4 * m + 2] + dz * u * hm;
-            }
-            else
-            {
-                pk.W = lp[pk.ell] * pk.lam;
-                pk.rx = p.src[0];
-                pk.ry = p.src[1];
-                pk.rz = p.src[2];
-            }
-            isotropic(pk.rng, pk.kx, pk.ky, pk.kz);
-            pk.nscatt = 0;
-            pk.taupath = 0.;
-            pk.tauCached = 0.;
-            pk.Wp = 0.;
-            pk.pix = -1;
-            pk.hint = -1;
-            pk.haveTau = false;
-            if (!(pk.W / pk.lam > 0))
-            {
-                finish();
-                return;
-            }
-            pk.Lthr = (pk.W / pk.lam) / p.min_weight_reduction;
-            begin_peel(false);
-        }
-
-        __device__ __forceinline__ void begin_peel(bool scattering)
-        {
-            pk.scat = scattering;
-            pk.f = (INST ? p.inst[pk.inst].fbeg : 0) - 1;
-            next_peel();
-        }
-
-        // peel-off toward each instrument in turn (MonteCarloSimulation.cpp:698-715, :2034-2092;
-        // FluxRecorder::detect skips the optical depth for packets outside the frame, :375)
-        __device__ __forceinline__ void next_peel()
-        {
-            int fb = 0, fe = p.nframes;
-            if constexpr (INST)
-            {
-                fb = p.inst[pk.inst].fbeg;
-                fe = p.inst[pk.inst].fend;
-            }
-            while (++pk.f < fe)
-            {
-                const FrameGeom& fg = p.frames[pk.f];
-                if constexpr (V == kPersp || V == kStoreRF)
-                {
-                    if (fg.persp)
-                    {
-                        // PerspectiveInstrument: toward the eye from this position
-                        // (PerspectiveInstrument.cpp:93-101), luminosity / zp^2 and the
-                        // optical depth over the path length zp (FluxRecorder.cpp:372-405)
-                        double zp;
-                        pk.pix = fg.persp_pixel(pk.rx, pk.ry, pk.rz, zp);
-                        const double dx = fg.eye[0] - pk.rx, dy = fg.eye[1] - pk.ry, dz = fg.eye[2] - pk.rz;
-                        const double D = sqrt(dx * dx + dy * dy + dz * dz);
-                        const double ox = dx / D, oy = dy / D, oz = dz / D;
-                        if (!fg.same_as_prev || pk.f == fb)
-                        {
-                            pk.haveTau = false;
-                            pk.Wp = pk.W;
-                            if (pk.scat) pk.Wp = pk.W * peel_phase(p.gpar[pk.ell], pk.kx * ox + pk.ky * oy + pk.kz * oz);
-                        }
-                        if (pk.pix < 0) continue;
-                        const double L = pk.Wp / pk.lam;
-                        if (pk.haveTau)
-                        {
-                            deposit(fg, L / (zp * zp), pk.tauCached);
-                            continue;
-                        }
-                        if (L <= 0)
-                        {
-                            pk.haveTau = true;
-                            pk.tauCached = INFINITY;
-                            continue;
-                        }
-                        T.taumax = log(L) + 745;
-                        atomicAdd(lds + 1, 1u);
-                        T.start(pk, ox, oy, oz, kPeel);
-                        T.slim = zp;
-                        return;
-                    }
-                }
-                if (!fg.same_as_prev || pk.f == fb)
-                {
-                    pk.haveTau = false;
-                    pk.Wp = pk.W;
-                    if (pk.scat)
-                    {
-                        double c = pk.kx * fg.kobs[0] + pk.ky * fg.kobs[1] + pk.kz * fg.kobs[2];
-                        pk.Wp = pk.W * peel_phase(p.gpar[pk.ell], c);
-                    }
-                }
-                pk.pix = fg.pixel(pk.rx, pk.ry, pk.rz);
-                if (pk.pix < 0) continue;
-                double L = pk.Wp / pk.lam;
-                if (pk.haveTau)
-                {
-                    deposit(fg, L, pk.tauCached);
-                    continue;
-                }
-                if (L <= 0)
-                {
-                    pk.haveTau = true;
-                    pk.tauCached = INFINITY;
-                    continue;
-                }
-                T.taumax = log(L) + 745;
-                atomicAdd(lds + 1, 1u);
-                T.start(pk, fg.kobs[0], fg.kobs[1], fg.kobs[2], kPeel);
-                return;
-            }
-            // all instruments done: scatter (after a scattering peel-off) and start a forced path
-            if (pk.scat)
-            {
-                if constexpr (V == kGuided)
-                {
-                    // the wave's guided scattering round follows (guided_scatter)
-                    T.phase = kScatter;
-                    T.ended = false;
-                    return;
-                }
-                scatter_hg(pk.rng, p.gpar[pk.ell], pk.kx, pk.ky, pk.kz);
-                pk.nscatt++;
-                atomicAdd(lds + 3, 1u);
-            }
-            atomicAdd(lds + 2, 1u);
-            T.start(pk, pk.kx, pk.ky, pk.kz, kFwd1);
-        }
-
-        // Guided scattering with the network outputs o of this packet's event: a direction from
-        // the defensive mixture (1 - mix) HG + mix guide, and the weight factor
-        // Phi_HG / p_mix that keeps the estimator unbiased.  The termination threshold takes the
-        // same factor, so the rule (MonteCarloSimulation.cpp:643-660) applies to the weight
-        // without the guide's factors (packets the guide down-weights are not cut early).
-        __device__ __forceinline__ void guided_scatter(const float* o)
-        {
-            LobesF Lb;
-            guide_lobes_f(o, Lb);
-            const double g = p.gpar[pk.ell];
-            const double kxo = pk.kx, kyo = pk.ky, kzo = pk.kz;
-            const bool useGuide = pk.rng.uniform() < p.guideMix;
-            if (useGuide)
-                guide_sample_f(Lb, pk.rng, pk.kx, pk.ky, pk.kz);
-            else
-                scatter_hg(pk.rng, g, pk.kx, pk.ky, pk.kz);
-            const double c = kxo * pk.kx + kyo * pk.ky + kzo * pk.kz;
-            const double phi = (fabs(g) < 1e-6 ? 1.0 : value_hg(g, c)) / (4 * M_PI);
-            const double q = guide_pdf_f(Lb, pk.kx, pk.ky, pk.kz);
-            const double ratio = phi / ((1.0 - p.guideMix) * phi + p.guideMix * q);
-            pk.W *= ratio;
-            pk.Lthr *= ratio;
-            pk.nscatt++;
-            atomicAdd(lds + 3, 1u);
-            atomicAdd(lds + 2, 1u);
-            T.start(pk, pk.kx, pk.ky, pk.kz, kFwd1);
-        }
-
-        // event handling once a traversal ended
-        __device__ __forceinline__ void transition()
-        {
-            // every traversal starts at the packet's position: its start cell is the hint for
-            // the next traversal from there (Voronoi and tetrahedral grids)
-            if constexpr (GT == kTetra || GT == kVoronoi)
-                if (T.gen.hint >= 0) pk.hint = T.gen.hint;
-            if (T.phase == kPeel)
-            {
-                pk.haveTau = true;
-                pk.tauCached = T.tau_value();
-                const FrameGeom& fg = p.frames[pk.f];
-                double L = pk.Wp / pk.lam;
-                if constexpr (V == kPersp || V == kStoreRF)
-                {
-                    if (fg.persp)
-                    {
-                        double zp;
-                        fg.persp_pixel(pk.rx, pk.ry, pk.rz, zp);
-                        L /= zp * zp;
-                    }
-                }
-                deposit(fg, L, pk.tauCached);
-                next_peel();
-            }
-            else if (T.phase == kFwd1)
-            {
-                // MonteCarloSimulation::simulateForcedPropagation (:1492-1544)
-                pk.taupath = T.tau_value();
-                if (pk.taupath <= 0.)
-                {
-                    pk.W *= 0.;
-                    finish();
-                    return;
-                }
-                double xi = p.xi;
-                double tint;
-                if (xi == 0.)
-                    tint = expon_cutoff(pk.rng, pk.taupath);
-                else
-                {
-                    tint = pk.rng.uniform() < xi ? pk.rng.uniform() * pk.taupath : expon_cutoff(pk.rng, pk.taupath);
-                    double pp = -exp(-tint) / expm1(-pk.taupath);
-                    double q = (1.0 - xi) * pp + xi / pk.taupath;
-                    pk.W *= pp / q;
-                }
-                atomicAdd(lds + 6, 1u);
-                T.start(pk, pk.kx, pk.ky, pk.kz, kFwd2);
-                T.tint = tint;
-                T.mint = -1;
-            }
-            else if (T.phase == kFwd2)
-            {
-                // SpatialGridPath::findInteractionPoint (SpatialGridPath.cpp:161-203): beyond the
-                // last segment, use the last segment's cell and exit distance
-                if (T.mint < 0 && !(T.tint < T.tau_value()))
-                {
-                    T.mint = T.lastm;
-                    T.sint = T.s_value();
-                }
-                int m = T.mint;
-                double n = m >= 0 ? p.dens[m] : 0.;
-                double ss = INST ? p.ssca[pk.ell] * p.inst[pk.inst].scale : p.ssca[pk.ell];
-                double ksca = n > 0. ? n * ss : 0.;
-                double kext = n > 0. ? n * T.section : 0.;
-                double albedo = kext > 0. ? ksca / kext : 0.;
-                pk.W *= -expm1(-pk.taupath) * albedo;
-                if constexpr (GT == kTetra || GT == kVoronoi)
-                    pk.hint = m;  // the interaction point lies in the cell of its segment
-                pk.rx += T.sint * pk.kx;
-                pk.ry += T.sint * pk.ky;
-                pk.rz += T.sint * pk.kz;
-                double L = pk.W / pk.lam;
-                if (L <= 0 || (L <= pk.Lthr && pk.nscatt >= p.min_scatt) || pk.nscatt >= kMaxScatterings)
-                {
-                    finish();
-                    return;
-                }
-                begin_peel(true);
-            }
-        }
-    };
-
-    __device__ __forceinline__ int lane_id() { return __lane_id(); }
-
-// Minimum waves per SIMD of the persistent kernel on Cartesian and octree grids.  Two waves
-// give it 256 VGPRs: the event transitions (FP64 transcendentals on a loaded packet) need far
-// more than the crossing loop, and on the octree at three or four waves the register spills to
-// scratch cost more than the extra latency hiding gains.
-#ifndef SKG_LIFECYCLE_WAVES
-#define SKG_LIFECYCLE_WAVES 2
-#endif
-#ifndef SKG_LIFECYCLE_WAVES_CART
-#define SKG_LIFECYCLE_WAVES_CART 2
-#endif
-
-    // Minimum waves per SIMD per grid type.  The Voronoi and tetra kernels hold less
-    // traversal state and stage no octree nodes in LDS (35 KB per workgroup): measured on the
-    // clumpy Voronoi workload 1735 / 1703 / 1800 ms per launch at 2 / 3 / 4 waves, on the
-    // batched tetra workload 6.72 / 5.80 / 5.27 s.  The octree kernel gains nothing from a
-    // third wave (147.4 vs 148.2 ms, with the top-node table halved to fit three workgroups),
-    // nor the Cartesian one (58.2 / 61.3 / 61.1 ms per 2^22-packet launch at 2 / 3 / 4 waves).
-    template<int GT> constexpr int lifecycle_waves()
-    {
-        return GT == kTetra ? 4 : GT == kVoronoi ? 3 : GT == kCartesian ? SKG_LIFECYCLE_WAVES_CART : SKG_LIFECYCLE_WAVES;
-    }
-
-    template<int GT, int FP, int V>
-    __global__ void __launch_bounds__(kLcThreads, lifecycle_waves<GT>()) lifecycle(SimParams p)
-    {
-        constexpr bool INST = V == kBatched;
-        __shared__ unsigned int wgcnt[8];
-        __shared__ double coldD[kColdF64 * kLcThreads];
-        __shared__ uint32_t coldU[kColdU32 * kLcThreads];
-        if (threadIdx.x < 8) wgcnt[threadIdx.x] = 0;
-        if constexpr (GT == kOctree)
-            stage_top(p.grid);  // other grids allocate no top-node LDS (stage_top ends with a barrier)
-        else
-            __syncthreads();  // the counters are zero before any wave adds to them
-        const int t = threadIdx.x;
-        Trav<GT, FP> T;
-        T.phase = kFetch;
-        T.ended = false;
-        const int lane = lane_id();
-        while (true)
-        {
-            // advance every busy lane by one cell crossing
-            bool busy = (T.phase == kPeel || T.phase == kFwd1 || T.phase == kFwd2) && !T.ended;
-            if (busy) T.template step<V == kStoreRF, V == kPersp || V == kStoreRF>(p.grid, wgcnt, p.rf, p.nlambda);
-
-            // wave-wide transition round once enough lanes wait
-            unsigned long long waiting = __ballot(T.ended || T.phase == kFetch || (V == kGuided && T.phase == kScatter));
-            unsigned long long active = __ballot(T.phase != kDone);
-            if (active == 0) break;
-            if (__popcll(waiting) >= p.threshold || waiting == active)
-            {
-                if (T.ended)
-                {
-                    Packet pk;
-                    pk.load<INST>(coldD, coldU, t, p);
-                    Events<GT, FP, V>{p, T, pk, wgcnt}.transition();
-                    pk.store(coldD, coldU, t);
-                }
-                if constexpr (V == kGuided)
-                {
-                    // one network evaluation for the lanes of the wave that wait to scatter, once
-                    // there are kGuideLanes of them or no lane of the wave is still traversing
-                    const bool sc = T.phase == kScatter;
-                    const unsigned long long scm = __ballot(sc);
-                    const unsigned long long trav =
-                        __ballot((T.phase == kPeel || T.phase == kFwd1 || T.phase == kFwd2) && !T.ended);
-                    if (scm && (__popcll(scm) >= kGuideLanes || trav == 0))
-                    {
-                        Packet pk;
-                        float feat[kGuideIn];
-                        for (int q = 0; q < kGuideIn; ++q) feat[q] = 0.f;
-                        if (sc)
-                        {
-                            pk.load<false>(coldD, coldU, t, p);
-                            guide_features(p.gframe, pk.rx, pk.ry, pk.rz, pk.kx, pk.ky, pk.kz, pk.lam, feat);
-                        }
-                        float o[kGuideOut];
-                        guide_wave_infer_regs(p.net, feat, sc, o);
-                        if (sc)
-                        {
-                            Events<GT, FP, V>{p, T, pk, wgcnt}.guided_scatter(o);
-                            pk.store(coldD, coldU, t);
-                        }
-                    }
-                }
-                unsigned long long need = __ballot(T.phase == kFetch);
-                if (need)
-                {
-                    int cntNeed = __popcll(need);
-                    int leader = __ffsll(static_cast<long long>(need)) - 1;
-                    unsigned long long base = 0;
-                    if (lane == leader) base = atomicAdd(p.queue, static_cast<unsigned long long>(cntNeed));
-                    base = __shfl(base, leader);
-                    if (T.phase == kFetch)
-                    {
-                        unsigned long long below = need & ((1ull << lane) - 1ull);
-                        unsigned long long mine = base + __popcll(below);
-                        if (mine < p.count)
-                        {
-                            Packet pk;
-                            Events<GT, FP, V>{p, T, pk, wgcnt}.launch(p.first + mine);
-                            pk.store(coldD, coldU, t);
-                        }
-                        else
-                            T.phase = kDone;
-                    }
-                }
-            }
-        }
-        // workgroup event counters to the global counters
-        __syncthreads();
-        if (threadIdx.x < 8 && wgcnt[threadIdx.x])
-            atomicAdd(p.counters + threadIdx.x, static_cast<unsigned long long>(wgcnt[threadIdx.x]));
-    }
 }
 
 // ==================================================================== host side
@@ -1125,7 +324,7 @@ struct skg_engine
     float lastMs = 0.f;
     // wavefront pipeline
     int mode = SKG_MODE_PIPELINE;
-    unsigned long long* ctl = nullptr;  // 8 control words followed by the 8 event counters
+    unsigned long long* ctl = nullptr;  // 8 control words followed by the kNumCounters event counters
     unsigned long long* hostCtl = nullptr;
     Pool pool{};
     void* poolMem = nullptr;
@@ -1143,6 +342,7 @@ struct skg_engine
     // path guide
     int guideMode = 0;
     double guideMix = 0.5;
+    uint32_t contKey = 0;  // continuation stream tag from the first scattering on (0: off)
     GuideFrame gframe{1.0, 0.0, 1.0};
     float* gTheta = nullptr;  // FP32 parameters, Adam moments and gradient (4 x kGuideParams)
     uint16_t* gBf = nullptr;  // packed BF16 tiles (W and W^T per layer)
@@ -1306,7 +506,7 @@ namespace
         e->frameSize = total;
         if (total) HIPCHK(e, hipMalloc(&e->frameBuf, total * sizeof(double)));
         if (total) HIPCHK(e, hipMemset(e->frameBuf, 0, total * sizeof(double)));
-        HIPCHK(e, hipMemset(e->counters, 0, 8 * sizeof(unsigned long long)));
+        HIPCHK(e, hipMemset(e->counters, 0, kNumCounters * sizeof(unsigned long long)));
         std::vector<FrameGeom> geo;
         fillFrameGeom(e, geo);
         if (!geo.empty()) HIPCHK(e, hipMemcpy(e->frameGeom, geo.data(), geo.size() * sizeof(FrameGeom), hipMemcpyHostToDevice));
@@ -1429,8 +629,8 @@ int skg_create(int device, skg_engine** out)
     if (st == hipSuccess) st = hipEventCreate(&e->ev0);
     if (st == hipSuccess) st = hipEventCreate(&e->ev1);
     if (st == hipSuccess) st = hipMalloc(&e->queue, sizeof(unsigned long long));
-    if (st == hipSuccess) st = hipMalloc(&e->ctl, 16 * sizeof(unsigned long long));
-    if (st == hipSuccess) st = hipMemset(e->ctl, 0, 16 * sizeof(unsigned long long));
+    if (st == hipSuccess) st = hipMalloc(&e->ctl, (kCtlCnt + kNumCounters) * sizeof(unsigned long long));
+    if (st == hipSuccess) st = hipMemset(e->ctl, 0, (kCtlCnt + kNumCounters) * sizeof(unsigned long long));
     if (st == hipSuccess) e->counters = e->ctl + kCtlCnt;
     if (st == hipSuccess) st = hipHostMalloc(&e->hostCtl, 16 * sizeof(unsigned long long), hipHostMallocDefault);
     if (st == hipSuccess) st = hipMalloc(&e->frameGeom, kMaxFrames * sizeof(FrameGeom));
@@ -1959,7 +1159,7 @@ int skg_frame_clear(skg_engine* e)
     int st = ensureFrames(e);
     if (st) return st;
     if (e->frameSize) HIPCHK(e, hipMemsetAsync(e->frameBuf, 0, e->frameSize * sizeof(double), e->stream));
-    HIPCHK(e, hipMemsetAsync(e->counters, 0, 8 * sizeof(unsigned long long), e->stream));
+    HIPCHK(e, hipMemsetAsync(e->counters, 0, kNumCounters * sizeof(unsigned long long), e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
     return SKG_OK;
 }
@@ -2290,6 +1490,7 @@ static int runPersistent(skg_engine* e, uint64_t first_history, uint64_t count)
     p.net = e->net;
     p.gframe = e->gframe;
     p.guideMix = e->guideMix;
+    p.contKey = e->contKey;
     p.rf = e->rf;
     p.persp = 0;
     for (const auto& f : e->frames) p.persp |= f.persp;
@@ -2598,6 +1799,8 @@ int skg_run_packets_async(skg_engine* e, uint64_t first_history, uint64_t count)
                 return fail(e, SKG_ERR_STATE, "skg_run_packets: an instance names a frame that does not exist");
     }
     if (count == 0) return SKG_OK;
+    if (e->mode == SKG_MODE_PIPELINE && e->contKey)
+        return fail(e, SKG_ERR_STATE, "skg_run_packets: continuation streams run in the persistent engine");
     if (e->mode == SKG_MODE_PIPELINE && e->rf)
         return fail(e, SKG_ERR_STATE, "skg_run_packets: radiation-field storage runs in the persistent engine");
     if (e->mode == SKG_MODE_PIPELINE && e->nent)
@@ -2616,6 +1819,14 @@ int skg_tune_pipeline(skg_engine* e, int32_t mode, int64_t pool_slots, int32_t t
     e->poolSlots = pool_slots;
     e->traceBlocks = trace_workgroups;
     e->graphIters = graph_iterations;
+    return SKG_OK;
+}
+
+int skg_continuation_key(skg_engine* e, uint32_t key)
+{
+    if (!e) return SKG_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->contKey = key;
     return SKG_OK;
 }
 
@@ -2731,31 +1942,16 @@ int skg_guide_infer(skg_engine* e, const float* features, int64_t n, float* out)
 int skg_guide_sample(skg_engine* e, const float* outputs, int64_t n, uint64_t seed, const double* at, double* dirs,
                      double* pdf, double* pdf_at)
 {
-    if (!e || n < 0 || (n && (!outputs || !dirs || !pdf || (at && !pdf_at))))
-        return fail(e, SKG_ERR_ARGUMENT, "skg_guide_sample: bad arguments");
-    for (int64_t i = 0; i < n * kGuideOut; ++i)
-        if (!std::isfinite(outputs[i])) return fail(e, SKG_ERR_ARGUMENT, "skg_guide_sample: outputs not finite");
-    std::lock_guard<std::mutex> lk(e->mu);
-    hipSetDevice(e->device);
-    if (n == 0) return SKG_OK;
-    float* dO = nullptr;
-    double *dAt = nullptr, *dRes = nullptr;
-    HIPCHK(e, hipMalloc(&dO, n * kGuideOut * sizeof(float)));
-    HIPCHK(e, hipMalloc(&dRes, n * 5 * sizeof(double)));
-    if (at) HIPCHK(e, hipMalloc(&dAt, n * 3 * sizeof(double)));
-    HIPCHK(e, hipMemcpy(dO, outputs, n * kGuideOut * sizeof(float), hipMemcpyHostToDevice));
-    if (at) HIPCHK(e, hipMemcpy(dAt, at, n * 3 * sizeof(double), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(guide_sample_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, e->stream, dO, n,
-                       seed, dAt, dRes, dRes + 3 * n, dRes + 4 * n);
-    HIPCHK(e, hipGetLastError());
-    HIPCHK(e, hipMemcpyAsync(dirs, dRes, n * 3 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(e, hipMemcpyAsync(pdf, dRes + 3 * n, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
-    if (at) HIPCHK(e, hipMemcpyAsync(pdf_at, dRes + 4 * n, n * sizeof(double), hipMemcpyDeviceToHost, e->stream));
-    HIPCHK(e, hipStreamSynchronize(e->stream));
-    hipFree(dO);
-    hipFree(dRes);
-    if (dAt) hipFree(dAt);
-    return SKG_OK;
+    // switched off: the first GPU run of this entry point ended in an abort whose cause has
+    // not been found; the guided life cycle checks the same sampler (test_gpu_guide.py)
+    (void)outputs;
+    (void)n;
+    (void)seed;
+    (void)at;
+    (void)dirs;
+    (void)pdf;
+    (void)pdf_at;
+    return fail(e, SKG_ERR_STATE, "skg_guide_sample: switched off (an unexplained abort on its first GPU run)");
 }
 
 int skg_guide_gradient(skg_engine* e, const float* records, int64_t n, float* grad, double* loss)
@@ -2941,7 +2137,7 @@ int skg_autotune(skg_engine* e, uint64_t first_history, uint64_t trial_packets, 
     double* saved = nullptr;
     double* savedRf = nullptr;
     const size_t rfSize = e->rf ? static_cast<size_t>(e->rfCells) * e->rfLambda : 0;
-    unsigned long long savedCnt[8];
+    unsigned long long savedCnt[kNumCounters];
     {
         std::lock_guard<std::mutex> lk(e->mu);
         hipSetDevice(e->device);
@@ -3036,15 +2232,17 @@ int skg_autotune(skg_engine* e, uint64_t first_history, uint64_t trial_packets, 
     return SKG_OK;
 }
 
-int skg_counters(skg_engine* e, int64_t* out8)
+int skg_counters(skg_engine* e, int64_t* out8) { return skg_counters_n(e, out8, 8); }
+
+int skg_counters_n(skg_engine* e, int64_t* out, int32_t n)
 {
-    if (!e || !out8) return SKG_ERR_ARGUMENT;
+    if (!e || !out || n < 1 || n > kNumCounters) return fail(e, SKG_ERR_ARGUMENT, "skg_counters_n: bad arguments");
     std::lock_guard<std::mutex> lk(e->mu);
     hipSetDevice(e->device);
-    unsigned long long c[8];
+    unsigned long long c[kNumCounters];
     HIPCHK(e, hipMemcpyAsync(c, e->counters, sizeof(c), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
-    for (int i = 0; i < 8; ++i) out8[i] = static_cast<int64_t>(c[i]);
+    for (int i = 0; i < n; ++i) out[i] = static_cast<int64_t>(c[i]);
     return SKG_OK;
 }
 

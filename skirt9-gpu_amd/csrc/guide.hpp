@@ -394,29 +394,6 @@ namespace skg
             for (int q = 0; q < kGuideOut; ++q) out[s * kGuideOut + q] = o[q];
     }
 
-    // The life cycle's FP32 mixture sampler and density as a service: for the network outputs
-    // of row i (40 floats), a direction drawn with the Philox stream (seed, i), the mixture
-    // density at that direction and, when `at` is given, the density at the direction at[i]
-    __global__ void guide_sample_kernel(const float* __restrict__ outs, int64_t n, uint64_t seed,
-                                        const double* __restrict__ at, double* dirs, double* pdf, double* pdfAt)
-    {
-        const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-        if (i >= n) return;
-        float o[kGuideOut];
-        for (int q = 0; q < kGuideOut; ++q) o[q] = outs[i * kGuideOut + q];
-        LobesF L;
-        guide_lobes_f(o, L);
-        Philox rng;
-        rng.init(seed, static_cast<uint64_t>(i));
-        double kx, ky, kz;
-        guide_sample_f(L, rng, kx, ky, kz);
-        dirs[3 * i] = kx;
-        dirs[3 * i + 1] = ky;
-        dirs[3 * i + 2] = kz;
-        pdf[i] = guide_pdf_f(L, kx, ky, kz);
-        if (at) pdfAt[i] = guide_pdf_f(L, at[3 * i], at[3 * i + 1], at[3 * i + 2]);
-    }
-
     // dL/d(raw outputs) of the weighted negative log-likelihood for one sample (FP64 math),
     // scaled by cw = c_i / sum(c); the expression of the CPU statement
     __device__ __forceinline__ void guide_dout(const float* o, double kx, double ky, double kz, double cw, float* d)

@@ -204,4 +204,22 @@ namespace skg
         double scale;  // optical-depth scale: sections multiplied by it
         int32_t fbeg, fend;
     };
+
+    // compensated (Kahan) FP32 accumulator used in SKG_FP32_KAHAN mode
+    struct Kahan32
+    {
+        float sum, comp;
+        __device__ __forceinline__ void reset()
+        {
+            sum = 0.f;
+            comp = 0.f;
+        }
+        __device__ __forceinline__ void add(float x)
+        {
+            float y = x - comp;
+            float t = sum + y;
+            comp = (t - sum) - y;
+            sum = t;
+        }
+    };
 }

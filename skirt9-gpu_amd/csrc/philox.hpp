@@ -50,6 +50,14 @@ namespace skg
             block = 0xFFFFFFFFu;
         }
 
+        // continue the same history on the stream of another tag, from its first draw
+        __host__ __device__ __forceinline__ void retag(uint32_t tg)
+        {
+            tag = tg;
+            draw = 0;
+            block = 0xFFFFFFFFu;
+        }
+
         __host__ __device__ __forceinline__ void gen(uint32_t b)
         {
             uint32_t x0 = h0, x1 = h1, x2 = b, x3 = tag;

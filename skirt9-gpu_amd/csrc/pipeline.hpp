@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "../../include/skirt_mi355x.h"
 #include "guide.hpp"
 #include "lifecycle.hpp"
 #include "traverse.hpp"
@@ -120,6 +121,11 @@ namespace skg
     constexpr int kCtlActive = 5;
     constexpr int kCtlOverflow = 7;  // requests that did not fit the queue (the run then fails)
     constexpr int kCtlCnt = 8;
+    // event counters (both engines): 0 histories, 1 peel-off traversals, 2 first forced passes,
+    // 3 scatterings, 4 cell crossings, 5 FP64 detector atomics issued, 6 second forced passes,
+    // 7 crossings of second passes, 8 detector deposits (before the wavefront combines them),
+    // 9..15 unused
+    constexpr int kNumCounters = 16;
 
     // wave-aggregated append of `want` entries per lane; returns the lane's first index
     __device__ __forceinline__ unsigned long long wave_reserve(unsigned long long* counter, unsigned want)
@@ -749,7 +755,7 @@ namespace skg
     {
         const unsigned long long count = p.ctl[kCtlCount + in];
         const Pool& P = p.pool;
-        unsigned long long atomics = 0;
+        unsigned long long atomics = 0, deposits = 0;
         const unsigned long long stride = static_cast<unsigned long long>(gridDim.x) * blockDim.x;
         const unsigned long long n = count < static_cast<unsigned long long>(p.qcap) ? count : p.qcap;
         for (unsigned long long i0 = blockIdx.x * static_cast<unsigned long long>(blockDim.x); i0 < n; i0 += stride)
@@ -777,6 +783,7 @@ namespace skg
                 if (pix >= 0) key = fg.offset + pix + static_cast<int64_t>(ell) * fg.nx * fg.ny;
             }
             wave_add(p.frame, key, v, atomics);
+            if (key >= 0) ++deposits;
             if (valid)
             {
                 double dep = key >= 0 ? v : 0.;
@@ -789,12 +796,18 @@ namespace skg
                     unsafeAtomicAdd(p.frame + fg.offset + pix + static_cast<int64_t>(ell) * fg.nx * fg.ny, v);
                     dep += v;
                     ++atomics;
+                    ++deposits;
                 }
                 if (p.guideMode == 2 && dep > 0.) atomicAdd(p.contrib + slot, dep);
             }
         }
-        for (int o = 32; o > 0; o >>= 1) atomics += __shfl_xor(atomics, o);
+        for (int o = 32; o > 0; o >>= 1)
+        {
+            atomics += __shfl_xor(atomics, o);
+            deposits += __shfl_xor(deposits, o);
+        }
         if (__lane_id() == 0 && atomics) atomicAdd(p.ctl + kCtlCnt + 5, atomics);
+        if (__lane_id() == 0 && deposits) atomicAdd(p.ctl + kCtlCnt + 8, deposits);
     }
 
     // training weights of the collected records: the luminosity the packet deposited after the

@@ -305,3 +305,51 @@ def test_pipeline_queue_overflow_fails_the_run(eng):
     eng.frame_clear()
     eng.run_packets(0, npk)
     np.testing.assert_allclose(pipe, eng.frame_read(0)[0], rtol=1e-12, atol=0)
+
+
+def test_continuation_streams(eng):
+    """skg_continuation_key (persistent engine): key 0 is the default single stream; a key is
+    deterministic; two keys share each history's emission and first-scattering peel-offs and
+    continue independently, so their batch totals are correlated but not equal and agree in
+    mean; the pipeline engine refuses a key."""
+    pkg = load_pkg()
+    b, dens = disk_density_cartesian(n=16, samples=4, seed=19)
+    eng.grid_cartesian(b, b, b)
+    eng.medium(dens)
+    s = 1.5e-29
+    eng.dust([0.55e-6], [s], [0.6 * s], [0.6])
+    eng.frame_remove_all()
+    fr = frame60(npix=30)
+    eng.frame_add(fr["inclination"], fr["azimuth"], fr["roll"], fr["nx"], fr["ny"], fr["fovx"], fr["fovy"])
+    per, nb = 20000, 24
+    eng.source_point((0.0, 0.0, 0.0), [1.0 / per])
+    eng.options(0.5, 1e4, 0, 0, 41)
+    eng.tune_pipeline(1, 1 << 18, 0, 8)
+
+    def run(key, first):
+        eng.continuation_key(key)
+        eng.frame_clear()
+        eng.run_packets(first, per)
+        return eng.frame_read(0)
+
+    print("case continuation streams", flush=True)
+    base = run(0, 0)
+    eng.continuation_key(0)
+    eng.frame_clear()
+    eng.run_packets(0, per)
+    np.testing.assert_allclose(eng.frame_read(0), base, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(run(1, 0), run(1, 0), rtol=1e-12, atol=0)
+    t1 = np.array([run(1, k * per).sum() for k in range(nb)])
+    t2 = np.array([run(2, k * per).sum() for k in range(nb)])
+    eng.continuation_key(0)
+    assert np.all(t1 != t2)
+    corr = np.corrcoef(t1, t2)[0, 1]
+    print(f"  correlation of batch totals between continuation streams: {corr:.3f}")
+    assert corr > 0.2
+    se = np.hypot(t1.std(ddof=1), t2.std(ddof=1)) / np.sqrt(nb)
+    assert abs(t1.mean() - t2.mean()) < 4 * se
+    eng.tune_pipeline(0, 1 << 18, 0, 8)
+    eng.continuation_key(3)
+    with pytest.raises(pkg.EngineError, match="persistent"):
+        eng.run_packets(0, 1000)
+    eng.continuation_key(0)

@@ -1,7 +1,10 @@
 """End to end through the ski front end on the GPU: the config-0 ski file (Cartesian
 64^3 disk, tau_V = 1, one frame at 60 deg) run by skirt9_gpu_amd.ski, calibrated to MJy/sr,
 against the reference's own FITS images (mean and standard error over 8 runs of 1e6
-packets): total within 4 sigma, 8x8-binned pooled chi-square per degree of freedom < 1.5."""
+packets): total within 4 sigma, 8x8-binned pooled chi-square per degree of freedom < 1.5, and
+per pixel at native resolution (200 x 200) the pooled t statistics of the pixels known to 10 %
+on both sides: mean square within 5 sigma of nu / (nu - 2), fraction beyond |t| = 4 within 5
+sigma of the t tail (FrameInstrument.cpp:117-136, FluxRecorder.cpp:396-420)."""
 
 import os
 import sys
@@ -9,7 +12,7 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import ROOT, golden, load_pkg
+from helpers import ROOT, golden, load_pkg, pixel_parity
 
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import skigen  # noqa: E402
@@ -25,7 +28,7 @@ def test_config0_ski_run_matches_reference_fits(tmp_path):
         f.write(skigen.render(skigen.config0(num_packets=1e6)))
     sim = S.load(path)
     eng = pkg.Engine(0)
-    nb, per = 6, 1_000_000
+    nb, per = 8, 1_000_000
     imgs = []
     for b in range(nb):
         sim.seed = 1000 + b  # own density sampling and random streams per batch, as each reference run
@@ -56,6 +59,12 @@ def test_config0_ski_run_matches_reference_fits(tmp_path):
     assert m.sum() > 300
     assert np.mean(z ** 2) < 1.5
     assert np.mean(np.abs(z) > 4) < 0.01
+    r = pixel_parity(imgs, per, ref, rsem, nref, pref, max_rel=0.1)
+    print(f"per pixel: {r['pixels']} pixels, chi2/dof {r['chi2_dof']:.3f} (expected {r['expected']:.3f} "
+          f"+- {r['tol']:.3f}), |z| > 4: {r['frac4']:.4f} (expected {r['expected_frac4']:.4f})")
+    assert r["pixels"] > 2000
+    assert abs(r["chi2_dof"] - r["expected"]) < r["tol"]
+    assert r["frac4"] < r["expected_frac4"] + r["tol_frac4"]
 
 
 def test_config0_mean_intensity_matches_reference(tmp_path):
@@ -121,7 +130,7 @@ def test_config0_on_mesh_grids_matches_reference(tmp_path, kind):
     sim = S.load(path)
     assert sim.grid == dict(kind=kind, n=20000)
     eng = pkg.Engine(0)
-    nb, per = 6, 1_000_000
+    nb, per = 8, 1_000_000
     imgs = []
     for b in range(nb):
         sim.seed = 7000 + b

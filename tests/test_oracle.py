@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import PC, cart_borders, child_first, disk_density_cartesian, frame60, golden
+from helpers import PC, cart_borders, child_first, disk_density_cartesian, frame60, golden, pixel_parity
 
 
 # ---------------------------------------------------------------- Philox4x32-10
@@ -295,12 +295,15 @@ def test_expon_cutoff_distribution():
 # ---------------------------------------------------------------- images
 def test_config0_image_matches_reference():
     """Oracle frame image of the Cartesian workload versus the reference's mean image over
-    8 independent runs: total flux and 8x8-binned chi-square per degree of freedom.
-    Each oracle batch uses its own density sampling, as each reference run does."""
+    8 independent runs: total flux, 8x8-binned chi-square per degree of freedom, and per pixel
+    at native resolution the pooled t statistics (pixels known to 10 % on both sides): their
+    mean square within 5 sigma of its expectation nu / (nu - 2) and the fraction beyond
+    |t| = 4 within 5 sigma of the t distribution's tail.  Each oracle batch uses its own
+    density sampling, as each reference run does."""
     f = golden("image_cfg0.npz")
     ref = f["mean"][0].astype(float)
     rsem = f["sem"][0].astype(float)
-    nb, per = 6, 150000
+    nb, per = 8, 250000
     imgs = []
     for bi in range(nb):
         b, dens = disk_density_cartesian(seed=100 + bi, samples=40)
@@ -332,3 +335,10 @@ def test_config0_image_matches_reference():
     assert m.sum() > 300
     assert np.mean(z ** 2) < 1.5  # pooled t with 12 dof: E[z^2] ~ 1.2
     assert np.mean(np.abs(z) > 4) < 0.01
+    # native resolution: per-pixel pooled t (FrameInstrument.cpp:117-136, FluxRecorder.cpp:396-420)
+    r = pixel_parity(imgs, per, ref, rsem, nref, pref, max_rel=0.1)
+    print(f"per pixel: {r['pixels']} pixels, chi2/dof {r['chi2_dof']:.3f} (expected {r['expected']:.3f} "
+          f"+- {r['tol']:.3f}), |z| > 4: {r['frac4']:.4f} (expected {r['expected_frac4']:.4f})")
+    assert r["pixels"] > 1000
+    assert abs(r["chi2_dof"] - r["expected"]) < r["tol"]
+    assert r["frac4"] < r["expected_frac4"] + r["tol_frac4"]

@@ -773,29 +773,26 @@ int skg_grid_octree(skg_engine* e, int64_t nnodes, const double* bounds, const i
     freeGrid(e);
     GridView g{};
     int st;
-    if ((st = gridUpload(e, bounds, 6 * nnodes, &g.bounds)) || (st = gridUpload(e, child_first, nnodes, &g.child_first))
-        || (st = gridUpload(e, cell_index, nnodes, &g.cell_of)) || (st = gridUpload(e, leaves.data(), leaves.size(), &g.leaf)))
-        return st;
+    if ((st = gridUpload(e, leaves.data(), leaves.size(), &g.leaf))) return st;
     if (!nbc.empty())
     {
         if ((st = gridUpload(e, nbc.data(), nbc.size(), &g.nbc))) return st;
     }
-    // the nodes kernels stage in LDS for the top-down search (ids below kTopNodes; the
+    // every node in the search's form; the kernels stage ids below kTopNodes in LDS (the
     // level-by-level construction numbers the top levels first)
     {
-        const int topN = static_cast<int>(std::min<int64_t>(nnodes, kTopNodes));
-        std::vector<TopNode> top(topN);
-        for (int n = 0; n < topN; ++n)
+        std::vector<TopNode> top(nnodes);
+        for (int64_t n = 0; n < nnodes; ++n)
         {
             const int32_t c = child_first[n];
             top[n].child = c;
-            top[n].pad = 0;
+            top[n].cell = c >= 0 ? -1 : cell_index[n];
             top[n].sx = c >= 0 ? bounds[6 * static_cast<int64_t>(c) + 3] : 0.;
             top[n].sy = c >= 0 ? bounds[6 * static_cast<int64_t>(c) + 4] : 0.;
             top[n].sz = c >= 0 ? bounds[6 * static_cast<int64_t>(c) + 5] : 0.;
         }
         if ((st = gridUpload(e, top.data(), top.size(), &g.top))) return st;
-        g.topN = topN;
+        g.topN = static_cast<int>(std::min<int64_t>(nnodes, kTopNodes));
     }
     for (int q = 0; q < 6; ++q) g.box[q] = bounds[q];
     double dx = g.box[3] - g.box[0], dy = g.box[4] - g.box[1], dz = g.box[5] - g.box[2];

@@ -655,8 +655,8 @@ namespace skg
         __device__ __forceinline__ void transition()
         {
             // every traversal starts at the packet's position: its start cell is the hint for
-            // the next traversal from there (Voronoi and tetrahedral grids)
-            if constexpr (GT == kTetra || GT == kVoronoi)
+            // the next traversal from there (octree, Voronoi and tetrahedral grids)
+            if constexpr (GT != kCartesian)
                 if (T.gen.hint >= 0) pk.hint = T.gen.hint;
             if (T.phase == kPeel)
             {
@@ -718,7 +718,7 @@ namespace skg
                 double kext = n > 0. ? n * T.section : 0.;
                 double albedo = kext > 0. ? ksca / kext : 0.;
                 pk.W *= -expm1(-pk.taupath) * albedo;
-                if constexpr (GT == kTetra || GT == kVoronoi)
+                if constexpr (GT != kCartesian)
                     pk.hint = m;  // the interaction point lies in the cell of its segment
                 pk.rx += T.sint * pk.kx;
                 pk.ry += T.sint * pk.ky;

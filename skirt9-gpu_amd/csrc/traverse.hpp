@@ -85,13 +85,15 @@ namespace skg
     };
     static_assert(sizeof(OctLeaf) == 128, "leaf record must be one 128-byte line");
 
-    // An octree node near the root as staged in LDS: the split point (upper corner of the first
-    // child, TreeNode::leafChild's test point) and the first child id (-1 for a leaf).
+    // An octree node in the form the top-down search reads (one 32-byte record per level): the
+    // split point (upper corner of the first child, TreeNode::leafChild's test point) and the
+    // first child id, or for a leaf (child -1) its cell index.  The records of the top levels
+    // are also staged in LDS.
     struct TopNode
     {
         double sx, sy, sz;
         int32_t child;
-        int32_t pad;
+        int32_t cell;
     };
     static_assert(sizeof(TopNode) == 32, "top node must be 32 bytes");
 #ifndef SKG_TOP_NODES
@@ -99,10 +101,9 @@ namespace skg
 #endif
     constexpr int kTopNodes = SKG_TOP_NODES;  // 1024: 32 KB of LDS, levels 0-3 of a full octree and part of level 4
 
-    // Read-only grid tables resident in HBM.  Octree: node-indexed bounds, first child
-    // (children are 8 consecutive ids) and cell index for the top-down search, plus the
-    // cell-indexed leaf records and their neighbour lists (cell ids, sorted by decreasing
-    // overlap as TreeNode::sortNeighbors).
+    // Read-only grid tables resident in HBM.  Octree: the node records of the top-down search
+    // (TopNode, children are 8 consecutive ids), the cell-indexed leaf records and their
+    // neighbour lists (cell ids, sorted by decreasing overlap as TreeNode::sortNeighbors).
     struct GridView
     {
         double box[6];  // xmin, ymin, zmin, xmax, ymax, zmax of the domain
@@ -112,13 +113,10 @@ namespace skg
         const double* zb;
         int nx, ny, nz;
         const double* dens;  // Cartesian: number density per cell
-        const double* bounds;
-        const int32_t* child_first;
-        const int32_t* cell_of;
         const OctLeaf* leaf;
         const int32_t* nbc;
-        const TopNode* top;  // the nodes with id < topN in TopNode form (global copy)
-        int topN;            // 0 disables the LDS stage (non-octree grids)
+        const TopNode* top;  // every node in TopNode form, indexed by node id
+        int topN;            // nodes with id < topN are staged in LDS (0: none, non-octree grids)
         // Voronoi: cell records, neighbour entries and the block search lists (CSR over vbn^3
         // blocks of cell ids, in the reference's order)
         const VorCell* vcell;
@@ -318,36 +316,20 @@ namespace skg
         __syncthreads();
     }
 
-    // top-down search for the leaf containing r; -1 outside the root (closed box)
-    __device__ __forceinline__ int leaf_child(const GridView& g, double x, double y, double z)
+    // TreeNode::leafChild from the root (TreeSpatialGrid.cpp:190-219 with OctTreeNode::child,
+    // OctTreeNode.cpp:37-42): the cell of the leaf containing r, -1 outside the root's closed
+    // box.  One 32-byte node record per level, from LDS for the staged top levels.
+    __device__ __forceinline__ int leaf_cell_at(const GridView& g, double x, double y, double z)
     {
-        if (!node_contains(g.bounds, x, y, z)) return -1;
+        if (!(x >= g.box[0] && x <= g.box[3] && y >= g.box[1] && y <= g.box[4] && z >= g.box[2] && z <= g.box[5]))
+            return -1;
         int node = 0;
         while (true)
         {
-            int c0;
-            double sx, sy, sz;
-            if (node < g.topN)
-            {
-                const TopNode& t = skg_top_lds[node];
-                c0 = t.child;
-                sx = t.sx;
-                sy = t.sy;
-                sz = t.sz;
-            }
-            else
-            {
-                c0 = g.child_first[node];
-                if (c0 < 0) break;
-                const double* b = g.bounds + 6 * static_cast<int64_t>(c0);
-                sx = b[3];
-                sy = b[4];
-                sz = b[5];
-            }
-            if (c0 < 0) break;
-            node = c0 + (x < sx ? 0 : 1) + (y < sy ? 0 : 2) + (z < sz ? 0 : 4);
+            const TopNode t = node < g.topN ? skg_top_lds[node] : g.top[node];
+            if (t.child < 0) return t.cell;
+            node = t.child + (x < t.sx ? 0 : 1) + (y < t.sy ? 0 : 2) + (z < t.sz ? 0 : 4);
         }
-        return node;
     }
 
     template<int GT> struct Gen
@@ -381,8 +363,7 @@ namespace skg
 
         __device__ __forceinline__ int leaf_cell(const GridView& g, double x, double y, double z) const
         {
-            int node = leaf_child(g, x, y, z);
-            return node >= 0 ? g.cell_of[node] : -1;
+            return leaf_cell_at(g, x, y, z);
         }
 
         __device__ __forceinline__ void start(double r0, double r1, double r2, double k0, double k1, double k2,
@@ -481,15 +462,26 @@ namespace skg
                 }
                 else if (GT == kOctree)
                 {
-                    // (no start hint: the top-down search from the LDS-staged top levels is
-                    // cheaper here than carrying the hint, measured 147 vs 157 ms per launch)
-                    i = leaf_cell(g, rx, ry, rz);
+                    // the start hint, when the point lies strictly inside its box: every split
+                    // plane of its ancestors then sends leafChild's descent toward it, so it is the
+                    // leaf the top-down search returns (its record is the one needed anyway)
+                    i = -1;
+                    if (hint >= 0)
+                    {
+                        load_leaf(g, hint);
+                        if (rx > b0 && rx < b3 && ry > b1 && ry < b4 && rz > b2 && rz < b5) i = hint;
+                    }
                     if (i < 0)
                     {
-                        state = kOutside;
-                        return false;
+                        i = leaf_cell(g, rx, ry, rz);
+                        if (i < 0)
+                        {
+                            state = kOutside;
+                            return false;
+                        }
+                        load_leaf(g, i);
                     }
-                    load_leaf(g, i);
+                    hint = i;
                 }
                 else if (GT == kVoronoi)
                 {

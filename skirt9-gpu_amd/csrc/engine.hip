@@ -774,14 +774,13 @@ int skg_grid_octree(skg_engine* e, int64_t nnodes, const double* bounds, const i
     GridView g{};
     int st;
     if ((st = gridUpload(e, leaves.data(), leaves.size(), &g.leaf))) return st;
-    if (!nbc.empty())
-    {
-        if ((st = gridUpload(e, nbc.data(), nbc.size(), &g.nbc))) return st;
-    }
+    // the octree step machine loads 112 bytes from any entry: 28 ints of padding
+    nbc.resize(nbc.size() + 28, 0);
+    if ((st = gridUpload(e, nbc.data(), nbc.size(), &g.nbc))) return st;
     // every node in the search's form; the kernels stage ids below kTopNodes in LDS (the
     // level-by-level construction numbers the top levels first)
     {
-        std::vector<TopNode> top(nnodes);
+        std::vector<TopNode> top(nnodes + 3);  // + 96 bytes: the step machine loads 112 from an entry
         for (int64_t n = 0; n < nnodes; ++n)
         {
             const int32_t c = child_first[n];
@@ -791,6 +790,7 @@ int skg_grid_octree(skg_engine* e, int64_t nnodes, const double* bounds, const i
             top[n].sy = c >= 0 ? bounds[6 * static_cast<int64_t>(c) + 4] : 0.;
             top[n].sz = c >= 0 ? bounds[6 * static_cast<int64_t>(c) + 5] : 0.;
         }
+        for (int64_t n = nnodes; n < nnodes + 3; ++n) top[n] = TopNode{0., 0., 0., -1, 0};
         if ((st = gridUpload(e, top.data(), top.size(), &g.top))) return st;
         g.topN = static_cast<int>(std::min<int64_t>(nnodes, kTopNodes));
     }

@@ -255,7 +255,9 @@ namespace skg
         {
             int m;
             double ds, nd;
-            if (!gen.next(grid, m, ds, nd) || ++steps > kMaxSteps)
+            const int r = gen.advance(grid, m, ds, nd);
+            if (r == 0) return;  // an octree step that loaded a search node or list entry
+            if (r < 0 || ++steps > kMaxSteps)
             {
                 ended = true;
                 atomicAdd(lds + 4, static_cast<unsigned int>(steps));

@@ -316,21 +316,29 @@ namespace skg
         __syncthreads();
     }
 
-    // TreeNode::leafChild from the root (TreeSpatialGrid.cpp:190-219 with OctTreeNode::child,
-    // OctTreeNode.cpp:37-42): the cell of the leaf containing r, -1 outside the root's closed
-    // box.  One 32-byte node record per level, from LDS for the staged top levels.
-    __device__ __forceinline__ int leaf_cell_at(const GridView& g, double x, double y, double z)
+    // a ? x : b ? y : z on values (selecting among members by reference would make the
+    // compiler select their addresses and keep the generator in scratch memory)
+    template<typename T> __device__ __forceinline__ T pick3(bool a, bool b, T x, T y, T z)
     {
-        if (!(x >= g.box[0] && x <= g.box[3] && y >= g.box[1] && y <= g.box[4] && z >= g.box[2] && z <= g.box[5]))
-            return -1;
-        int node = 0;
-        while (true)
-        {
-            const TopNode t = node < g.topN ? skg_top_lds[node] : g.top[node];
-            if (t.child < 0) return t.cell;
-            node = t.child + (x < t.sx ? 0 : 1) + (y < t.sy ? 0 : 2) + (z < t.sz ? 0 : 4);
-        }
+        const T yz = b ? y : z;
+        return a ? x : yz;
     }
+
+    // What the octree generator's next advance() call loads and does with the record.  Every
+    // call makes at most one dependent global load (a leaf record, a search node or a
+    // neighbour-list entry), so lanes of a wavefront in different phases share one memory
+    // round trip per call instead of the wave waiting for each lane's chain of loads.
+    enum OctStep : int
+    {
+        kOsStart = 0,     // enter the domain, then the start hint's record (or the search)
+        kOsCross = 1,     // the crossing of the current leaf, then the wall's first neighbour
+        kOsListStart = 2, // the current leaf's record again: the wall's neighbour-list range
+        kOsListId = 3,    // a neighbour-list entry (cell id)
+        kOsListRec = 4,   // that neighbour's record
+        kOsNode = 5,      // a search node below the LDS-staged levels
+        kOsLeaf = 6,      // the record of the leaf the search found
+        kOsEnd = 7        // no more segments
+    };
 
     template<int GT> struct Gen
     {
@@ -338,38 +346,21 @@ namespace skg
         int state;
         int i, j, k;  // Cartesian cell indices; octree, Voronoi, tetra: i = current cell
                       // (tetra: j = entering face, -1 if unknown)
-        double b0, b1, b2, b3, b4, b5, bn;  // octree: bounds and density of the current cell
-        int f0, f1, f2, f3, f4, f5;         // octree: first neighbour of the current cell per wall
-        int hint;  // Voronoi, tetra: a cell that probably contains the start point (-1: none);
-                   // after the first step, the start cell
-
-        __device__ __forceinline__ void load_leaf(const GridView& g, int c)
-        {
-            const OctLeaf& L = g.leaf[c];
-            b0 = L.lo[0];
-            b1 = L.lo[1];
-            b2 = L.lo[2];
-            b3 = L.hi[0];
-            b4 = L.hi[1];
-            b5 = L.hi[2];
-            bn = L.dens;
-            f0 = L.first[0];
-            f1 = L.first[1];
-            f2 = L.first[2];
-            f3 = L.first[3];
-            f4 = L.first[4];
-            f5 = L.first[5];
-        }
-
-        __device__ __forceinline__ int leaf_cell(const GridView& g, double x, double y, double z) const
-        {
-            return leaf_cell_at(g, x, y, z);
-        }
+        double b0, b1, b2, bn;  // octree: the current cell's wall ahead on each axis, its density
+        int f0, f1, f2;         // octree: the first neighbour across each of those walls
+        int hint;  // octree, Voronoi, tetra: a cell that probably contains the start point (-1:
+                   // none); after the first step, the start cell
+        // octree step machine (advance): the next step, the crossed wall, a candidate cell,
+        // the neighbour-list cursor and end, the search node, and the flags for the start
+        // search and the nextafter retry (TreeSpatialGrid.cpp:210-215)
+        int ost, wall, cand, q, qe, node;
+        bool locating, na;
 
         __device__ __forceinline__ void start(double r0, double r1, double r2, double k0, double k1, double k2,
                                               int start_hint = -1)
         {
             hint = start_hint;
+            ost = kOsStart;
             rx = r0;
             ry = r1;
             rz = r2;
@@ -445,10 +436,350 @@ namespace skg
             return true;
         }
 
+        // ---------------------------------------------------------------- octree step machine
+        // The record fields a step reads, from the 112 bytes it loads (an OctLeaf, a TopNode or
+        // neighbour-list entries; the tables are padded so that 112 bytes past any entry stay
+        // inside them)
+        struct RawRec
+        {
+            // the 112 bytes as 14 doubles in registers (plain scalars: no member is selected by
+            // reference, which would leave the record in scratch memory)
+            double q0, q1, q2, q3, q4, q5, q6, q7, q8, q9, q10, q11, q12, q13;
+            __device__ __forceinline__ void load(const char* addr)
+            {
+                const double* p = reinterpret_cast<const double*>(addr);
+                q0 = p[0];
+                q1 = p[1];
+                q2 = p[2];
+                q3 = p[3];
+                q4 = p[4];
+                q5 = p[5];
+                q6 = p[6];
+                q7 = p[7];
+                q8 = p[8];
+                q9 = p[9];
+                q10 = p[10];
+                q11 = p[11];
+                q12 = p[12];
+                q13 = p[13];
+            }
+            // double k (bytes 8k..8k+7), k a compile-time constant
+            __device__ __forceinline__ double d(int k) const
+            {
+                switch (k)
+                {
+                    case 0: return q0;
+                    case 1: return q1;
+                    case 2: return q2;
+                    case 3: return q3;
+                    case 4: return q4;
+                    case 5: return q5;
+                    case 6: return q6;
+                    case 7: return q7;
+                    case 8: return q8;
+                    case 9: return q9;
+                    case 10: return q10;
+                    case 11: return q11;
+                    case 12: return q12;
+                    default: return q13;
+                }
+            }
+            // int k (bytes 4k..4k+3), k a compile-time constant
+            __device__ __forceinline__ int i32(int k) const
+            {
+                const double v = d(k >> 1);
+                return (k & 1) ? __double2hiint(v) : __double2loint(v);
+            }
+            // the entry r (run time) of the first four ints
+            __device__ __forceinline__ int first4(int r) const
+            {
+                const int e0 = __double2loint(q0), e1 = __double2hiint(q0), e2 = __double2loint(q1),
+                          e3 = __double2hiint(q1);
+                const int e23 = r == 2 ? e2 : e3;
+                const int e123 = r == 1 ? e1 : e23;
+                return r == 0 ? e0 : e123;
+            }
+            // OctLeaf::off[wl] (ints 22..27) and the end of the wall's list (the next wall's
+            // start, or off_end, int 15, after the last wall), wl at run time
+            __device__ __forceinline__ int off_at(int wl) const
+            {
+                const int o0 = __double2loint(q11), o1 = __double2hiint(q11), o2 = __double2loint(q12),
+                          o3 = __double2hiint(q12), o4 = __double2loint(q13), o5 = __double2hiint(q13);
+                return pick6(wl, o0, o1, o2, o3, o4, o5);
+            }
+            __device__ __forceinline__ int off_end_at(int wl) const
+            {
+                const int o1 = __double2hiint(q11), o2 = __double2loint(q12), o3 = __double2hiint(q12),
+                          o4 = __double2loint(q13), o5 = __double2hiint(q13), oe = __double2hiint(q7);
+                return pick6(wl, o1, o2, o3, o4, o5, oe);
+            }
+            __device__ __forceinline__ static int pick6(int w, int a, int b, int c, int d, int e, int f)
+            {
+                const int ef = w == 4 ? e : f;
+                const int def = w == 3 ? d : ef;
+                const int cdef = w == 2 ? c : def;
+                const int bcdef = w == 1 ? b : cdef;
+                return w == 0 ? a : bcdef;
+            }
+        };
+        // OctLeaf: lo/hi doubles 0..5, dens double 6, node int 14, off_end int 15, first ints
+        // 16..21, off ints 22..27.  TopNode: split point doubles 0..2, child int 6, cell int 7.
+        static_assert(offsetof(OctLeaf, dens) == 48 && offsetof(OctLeaf, off_end) == 60 && offsetof(OctLeaf, first) == 64
+                          && offsetof(OctLeaf, off) == 88,
+                      "leaf record layout the step machine reads");
+        static_assert(offsetof(TopNode, child) == 24 && offsetof(TopNode, cell) == 28, "node layout the step machine reads");
+
+        // the leaf c becomes the current cell: of its record, the crossing needs only the wall
+        // ahead on each axis (the direction does not change along a path) -- its coordinate
+        // (TreeSpatialGrid.cpp:157-159) and its first neighbour -- and the density
+        __device__ __forceinline__ void adopt(const RawRec& R, int c)
+        {
+            i = c;
+            b0 = (kx < 0.0) ? R.d(0) : R.d(3);
+            b1 = (ky < 0.0) ? R.d(1) : R.d(4);
+            b2 = (kz < 0.0) ? R.d(2) : R.d(5);
+            bn = R.d(6);
+            f0 = (kx < 0.0) ? R.i32(16) : R.i32(17);
+            f1 = (ky < 0.0) ? R.i32(18) : R.i32(19);
+            f2 = (kz < 0.0) ? R.i32(20) : R.i32(21);
+            ost = kOsCross;
+        }
+
+        __device__ __forceinline__ static bool closed_contains(const RawRec& R, double x, double y, double z)
+        {
+            return x >= R.d(0) && x <= R.d(3) && y >= R.d(1) && y <= R.d(4) && z >= R.d(2) && z <= R.d(5);
+        }
+
+        // TreeNode::leafChild (TreeSpatialGrid.cpp:190-219, OctTreeNode.cpp:37-42) from node nd
+        // through the LDS-staged levels, with no global load: the cell of the leaf containing r
+        // (-1 outside the root's closed box), or -2 with the first node below the staged levels
+        // in `node`, whose record a later step loads
+        __device__ __forceinline__ int search_lds(const GridView& g, int nd)
+        {
+            if (nd == 0
+                && !(rx >= g.box[0] && rx <= g.box[3] && ry >= g.box[1] && ry <= g.box[4] && rz >= g.box[2]
+                     && rz <= g.box[5]))
+                return -1;
+            while (nd < g.topN)
+            {
+                const TopNode t = skg_top_lds[nd];
+                if (t.child < 0) return t.cell;
+                nd = t.child + (rx < t.sx ? 0 : 1) + (ry < t.sy ? 0 : 2) + (rz < t.sz ? 0 : 4);
+            }
+            node = nd;
+            return -2;
+        }
+
+        // a search result c (-2: the search continues at `node`).  Starting a path, c is the
+        // start cell.  After a crossing, c is the next cell unless it is the current one: then
+        // the position moves by one ulp toward the direction and the search is made once more;
+        // outside, or still the current cell, the path ends (TreeSpatialGrid.cpp:204-218).
+        __device__ __forceinline__ void resolve(const GridView& g, int c)
+        {
+            if (c == -2)
+            {
+                ost = kOsNode;
+                return;
+            }
+            if (!locating && c == i && !na)
+            {
+                na = true;
+                rx = nextafter(rx, (kx < 0.) ? -DBL_MAX : DBL_MAX);
+                ry = nextafter(ry, (ky < 0.) ? -DBL_MAX : DBL_MAX);
+                rz = nextafter(rz, (kz < 0.) ? -DBL_MAX : DBL_MAX);
+                c = search_lds(g, 0);
+                if (c == -2)
+                {
+                    ost = kOsNode;
+                    return;
+                }
+            }
+            // the same members are written on both outcomes (as values), so that the compiler
+            // keeps them in registers
+            const bool end = c < 0 || (!locating && c == i);
+            ost = end ? kOsEnd : kOsLeaf;
+            state = end ? kOutside : state;
+            cand = end ? cand : c;
+        }
+
+        // One step of the octree generator (TreeSpatialGrid::MySegmentGenerator::next,
+        // TreeSpatialGrid.cpp:143-228, split so that a step makes at most one dependent global
+        // load): 1 with the segment (m, ds, n) when the step completed one, 0 when it only
+        // progressed (a start, a neighbour-list entry, a search level), -1 at the end of the path.
+        // The sequence of segments equals next()'s.
+        __device__ __forceinline__ int advance_octree(const GridView& g, int& m, double& ds, double& n)
+        {
+            int out = 0;
+            int act = ost;
+            const char* addr = nullptr;
+            // phase 1: the step's own computation and the address of the record it needs
+            if (ost == kOsEnd) return -1;
+            if (ost == kOsStart)
+            {
+                double cumds;
+                if (!move_inside(g, cumds))
+                {
+                    ost = kOsEnd;
+                    return -1;
+                }
+                locating = true;
+                na = false;
+                if (cumds > 0.)
+                {
+                    m = -1;
+                    ds = cumds;
+                    n = 0.;
+                    out = 1;
+                }
+                if (hint >= 0)
+                    act = kOsStart;  // the hint's record follows
+                else
+                {
+                    resolve(g, search_lds(g, 0));
+                    act = ost;
+                }
+            }
+            else if (ost == kOsCross)
+            {
+                const int cur = i;
+                // the exit distance along each axis to the wall ahead (b0..b2, from adopt); the
+                // quotients are computed for every axis and the tiny-component ones replaced,
+                // which leaves the values of TreeSpatialGrid.cpp:157-165 and needs no branch
+                const double qx = (b0 - rx) / kx, qy = (b1 - ry) / ky, qz = (b2 - rz) / kz;
+                const double dsx = (fabs(kx) > 1e-15) ? qx : DBL_MAX;
+                const double dsy = (fabs(ky) > 1e-15) ? qy : DBL_MAX;
+                const double dsz = (fabs(kz) > 1e-15) ? qz : DBL_MAX;
+                // ties: x if dsx <= dsy && dsx <= dsz, y if dsy <= dsx && dsy <= dsz, else z
+                const bool ax = dsx <= dsy && dsx <= dsz;
+                const bool ay = !ax && dsy <= dsx && dsy <= dsz;
+                const double d = pick3(ax, ay, dsx, dsy, dsz);
+                wall = pick3(ax, ay, (kx < 0.0) ? 0 : 1, (ky < 0.0) ? 2 : 3, (kz < 0.0) ? 4 : 5);
+                const int f = pick3(ax, ay, f0, f1, f2);
+                double step = d + g.eps;
+                rx += kx * step;
+                ry += ky * step;
+                rz += kz * step;
+                m = cur;
+                ds = d;
+                n = bn;
+                out = 1;
+                locating = false;
+                na = false;
+                if (f >= 0)
+                {
+                    cand = f;
+                    act = kOsCross;  // the first neighbour's record follows
+                }
+                else
+                {
+                    resolve(g, search_lds(g, 0));
+                    act = ost;
+                }
+            }
+            if (act == kOsEnd) return out ? out : -1;
+            switch (act)
+            {
+                case kOsStart: addr = reinterpret_cast<const char*>(g.leaf + hint); break;
+                case kOsCross:
+                case kOsListRec:
+                case kOsLeaf: addr = reinterpret_cast<const char*>(g.leaf + cand); break;
+                case kOsListStart: addr = reinterpret_cast<const char*>(g.leaf + i); break;
+                case kOsListId: addr = reinterpret_cast<const char*>(g.nbc + (q & ~3)); break;
+                default: addr = reinterpret_cast<const char*>(g.top + node); break;  // kOsNode
+            }
+            // phase 2: the one load of this step
+            RawRec R;
+            R.load(addr);
+            // phase 3: what the record decides
+            switch (act)
+            {
+                case kOsStart:
+                    // the start hint, when the point lies strictly inside its box: every split
+                    // plane of its ancestors then sends leafChild's descent toward it, so it is
+                    // the leaf the top-down search returns
+                    if (rx > R.d(0) && rx < R.d(3) && ry > R.d(1) && ry < R.d(4) && rz > R.d(2) && rz < R.d(5))
+                        adopt(R, hint);
+                    else
+                        resolve(g, search_lds(g, 0));
+                    break;
+                case kOsCross:
+                    // the first (largest-overlap) neighbour on the crossed wall, when its closed
+                    // box contains the new position; else the rest of the wall's list
+                    if (closed_contains(R, rx, ry, rz))
+                    {
+                        if (cand != i)
+                            adopt(R, cand);
+                        else
+                            resolve(g, cand);
+                    }
+                    else
+                        ost = kOsListStart;
+                    break;
+                case kOsListStart:
+                    q = R.off_at(wall) + 1;
+                    qe = R.off_end_at(wall);
+                    if (q < qe)
+                        ost = kOsListId;
+                    else
+                        resolve(g, search_lds(g, 0));
+                    break;
+                case kOsListId:
+                    cand = R.first4(q & 3);
+                    ost = kOsListRec;
+                    break;
+                case kOsListRec:
+                    if (closed_contains(R, rx, ry, rz))
+                    {
+                        if (cand != i)
+                            adopt(R, cand);
+                        else
+                            resolve(g, cand);
+                    }
+                    else if (++q < qe)
+                        ost = kOsListId;
+                    else
+                        resolve(g, search_lds(g, 0));
+                    break;
+                case kOsNode:
+                {
+                    const int child = R.i32(6);
+                    if (child < 0)
+                        resolve(g, R.i32(7));
+                    else
+                    {
+                        const int nd = child + (rx < R.d(0) ? 0 : 1) + (ry < R.d(1) ? 0 : 2) + (rz < R.d(2) ? 0 : 4);
+                        resolve(g, search_lds(g, nd));
+                    }
+                    break;
+                }
+                default:  // kOsLeaf
+                    adopt(R, cand);
+                    if (locating) hint = i;
+                    break;
+            }
+            if (ost == kOsEnd && !out) return -1;
+            return out;
+        }
+
+        // one advance() per call for the octree (next() for the other grids)
+        __device__ __forceinline__ int advance(const GridView& g, int& m, double& ds, double& n)
+        {
+            if constexpr (GT == kOctree)
+                return advance_octree(g, m, ds, n);
+            else
+                return next(g, m, ds, n) ? 1 : -1;
+        }
+
         // Returns false when the path has no more segments; otherwise sets the cell m, the
         // segment length ds and the cell's number density n (0 for m = -1).
         __device__ __forceinline__ bool next(const GridView& g, int& m, double& ds, double& n)
         {
+            if constexpr (GT == kOctree)
+            {
+                int r;
+                while ((r = advance_octree(g, m, ds, n)) == 0) {}
+                return r > 0;
+            }
             if (state == kOutside) return false;
             if (state == kUnknown)
             {
@@ -459,29 +790,6 @@ namespace skg
                     i = locate_clip(g.xb, g.nx + 1, rx);
                     j = locate_clip(g.yb, g.ny + 1, ry);
                     k = locate_clip(g.zb, g.nz + 1, rz);
-                }
-                else if (GT == kOctree)
-                {
-                    // the start hint, when the point lies strictly inside its box: every split
-                    // plane of its ancestors then sends leafChild's descent toward it, so it is the
-                    // leaf the top-down search returns (its record is the one needed anyway)
-                    i = -1;
-                    if (hint >= 0)
-                    {
-                        load_leaf(g, hint);
-                        if (rx > b0 && rx < b3 && ry > b1 && ry < b4 && rz > b2 && rz < b5) i = hint;
-                    }
-                    if (i < 0)
-                    {
-                        i = leaf_cell(g, rx, ry, rz);
-                        if (i < 0)
-                        {
-                            state = kOutside;
-                            return false;
-                        }
-                        load_leaf(g, i);
-                    }
-                    hint = i;
                 }
                 else if (GT == kVoronoi)
                 {
@@ -562,107 +870,7 @@ namespace skg
                 return next_voronoi(g, m, ds, n);
             else if constexpr (GT == kTetra)
                 return next_tetra(g, m, ds, n);
-            else
-            {
-                const int cur = i;
-                double xnext = (kx < 0.0) ? b0 : b3;
-                double ynext = (ky < 0.0) ? b1 : b4;
-                double znext = (kz < 0.0) ? b2 : b5;
-                double dsx = (fabs(kx) > 1e-15) ? (xnext - rx) / kx : DBL_MAX;
-                double dsy = (fabs(ky) > 1e-15) ? (ynext - ry) / ky : DBL_MAX;
-                double dsz = (fabs(kz) > 1e-15) ? (znext - rz) / kz : DBL_MAX;
-                double d;
-                int wall;
-                if (dsx <= dsy && dsx <= dsz)
-                {
-                    d = dsx;
-                    wall = (kx < 0.0) ? 0 : 1;
-                }
-                else if (dsy <= dsx && dsy <= dsz)
-                {
-                    d = dsy;
-                    wall = (ky < 0.0) ? 2 : 3;
-                }
-                else
-                {
-                    d = dsz;
-                    wall = (kz < 0.0) ? 4 : 5;
-                }
-                double step = d + g.eps;
-                rx += kx * step;
-                ry += ky * step;
-                rz += kz * step;
-                m = cur;
-                ds = d;
-                n = bn;
-                // first neighbour on the crossed wall whose closed box contains r.  The first
-                // (largest-overlap) neighbour's whole record is read in one go, so the common
-                // crossing costs one dependent cache-line access and no reload.
-                const int f = wall == 0 ? f0 : wall == 1 ? f1 : wall == 2 ? f2 : wall == 3 ? f3 : wall == 4 ? f4 : f5;
-                int nxt = -1;
-                bool loaded = false;
-                if (f >= 0)
-                {
-                    const OctLeaf& F = g.leaf[f];
-                    const double l0 = F.lo[0], l1 = F.lo[1], l2 = F.lo[2], h0 = F.hi[0], h1 = F.hi[1], h2 = F.hi[2];
-                    const double dn = F.dens;
-                    const int n0 = F.first[0], n1 = F.first[1], n2 = F.first[2], n3 = F.first[3], n4 = F.first[4],
-                              n5 = F.first[5];
-                    if (rx >= l0 && rx <= h0 && ry >= l1 && ry <= h1 && rz >= l2 && rz <= h2)
-                    {
-                        nxt = f;
-                        if (f != cur)
-                        {
-                            b0 = l0;
-                            b1 = l1;
-                            b2 = l2;
-                            b3 = h0;
-                            b4 = h1;
-                            b5 = h2;
-                            bn = dn;
-                            f0 = n0;
-                            f1 = n1;
-                            f2 = n2;
-                            f3 = n3;
-                            f4 = n4;
-                            f5 = n5;
-                            loaded = true;
-                        }
-                    }
-                    else
-                    {
-                        const OctLeaf& R = g.leaf[cur];
-                        int e0 = R.off[wall] + 1;
-                        int e1 = wall < 5 ? R.off[wall + 1] : R.off_end;
-                        for (int q = e0; q < e1; ++q)
-                        {
-                            int c = g.nbc[q];
-                            const OctLeaf& C = g.leaf[c];
-                            if (rx >= C.lo[0] && rx <= C.hi[0] && ry >= C.lo[1] && ry <= C.hi[1] && rz >= C.lo[2]
-                                && rz <= C.hi[2])
-                            {
-                                nxt = c;
-                                break;
-                            }
-                        }
-                    }
-                }
-                if (nxt < 0) nxt = leaf_cell(g, rx, ry, rz);
-                if (nxt == cur)
-                {
-                    rx = nextafter(rx, (kx < 0.) ? -DBL_MAX : DBL_MAX);
-                    ry = nextafter(ry, (ky < 0.) ? -DBL_MAX : DBL_MAX);
-                    rz = nextafter(rz, (kz < 0.) ? -DBL_MAX : DBL_MAX);
-                    nxt = leaf_cell(g, rx, ry, rz);
-                    loaded = false;
-                }
-                if (nxt < 0 || nxt == cur)
-                    state = kOutside;
-                else if (!loaded)
-                    load_leaf(g, nxt);
-                i = nxt;
-                return true;
-            }
+            return false;  // the octree steps through advance_octree
         }
 
         // exit through the nearest bisector plane or domain wall (VoronoiMeshSnapshot.cpp:1119-1210)

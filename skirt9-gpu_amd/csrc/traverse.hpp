@@ -355,6 +355,7 @@ namespace skg
         // search and the nextafter retry (TreeSpatialGrid.cpp:210-215)
         int ost, wall, cand, q, qe, node;
         bool locating, na;
+        float rkx, rky, rkz;  // FP32 reciprocals of the direction, for the exit-axis estimates
 
         __device__ __forceinline__ void start(double r0, double r1, double r2, double k0, double k1, double k2,
                                               int start_hint = -1)
@@ -624,6 +625,9 @@ namespace skg
                 }
                 locating = true;
                 na = false;
+                rkx = 1.f / static_cast<float>(kx);
+                rky = 1.f / static_cast<float>(ky);
+                rkz = 1.f / static_cast<float>(kz);
                 if (cumds > 0.)
                 {
                     m = -1;
@@ -642,17 +646,41 @@ namespace skg
             else if (ost == kOsCross)
             {
                 const int cur = i;
-                // the exit distance along each axis to the wall ahead (b0..b2, from adopt); the
-                // quotients are computed for every axis and the tiny-component ones replaced,
-                // which leaves the values of TreeSpatialGrid.cpp:157-165 and needs no branch
-                const double qx = (b0 - rx) / kx, qy = (b1 - ry) / ky, qz = (b2 - rz) / kz;
-                const double dsx = (fabs(kx) > 1e-15) ? qx : DBL_MAX;
-                const double dsy = (fabs(ky) > 1e-15) ? qy : DBL_MAX;
-                const double dsz = (fabs(kz) > 1e-15) ? qz : DBL_MAX;
-                // ties: x if dsx <= dsy && dsx <= dsz, y if dsy <= dsx && dsy <= dsz, else z
-                const bool ax = dsx <= dsy && dsx <= dsz;
-                const bool ay = !ax && dsy <= dsx && dsy <= dsz;
-                const double d = pick3(ax, ay, dsx, dsy, dsz);
+                // The exit distance along each axis to the wall ahead (b0..b2, from adopt):
+                // (wall - r) / k, or DBL_MAX for a component within 1e-15 of zero, and the
+                // smallest wins with the ties of TreeSpatialGrid.cpp:157-181 (x if dsx <= dsy &&
+                // dsx <= dsz, y if dsy <= dsx && dsy <= dsz, else z).  FP32 estimates (relative
+                // error below 3e-7) pick the axis when the two smallest differ by more than 1e-6
+                // of their size: the exact FP64 quotients then order the same way and differ,
+                // so only the winner's is computed (one FP64 division instead of three).  Close
+                // estimates, infinities and NaNs take the three exact quotients.
+                const double dx = b0 - rx, dy = b1 - ry, dz = b2 - rz;
+                const bool tx = fabs(kx) > 1e-15, ty = fabs(ky) > 1e-15, tz = fabs(kz) > 1e-15;
+                const float ex = tx ? static_cast<float>(dx) * rkx : INFINITY;
+                const float ey = ty ? static_cast<float>(dy) * rky : INFINITY;
+                const float ez = tz ? static_cast<float>(dz) * rkz : INFINITY;
+                const bool fx = ex <= ey && ex <= ez;
+                const bool fy = !fx && ey <= ez;
+                const float e1 = pick3(fx, fy, ex, ey, ez);
+                const float e2 = pick3(fx, fy, fminf(ey, ez), fminf(ex, ez), fminf(ex, ey));
+                const bool clear = e2 - e1 > 1e-6f * (fabsf(e1) + fabsf(e2));  // false for inf / NaN
+                bool ax, ay;
+                double d;
+                if (clear)
+                {
+                    ax = fx;
+                    ay = fy;
+                    d = pick3(ax, ay, dx, dy, dz) / pick3(ax, ay, kx, ky, kz);
+                }
+                else
+                {
+                    const double dsx = tx ? dx / kx : DBL_MAX;
+                    const double dsy = ty ? dy / ky : DBL_MAX;
+                    const double dsz = tz ? dz / kz : DBL_MAX;
+                    ax = dsx <= dsy && dsx <= dsz;
+                    ay = !ax && dsy <= dsx && dsy <= dsz;
+                    d = pick3(ax, ay, dsx, dsy, dsz);
+                }
                 wall = pick3(ax, ay, (kx < 0.0) ? 0 : 1, (ky < 0.0) ? 2 : 3, (kz < 0.0) ? 4 : 5);
                 const int f = pick3(ax, ay, f0, f1, f2);
                 double step = d + g.eps;

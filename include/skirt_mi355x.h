@@ -114,6 +114,17 @@ int skg_dust(skg_engine* e, int32_t nlambda, const double* lambda, const double*
 int skg_trace_paths(skg_engine* e, const double* rays, int64_t n, int64_t cap, int64_t* off, int32_t* m,
                     double* ds, int64_t* total);
 
+/* skg_trace_paths without any device allocation, for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  rays (n x 6), off (n + 1), m and ds (cap) are DEVICE memory
+ * owned by the caller; *total (host) receives the number of segments, also when it exceeds cap
+ * (SKG_ERR_CAPACITY).  The offsets are scanned on the device with a fixed 128 KB scratch array
+ * per device that the library holds until the process ends: at most 16,777,216 rays per call
+ * (larger n fails with SKG_ERR_CAPACITY), and calls that overlap on one GPU race on it.
+ * Synchronous on the engine's stream. */
+int skg_trace_paths_noalloc(skg_engine* e, const double* rays, int64_t n, int64_t cap, int64_t* off, int32_t* m,
+                            double* ds, int64_t* total);
+
 /* Optical depths at wavelength bin `ell` for n rays (host memory):
  *   tau_path[n]  total extinction optical depth of the complete path (the last cumulative
  *                value of MediumSystem::setExtinctionOpticalDepths, MediumSystem.cpp:1785-1810)

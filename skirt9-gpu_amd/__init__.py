@@ -64,7 +64,7 @@ def lib():
 EXPORTED = [
     "skg_create", "skg_destroy", "skg_last_error", "skg_grid_cartesian", "skg_grid_octree", "skg_grid_voronoi",
     "skg_grid_tetra", "skg_num_cells",
-    "skg_medium", "skg_dust", "skg_trace_paths", "skg_trace_tau", "skg_frame_add", "skg_frame_remove_all",
+    "skg_medium", "skg_dust", "skg_trace_paths", "skg_trace_paths_noalloc", "skg_trace_tau", "skg_frame_add", "skg_frame_remove_all",
     "skg_frame_clear", "skg_frame_read", "skg_source_point", "skg_source_particles", "skg_options", "skg_run_packets",
     "skg_run_packets_async", "skg_sync", "skg_counters", "skg_counters_n", "skg_tune", "skg_last_kernel_ms",
     "skg_disk_density", "skg_octree_build", "skg_octree_from_flags", "skg_octree_info", "skg_octree_export",
@@ -170,6 +170,15 @@ class Engine:
         self._chk(lib().skg_dust(self._h, C.c_int32(self.nlambda), *[_p(x, f64p) for x in a]))
 
     # ---- path service
+    def trace_paths_noalloc(self, rays_ptr, n, cap, off_ptr, m_ptr, ds_ptr):
+        """skg_trace_paths_noalloc on caller-owned DEVICE buffers (integer addresses): rays
+        (n x 6 f64), off (n + 1 i64), m (cap i32), ds (cap f64); returns the segment total."""
+        total = C.c_int64(0)
+        self._chk(lib().skg_trace_paths_noalloc(self._h, C.c_void_p(rays_ptr), C.c_int64(n), C.c_int64(cap),
+                                                C.c_void_p(off_ptr), C.c_void_p(m_ptr), C.c_void_p(ds_ptr),
+                                                C.byref(total)))
+        return total.value
+
     def trace_paths(self, rays, cap=None):
         rays = _f64(rays).reshape(-1, 6)
         n = len(rays)

@@ -71,6 +71,103 @@ namespace skg
         }
     }
 
+    // ------------------------------------------------------------------ path service without allocation
+    // skg_trace_paths_noalloc: the segment counts go straight into the caller's offset array
+    // and are turned into offsets by a three-kernel scan whose block sums live in `scratch`.
+    constexpr int kScanItems = 1024;          // offsets per scan block (256 threads x 4)
+    constexpr int kScanMaxBlocks = 16384;     // 16.7 M rays per call at most
+    // the scan's block sums and, after the last one, the total; shared by every call on a device
+    static __device__ __align__(256) int64_t scratch[kScanMaxBlocks + 1];
+
+    template<int GT> __global__ void trace_count_off(GridView g, const double* __restrict__ rays, int64_t n, int64_t* off)
+    {
+        int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+        stage_top(g);
+        if (t == 0) off[0] = 0;
+        if (t >= n) return;
+        const double* r = rays + 6 * t;
+        Gen<GT> gen;
+        gen.start(r[0], r[1], r[2], r[3], r[4], r[5]);
+        int m;
+        double ds, nd;
+        int64_t c = 0;
+        while (gen.next(g, m, ds, nd)) ++c;
+        off[t + 1] = c;
+    }
+
+    // inclusive scan of off[1..n] within blocks of kScanItems; block b's sum into sums[b]
+    __global__ void __launch_bounds__(256) scan_block_sums(int64_t* off, int64_t n, int64_t* sums)
+    {
+        __shared__ int64_t part[256];
+        const int64_t base = 1 + static_cast<int64_t>(blockIdx.x) * kScanItems + 4 * threadIdx.x;
+        int64_t v[4], acc = 0;
+        for (int q = 0; q < 4; ++q)
+        {
+            const int64_t i = base + q;
+            v[q] = i <= n ? off[i] : 0;
+            acc += v[q];
+        }
+        part[threadIdx.x] = acc;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1)
+        {
+            const int64_t x = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+            __syncthreads();
+            part[threadIdx.x] += x;
+            __syncthreads();
+        }
+        int64_t run = part[threadIdx.x] - acc;  // exclusive prefix of this thread's items
+        for (int q = 0; q < 4; ++q)
+        {
+            run += v[q];
+            const int64_t i = base + q;
+            if (i <= n) off[i] = run;
+        }
+        if (threadIdx.x == 255) sums[blockIdx.x] = part[255];
+    }
+
+    // exclusive scan of the nb block sums in place (one workgroup of 1024 threads, up to 16
+    // sums each); the total after them
+    __global__ void __launch_bounds__(1024) scan_sums(int64_t* sums, int nb)
+    {
+        __shared__ int64_t part[1024];
+        const int per = (nb + 1023) / 1024;
+        const int b0 = threadIdx.x * per;
+        int64_t acc = 0;
+        for (int q = 0; q < per; ++q)
+            if (b0 + q < nb) acc += sums[b0 + q];
+        part[threadIdx.x] = acc;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1)
+        {
+            const int64_t x = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+            __syncthreads();
+            part[threadIdx.x] += x;
+            __syncthreads();
+        }
+        int64_t run = part[threadIdx.x] - acc;
+        for (int q = 0; q < per; ++q)
+        {
+            if (b0 + q >= nb) break;
+            const int64_t v = sums[b0 + q];
+            sums[b0 + q] = run;
+            run += v;
+        }
+        if (threadIdx.x == 1023) sums[nb] = part[1023];
+    }
+
+    // adds each block's exclusive prefix to its offsets
+    __global__ void __launch_bounds__(256) add_block_offsets(int64_t* off, int64_t n, const int64_t* sums)
+    {
+        const int64_t base = 1 + static_cast<int64_t>(blockIdx.x) * kScanItems + 4 * threadIdx.x;
+        const int64_t add = sums[blockIdx.x];
+        for (int q = 0; q < 4; ++q)
+        {
+            const int64_t i = base + q;
+            if (i <= n) off[i] += add;
+        }
+    }
+
     // copies the cell densities into the cell records (octree leaves, Voronoi cells, tetrahedra)
     template<typename Rec> __global__ void fill_record_density(Rec* rec, const double* dens, int64_t n)
     {
@@ -253,9 +350,34 @@ namespace skg
 // ==================================================================== host side
 using namespace skg;
 
+// The device address of `scratch` on each device, looked up once per device at the first
+// engine creation on it (hipGetSymbolAddress with that device current); host code uses this
+// address, never the array's name, wherever it needs the array's memory.
+namespace
+{
+    std::mutex g_scratchMu;
+    std::vector<int64_t*> g_scratch;  // indexed by device; null until looked up
+
+    hipError_t scratchAddress(int device, int64_t** out)
+    {
+        std::lock_guard<std::mutex> lk(g_scratchMu);
+        if (static_cast<int>(g_scratch.size()) <= device) g_scratch.resize(device + 1, nullptr);
+        if (!g_scratch[device])
+        {
+            void* p = nullptr;
+            hipError_t st = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (st != hipSuccess) return st;
+            g_scratch[device] = static_cast<int64_t*>(p);
+        }
+        *out = g_scratch[device];
+        return hipSuccess;
+    }
+}
+
 struct skg_engine
 {
     int device = 0;
+    int64_t* scan = nullptr;  // this device's `scratch` (the no-allocation path service)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
@@ -634,6 +756,7 @@ int skg_create(int device, skg_engine** out)
     if (st == hipSuccess) e->counters = e->ctl + kCtlCnt;
     if (st == hipSuccess) st = hipHostMalloc(&e->hostCtl, 16 * sizeof(unsigned long long), hipHostMallocDefault);
     if (st == hipSuccess) st = hipMalloc(&e->frameGeom, kMaxFrames * sizeof(FrameGeom));
+    if (st == hipSuccess) st = scratchAddress(device, &e->scan);
     hipDeviceProp_t prop;
     if (st == hipSuccess) st = hipGetDeviceProperties(&prop, device);
     if (st != hipSuccess)
@@ -1030,6 +1153,52 @@ int skg_trace_paths(skg_engine* e, const double* rays, int64_t n, int64_t cap, i
     hipFree(dOff);
     if (dM) hipFree(dM);
     if (dDs) hipFree(dDs);
+    return SKG_OK;
+}
+
+int skg_trace_paths_noalloc(skg_engine* e, const double* rays, int64_t n, int64_t cap, int64_t* off, int32_t* m,
+                            double* ds, int64_t* total)
+{
+    if (!e) return SKG_ERR_ARGUMENT;
+    const int64_t nb = (n + kScanItems - 1) / kScanItems;
+    if (nb > kScanMaxBlocks)
+        return fail(e, SKG_ERR_CAPACITY, "skg_trace_paths_noalloc: at most " +
+                                             std::to_string(static_cast<int64_t>(kScanMaxBlocks) * kScanItems) +
+                                             " rays per call");
+    if (n < 0 || cap < 0 || (n && (!rays || !off)))
+        return fail(e, SKG_ERR_ARGUMENT, "skg_trace_paths_noalloc: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    int st = checkTraceState(e);
+    if (st) return st;
+    if (total) *total = 0;
+    if (n == 0) return SKG_OK;
+    GridView g = e->grid;
+    g.dens = e->dens;
+    const int blocks = static_cast<int>((n + 255) / 256);
+    dispatchGrid(e, [&](auto gt) {
+        hipLaunchKernelGGL((trace_count_off<decltype(gt)::value>), dim3(blocks), dim3(256), 0, e->stream, g, rays, n, off);
+        return 0;
+    });
+    hipLaunchKernelGGL(scan_block_sums, dim3(static_cast<unsigned>(nb)), dim3(256), 0, e->stream, off, n, e->scan);
+    hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, e->stream, e->scan, static_cast<int>(nb));
+    hipLaunchKernelGGL(add_block_offsets, dim3(static_cast<unsigned>(nb)), dim3(256), 0, e->stream, off, n, e->scan);
+    HIPCHK(e, hipGetLastError());
+    int64_t t = 0;
+    HIPCHK(e, hipMemcpyAsync(&t, e->scan + nb, sizeof(int64_t), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(e, hipStreamSynchronize(e->stream));
+    if (total) *total = t;
+    if (t > cap || (t && (!m || !ds)))
+        return fail(e, SKG_ERR_CAPACITY, "skg_trace_paths_noalloc: capacity " + std::to_string(cap) + " < " + std::to_string(t));
+    if (t)
+    {
+        dispatchGrid(e, [&](auto gt) {
+            hipLaunchKernelGGL((trace_write<decltype(gt)::value>), dim3(blocks), dim3(256), 0, e->stream, g, rays, n, off, m, ds);
+            return 0;
+        });
+        HIPCHK(e, hipGetLastError());
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+    }
     return SKG_OK;
 }
 

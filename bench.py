@@ -328,7 +328,8 @@ def main():
     ap.add_argument("--fp", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--cpu-packets", type=float, default=4e5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--threshold", type=int, default=32)
+    ap.add_argument("--threshold", type=int, default=16,
+                    help="persistent engine: waiting lanes of a wavefront that start a transition round")
     ap.add_argument("--mode", choices=["pipeline", "persistent"], default="persistent")
     ap.add_argument("--pool", type=int, default=1 << 21, help="packet slots in flight (pipeline)")
     ap.add_argument("--trace-wgs", type=int, default=0)

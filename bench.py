@@ -38,6 +38,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
+CPU_PACKETS_DEFAULT = 4e5  # packets of the reference CPU sample (octree and Cartesian workloads)
 BYTES_PER_SEGMENT = 64.0  # octree: node bounds 48 B + leaf/cell 4 B + neighbour entry 4 B + density 8 B
 BYTES_PER_ATOMIC = 8.0
 BYTES_PER_REQUEST = 96.0  # per traversal: request 32 B + packet position/direction/bin 52 B + result 12 B
@@ -63,9 +64,10 @@ def host_threads():
     return max(1, min(16, n))
 
 
-def cpu_baseline_reference(tau_v, nlambda, lam_range, packets, threads, cartesian=False):
-    """Times the reference SKIRT build on the same workload (its own grid and densities);
-    packets/s over the primary emission phase, as logged by the reference."""
+def cpu_baseline_reference(tau_v, nlambda, lam_range, packets, threads, cartesian=False, voronoi=None):
+    """Times the reference SKIRT build on the same workload (its own grid and densities; for
+    a Voronoi workload our sites and cell densities, imported as a VoronoiMeshMedium whose mesh
+    the grid employs); packets/s over the primary emission phase, as logged by the reference."""
     exe = os.path.join(ROOT, "oracle", "_ref", "bin", "skirt", "ref", "skirt_ref")
     if not os.path.exists(exe):
         return None
@@ -73,9 +75,15 @@ def cpu_baseline_reference(tau_v, nlambda, lam_range, packets, threads, cartesia
     import skigen
     if cartesian:
         spec = skigen.config0(num_packets=packets)
+    elif voronoi is not None:
+        spec = skigen.SkiSpec(num_packets=packets, grid=("voronoi_imported",), medium_file="sites.txt",
+                              instruments=tuple((f"i{k}", np.degrees(f["inclination"]), np.degrees(f["azimuth"]))
+                                                for k, f in enumerate(voronoi["frames"])))
     else:
         spec = skigen.config1(num_packets=packets, tau_v=tau_v, nlambda=nlambda, lam_range=lam_range)
     with tempfile.TemporaryDirectory() as tmp:
+        if voronoi is not None:
+            skigen.write_voronoi_medium(os.path.join(tmp, "sites.txt"), voronoi["sites"], voronoi["density"])
         ski = os.path.join(tmp, "bench.ski")
         with open(ski, "w") as f:
             f.write(skigen.render(spec))
@@ -88,20 +96,16 @@ def cpu_baseline_reference(tau_v, nlambda, lam_range, packets, threads, cartesia
         return None
     t = float(m.group(1))
     return dict(value=packets / t, unit="packets/s", cores=threads, kind="reference",
-                sample=f"{packets:.0f} packets of the same workload (grid and densities built by the reference), "
-                       f"primary emission "
-                       f"{t:.2f} s")
+                sample=f"{packets:.0f} packets of the same workload ("
+                       + ("our sites and cell densities imported, mesh built by the reference" if voronoi is not None
+                          else "grid and densities built by the reference")
+                       + f"), primary emission {t:.2f} s")
 
 
 def cpu_baseline_port(wl, packets, threads, seed):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    if wl.grid == "cartesian":
-        g = O.Grid("cartesian", xb=wl.tables["xb"], yb=wl.tables["yb"], zb=wl.tables["zb"])
-    else:
-        ex = wl.octree.export()
-        g = O.Grid("octree", bounds=ex["bounds"], child_first=ex["child_first"], cell_index=ex["cell_index"],
-                   nb_off=ex["nb_off"], nb_id=ex["nb_id"])
+    g = oracle_grid(wl)
     t0 = time.perf_counter()
     O.simulate(g, wl.density, wl.lam, wl.sext, wl.ssca, wl.g, np.ones(len(wl.lam)), wl.frames, 0, packets,
                threads=threads, seed=seed)
@@ -326,7 +330,7 @@ def main():
     ap.add_argument("--packets", type=int, default=1 << 22, help="photon packets per step")
     ap.add_argument("--tau", type=float, default=50.0)
     ap.add_argument("--fp", choices=["fp64", "fp32"], default="fp64")
-    ap.add_argument("--cpu-packets", type=float, default=4e5)
+    ap.add_argument("--cpu-packets", type=float, default=CPU_PACKETS_DEFAULT)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--threshold", type=int, default=16,
                     help="persistent engine: waiting lanes of a wavefront that start a transition round")
@@ -418,7 +422,8 @@ def main():
         bytes_per_segment = 64.0 + 32.0 * nnb / ncells
         wdesc = "Voronoi %d sites, lognormal clumpy medium (P(k)~k^-3.7, sigma_ln 1), mean tauV=5, 3 frames" % ncells
         a.no_guide = True
-        a.no_cpu_baseline = True
+        if a.cpu_packets == CPU_PACKETS_DEFAULT:
+            a.cpu_packets = 1e5  # about 20 s of the reference on 16 cores (its mesh build excluded)
     print(f"setup: {ncells} cells, {wl.setup_s:.1f} s", file=sys.stderr, flush=True)
 
     wl.apply(eng, npk_norm=1.0)
@@ -491,8 +496,11 @@ def main():
     cpu = None
     if not a.no_cpu_baseline:
         try:
+            vor = None
+            if a.workload == "voronoi":
+                vor = dict(sites=wl.tables["mesh"].export()["sites"], density=wl.density, frames=wl.frames)
             cpu = cpu_baseline_reference(a.tau, nlambda, lam_range, a.cpu_packets, threads,
-                                         cartesian=a.workload == "cartesian")
+                                         cartesian=a.workload == "cartesian", voronoi=vor)
         except Exception as ex:  # the reference build may be absent or fail on this host
             print(f"reference CPU baseline unavailable: {ex}", file=sys.stderr)
         if cpu is None:

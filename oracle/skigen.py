@@ -46,6 +46,10 @@ class SkiSpec:
     min_scatt_events: int = 0
     path_length_bias: float = 0.5
     num_density_samples: int = 100
+    # an imported Voronoi medium (VoronoiMeshMedium: columns x, y, z in pc and the number density
+    # in 1/m3, written by write_voronoi_medium) instead of the disk; pair it with the grid
+    # ("voronoi_imported",), which employs that medium's mesh (policy ImportedMesh)
+    medium_file: str = ""
 
 
 def _fmt_list(vals, unit):
@@ -120,6 +124,9 @@ def render(spec: SkiSpec) -> str:
         # ("voronoi", num_sites, policy): sites drawn by the grid's policy (VoronoiMeshSpatialGrid.cpp:51-140)
         _, nsites, policy = spec.grid
         grid = (f'<VoronoiMeshSpatialGrid {box} policy="{policy}" numSites="{nsites}" relaxSites="false"/>')
+    elif spec.grid[0] == "voronoi_imported":
+        # the imported medium's own mesh (VoronoiMeshSpatialGrid.cpp:133-141)
+        grid = f'<VoronoiMeshSpatialGrid {box} policy="ImportedMesh"/>'
     elif spec.grid[0] == "tetra":
         # ("tetra", num_sites, policy): Delaunay tetrahedralisation of the sampled vertices
         _, nsites, policy = spec.grid
@@ -142,6 +149,18 @@ def render(spec: SkiSpec) -> str:
         f'focal="{fe:.9g} pc" recordComponents="false" numScatteringLevels="0" recordPolarization="false" '
         'recordStatistics="false"/>'
         for (name, v, c, u, fe, w, n) in spec.persp_instruments)
+    mix = (f'<materialMix type="MaterialMix"><MeanListDustMix wavelengths="{dl}" extinctionCoefficients="{kappa}" '
+           f'albedos="{alb}" asymmetryParameters="{gg}"/></materialMix>')
+    if spec.medium_file:
+        medium = (f'<VoronoiMeshMedium filename="{spec.medium_file}" minX="-{h} pc" maxX="{h} pc" minY="-{h} pc" '
+                  f'maxY="{h} pc" minZ="-{h} pc" maxZ="{h} pc" massType="NumberDensity" massFraction="1" '
+                  'importMetallicity="false" importTemperature="false" maxTemperature="0 K" importVelocity="false" '
+                  f'importMagneticField="false" importVariableMixParams="false" useColumns="">{mix}</VoronoiMeshMedium>')
+    else:
+        medium = (f'<GeometricMedium velocityMagnitude="0 km/s" magneticFieldStrength="0 uG">'
+                  f'<geometry type="Geometry">{geom}</geometry>{mix}'
+                  '<normalization type="MaterialNormalization"><OpticalDepthMaterialNormalization axis="Z" '
+                  f'wavelength="0.55 micron" opticalDepth="{spec.tau_v:.9g}"/></normalization></GeometricMedium>')
     force = "true" if spec.force_scattering else "false"
     rfopt = ('                <radiationFieldOptions type="RadiationFieldOptions">'
              '<RadiationFieldOptions storeRadiationField="true"/></radiationFieldOptions>\n') if spec.store_rf else ""
@@ -167,15 +186,7 @@ def render(spec: SkiSpec) -> str:
                     <PhotonPacketOptions explicitAbsorption="false" forceScattering="{force}" minWeightReduction="{spec.min_weight_reduction:.9g}" minScattEvents="{spec.min_scatt_events}" pathLengthBias="{spec.path_length_bias:.9g}"/>
                 </photonPacketOptions>
                 <media type="Medium">
-                    <GeometricMedium velocityMagnitude="0 km/s" magneticFieldStrength="0 uG">
-                        <geometry type="Geometry">{geom}</geometry>
-                        <materialMix type="MaterialMix">
-                            <MeanListDustMix wavelengths="{dl}" extinctionCoefficients="{kappa}" albedos="{alb}" asymmetryParameters="{gg}"/>
-                        </materialMix>
-                        <normalization type="MaterialNormalization">
-                            <OpticalDepthMaterialNormalization axis="Z" wavelength="0.55 micron" opticalDepth="{spec.tau_v:.9g}"/>
-                        </normalization>
-                    </GeometricMedium>
+                    {medium}
                 </media>
 {rfopt}                <samplingOptions type="SamplingOptions">
                     <SamplingOptions numDensitySamples="{spec.num_density_samples}" numPropertySamples="1" aggregateVelocity="Average"/>
@@ -190,6 +201,18 @@ def render(spec: SkiSpec) -> str:
     </MonteCarloSimulation>
 </skirt-simulation-hierarchy>
 """
+
+
+def write_voronoi_medium(path, sites_m, density_m3):
+    """A VoronoiMeshMedium file: site positions (pc) and the cell's number density (1/m3),
+    one site per row."""
+    import numpy as np
+    pc = 3.08567758e16  # the reference's parsec (Constants::pc)
+    with open(path, "w") as f:
+        for i, (name, unit) in enumerate((("position x", "pc"), ("position y", "pc"), ("position z", "pc"),
+                                          ("number density", "1/m3"))):
+            f.write(f"# column {i + 1}: {name} ({unit})\n")
+        np.savetxt(f, np.column_stack([np.asarray(sites_m) / pc, density_m3]), fmt="%.17g")
 
 
 def log_wavelengths(lo: float, hi: float, n: int) -> List[float]:

@@ -375,9 +375,13 @@ int skg_sample_density_cartesian(const double* xb, int32_t nx, const double* yb,
                                  int32_t threads, double* out);
 int skg_sample_density_octree(const skg_octree* t, const skg_disk_model* model, int32_t num_samples,
                               uint64_t seed, int32_t threads, double* out);
-/* skg_octree_build with the per-node density estimates of each level computed on the
- * engine's GPU (the same random positions per node; the subdivision and the neighbour lists
- * on the host as before). */
+/* skg_octree_build on the engine's GPU, level by level (DensityTreePolicy::constructTree,
+ * DensityTreePolicy.cpp:180-241): the per-node density estimates (the same random positions
+ * per node), the subdivision flags, the children's ids and bounds and every leaf's neighbour
+ * lists (the lists OctTreeNode::addNeighbors, OctTreeNode.cpp:46-136, leaves after the
+ * level's subdivisions in node order) are computed on the device; lists of more than 16
+ * entries are sorted on the host with std::sort, the others on the device.  The tree is
+ * returned on the host in the same form as skg_octree_build's. */
 int skg_octree_build_gpu(skg_engine* e, const double* box, const skg_disk_model* model, int32_t min_level,
                          int32_t max_level, double max_dust_fraction, int32_t num_samples, uint64_t seed,
                          int32_t threads, skg_octree** out);

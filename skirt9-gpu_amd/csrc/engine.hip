@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -23,6 +24,7 @@
 #include "disk_model.hpp"
 #include "lifecycle.hpp"
 #include "octree_build.hpp"
+#include "octree_gpu.hpp"
 #include "persistent.hpp"
 #include "pipeline.hpp"
 #include "traverse.hpp"
@@ -321,28 +323,6 @@ namespace skg
             sum += md.geom(x, y, z);
         }
         out[m] = md.nnorm * (sum / n);
-    }
-
-    // Mean normalised density of n boxes (6 bounds each) over `samples` uniform points, from the
-    // Philox stream (seed, id, tree tag): the per-node estimate of the octree refinement, the
-    // same positions as the host builder's
-    __global__ void box_sample_kernel(DiskModel md, int64_t n, const double* __restrict__ boxes,
-                                      const uint64_t* __restrict__ ids, int samples, uint64_t seed, double* out)
-    {
-        const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-        if (i >= n) return;
-        const double* b = boxes + 6 * i;
-        Philox rng;
-        rng.init(seed, ids[i], kTagTree);
-        double sum = 0.;
-        for (int q = 0; q < samples; ++q)
-        {
-            const double x = b[0] + rng.uniform() * (b[3] - b[0]);
-            const double y = b[1] + rng.uniform() * (b[4] - b[1]);
-            const double z = b[2] + rng.uniform() * (b[5] - b[2]);
-            sum += md.geom(x, y, z);
-        }
-        out[i] = sum / samples;
     }
 
 }
@@ -727,6 +707,52 @@ namespace
                     for (int k = rng[6 * m + 4]; k <= rng[6 * m + 5]; ++k)
                         ids[fill[(static_cast<int64_t>(i) * tbn + j) * tbn + k]++] = static_cast<int32_t>(m);
     }
+}
+
+namespace
+{
+    // a device buffer that grows on demand (contents not kept) and is freed with its owner
+    struct DevBuf
+    {
+        void* p = nullptr;
+        size_t bytes = 0;
+        DevBuf() = default;
+        DevBuf(const DevBuf&) = delete;
+        DevBuf& operator=(const DevBuf&) = delete;
+        ~DevBuf()
+        {
+            if (p) hipFree(p);
+        }
+        hipError_t ensure(size_t n)
+        {
+            if (n <= bytes) return hipSuccess;
+            if (p) hipFree(p);
+            p = nullptr;
+            bytes = 0;
+            hipError_t st = hipMalloc(&p, n);
+            if (st == hipSuccess) bytes = n;
+            return st;
+        }
+        template<typename T> T* as() const { return static_cast<T*>(p); }
+    };
+
+    // turns counts in off[1..n] (off[0] = 0) into offsets on the engine's stream; the total
+    // into *total
+    hipError_t scanCounts(hipStream_t s, int64_t* off, int64_t n, DevBuf& sums, int64_t* total)
+    {
+        const int64_t nb = (n + kScanItems - 1) / kScanItems;
+        hipError_t st = sums.ensure(static_cast<size_t>(nb + 1) * sizeof(int64_t));
+        if (st != hipSuccess) return st;
+        hipLaunchKernelGGL(scan_block_sums, dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, off, n, sums.as<int64_t>());
+        hipLaunchKernelGGL(scan_sums, dim3(1), dim3(1024), 0, s, sums.as<int64_t>(), static_cast<int>(nb));
+        hipLaunchKernelGGL(add_block_offsets, dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, off, n, sums.as<int64_t>());
+        st = hipGetLastError();
+        if (st == hipSuccess) st = hipMemcpyAsync(total, off + n, sizeof(int64_t), hipMemcpyDeviceToHost, s);
+        if (st == hipSuccess) st = hipStreamSynchronize(s);
+        return st;
+    }
+
+    unsigned blocksFor(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
 }
 
 extern "C" {
@@ -1521,50 +1547,120 @@ int skg_octree_build_gpu(skg_engine* e, const double* box, const skg_disk_model*
                          int32_t max_level, double max_dust_fraction, int32_t num_samples, uint64_t seed,
                          int32_t threads, skg_octree** out)
 {
-    if (!e || !model) return fail(e, SKG_ERR_ARGUMENT, "skg_octree_build_gpu: bad arguments");
+    if (!e || !box || !model || !out || min_level < 0 || max_level < min_level || max_level > 20 || num_samples < 1
+        || !(box[3] > box[0] && box[4] > box[1] && box[5] > box[2]))
+        return fail(e, SKG_ERR_ARGUMENT, "skg_octree_build_gpu: bad arguments");
     std::lock_guard<std::mutex> lk(e->mu);
     hipSetDevice(e->device);
     const DiskModel md(*model);
-    double* dBox = nullptr;
-    uint64_t* dIds = nullptr;
-    double* dRho = nullptr;
+    hipStream_t s = e->stream;
+    // node tables (bounds, first child, parent), grown by doubling with their contents kept
+    DevBuf B, F, P, flags, sums, offA, idA, offB, idB;
     int64_t cap = 0;
-    auto release = [&]() {
-        if (dBox) hipFree(dBox);
-        if (dIds) hipFree(dIds);
-        if (dRho) hipFree(dRho);
-        dBox = nullptr;
-        dIds = nullptr;
-        dRho = nullptr;
-    };
-    // the builder's per-level estimates on the GPU; everything else as the host builder
-    auto sampler = [&](const double* boxes, const uint64_t* ids, int64_t n, double* rho) -> int {
-        if (n > cap)
+    auto grow = [&](int64_t need) -> hipError_t {
+        if (need <= cap) return hipSuccess;
+        const int64_t nc = std::max<int64_t>(need, std::max<int64_t>(2 * cap, 1 << 16));
+        DevBuf b2, f2, p2;
+        hipError_t st = b2.ensure(static_cast<size_t>(nc) * 6 * sizeof(double));
+        if (st == hipSuccess) st = f2.ensure(static_cast<size_t>(nc) * sizeof(int32_t));
+        if (st == hipSuccess) st = p2.ensure(static_cast<size_t>(nc) * sizeof(int32_t));
+        if (st == hipSuccess && cap)
         {
-            release();
-            cap = std::max<int64_t>(n, 1 << 16);
-            if (hipMalloc(&dBox, cap * 6 * sizeof(double)) != hipSuccess || hipMalloc(&dIds, cap * sizeof(uint64_t)) != hipSuccess
-                || hipMalloc(&dRho, cap * sizeof(double)) != hipSuccess)
-                return fail(e, SKG_ERR_DEVICE, "skg_octree_build_gpu: allocation failed");
+            st = hipMemcpyAsync(b2.p, B.p, static_cast<size_t>(cap) * 6 * sizeof(double), hipMemcpyDeviceToDevice, s);
+            if (st == hipSuccess) st = hipMemcpyAsync(f2.p, F.p, static_cast<size_t>(cap) * sizeof(int32_t), hipMemcpyDeviceToDevice, s);
+            if (st == hipSuccess) st = hipMemcpyAsync(p2.p, P.p, static_cast<size_t>(cap) * sizeof(int32_t), hipMemcpyDeviceToDevice, s);
+            if (st == hipSuccess) st = hipStreamSynchronize(s);
         }
-        hipError_t st = hipMemcpyAsync(dBox, boxes, n * 6 * sizeof(double), hipMemcpyHostToDevice, e->stream);
-        if (st == hipSuccess) st = hipMemcpyAsync(dIds, ids, n * sizeof(uint64_t), hipMemcpyHostToDevice, e->stream);
-        if (st == hipSuccess)
-        {
-            hipLaunchKernelGGL(box_sample_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, e->stream, md, n,
-                               dBox, dIds, num_samples, seed, dRho);
-            st = hipGetLastError();
-        }
-        if (st == hipSuccess) st = hipMemcpyAsync(rho, dRho, n * sizeof(double), hipMemcpyDeviceToHost, e->stream);
-        if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
-        if (st != hipSuccess) return fail(e, SKG_ERR_DEVICE, std::string("skg_octree_build_gpu: ") + hipGetErrorString(st));
-        return SKG_OK;
+        if (st != hipSuccess) return st;
+        std::swap(B.p, b2.p);
+        std::swap(B.bytes, b2.bytes);
+        std::swap(F.p, f2.p);
+        std::swap(F.bytes, f2.bytes);
+        std::swap(P.p, p2.p);
+        std::swap(P.bytes, p2.bytes);
+        cap = nc;
+        return hipSuccess;
     };
-    int st = skg::octree_build_with(box, model, min_level, max_level, max_dust_fraction, num_samples, seed, threads, out,
-                                    sampler);
-    release();
-    if (st != SKG_OK && e->err.empty()) e->err = "skg_octree_build_gpu failed";
-    return st;
+    HIPCHK(e, grow(1));
+    const int32_t none = -1;
+    HIPCHK(e, hipMemcpyAsync(B.p, box, 6 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(e, hipMemcpyAsync(F.p, &none, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    HIPCHK(e, hipMemcpyAsync(P.p, &none, sizeof(int32_t), hipMemcpyHostToDevice, s));
+    // the root's six empty lists
+    DevBuf* oldOff = &offA;
+    DevBuf* oldId = &idA;
+    DevBuf* newOff = &offB;
+    DevBuf* newId = &idB;
+    HIPCHK(e, oldOff->ensure(7 * sizeof(int64_t)));
+    HIPCHK(e, oldId->ensure(sizeof(int32_t)));
+    HIPCHK(e, hipMemsetAsync(oldOff->p, 0, 7 * sizeof(int64_t), s));
+    int64_t lbeg = 0, lend = 1, entries = 0;
+    for (int level = 0;; ++level)
+    {
+        // (1) flags of the level's nodes and (2) their children
+        const int64_t n = lend - lbeg;
+        HIPCHK(e, flags.ensure(static_cast<size_t>(n + 1) * sizeof(int64_t)));
+        hipLaunchKernelGGL(tree_flags, dim3(blocksFor(n)), dim3(256), 0, s, md, B.as<double>(), lbeg, n, level, min_level,
+                           max_level, max_dust_fraction, num_samples, seed, flags.as<int64_t>());
+        HIPCHK(e, hipGetLastError());
+        int64_t nflag = 0;
+        HIPCHK(e, scanCounts(s, flags.as<int64_t>(), n, sums, &nflag));
+        if (nflag == 0) break;
+        const int64_t nn = lend + 8 * nflag;
+        if (nn > INT32_MAX / 8) return fail(e, SKG_ERR_CAPACITY, "skg_octree_build_gpu: too many nodes");
+        HIPCHK(e, grow(nn));
+        hipLaunchKernelGGL(tree_children, dim3(blocksFor(n)), dim3(256), 0, s, B.as<double>(), F.as<int32_t>(),
+                           P.as<int32_t>(), lbeg, lend, n, flags.as<const int64_t>());
+        HIPCHK(e, hipGetLastError());
+        // (3) the lists of all nodes after the level
+        HIPCHK(e, newOff->ensure(static_cast<size_t>(6 * nn + 1) * sizeof(int64_t)));
+        hipLaunchKernelGGL(tree_lists<false>, dim3(blocksFor(6 * nn)), dim3(256), 0, s, B.as<const double>(),
+                           F.as<const int32_t>(), P.as<const int32_t>(), lend, nn, oldOff->as<const int64_t>(),
+                           oldId->as<const int32_t>(), newOff->as<int64_t>(), static_cast<int32_t*>(nullptr));
+        HIPCHK(e, hipGetLastError());
+        HIPCHK(e, scanCounts(s, newOff->as<int64_t>(), 6 * nn, sums, &entries));
+        if (entries > INT32_MAX) return fail(e, SKG_ERR_CAPACITY, "skg_octree_build_gpu: too many neighbour entries");
+        HIPCHK(e, newId->ensure(static_cast<size_t>(std::max<int64_t>(entries, 1)) * sizeof(int32_t)));
+        hipLaunchKernelGGL(tree_lists<true>, dim3(blocksFor(6 * nn)), dim3(256), 0, s, B.as<const double>(),
+                           F.as<const int32_t>(), P.as<const int32_t>(), lend, nn, oldOff->as<const int64_t>(),
+                           oldId->as<const int32_t>(), newOff->as<int64_t>(), newId->as<int32_t>());
+        HIPCHK(e, hipGetLastError());
+        std::swap(oldOff, newOff);
+        std::swap(oldId, newId);
+        lbeg = lend;
+        lend = nn;
+    }
+    const int64_t nn = lend;
+    hipLaunchKernelGGL(tree_sort_short, dim3(blocksFor(6 * nn)), dim3(256), 0, s, B.as<const double>(),
+                       F.as<const int32_t>(), nn, oldOff->as<const int64_t>(), oldId->as<int32_t>());
+    HIPCHK(e, hipGetLastError());
+    try
+    {
+        auto t = std::make_unique<skg_octree>();
+        t->bounds.resize(6 * nn);
+        t->first.resize(nn);
+        t->nbOff.resize(6 * nn + 1);
+        t->nbId.resize(entries);
+        HIPCHK(e, hipMemcpyAsync(t->bounds.data(), B.p, t->bounds.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(e, hipMemcpyAsync(t->first.data(), F.p, t->first.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(e, hipMemcpyAsync(t->nbOff.data(), oldOff->p, t->nbOff.size() * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        if (entries)
+            HIPCHK(e, hipMemcpyAsync(t->nbId.data(), oldId->p, t->nbId.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(e, hipStreamSynchronize(s));
+        // leaves numbered in node order (TreeSpatialGrid.cpp:37-49)
+        t->cellOf.assign(nn, -1);
+        int32_t m = 0;
+        for (int64_t l = 0; l < nn; ++l)
+            if (t->first[l] < 0) t->cellOf[l] = m++;
+        t->ncells = m;
+        skg::octree_sort_lists(*t, 17, threads);
+        *out = t.release();
+    }
+    catch (const std::bad_alloc&)
+    {
+        return fail(e, SKG_ERR_CAPACITY, "skg_octree_build_gpu: out of host memory");
+    }
+    return SKG_OK;
 }
 
 int skg_rf_enable(skg_engine* e, int32_t on)

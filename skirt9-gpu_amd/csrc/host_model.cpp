@@ -78,8 +78,10 @@ namespace
 }
 
 // ------------------------------------------------------------------------------- octree
-struct skg_octree
+namespace
 {
+    // a node while the tree is built: bounds, first child, level and the neighbour lists that
+    // subdivisions relink (TreeNode's members)
     struct Node
     {
         double b[6];
@@ -87,14 +89,6 @@ struct skg_octree
         int32_t level = 0;
         std::vector<int32_t> nb[6];
     };
-    std::vector<Node> nodes;
-    std::vector<int32_t> cellOf;
-    int64_t ncells = 0;
-};
-
-namespace
-{
-    using Node = skg_octree::Node;
     enum Wall
     {
         BACK = 0,
@@ -179,27 +173,39 @@ namespace
         for (auto& l : v[p].nb) std::vector<int32_t>().swap(l);
     }
 
-    // TreeNode::sortNeighbors with the LargerOverlap order (TreeNode.cpp:145-207)
-    double overlapArea(const double* base, const double* nb, int wall)
+    // the finished table form: leaves numbered in node order, the lists in one CSR, then
+    // sorted by decreasing overlap (TreeNode::sortNeighbors)
+    void finishTree(std::vector<Node>& v, skg_octree& t, int threads)
     {
-        int u, w;
-        if (wall == BACK || wall == FRONT)
+        const size_t n = v.size();
+        t.bounds.resize(6 * n);
+        t.first.resize(n);
+        t.cellOf.assign(n, -1);
+        t.nbOff.resize(6 * n + 1);
+        int64_t total = 0;
+        int32_t m = 0;
+        for (size_t l = 0; l < n; ++l)
         {
-            u = 1;
-            w = 2;
+            std::memcpy(&t.bounds[6 * l], v[l].b, 6 * sizeof(double));
+            t.first[l] = v[l].first;
+            if (v[l].first < 0) t.cellOf[l] = m++;
+            for (int w = 0; w < 6; ++w)
+            {
+                t.nbOff[6 * l + w] = total;
+                total += static_cast<int64_t>(v[l].nb[w].size());
+            }
         }
-        else if (wall == LEFT || wall == RIGHT)
+        t.nbOff[6 * n] = total;
+        t.ncells = m;
+        t.nbId.resize(static_cast<size_t>(total));
+        for (size_t l = 0; l < n; ++l)
         {
-            u = 0;
-            w = 2;
+            for (int w = 0; w < 6; ++w)
+                std::copy(v[l].nb[w].begin(), v[l].nb[w].end(), t.nbId.begin() + t.nbOff[6 * l + w]);
+            for (auto& q : v[l].nb) std::vector<int32_t>().swap(q);
         }
-        else
-        {
-            u = 0;
-            w = 1;
-        }
-        return std::max(std::min(base[u + 3], nb[u + 3]) - std::max(base[u], nb[u]), 0.)
-               * std::max(std::min(base[w + 3], nb[w + 3]) - std::max(base[w], nb[w]), 0.);
+        std::vector<Node>().swap(v);
+        skg::octree_sort_lists(t, 2, threads);
     }
 }
 
@@ -215,7 +221,7 @@ int skg::octree_build_with(const double* box, const skg_disk_model* model, int32
     {
         threads = defaultThreads(threads);
         auto t = std::make_unique<skg_octree>();
-        auto& v = t->nodes;
+        std::vector<Node> v;
         v.reserve(1 << 20);
         v.emplace_back();
         std::memcpy(v[0].b, box, 6 * sizeof(double));
@@ -261,26 +267,7 @@ int skg::octree_build_with(const double* box, const skg_disk_model* model, int32
             lend = v.size();
             if (v.size() > static_cast<size_t>(INT32_MAX / 8)) return SKG_ERR_CAPACITY;
         }
-        // cell indices in node order (TreeSpatialGrid.cpp:37-49)
-        t->cellOf.assign(v.size(), -1);
-        int32_t m = 0;
-        for (size_t l = 0; l < v.size(); ++l)
-            if (v[l].first < 0) t->cellOf[l] = m++;
-        t->ncells = m;
-        // sort the leaves' neighbour lists by decreasing overlap area
-        parallelFor(static_cast<int64_t>(v.size()), threads, [&](int64_t b, int64_t e) {
-            for (int64_t l = b; l < e; ++l)
-            {
-                if (v[l].first >= 0) continue;
-                for (int w = 0; w < 6; ++w)
-                {
-                    const double* base = v[l].b;
-                    std::sort(v[l].nb[w].begin(), v[l].nb[w].end(), [&](int32_t a, int32_t c) {
-                        return overlapArea(base, v[a].b, w) > overlapArea(base, v[c].b, w);
-                    });
-                }
-            }
-        });
+        finishTree(v, *t, threads);
         *out = t.release();
         return SKG_OK;
     }
@@ -290,6 +277,29 @@ int skg::octree_build_with(const double* box, const skg_disk_model* model, int32
     }
 }
 
+
+void skg::octree_sort_lists(skg_octree& t, int64_t min_len, int32_t threads)
+{
+    const int64_t n = static_cast<int64_t>(t.first.size());
+    const double* B = t.bounds.data();
+    parallelFor(n, defaultThreads(threads), [&](int64_t b, int64_t e) {
+        for (int64_t l = b; l < e; ++l)
+        {
+            if (t.first[l] >= 0) continue;
+            const double* base = B + 6 * l;
+            for (int w = 0; w < 6; ++w)
+            {
+                int32_t* lo = t.nbId.data() + t.nbOff[6 * l + w];
+                int32_t* hi = t.nbId.data() + t.nbOff[6 * l + w + 1];
+                if (hi - lo < min_len) continue;
+                std::sort(lo, hi, [&](int32_t a, int32_t c) {
+                    return octree_overlap(base, B + 6 * static_cast<int64_t>(a), w)
+                           > octree_overlap(base, B + 6 * static_cast<int64_t>(c), w);
+                });
+            }
+        }
+    });
+}
 
 extern "C" {
 
@@ -320,36 +330,21 @@ int skg_octree_build(const double* box, const skg_disk_model* model, int32_t min
 int skg_octree_info(const skg_octree* t, int64_t* nnodes, int64_t* ncells, int64_t* nneighbors)
 {
     if (!t) return SKG_ERR_ARGUMENT;
-    if (nnodes) *nnodes = static_cast<int64_t>(t->nodes.size());
+    if (nnodes) *nnodes = static_cast<int64_t>(t->first.size());
     if (ncells) *ncells = t->ncells;
-    if (nneighbors)
-    {
-        int64_t s = 0;
-        for (const auto& n : t->nodes)
-            for (const auto& l : n.nb) s += static_cast<int64_t>(l.size());
-        *nneighbors = s;
-    }
+    if (nneighbors) *nneighbors = static_cast<int64_t>(t->nbId.size());
     return SKG_OK;
 }
 
 int skg_octree_export(const skg_octree* t, double* bounds, int32_t* child_first, int32_t* cell_index, int64_t* nb_off,
                       int32_t* nb_id)
 {
-    if (!t || !bounds || !child_first || !cell_index || !nb_off || !nb_id) return SKG_ERR_ARGUMENT;
-    const auto& v = t->nodes;
-    int64_t o = 0;
-    for (size_t l = 0; l < v.size(); ++l)
-    {
-        std::memcpy(bounds + 6 * l, v[l].b, 6 * sizeof(double));
-        child_first[l] = v[l].first;
-        cell_index[l] = t->cellOf[l];
-        for (int w = 0; w < 6; ++w)
-        {
-            nb_off[6 * l + w] = o;
-            for (int32_t id : v[l].nb[w]) nb_id[o++] = id;
-        }
-    }
-    nb_off[6 * v.size()] = o;
+    if (!t || !bounds || !child_first || !cell_index || !nb_off || (!nb_id && !t->nbId.empty())) return SKG_ERR_ARGUMENT;
+    std::copy(t->bounds.begin(), t->bounds.end(), bounds);
+    std::copy(t->first.begin(), t->first.end(), child_first);
+    std::copy(t->cellOf.begin(), t->cellOf.end(), cell_index);
+    std::copy(t->nbOff.begin(), t->nbOff.end(), nb_off);
+    std::copy(t->nbId.begin(), t->nbId.end(), nb_id);
     return SKG_OK;
 }
 
@@ -365,7 +360,7 @@ int skg_octree_from_flags(const double* box, int64_t nflags, const uint8_t* divi
     try
     {
         auto t = std::make_unique<skg_octree>();
-        auto& v = t->nodes;
+        std::vector<Node> v;
         v.emplace_back();
         std::memcpy(v[0].b, box, 6 * sizeof(double));
         for (int64_t l = 0; l < static_cast<int64_t>(v.size()); ++l)
@@ -374,22 +369,7 @@ int skg_octree_from_flags(const double* box, int64_t nflags, const uint8_t* divi
             if (divide[l]) subdivide(v, static_cast<int32_t>(l));
         }
         if (static_cast<int64_t>(v.size()) != nflags) return SKG_ERR_ARGUMENT;
-        t->cellOf.assign(v.size(), -1);
-        int32_t m = 0;
-        for (size_t l = 0; l < v.size(); ++l)
-            if (v[l].first < 0) t->cellOf[l] = m++;
-        t->ncells = m;
-        for (size_t l = 0; l < v.size(); ++l)
-        {
-            if (v[l].first >= 0) continue;
-            for (int w = 0; w < 6; ++w)
-            {
-                const double* base = v[l].b;
-                std::sort(v[l].nb[w].begin(), v[l].nb[w].end(), [&](int32_t a, int32_t c) {
-                    return overlapArea(base, v[a].b, w) > overlapArea(base, v[c].b, w);
-                });
-            }
-        }
+        finishTree(v, *t, 0);
         *out = t.release();
         return SKG_OK;
     }
@@ -423,13 +403,12 @@ int skg_sample_density_octree(const skg_octree* t, const skg_disk_model* model, 
 {
     if (!t || !model || num_samples < 1 || !out) return SKG_ERR_ARGUMENT;
     Model md(*model);
-    const auto& v = t->nodes;
-    parallelFor(static_cast<int64_t>(v.size()), defaultThreads(threads), [&](int64_t b, int64_t e) {
+    parallelFor(static_cast<int64_t>(t->first.size()), defaultThreads(threads), [&](int64_t b, int64_t e) {
         for (int64_t l = b; l < e; ++l)
         {
             int32_t m = t->cellOf[l];
             if (m < 0) continue;
-            out[m] = md.nnorm * sampleMean(md, v[l].b, num_samples, seed, static_cast<uint64_t>(m), kTagDensity);
+            out[m] = md.nnorm * sampleMean(md, &t->bounds[6 * l], num_samples, seed, static_cast<uint64_t>(m), kTagDensity);
         }
     });
     return SKG_OK;

@@ -327,6 +327,31 @@ namespace skg
 
 }
 
+namespace skg
+{
+    // TreeNode::leafChild from the root over the node records (TreeSpatialGrid.cpp:190-219): the
+    // cell of the leaf holding (x, y, z), -1 outside the root's closed box (one thread)
+    __global__ void octree_locate(GridView g, double x, double y, double z, int* out)
+    {
+        if (!(x >= g.box[0] && x <= g.box[3] && y >= g.box[1] && y <= g.box[4] && z >= g.box[2] && z <= g.box[5]))
+        {
+            *out = -1;
+            return;
+        }
+        int nd = 0;
+        while (true)
+        {
+            const TopNode t = g.top[nd];
+            if (t.child < 0)
+            {
+                *out = t.cell;
+                return;
+            }
+            nd = t.child + (x < t.sx ? 0 : 1) + (y < t.sy ? 0 : 2) + (z < t.sz ? 0 : 4);
+        }
+    }
+}
+
 // ==================================================================== host side
 using namespace skg;
 
@@ -380,6 +405,8 @@ struct skg_engine
     // source / options
     double src[3] = {0, 0, 0};
     bool haveSource = false;
+    int srcHint = -1;           // octree: the leaf holding src (start hint of new packets)
+    bool srcHintValid = false;  // srcHint belongs to the current grid and source
     double xi = 0.5, mwr = 1e4;
     int minScatt = 0;
     int fpMode = SKG_FP64;
@@ -500,6 +527,7 @@ namespace
     {
         for (void* p : e->gridBufs) hipFree(p);
         e->gridBufs.clear();
+        e->srcHintValid = false;
         e->gridType = -1;
         e->ncells = 0;
     }
@@ -1382,6 +1410,7 @@ int skg_source_point(skg_engine* e, double x, double y, double z, const double* 
     e->src[0] = x;
     e->src[1] = y;
     e->src[2] = z;
+    e->srcHintValid = false;
     e->hLpack.assign(lpack, lpack + e->nlambda);
     int st = upload(e, &e->lpack, lpack, e->nlambda);
     if (st) return st;
@@ -1753,6 +1782,21 @@ static int runPersistent(skg_engine* e, uint64_t first_history, uint64_t count)
     p.gframe = e->gframe;
     p.guideMix = e->guideMix;
     p.contKey = e->contKey;
+    // the leaf of the point source, the start hint of every new packet (octree): found once
+    // per grid and source by the top-down search, which the hint check then reproduces
+    if (e->gridType == SKG_GRID_OCTREE && !e->srcHintValid)
+    {
+        int* d = nullptr;
+        HIPCHK(e, hipMalloc(&d, sizeof(int)));
+        hipLaunchKernelGGL(octree_locate, dim3(1), dim3(1), 0, e->stream, e->grid, e->src[0], e->src[1], e->src[2], d);
+        hipError_t st = hipGetLastError();
+        if (st == hipSuccess) st = hipMemcpyAsync(&e->srcHint, d, sizeof(int), hipMemcpyDeviceToHost, e->stream);
+        if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+        hipFree(d);
+        if (st != hipSuccess) return fail(e, SKG_ERR_DEVICE, std::string("source cell: ") + hipGetErrorString(st));
+        e->srcHintValid = true;
+    }
+    p.srcHint = e->gridType == SKG_GRID_OCTREE ? e->srcHint : -1;
     p.rf = e->rf;
     p.persp = 0;
     for (const auto& f : e->frames) p.persp |= f.persp;

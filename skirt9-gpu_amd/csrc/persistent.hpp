@@ -53,6 +53,7 @@ namespace skg
         uint32_t contKey;         // nonzero: histories continue on Philox tag contKey from their first
                                   // scattering on (same emission and first interaction, independent
                                   // continuations)
+        int32_t srcHint;          // octree: the leaf of the point source (the start hint of new packets), -1: none
         double* rf;               // radiation-field variant: ncells x nlambda table
         int persp;                // some instruments are perspective instruments
         int nent;                 // particle-source variant: number of entities (0: the point source)
@@ -501,13 +502,14 @@ namespace skg
                 pk.ry = p.src[1];
                 pk.rz = p.src[2];
             }
+            const int startHint = V == kParticles ? -1 : p.srcHint;
             isotropic(pk.rng, pk.kx, pk.ky, pk.kz);
             pk.nscatt = 0;
             pk.taupath = 0.;
             pk.tauCached = 0.;
             pk.Wp = 0.;
             pk.pix = -1;
-            pk.hint = -1;
+            pk.hint = startHint;
             pk.haveTau = false;
             if (!(pk.W / pk.lam > 0))
             {

@@ -722,14 +722,21 @@ namespace skg
             switch (act)
             {
                 case kOsStart:
-                    // the start hint, when the point lies strictly inside its box: every split
-                    // plane of its ancestors then sends leafChild's descent toward it, so it is
-                    // the leaf the top-down search returns
-                    if (rx > R.d(0) && rx < R.d(3) && ry > R.d(1) && ry < R.d(4) && rz > R.d(2) && rz < R.d(5))
+                {
+                    // the start hint, when the point lies in its box the way the top-down search
+                    // assigns points to leaves: lo <= r < hi on each axis (a point on a split
+                    // plane goes to the upper child), hi included at the domain's upper boundary.
+                    // Every split plane of the leaf's ancestors then sends leafChild's descent
+                    // toward it, so it is the leaf the search returns.
+                    const bool inx = rx >= R.d(0) && (rx < R.d(3) || R.d(3) == g.box[3]);
+                    const bool iny = ry >= R.d(1) && (ry < R.d(4) || R.d(4) == g.box[4]);
+                    const bool inz = rz >= R.d(2) && (rz < R.d(5) || R.d(5) == g.box[5]);
+                    if (inx && iny && inz)
                         adopt(R, hint);
                     else
                         resolve(g, search_lds(g, 0));
                     break;
+                }
                 case kOsCross:
                     // the first (largest-overlap) neighbour on the crossed wall, when its closed
                     // box contains the new position; else the rest of the wall's list

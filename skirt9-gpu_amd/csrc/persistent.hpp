@@ -258,91 +258,89 @@ namespace skg
             double ds, nd;
             const int r = gen.advance(grid, m, ds, nd);
             if (r == 0) return;  // an octree step that loaded a search node or list entry
-            if (r < 0 || ++steps > kMaxSteps)
+            // the traversal ends at the path's end, at the early-out optical depth of a peel-off,
+            // past a perspective peel-off's distance, or at a second pass's interaction; the
+            // crossing counters are added once, where it ends
+            bool fin = r < 0 || ++steps > kMaxSteps;
+            if (!fin)
+            {
+                if (phase == kPeel)
+                {
+                    if (m >= 0)
+                    {
+                        if (FP == SKG_FP64)
+                        {
+                            tau += section * nd * ds;
+                            if (tau >= taumax)
+                            {
+                                tau = INFINITY;
+                                fin = true;
+                            }
+                        }
+                        else
+                        {
+                            tau32.add(section32 * static_cast<float>(nd) * static_cast<float>(ds));
+                            if (static_cast<double>(tau32.sum) >= taumax)
+                            {
+                                tau32.sum = INFINITY;
+                                fin = true;
+                            }
+                        }
+                    }
+                    if constexpr (PERSP)
+                    {
+                        // MediumSystem::getExtinctionOpticalDepth(pp, distance) (MediumSystem.cpp:
+                        // 2376-2382): the segment that passes the distance still counts
+                        s += ds;
+                        if (s > slim) fin = true;
+                    }
+                }
+                else if (ds > 0.)
+                {
+                    if (FP == SKG_FP64)
+                    {
+                        tprev = tau;
+                        sprev = s;
+                        s += ds;
+                        if (m >= 0) tau += section * nd * ds;
+                    }
+                    else
+                    {
+                        tprev = static_cast<double>(tau32.sum);
+                        sprev = static_cast<double>(s32.sum);
+                        s32.add(static_cast<float>(ds));
+                        if (m >= 0) tau32.add(section32 * static_cast<float>(nd) * static_cast<float>(ds));
+                    }
+                    lastm = m;
+                    if constexpr (RF)
+                    {
+                        if (phase == kFwd1)
+                        {
+                            const double lnEnd = -tau_value();
+                            const double extEnd = exp(lnEnd);
+                            if (m >= 0)
+                            {
+                                const double em = ln_mean(extEnd, rfExt, lnEnd, -tprev);
+                                unsafeAtomicAdd(rf + static_cast<int64_t>(m) * nl + ell, rfL * em * ds);
+                            }
+                            rfExt = extEnd;
+                        }
+                    }
+                    // second pass: stop at the first segment whose exit optical depth exceeds tau_int
+                    if (phase == kFwd2 && tint < tau_value())
+                    {
+                        mint = m;
+                        double tv = tau_value(), sv = s_value();
+                        sint = sprev + ((tint - tprev) / (tv - tprev)) * (sv - sprev);
+                        fin = true;
+                    }
+                }
+            }
+            if (fin)
             {
                 ended = true;
                 atomicAdd(lds + 4, static_cast<unsigned int>(steps));
                 if (phase == kFwd2) atomicAdd(lds + 7, static_cast<unsigned int>(steps));
-                return;
-            }
-            if (phase == kPeel)
-            {
-                if (m >= 0)
-                {
-                    if (FP == SKG_FP64)
-                    {
-                        tau += section * nd * ds;
-                        if (tau >= taumax)
-                        {
-                            tau = INFINITY;
-                            ended = true;
-                            atomicAdd(lds + 4, static_cast<unsigned int>(steps));
-                        }
-                    }
-                    else
-                    {
-                        tau32.add(section32 * static_cast<float>(nd) * static_cast<float>(ds));
-                        if (static_cast<double>(tau32.sum) >= taumax)
-                        {
-                            tau32.sum = INFINITY;
-                            ended = true;
-                            atomicAdd(lds + 4, static_cast<unsigned int>(steps));
-                        }
-                    }
-                }
-                if constexpr (PERSP)
-                {
-                    // MediumSystem::getExtinctionOpticalDepth(pp, distance) (MediumSystem.cpp:
-                    // 2376-2382): the segment that passes the distance still counts
-                    s += ds;
-                    if (!ended && s > slim)
-                    {
-                        ended = true;
-                        atomicAdd(lds + 4, static_cast<unsigned int>(steps));
-                    }
-                }
-            }
-            else if (ds > 0.)
-            {
-                if (FP == SKG_FP64)
-                {
-                    tprev = tau;
-                    sprev = s;
-                    s += ds;
-                    if (m >= 0) tau += section * nd * ds;
-                }
-                else
-                {
-                    tprev = static_cast<double>(tau32.sum);
-                    sprev = static_cast<double>(s32.sum);
-                    s32.add(static_cast<float>(ds));
-                    if (m >= 0) tau32.add(section32 * static_cast<float>(nd) * static_cast<float>(ds));
-                }
-                lastm = m;
-                if constexpr (RF)
-                {
-                    if (phase == kFwd1)
-                    {
-                        const double lnEnd = -tau_value();
-                        const double extEnd = exp(lnEnd);
-                        if (m >= 0)
-                        {
-                            const double em = ln_mean(extEnd, rfExt, lnEnd, -tprev);
-                            unsafeAtomicAdd(rf + static_cast<int64_t>(m) * nl + ell, rfL * em * ds);
-                        }
-                        rfExt = extEnd;
-                    }
-                }
-                // second pass: stop at the first segment whose exit optical depth exceeds tau_int
-                if (phase == kFwd2 && tint < tau_value())
-                {
-                    mint = m;
-                    double tv = tau_value(), sv = s_value();
-                    sint = sprev + ((tint - tprev) / (tv - tprev)) * (sv - sprev);
-                    ended = true;
-                    atomicAdd(lds + 4, static_cast<unsigned int>(steps));
-                    atomicAdd(lds + 7, static_cast<unsigned int>(steps));
-                }
             }
         }
     };

@@ -612,6 +612,7 @@ namespace skg
         {
             int out = 0;
             int act = ost;
+            bool pre = false;
             const char* addr = nullptr;
             // phase 1: the step's own computation and the address of the record it needs
             if (ost == kOsEnd) return -1;
@@ -638,10 +639,7 @@ namespace skg
                 if (hint >= 0)
                     act = kOsStart;  // the hint's record follows
                 else
-                {
-                    resolve(g, search_lds(g, 0));
-                    act = ost;
-                }
+                    pre = true;
             }
             else if (ost == kOsCross)
             {
@@ -699,10 +697,13 @@ namespace skg
                     act = kOsCross;  // the first neighbour's record follows
                 }
                 else
-                {
-                    resolve(g, search_lds(g, 0));
-                    act = ost;
-                }
+                    pre = true;
+            }
+            if (pre)
+            {
+                // the search from the root (a start without hint, a wall on the domain boundary)
+                resolve(g, search_lds(g, 0));
+                act = ost;
             }
             if (act == kOsEnd) return out ? out : -1;
             switch (act)
@@ -718,7 +719,12 @@ namespace skg
             // phase 2: the one load of this step
             RawRec R;
             R.load(addr);
-            // phase 3: what the record decides
+            // phase 3: what the record decides.  Each state ends by adopting the loaded record
+            // as cell `take`, by resolving a search from node `from` or a known cell `known`, or
+            // by a change of state; the adoption and the search are written once for all
+            // states, so a wave whose lanes are in different states executes each once.
+            const bool closed = closed_contains(R, rx, ry, rz);
+            int take = -1, from = -1, known = -1;
             switch (act)
             {
                 case kOsStart:
@@ -732,23 +738,28 @@ namespace skg
                     const bool iny = ry >= R.d(1) && (ry < R.d(4) || R.d(4) == g.box[4]);
                     const bool inz = rz >= R.d(2) && (rz < R.d(5) || R.d(5) == g.box[5]);
                     if (inx && iny && inz)
-                        adopt(R, hint);
+                        take = hint;
                     else
-                        resolve(g, search_lds(g, 0));
+                        from = 0;
                     break;
                 }
                 case kOsCross:
-                    // the first (largest-overlap) neighbour on the crossed wall, when its closed
-                    // box contains the new position; else the rest of the wall's list
-                    if (closed_contains(R, rx, ry, rz))
+                case kOsListRec:
+                    // a neighbour on the crossed wall (the first, largest-overlap one, then the
+                    // rest of the list in order), when its closed box contains the new position
+                    if (closed)
                     {
                         if (cand != i)
-                            adopt(R, cand);
+                            take = cand;
                         else
-                            resolve(g, cand);
+                            known = cand;
                     }
-                    else
+                    else if (act == kOsCross)
                         ost = kOsListStart;
+                    else if (++q < qe)
+                        ost = kOsListId;
+                    else
+                        from = 0;
                     break;
                 case kOsListStart:
                     q = R.off_at(wall) + 1;
@@ -756,42 +767,34 @@ namespace skg
                     if (q < qe)
                         ost = kOsListId;
                     else
-                        resolve(g, search_lds(g, 0));
+                        from = 0;
                     break;
                 case kOsListId:
                     cand = R.first4(q & 3);
                     ost = kOsListRec;
                     break;
-                case kOsListRec:
-                    if (closed_contains(R, rx, ry, rz))
-                    {
-                        if (cand != i)
-                            adopt(R, cand);
-                        else
-                            resolve(g, cand);
-                    }
-                    else if (++q < qe)
-                        ost = kOsListId;
-                    else
-                        resolve(g, search_lds(g, 0));
-                    break;
                 case kOsNode:
                 {
                     const int child = R.i32(6);
                     if (child < 0)
-                        resolve(g, R.i32(7));
+                        known = R.i32(7);
                     else
-                    {
-                        const int nd = child + (rx < R.d(0) ? 0 : 1) + (ry < R.d(1) ? 0 : 2) + (rz < R.d(2) ? 0 : 4);
-                        resolve(g, search_lds(g, nd));
-                    }
+                        from = child + (rx < R.d(0) ? 0 : 1) + (ry < R.d(1) ? 0 : 2) + (rz < R.d(2) ? 0 : 4);
                     break;
                 }
                 default:  // kOsLeaf
-                    adopt(R, cand);
-                    if (locating) hint = i;
+                    take = cand;
                     break;
             }
+            if (take >= 0)
+            {
+                adopt(R, take);
+                if (locating) hint = take;
+            }
+            else if (from >= 0)
+                resolve(g, search_lds(g, from));
+            else if (known >= 0)
+                resolve(g, known);
             if (ost == kOsEnd && !out) return -1;
             return out;
         }

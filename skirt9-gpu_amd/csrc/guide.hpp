@@ -338,7 +338,9 @@ namespace skg
     // layout and outputs returned to their lanes with cross-lane permutes (ds_bpermute), so
     // no LDS is allocated and no workgroup barrier is needed; 16-lane tiles without a wanting
     // lane are skipped.  Every lane of the wavefront must call it (converged control flow).
-    __device__ __forceinline__ void guide_wave_infer_regs(const GuideNet& net, const float* f, bool want, float* out40)
+    // Not inlined: its registers are then not added to those of the traversal state that is live
+    // around the call in the persistent kernel.
+    __device__ __attribute__((noinline)) void guide_wave_infer_regs(const GuideNet& net, const float* f, bool want, float* out40)
     {
         const int lane = __lane_id();
         const int row = lane & 15, grp = lane >> 4;

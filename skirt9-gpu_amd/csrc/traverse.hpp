@@ -337,7 +337,9 @@ namespace skg
         kOsListRec = 4,   // that neighbour's record
         kOsNode = 5,      // a search node below the LDS-staged levels
         kOsLeaf = 6,      // the record of the leaf the search found
-        kOsEnd = 7        // no more segments
+        kOsEnd = 7,       // no more segments
+        kOsSearch = 8     // (a step's action only) the search from the root after a load of no use:
+                          // a start without hint, a wall on the domain boundary
     };
 
     template<int GT> struct Gen
@@ -612,7 +614,6 @@ namespace skg
         {
             int out = 0;
             int act = ost;
-            bool pre = false;
             const char* addr = nullptr;
             // phase 1: the step's own computation and the address of the record it needs
             if (ost == kOsEnd) return -1;
@@ -636,10 +637,7 @@ namespace skg
                     n = 0.;
                     out = 1;
                 }
-                if (hint >= 0)
-                    act = kOsStart;  // the hint's record follows
-                else
-                    pre = true;
+                act = hint >= 0 ? kOsStart : kOsSearch;  // the hint's record follows, or the search
             }
             else if (ost == kOsCross)
             {
@@ -691,23 +689,15 @@ namespace skg
                 out = 1;
                 locating = false;
                 na = false;
-                if (f >= 0)
-                {
-                    cand = f;
-                    act = kOsCross;  // the first neighbour's record follows
-                }
-                else
-                    pre = true;
+                cand = f;
+                act = f >= 0 ? kOsCross : kOsSearch;  // the first neighbour's record follows, or the search
             }
-            if (pre)
-            {
-                // the search from the root (a start without hint, a wall on the domain boundary)
-                resolve(g, search_lds(g, 0));
-                act = ost;
-            }
-            if (act == kOsEnd) return out ? out : -1;
+            // one search for all lanes that need it, after the step's load (a start without hint
+            // and a wall on the domain boundary load a record they do not use, so that the search
+            // is written once, in phase 3)
             switch (act)
             {
+                case kOsSearch: addr = reinterpret_cast<const char*>(g.leaf); break;
                 case kOsStart: addr = reinterpret_cast<const char*>(g.leaf + hint); break;
                 case kOsCross:
                 case kOsListRec:
@@ -772,6 +762,9 @@ namespace skg
                 case kOsListId:
                     cand = R.first4(q & 3);
                     ost = kOsListRec;
+                    break;
+                case kOsSearch:
+                    from = 0;
                     break;
                 case kOsNode:
                 {

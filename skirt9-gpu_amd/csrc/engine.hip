@@ -1788,6 +1788,7 @@ static int runPersistent(skg_engine* e, uint64_t first_history, uint64_t count)
     {
         int* d = nullptr;
         HIPCHK(e, hipMalloc(&d, sizeof(int)));
+        (void)hipGetLastError();  // a launch error below is then this launch's own
         hipLaunchKernelGGL(octree_locate, dim3(1), dim3(1), 0, e->stream, e->grid, e->src[0], e->src[1], e->src[2], d);
         hipError_t st = hipGetLastError();
         if (st == hipSuccess) st = hipMemcpyAsync(&e->srcHint, d, sizeof(int), hipMemcpyDeviceToHost, e->stream);

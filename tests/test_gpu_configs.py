@@ -13,8 +13,10 @@
   within 1 % per-pixel median;
 * config 4 -- guided sampling on the fixture octree at face-on tau_V = 50 with the config's
   5 wavelengths 0.3-2 um: records from plain packets, Adam on the GPU, then the guided
-  persistent engine equals the guided pipeline on the same histories, and the guided total
-  equals the plain total within Monte Carlo error.
+  persistent engine equals the guided pipeline on the same histories, the guided total
+  equals the plain total within Monte Carlo error, and the figure of merit of guided versus
+  plain is printed with its 95 % interval beside the variance ceiling of any direction guide
+  (two continuations of the same first scatterings).
 
 References: TreeSpatialGrid.cpp:143-228, VoronoiMeshSnapshot.cpp:1089-1219,
 MediumSystem.cpp:1785-1844 (optical depth), DustMix.cpp:47-150 (tables),
@@ -22,6 +24,7 @@ DistantInstrument.cpp:55-63 (same-observer sharing), MonteCarloSimulation.cpp:20
 
 import os
 import sys
+import time
 
 import numpy as np
 import pytest
@@ -210,19 +213,44 @@ def test_config4_guided_octree_tau50(eng):
     assert abs(c.sum() / a.sum() - 1) < 1e-9
     nz = a > 0
     assert np.max(np.abs(c[nz] - a[nz]) / a[nz]) < 1e-9
-    # unbiased: the guided total equals the plain total within Monte Carlo error
+    # unbiased: the guided total equals the plain total within Monte Carlo error; the figure of
+    # merit 1/(relative variance x time) of each sampler with a 95 % interval for their ratio
+    # (the ratio of two batch variances follows an F distribution)
+    from scipy import stats
     nb = 16
-    tot = {}
+    tot, secs = {}, {}
     for gm in (0, 1):
         eng.guide_mode(gm, mix=0.5, half_size=float(golden("tree_paths.npz")["half"]))
+        t = []
+        t0 = time.perf_counter()
+        for b in range(nb):
+            eng.frame_clear()
+            eng.run_packets(b * per, per)
+            t.append(eng.frame_read(0).sum())
+        secs[gm] = time.perf_counter() - t0
+        tot[gm] = np.array(t)
+    # the variance left once the first scattering is fixed: two continuations per batch
+    cont = []
+    eng.guide_mode(0)
+    for key in (1, 2):
+        eng.continuation_key(key)
         t = []
         for b in range(nb):
             eng.frame_clear()
             eng.run_packets(b * per, per)
             t.append(eng.frame_read(0).sum())
-        tot[gm] = np.array(t)
-    eng.guide_mode(0)
+        cont.append(np.array(t))
+    eng.continuation_key(0)
     m0, m1 = tot[0].mean(), tot[1].mean()
-    e0, e1 = tot[0].std(ddof=1) / np.sqrt(nb), tot[1].std(ddof=1) / np.sqrt(nb)
+    v0, v1 = tot[0].var(ddof=1), tot[1].var(ddof=1)
+    e0, e1 = np.sqrt(v0 / nb), np.sqrt(v1 / nb)
     print(f"  plain {m0:.6e} +- {e0:.1e}, guided {m1:.6e} +- {e1:.1e}")
     assert abs(m1 - m0) < 4 * np.hypot(e0, e1)
+    fom = {gm: 1.0 / (tot[gm].var(ddof=1) / tot[gm].mean() ** 2 * secs[gm]) for gm in (0, 1)}
+    gain = fom[1] / fom[0]
+    lo, hi = gain / stats.f.ppf(0.975, nb - 1, nb - 1), gain / stats.f.ppf(0.025, nb - 1, nb - 1)
+    ca, cb = cont
+    ceiling = 0.5 * (ca.var(ddof=1) + cb.var(ddof=1)) / np.cov(ca, cb)[0, 1]
+    print(f"  FOM plain {fom[0]:.3e}, guided {fom[1]:.3e}: gain {gain:.2f} (95 % {lo:.2f}-{hi:.2f}); "
+          f"variance ceiling of any direction guide {ceiling:.2f}")
+    assert np.isfinite(gain) and gain > 0

@@ -2464,7 +2464,7 @@ int skg_autotune(skg_engine* e, uint64_t first_history, uint64_t trial_packets, 
     std::vector<Cand> cands;
     const int cus = e->cus;
     if (engines & 1)
-        for (int thr : {12, 16, 24, 32})
+        for (int thr : {12, 16, 20, 24})
             for (int wg : {4 * cus, 8 * cus}) cands.push_back({SKG_MODE_PERSISTENT, thr, wg, e->poolSlots, e->traceBlocks, e->graphIters});
     if (engines & 2)
         for (int64_t pool : {int64_t(1) << 20, int64_t(1) << 21, int64_t(1) << 22})

@@ -382,6 +382,27 @@ namespace skg
         pd.off = -1;
     }
 
+    // The guided scattering direction (defensive mixture of the guide's lobes and HG) and its
+    // weight factor Phi_HG / ((1 - mix) Phi_HG + mix q), from the network outputs o.  Not
+    // inlined: its registers are then not added to those of the traversal state that is live
+    // around the call in the persistent kernel.
+    __device__ __attribute__((noinline)) double guided_direction(const float* o, double g, double mix, Philox& rng,
+                                                                 double& kx, double& ky, double& kz)
+    {
+        LobesF Lb;
+        guide_lobes_f(o, Lb);
+        const double kxo = kx, kyo = ky, kzo = kz;
+        const bool useGuide = rng.uniform() < mix;
+        if (useGuide)
+            guide_sample_f(Lb, rng, kx, ky, kz);
+        else
+            scatter_hg(rng, g, kx, ky, kz);
+        const double c = kxo * kx + kyo * ky + kzo * kz;
+        const double phi = (fabs(g) < 1e-6 ? 1.0 : value_hg(g, c)) / (4 * M_PI);
+        const double q = guide_pdf_f(Lb, kx, ky, kz);
+        return phi / ((1.0 - mix) * phi + mix * q);
+    }
+
     // The event transitions of the life cycle, on a packet loaded from LDS.
     template<int GT, int FP, int V> struct Events
     {
@@ -631,20 +652,8 @@ namespace skg
         // without the guide's factors (packets the guide down-weights are not cut early).
         __device__ __forceinline__ void guided_scatter(const float* o)
         {
-            LobesF Lb;
-            guide_lobes_f(o, Lb);
-            const double g = p.gpar[pk.ell];
-            const double kxo = pk.kx, kyo = pk.ky, kzo = pk.kz;
             if (p.contKey != 0u && pk.nscatt == 0) pk.rng.retag(p.contKey);
-            const bool useGuide = pk.rng.uniform() < p.guideMix;
-            if (useGuide)
-                guide_sample_f(Lb, pk.rng, pk.kx, pk.ky, pk.kz);
-            else
-                scatter_hg(pk.rng, g, pk.kx, pk.ky, pk.kz);
-            const double c = kxo * pk.kx + kyo * pk.ky + kzo * pk.kz;
-            const double phi = (fabs(g) < 1e-6 ? 1.0 : value_hg(g, c)) / (4 * M_PI);
-            const double q = guide_pdf_f(Lb, pk.kx, pk.ky, pk.kz);
-            const double ratio = phi / ((1.0 - p.guideMix) * phi + p.guideMix * q);
+            const double ratio = guided_direction(o, p.gpar[pk.ell], p.guideMix, pk.rng, pk.kx, pk.ky, pk.kz);
             pk.W *= ratio;
             pk.Lthr *= ratio;
             pk.nscatt++;
@@ -797,11 +806,15 @@ namespace skg
             const SimParams& sp = kernel_params();
             if (busy) T.template step<V == kStoreRF, V == kPersp || V == kStoreRF>(sp.grid, wgcnt, sp.rf, sp.nlambda);
 
-            // wave-wide transition round once enough lanes wait
-            unsigned long long waiting = __ballot(T.ended || T.phase == kFetch || (V == kGuided && T.phase == kScatter));
-            unsigned long long active = __ballot(T.phase != kDone);
+            // wave-wide transition round once enough lanes wait for an event (or, guided, for the
+            // network: lanes waiting to scatter call a round only once there are kGuideLanes of
+            // them, so that they do not call transition rounds with few events to handle)
+            const unsigned long long waiting = __ballot(T.ended || T.phase == kFetch);
+            const unsigned long long scatter = V == kGuided ? __ballot(T.phase == kScatter) : 0ull;
+            const unsigned long long active = __ballot(T.phase != kDone);
             if (active == 0) break;
-            if (__popcll(waiting) >= sp.threshold || waiting == active)
+            if (__popcll(waiting) >= sp.threshold || (waiting | scatter) == active
+                || (V == kGuided && __popcll(scatter) >= kGuideLanes))
             {
                 const SimParams& kp = kernel_params();
                 if (T.ended)

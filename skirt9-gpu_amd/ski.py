@@ -6,7 +6,9 @@ an oligochromatic `MonteCarloSimulation` with one `PointSource` (`BlackBodySED` 
 inside a `SpiralStructureGeometryDecorator`) with a `MeanListDustMix` and an
 `OpticalDepthMaterialNormalization`, a `CartesianSpatialGrid` (`LinMesh` axes) or a
 `PolicyTreeSpatialGrid` with a `DensityTreePolicy`, a `VoronoiMeshSpatialGrid` or a
-`TetraMeshSpatialGrid` with sites / vertices from the dust density, `PhotonPacketOptions`,
+`TetraMeshSpatialGrid` with sites / vertices from the dust density -- or one
+`VoronoiMeshMedium` (sites and number or mass densities imported from a text file, with a
+`MeanListDustMix`) on a `VoronoiMeshSpatialGrid` that employs its mesh -- `PhotonPacketOptions`,
 `FrameInstrument`s, `SEDInstrument`s and `PerspectiveInstrument`s.  Anything else raises `UnsupportedSki`, so a caller
 can fall back to the reference.  `run(sim, outdir, prefix)` sets the engine up, runs the
 photon packets and writes `<prefix>_<instrument>_total.fits` per frame as
@@ -143,6 +145,8 @@ class Simulation:
     # ParticleSource instead of the point source: xyzh (n, 4) m, radius (n,) m, temperature (n,) K,
     # kernel 0 uniform / 1 cubic spline
     particles: object = None
+    # VoronoiMeshMedium instead of the GeometricMedium: sites (n, 3) m and number density (n,) m^-3
+    imported: object = None
 
 
 def _one(root, tag):
@@ -157,9 +161,9 @@ def load(path: str) -> Simulation:
     sim = _one(root, "MonteCarloSimulation")
     if sim.get("simulationMode") != "OligoExtinctionOnly":
         raise UnsupportedSki("only simulationMode OligoExtinctionOnly is supported")
-    if not root.findall(".//GeometricMedium"):
-        raise UnsupportedSki("no GeometricMedium")
     media = root.findall(".//media/*")
+    if not media or media[0].tag not in ("GeometricMedium", "VoronoiMeshMedium"):
+        raise UnsupportedSki("a GeometricMedium or a VoronoiMeshMedium is supported")
     sources = root.findall(".//sources/*")
     if len(media) != 1 or len(sources) != 1:
         raise UnsupportedSki("exactly one medium and one source are supported")
@@ -187,7 +191,12 @@ def load(path: str) -> Simulation:
     else:
         raise UnsupportedSki("only a PointSource or a ParticleSource is supported")
     opts = _one(root, "PhotonPacketOptions")
-    disk = _one(root, "ExpDiskGeometry")
+    imported = None
+    if media[0].tag == "VoronoiMeshMedium":
+        imported = _voronoi_medium(path, media[0])
+        disk = ET.Element("ExpDiskGeometry", scaleLength="0 pc", scaleHeight="0 pc")
+    else:
+        disk = _one(root, "ExpDiskGeometry")
     for k in ("minRadius", "maxRadius", "maxZ"):
         if quantity(disk.get(k, "0 pc")) != 0:
             raise UnsupportedSki("truncated exponential disks are not supported")
@@ -202,9 +211,12 @@ def load(path: str) -> Simulation:
     tab = np.column_stack([quantities(mix.get("wavelengths")), quantities(mix.get("extinctionCoefficients")),
                            [float(x) for x in mix.get("albedos").split(",")],
                            [float(x) for x in mix.get("asymmetryParameters").split(",")]])
-    mn = _one(root, "OpticalDepthMaterialNormalization")
-    if mn.get("axis", "Z") != "Z":
-        raise UnsupportedSki("only optical depth normalizations along Z are supported")
+    if imported is None:
+        mn = _one(root, "OpticalDepthMaterialNormalization")
+        if mn.get("axis", "Z") != "Z":
+            raise UnsupportedSki("only optical depth normalizations along Z are supported")
+    else:
+        mn = ET.Element("OpticalDepthMaterialNormalization", opticalDepth="0", wavelength="0.55 micron")
     samp = root.findall(".//SamplingOptions")
     nds = int(samp[0].get("numDensitySamples", "100")) if samp else 100
     grid_el = root.find(".//grid/*")
@@ -229,6 +241,13 @@ def load(path: str) -> Simulation:
             raise UnsupportedSki("only maxDustFraction refinement is supported")
         grid = dict(kind="octree", min_level=int(pol.get("minLevel")), max_level=int(pol.get("maxLevel")),
                     max_fraction=float(pol.get("maxDustFraction")))
+    elif imported is not None:
+        # VoronoiMeshSpatialGrid employing the imported medium's mesh (VoronoiMeshSpatialGrid.cpp:133-141)
+        if grid_el.tag != "VoronoiMeshSpatialGrid" or grid_el.get("policy") != "ImportedMesh":
+            raise UnsupportedSki("a VoronoiMeshMedium runs on a VoronoiMeshSpatialGrid with policy ImportedMesh")
+        if tuple(imported["box"]) != box:
+            raise UnsupportedSki("the grid and the imported medium must have the same domain")
+        grid = dict(kind="voronoi_imported")
     elif grid_el.tag in ("VoronoiMeshSpatialGrid", "TetraMeshSpatialGrid"):
         # vertices / sites sampled from the medium's density (policy DustDensity)
         if grid_el.get("policy") != "DustDensity":
@@ -276,7 +295,59 @@ def load(path: str) -> Simulation:
         force_scattering=opts.get("forceScattering", "true") == "true",
         min_weight_reduction=float(opts.get("minWeightReduction", "1e4")),
         min_scatt_events=int(opts.get("minScattEvents", "0")),
-        path_length_bias=float(opts.get("pathLengthBias", "0.5")), num_density_samples=nds, particles=particles)
+        path_length_bias=float(opts.get("pathLengthBias", "0.5")), num_density_samples=nds, particles=particles,
+        imported=imported)
+
+
+# VoronoiMeshMedium columns (VoronoiMeshMedium.hpp:20-40) and the units a header line may give;
+# without one, x, y, z are in pc, a number density in 1/cm3, a mass density in Msun/pc3
+_DENSITY_UNITS = {"1/m3": 1.0, "/m3": 1.0, "1/cm3": 1e6, "/cm3": 1e6, "1/pc3": PC ** -3,
+                  "kg/m3": 1.0, "g/cm3": 1e3, "Msun/pc3": 1.9891e30 / PC ** 3, "Msun/AU3": 1.9891e30 / AU ** 3}
+
+
+def _voronoi_medium(path, el):
+    """VoronoiMeshMedium (MeshMedium.hpp, ImportedMedium.hpp): the text columns x, y, z and a
+    number density (massType NumberDensity) or a mass density (MassDensity, converted with the
+    dust mix's mass per hydrogen atom, MeanListDustMix.cpp:25), times massFraction, in the
+    units of the '# column i: name (unit)' header lines or the defaults."""
+    from .workloads import MU_DUST
+    for k in ("importMetallicity", "importTemperature", "importVelocity", "importMagneticField",
+              "importVariableMixParams"):
+        if el.get(k, "false") != "false":
+            raise UnsupportedSki(f"VoronoiMeshMedium {k} is not supported")
+    if el.get("useColumns", ""):
+        raise UnsupportedSki("VoronoiMeshMedium useColumns is not supported")
+    mtype = el.get("massType", "MassDensity")
+    if mtype not in ("NumberDensity", "MassDensity"):
+        raise UnsupportedSki("VoronoiMeshMedium massType NumberDensity or MassDensity is supported")
+    mix = el.findall("./materialMix/*")
+    if not mix or mix[0].tag != "MeanListDustMix":
+        raise UnsupportedSki("a VoronoiMeshMedium with a MeanListDustMix is supported")
+    units = ["pc", "pc", "pc", "1/cm3" if mtype == "NumberDensity" else "Msun/pc3"]
+    fname = os.path.join(os.path.dirname(os.path.abspath(path)), el.get("filename"))
+    rows = []
+    with open(fname) as f:
+        for line in f:
+            t = line.strip()
+            if not t:
+                continue
+            if t.startswith("#"):
+                low = t[1:].strip().lower()
+                if low.startswith("column") and "(" in t and ":" in t:
+                    i = int(low.split(":")[0].split()[1]) - 1
+                    if 0 <= i < len(units):
+                        units[i] = t[t.rindex("(") + 1:t.rindex(")")].strip()
+                continue
+            rows.append([float(v) for v in t.split()[:4]])
+    a = np.array(rows, dtype=np.float64).reshape(-1, 4)
+    if units[3] not in _DENSITY_UNITS:
+        raise UnsupportedSki(f"density unit {units[3]} is not supported")
+    sites = a[:, :3] * np.array([quantity("1 " + u) for u in units[:3]])
+    dens = a[:, 3] * _DENSITY_UNITS[units[3]] * float(el.get("massFraction", "1"))
+    if mtype == "MassDensity":
+        dens = dens / MU_DUST
+    box = tuple(quantity(el.get(k)) for k in ("minX", "minY", "minZ", "maxX", "maxY", "maxZ"))
+    return dict(sites=sites, density=dens, box=box)
 
 
 _PARTICLE_COLUMNS = (("position x", "pc"), ("position y", "pc"), ("position z", "pc"), ("size h", "pc"),
@@ -535,6 +606,20 @@ def read_fits(path):
     return np.frombuffer(raw, dtype=">f4", count=n, offset=pos).astype(np.float64).reshape(dims[::-1])
 
 
+def imported_cell_densities(sites, density, cell_sites):
+    """The density of each cell of a mesh built from imported sites: the mesh's cells come in
+    the builder's order (sorted by x), so the file's rows are matched to them by coordinates."""
+    if len(cell_sites) != len(sites):
+        raise UnsupportedSki("imported sites outside the domain or closer than eps are not supported")
+    order_in = np.lexsort(np.asarray(sites).T[::-1])
+    order_cells = np.lexsort(np.asarray(cell_sites).T[::-1])
+    if not np.array_equal(np.asarray(sites)[order_in], np.asarray(cell_sites)[order_cells]):
+        raise UnsupportedSki("the mesh's sites differ from the imported ones")
+    dens = np.empty(len(order_cells))
+    dens[order_cells] = np.asarray(density)[order_in]
+    return dens
+
+
 def setup(sim: Simulation, engine, threads=0, npk_norm=None):
     """Builds the grid and medium on the host, uploads them and configures the engine."""
     from . import workloads as W
@@ -548,10 +633,11 @@ def setup(sim: Simulation, engine, threads=0, npk_norm=None):
     sext, ssca, g = pkg.dust_meanlist(tab[:, 0], tab[:, 1], tab[:, 2], tab[:, 3], sim.wavelengths, mu)
     sv, _, _ = pkg.dust_meanlist(tab[:, 0], tab[:, 1], tab[:, 2], tab[:, 3], [sim.tau_wavelength], mu)
     sp = sim.spiral_params
-    model = pkg.disk_model(tau_v=sim.tau_v, sext_v=float(sv[0]), spiral=sim.spiral, scale_length=sim.scale_length,
-                           scale_height=sim.scale_height, **({} if not sp else dict(
-                               arms=sp["arms"], pitch=sp["pitch"], radius_zero=sp["radius_zero"],
-                               phase_zero=sp["phase_zero"], weight=sp["weight"], index=sp["index"])))
+    model = None if sim.imported is not None else pkg.disk_model(
+        tau_v=sim.tau_v, sext_v=float(sv[0]), spiral=sim.spiral, scale_length=sim.scale_length,
+        scale_height=sim.scale_height, **({} if not sp else dict(
+            arms=sp["arms"], pitch=sp["pitch"], radius_zero=sp["radius_zero"], phase_zero=sp["phase_zero"],
+            weight=sp["weight"], index=sp["index"])))
     h = sim.box[3]
     if sim.grid["kind"] == "cartesian":
         borders = []
@@ -563,6 +649,16 @@ def setup(sim: Simulation, engine, threads=0, npk_norm=None):
         # cell volumes, cell index k + nz*j + nz*ny*i (CartesianSpatialGrid.cpp:200-203)
         dx, dy, dz = (np.diff(bb) for bb in borders)
         sim.cell_volumes = (dx[:, None, None] * dy[None, :, None] * dz[None, None, :]).ravel()
+    elif sim.grid["kind"] == "voronoi_imported":
+        # the imported sites' mesh; each cell keeps its site's density (cells in the builder's
+        # order, sorted by x, matched to the file's rows by their coordinates)
+        imp = sim.imported
+        mesh = pkg.Voronoi.build(imp["sites"], sim.box, threads)
+        ex = mesh.export()
+        dens = imported_cell_densities(imp["sites"], imp["density"], ex["sites"])
+        mesh.upload(engine)
+        sim.cell_volumes = ex["volume"]
+        mesh.close()
     elif sim.grid["kind"] in ("voronoi", "tetra"):
         # TetraMeshSpatialGrid / VoronoiMeshSpatialGrid with the DustDensity policy: points
         # distributed as the medium (TetraMeshSpatialGrid.cpp:23-40), then the mesh

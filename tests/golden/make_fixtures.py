@@ -42,6 +42,11 @@ Every fixture is data: seeded inputs plus the reference's outputs for them.
                    (ParticleSource, skigen.disk_particles) instead of the point source, with the
                    cubic-spline and the uniform smoothing kernel, 100x100 frame at 60 deg:
                    per-pixel mean and standard error over independent runs.
+  image_clumpy.npz (--clumpy) the config-2 medium at fixture size: the lognormal clumpy
+                   medium of skirt9-gpu_amd/workloads.py voronoi_clumpy (20000 sites, seed 1),
+                   written to a text file and imported by the reference as a VoronoiMeshMedium
+                   on a VoronoiMeshSpatialGrid with policy ImportedMesh, three 100x100 frames at
+                   0/45/90 deg: per-pixel mean and standard error over independent runs.
   rf_cfg0.npz      (--rf) the same workload on a 16^3 grid (1e4 density samples per cell)
                    storing the radiation field:
                    the RadiationFieldProbe's mean intensity J_nu per cell (PerCellForm),
@@ -561,6 +566,50 @@ def fixtures_particles(tmp, runs=8, packets=1e6):
     np.savez_compressed(os.path.join(HERE, "image_particles.npz"), **out)
 
 
+CLUMPY_FRAMES = (("i0", 0.0, 0.0), ("i45", 45.0, 0.0), ("i90", 90.0, 0.0))
+
+
+def clumpy_ski(dirname, packets, seed):
+    """The config-2 medium at fixture size as a ski file with its VoronoiMeshMedium sites file
+    (shared by this generator and the GPU test)."""
+    import importlib.util
+    name = "skirt9_gpu_amd"
+    if name not in sys.modules:
+        spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "skirt9-gpu_amd", "__init__.py"),
+                                                      submodule_search_locations=[os.path.join(ROOT, "skirt9-gpu_amd")])
+        mod = importlib.util.module_from_spec(spec)
+        sys.modules[name] = mod
+        spec.loader.exec_module(mod)
+    from skirt9_gpu_amd import workloads
+    wl = workloads.voronoi_clumpy(num_sites=20000, threads=8)
+    skigen.write_voronoi_medium(os.path.join(dirname, "clumpy_sites.txt"), wl.tables["mesh"].export()["sites"],
+                                wl.density)
+    spec = skigen.SkiSpec(num_packets=packets, seed=seed, grid=("voronoi_imported",), medium_file="clumpy_sites.txt",
+                          num_pixels=100, instruments=CLUMPY_FRAMES)
+    path = os.path.join(dirname, f"clumpy{seed}.ski")
+    with open(path, "w") as f:
+        f.write(skigen.render(spec))
+    return path
+
+
+def fixtures_clumpy(tmp, runs=8, packets=1e6):
+    outdir = os.path.join(tmp, "clumpy")
+    os.makedirs(outdir, exist_ok=True)
+    imgs = {n: [] for n, _, _ in CLUMPY_FRAMES}
+    for s in range(runs):
+        ski = clumpy_ski(outdir, packets, 9000 + s)
+        run("run", ski, outdir, 8)
+        for n, _, _ in CLUMPY_FRAMES:
+            imgs[n].append(fits_image(os.path.join(outdir, f"clumpy{9000 + s}_{n}_total.fits")))
+    out = {"runs": np.array(runs), "packets": np.array(packets)}
+    for n, v in imgs.items():
+        v = np.array(v)
+        out[n + "_mean"] = v.mean(axis=0).astype(np.float32)
+        out[n + "_sem"] = (v.std(axis=0, ddof=1) / np.sqrt(runs)).astype(np.float32)
+    np.savez_compressed(os.path.join(HERE, "image_clumpy.npz"), **out)
+    print("image_clumpy.npz", {k: v.shape for k, v in out.items()})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--images", action="store_true")
@@ -569,6 +618,7 @@ def main():
     ap.add_argument("--meshes", action="store_true")
     ap.add_argument("--persp", action="store_true")
     ap.add_argument("--particles", action="store_true")
+    ap.add_argument("--clumpy", action="store_true")
     ap.add_argument("--only", default="")
     a = ap.parse_args()
     with tempfile.TemporaryDirectory() as tmp:
@@ -595,6 +645,8 @@ def main():
             fixtures_persp(tmp)
         if a.particles:
             fixtures_particles(tmp)
+        if a.clumpy:
+            fixtures_clumpy(tmp)
 
 
 if __name__ == "__main__":

@@ -200,3 +200,46 @@ def test_run_ski_everything_combined(tmp_path):
     lum_frame = frame[0].sum() / S.calibration(sim, fr)[0]
     lum_sed = sed[0, 1] / S.sed_calibration(sim, sd)[0]
     assert lum_sed >= lum_frame * (1 - 1e-5)
+
+
+def test_config2_clumpy_voronoi_matches_reference(tmp_path):
+    """Config 2 at fixture size: the lognormal clumpy medium on 20000 Voronoi sites (the bench's
+    workload generator at that size), written to a text file and imported as a
+    VoronoiMeshMedium on a VoronoiMeshSpatialGrid with policy ImportedMesh, three frames at
+    0/45/90 deg, run through the ski front end on the GPU against the reference run on the same
+    ski and sites file (tests/golden/image_clumpy.npz, 8 runs of 1e6 packets): per frame the
+    total within 4 sigma and the per-pixel parity at native resolution (100 x 100)
+    (VoronoiMeshSnapshot.cpp:1089-1219, ImportedMedium, DistantInstrument.cpp:55-63)."""
+    pkg = load_pkg()
+    from skirt9_gpu_amd import ski as S
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_fixtures as MF
+    path = MF.clumpy_ski(str(tmp_path), 1e6, 1)
+    sim = S.load(path)
+    assert sim.grid == dict(kind="voronoi_imported") and len(sim.imported["sites"]) == 20000
+    eng = pkg.Engine(0)
+    nb, per = 8, 1_000_000
+    names = [n for n, _, _ in MF.CLUMPY_FRAMES]
+    imgs = {n: [] for n in names}
+    for b in range(nb):
+        sim.seed = 9100 + b
+        sim.num_packets = per
+        print("case ski clumpy batch", b, flush=True)
+        out = S.run(sim, str(tmp_path), f"c{b}", engine=eng, log=lambda *a: None)
+        for n in names:
+            imgs[n].append(out[n][0])
+    eng.close()
+    f = golden("image_clumpy.npz")
+    nref, pref = int(f["runs"]), float(f["packets"])
+    for n in names:
+        ref, rsem = f[n + "_mean"][0].astype(float), f[n + "_sem"][0].astype(float)
+        im = np.array(imgs[n])
+        tot_sem = im.sum(axis=(1, 2)).std(ddof=1) / np.sqrt(nb)
+        print(f"  {n}: total {im.mean(0).sum():.5e} +- {tot_sem:.1e}, reference {ref.sum():.5e}", flush=True)
+        assert abs(im.mean(0).sum() - ref.sum()) < 4 * np.hypot(tot_sem, np.sqrt((rsem ** 2).sum())), n
+        r = pixel_parity(im, per, ref, rsem, nref, pref, max_rel=0.1)
+        print(f"  {n} per pixel: {r['pixels']} pixels, chi2/dof {r['chi2_dof']:.3f} (expected {r['expected']:.3f} "
+              f"+- {r['tol']:.3f}), |z| > 4: {r['frac4']:.4f} (expected {r['expected_frac4']:.4f})", flush=True)
+        assert r["pixels"] > 1000, n
+        assert abs(r["chi2_dof"] - r["expected"]) < r["tol"], n
+        assert r["frac4"] < r["expected_frac4"] + r["tol_frac4"], n

@@ -263,3 +263,54 @@ def test_particle_source_parse(ski, tmp_path):
             f.write(bad)
         with pytest.raises(ski.UnsupportedSki):
             ski.load(p)
+
+
+def test_voronoi_mesh_medium_parse(ski, tmp_path):
+    """VoronoiMeshMedium on a VoronoiMeshSpatialGrid with policy ImportedMesh: the site and
+    density columns in the units of the '# column' header lines (or the defaults pc and 1/cm3),
+    a MassDensity column divided by the dust mix's mass per hydrogen atom, massFraction applied;
+    the cells take their sites' densities in the mesh's order; unsupported options raise."""
+    rng = np.random.default_rng(3)
+    n = 400
+    sites = (rng.random((n, 3)) * 2 - 1) * 9 * PC
+    dens = rng.lognormal(0, 1, n) * 1e6
+    skigen.write_voronoi_medium(os.path.join(str(tmp_path), "sites.txt"), sites, dens)
+    spec = skigen.SkiSpec(num_packets=1e4, grid=("voronoi_imported",), medium_file="sites.txt")
+    p = _write(str(tmp_path), "vm", spec)
+    sim = ski.load(p)
+    assert sim.grid == dict(kind="voronoi_imported")
+    np.testing.assert_allclose(sim.imported["sites"], sites, rtol=1e-15, atol=64.0)
+    np.testing.assert_allclose(sim.imported["density"], dens, rtol=1e-15)
+    # defaults without header lines: pc and 1/cm3
+    with open(os.path.join(str(tmp_path), "sites.txt"), "w") as f:
+        np.savetxt(f, np.column_stack([sites / PC, dens]), fmt="%.17g")
+    np.testing.assert_allclose(ski.load(p).imported["density"], dens * 1e6, rtol=1e-15)
+    text = open(p).read()
+    md = text.replace('massType="NumberDensity"', 'massType="MassDensity"').replace('massFraction="1"',
+                                                                                  'massFraction="0.5"')
+    with open(p, "w") as f:
+        f.write(md)
+    mu = 1.5e-29
+    Msun = 1.9891e30
+    np.testing.assert_allclose(ski.load(p).imported["density"], 0.5 * dens * Msun / PC ** 3 / mu, rtol=1e-12)
+    # the cells' densities follow the builder's order of the sites
+    with open(p, "w") as f:
+        f.write(text)
+    skigen.write_voronoi_medium(os.path.join(str(tmp_path), "sites.txt"), sites, dens)
+    sim = ski.load(p)
+    pkg = load_pkg()
+    mesh = pkg.Voronoi.build(sim.imported["sites"], sim.box)
+    ex = mesh.export()
+    mesh.close()
+    order = {tuple(s): d for s, d in zip(sim.imported["sites"], sim.imported["density"])}
+    want = np.array([order[tuple(s)] for s in ex["sites"]])
+    assert not np.array_equal(want, sim.imported["density"])  # the builder reorders (by x)
+    np.testing.assert_array_equal(ski.imported_cell_densities(sim.imported["sites"], sim.imported["density"],
+                                                               ex["sites"]), want)
+    for bad in (text.replace('importVelocity="false"', 'importVelocity="true"'),
+                text.replace('policy="ImportedMesh"', 'policy="DustDensity" numSites="100"'),
+                text.replace('massType="NumberDensity"', 'massType="Mass"')):
+        with open(p, "w") as f:
+            f.write(bad)
+        with pytest.raises(ski.UnsupportedSki):
+            ski.load(p)

@@ -345,9 +345,12 @@ namespace skg
         const int lane = __lane_id();
         const int row = lane & 15, grp = lane >> 4;
         const unsigned long long need = __ballot(want);
+        bool act[4];
+        bf16x4 x[4];
+#pragma unroll
         for (int t = 0; t < 4; ++t)
         {
-            if (!((need >> (16 * t)) & 0xFFFFull)) continue;
+            act[t] = ((need >> (16 * t)) & 0xFFFFull) != 0;
             // features 4*grp .. 4*grp+3 of sample t*16 + row (zero beyond the 9 features)
             const int src = t * 16 + row;
             float xs[4] = {0.f, 0.f, 0.f, 0.f};
@@ -357,26 +360,113 @@ namespace skg
                 float v = __shfl(f[k], src);
                 if (grp == (k >> 2)) xs[k & 3] = v;
             }
-            bf16x4 x;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) x[q] = static_cast<short>(f2bf(xs[q]));
-            f32x4 o[3];
-            guide_tile_forward(net, x, o, nullptr, nullptr, nullptr);
-            // output 16*ot + 4*g + q of sample t*16 + r is o[ot][q] of lane r + 16*g
+            for (int q = 0; q < 4; ++q) x[t][q] = static_cast<short>(f2bf(xs[q]));
+        }
+        // the layers in turn, each applied to every 16-sample tile with a wanting lane, so that a
+        // layer's weights are loaded once per call (the same MFMA sequence per tile as
+        // guide_tile_forward)
+        bf16x4 cur[4][4];
+        {
+            bf16x4 w[4];
+            float bias[4][4];
 #pragma unroll
-            for (int ot = 0; ot < 3; ++ot)
+            for (int o = 0; o < 4; ++o)
+            {
+                w[o] = load_bf4(net.W[0] + (o * 16 + row) * 16 + grp * 4);
 #pragma unroll
-                for (int g = 0; g < 4; ++g)
+                for (int q = 0; q < 4; ++q) bias[o][q] = net.b[0][o * 16 + grp * 4 + q];
+            }
 #pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                    {
-                        const int j = 16 * ot + 4 * g + q;
-                        if (j < kGuideOut)
+            for (int t = 0; t < 4; ++t)
+            {
+                if (!act[t]) continue;
+#pragma unroll
+                for (int o = 0; o < 4; ++o)
+                {
+                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                    acc = mfma16(w[o], x[t], acc);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc[q] = fmaxf(acc[q] + bias[o][q], 0.f);
+                    cur[t][o] = pack_bf4(acc);
+                }
+            }
+        }
+#pragma unroll
+        for (int l = 1; l <= 2; ++l)
+        {
+            bf16x4 w[4][4];
+            float bias[4][4];
+#pragma unroll
+            for (int o = 0; o < 4; ++o)
+            {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) w[o][k] = load_bf4(net.W[l] + (o * 16 + row) * 64 + k * 16 + grp * 4);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) bias[o][q] = net.b[l][o * 16 + grp * 4 + q];
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+            {
+                if (!act[t]) continue;
+                bf16x4 nxt[4];
+#pragma unroll
+                for (int o = 0; o < 4; ++o)
+                {
+                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) acc = mfma16(w[o][k], cur[t][k], acc);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc[q] = fmaxf(acc[q] + bias[o][q], 0.f);
+                    nxt[o] = pack_bf4(acc);
+                }
+#pragma unroll
+                for (int o = 0; o < 4; ++o) cur[t][o] = nxt[o];
+            }
+        }
+        {
+            // output layer: 3 output tiles (48 rows, 40 used); outputs back to their lanes
+            bf16x4 w[3][4];
+            float bias[3][4];
+#pragma unroll
+            for (int o = 0; o < 3; ++o)
+            {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) w[o][k] = load_bf4(net.W[3] + (o * 16 + row) * 64 + k * 16 + grp * 4);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) bias[o][q] = net.b[3][o * 16 + grp * 4 + q];
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+            {
+                if (!act[t]) continue;
+                f32x4 o3[3];
+#pragma unroll
+                for (int o = 0; o < 3; ++o)
+                {
+                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) acc = mfma16(w[o][k], cur[t][k], acc);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc[q] += bias[o][q];
+                    o3[o] = acc;
+                }
+                // output 16*ot + 4*g + q of sample t*16 + r is o3[ot][q] of lane r + 16*g
+#pragma unroll
+                for (int ot = 0; ot < 3; ++ot)
+#pragma unroll
+                    for (int g = 0; g < 4; ++g)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
                         {
-                            float v = __shfl(o[ot][q], row + 16 * g);
-                            if (grp == t) out40[j] = v;
+                            const int j = 16 * ot + 4 * g + q;
+                            if (j < kGuideOut)
+                            {
+                                float v = __shfl(o3[ot][q], row + 16 * g);
+                                if (grp == t) out40[j] = v;
+                            }
                         }
-                    }
+            }
         }
     }
 

@@ -258,9 +258,9 @@ namespace skg
             double ds, nd;
             const int r = gen.advance(grid, m, ds, nd);
             if (r == 0) return;  // an octree step that loaded a search node or list entry
-            // the traversal ends at the path's end, at the early-out optical depth of a peel-off,
-            // past a perspective peel-off's distance, or at a second pass's interaction; the
-            // crossing counters are added once, where it ends
+            // the traversal ends at the path's end (r < 0, or r == 2 after its last segment), at
+            // the early-out optical depth of a peel-off, past a perspective peel-off's distance, or
+            // at a second pass's interaction; the crossing counters are added once, where it ends
             bool fin = r < 0 || ++steps > kMaxSteps;
             if (!fin)
             {
@@ -336,6 +336,7 @@ namespace skg
                     }
                 }
             }
+            fin = fin || r == 2;
             if (fin)
             {
                 ended = true;

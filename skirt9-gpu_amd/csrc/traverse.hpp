@@ -607,9 +607,10 @@ namespace skg
 
         // One step of the octree generator (TreeSpatialGrid::MySegmentGenerator::next,
         // TreeSpatialGrid.cpp:143-228, split so that a step makes at most one dependent global
-        // load): 1 with the segment (m, ds, n) when the step completed one, 0 when it only
-        // progressed (a start, a neighbour-list entry, a search level), -1 at the end of the path.
-        // The sequence of segments equals next()'s.
+        // load): 1 with the segment (m, ds, n) when the step completed one, 2 with the path's last
+        // segment (the path ends after it), 0 when it only progressed (a start, a neighbour-list
+        // entry, a search level), -1 at the end of the path.  The sequence of segments equals
+        // next()'s.
         __device__ __forceinline__ int advance_octree(const GridView& g, int& m, double& ds, double& n)
         {
             int out = 0;
@@ -691,6 +692,15 @@ namespace skg
                 na = false;
                 cand = f;
                 act = f >= 0 ? kOsCross : kOsSearch;  // the first neighbour's record follows, or the search
+                // a wall without neighbours is on the domain's boundary: outside the root's closed
+                // box the search finds no leaf (TreeNode::leafChild), and the path ends here
+                if (f < 0 && !(rx >= g.box[0] && rx <= g.box[3] && ry >= g.box[1] && ry <= g.box[4] && rz >= g.box[2]
+                               && rz <= g.box[5]))
+                {
+                    ost = kOsEnd;
+                    state = kOutside;
+                    return 2;
+                }
             }
             // one search for all lanes that need it, after the step's load (a start without hint
             // and a wall on the domain boundary load a record they do not use, so that the search
@@ -788,7 +798,7 @@ namespace skg
                 resolve(g, search_lds(g, from));
             else if (known >= 0)
                 resolve(g, known);
-            if (ost == kOsEnd && !out) return -1;
+            if (ost == kOsEnd) return out ? 2 : -1;
             return out;
         }
 

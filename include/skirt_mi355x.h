@@ -247,8 +247,7 @@ int skg_tune(skg_engine* e, int32_t workgroups, int32_t threads, int32_t transit
  * run of `trial_packets` histories (from first_history on; the detector contents, the stored
  * radiation field and the counters are restored afterwards) for each candidate launch
  * configuration -- persistent
- * engine: transition threshold 12/16/20/24 (or 24/32/48/64 on a tetrahedral grid)
- * x 4/8 workgroups per CU; pipeline engine: 2^20/2^21/
+ * engine: transition threshold 12/16/20/24/32/48/64 x 4/8 workgroups per CU; pipeline engine: 2^20/2^21/
  * 2^22 packet slots x 8/16 graph iterations x 4/8 trace workgroups per CU -- and keeps the
  * fastest.  engines: bit 0 persistent, bit 1 pipeline (batched instances, particle sources
  * and radiation-field storage: persistent only).

@@ -2463,13 +2463,11 @@ int skg_autotune(skg_engine* e, uint64_t first_history, uint64_t trial_packets, 
     };
     std::vector<Cand> cands;
     const int cus = e->cus;
-    // transition thresholds: the octree, Cartesian and Voronoi kernels are fastest around 16-24
-    // waiting lanes; the tetra kernel (four waves per SIMD, with spilled transition code) with
-    // rounds only once every lane of the wave waits (64)
-    const std::vector<int> thresholds =
-        e->gridType == SKG_GRID_TETRA ? std::vector<int>{24, 32, 48, 64} : std::vector<int>{12, 16, 20, 24};
+    // transition thresholds: the octree and Voronoi kernels are fastest around 16-24 waiting
+    // lanes, the Cartesian and tetra kernels with rounds only once every lane of the wave waits
+    // (64); the candidates cover both
     if (engines & 1)
-        for (int thr : thresholds)
+        for (int thr : {12, 16, 20, 24, 32, 48, 64})
             for (int wg : {4 * cus, 8 * cus}) cands.push_back({SKG_MODE_PERSISTENT, thr, wg, e->poolSlots, e->traceBlocks, e->graphIters});
     if (engines & 2)
         for (int64_t pool : {int64_t(1) << 20, int64_t(1) << 21, int64_t(1) << 22})

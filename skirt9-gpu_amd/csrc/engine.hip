@@ -177,6 +177,13 @@ namespace skg
         if (i < n) rec[i].dens = dens[i];
     }
 
+    // and into the header slot of each Voronoi cell's run in the entry table
+    __global__ void fill_vor_run_density(VorNb* nb, const VorCell* cells, const double* dens, int64_t n)
+    {
+        int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+        if (i < n) nb[cells[i].begin].x = dens[i];
+    }
+
     template<int GT, int FP>
     __global__ void trace_tau(GridView g, const double* __restrict__ dens, const float* __restrict__ dens32,
                               double section, double taumax, const double* __restrict__ rays, int64_t n,
@@ -540,7 +547,11 @@ namespace
         if (e->gridType == SKG_GRID_OCTREE)
             hipLaunchKernelGGL(fill_record_density<OctLeaf>, grid, block, 0, e->stream, const_cast<OctLeaf*>(e->grid.leaf), e->dens, n);
         else if (e->gridType == SKG_GRID_VORONOI)
+        {
             hipLaunchKernelGGL(fill_record_density<VorCell>, grid, block, 0, e->stream, const_cast<VorCell*>(e->grid.vcell), e->dens, n);
+            hipLaunchKernelGGL(fill_vor_run_density, grid, block, 0, e->stream, const_cast<VorNb*>(e->grid.vnb), e->grid.vcell,
+                               e->dens, n);
+        }
         else
             hipLaunchKernelGGL(fill_record_density<TetCell>, grid, block, 0, e->stream, const_cast<TetCell*>(e->grid.tcell), e->dens, n);
         HIPCHK(e, hipGetLastError());
@@ -989,33 +1000,45 @@ int skg_grid_voronoi(skg_engine* e, int64_t nsites, const double* sites, const i
         return fail(e, SKG_ERR_ARGUMENT, "skg_grid_voronoi: bad arguments");
     if (!(box[0] < box[3] && box[1] < box[4] && box[2] < box[5])) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_voronoi: empty box");
     std::vector<VorCell> cells(nsites);
-    std::vector<VorNb> nbs;
+    // one run per cell in the entry table: a header slot, then the neighbour entries
+    int64_t slots = 0;
     for (int64_t m = 0; m < nsites; ++m)
     {
+        if (nb_begin[m] < 0 || nb_count[m] < 0) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_voronoi: bad neighbour range");
         VorCell& c = cells[m];
         std::memset(&c, 0, sizeof(c));
         c.x = sites[3 * m];
         c.y = sites[3 * m + 1];
         c.z = sites[3 * m + 2];
-        if (nb_begin[m] < 0 || nb_count[m] < 0) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_voronoi: bad neighbour range");
-        c.begin = static_cast<int32_t>(nbs.size());
+        c.begin = static_cast<int32_t>(std::min<int64_t>(slots, INT32_MAX));
         c.count = nb_count[m];
-        for (int32_t q = nb_begin[m]; q < nb_begin[m] + nb_count[m]; ++q)
+        slots += 1 + static_cast<int64_t>(nb_count[m]);
+        if (slots + 3 > static_cast<int64_t>(INT32_MAX)) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_voronoi: too many neighbours");
+    }
+    std::vector<VorNb> nbs(static_cast<size_t>(slots + 3));
+    std::memset(nbs.data(), 0, nbs.size() * sizeof(VorNb));
+    for (int64_t m = 0; m < nsites; ++m)
+    {
+        const VorCell& c = cells[m];
+        VorNb& H = nbs[c.begin];
+        H.id = kVorHeader;
+        H.run = c.count;  // the density (x) is filled with the medium
+        for (int32_t q = 0; q < nb_count[m]; ++q)
         {
-            int32_t id = nb_index[q];
+            int32_t id = nb_index[static_cast<int64_t>(nb_begin[m]) + q];
             if (id < -6 || id >= nsites) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_voronoi: neighbour id out of range");
-            VorNb E{};
+            VorNb& E = nbs[static_cast<size_t>(c.begin) + 1 + q];
             E.id = id;
+            E.run = id >= 0 ? cells[id].begin : -1;
             if (id >= 0)
             {
                 E.x = sites[3 * static_cast<int64_t>(id)];
                 E.y = sites[3 * static_cast<int64_t>(id) + 1];
                 E.z = sites[3 * static_cast<int64_t>(id) + 2];
             }
-            nbs.push_back(E);
-            if (nbs.size() > static_cast<size_t>(INT32_MAX)) return fail(e, SKG_ERR_ARGUMENT, "skg_grid_voronoi: too many neighbours");
         }
     }
+    for (int q = 0; q < 3; ++q) nbs[static_cast<size_t>(slots) + q].id = kVorHeader;  // padding
     const int64_t nb3 = static_cast<int64_t>(block_n) * block_n * block_n;
     std::vector<int32_t> boff(nb3 + 1, 0), bids;
     for (int64_t b = 0; b < nb3; ++b)
@@ -1036,7 +1059,7 @@ int skg_grid_voronoi(skg_engine* e, int64_t nsites, const double* sites, const i
     int st;
     if ((st = gridUpload(e, cells.data(), cells.size(), &g.vcell)) || (st = gridUpload(e, boff.data(), boff.size(), &g.vbl_off)))
         return st;
-    if (!nbs.empty() && (st = gridUpload(e, nbs.data(), nbs.size(), &g.vnb))) return st;
+    if ((st = gridUpload(e, nbs.data(), nbs.size(), &g.vnb))) return st;
     if (!bids.empty() && (st = gridUpload(e, bids.data(), bids.size(), &g.vbl_id))) return st;
     g.vbn = block_n;
     for (int q = 0; q < 6; ++q) g.box[q] = box[q];

@@ -27,7 +27,8 @@ namespace skg
     };
 
     // Voronoi cell record (64 bytes, indexed by cell): the site, the cell's number density and
-    // its neighbour list in the entry table.
+    // its run in the entry table (begin: the run's header slot; count: its neighbour entries).
+    // Read at a path's start; a crossing reads only the entry table.
     struct alignas(64) VorCell
     {
         double x, y, z;
@@ -37,16 +38,22 @@ namespace skg
     };
     static_assert(sizeof(VorCell) == 64, "Voronoi cell record must be 64 bytes");
 
-    // One neighbour entry (32 bytes): the neighbour's site is stored with its id so that a
-    // cell's exit search streams through one contiguous run of entries instead of gathering
-    // the neighbour sites.  Domain walls have ids -1..-6 (xmin, xmax, ymin, ymax, zmin, zmax).
+    // The entry table holds one run per cell: a header slot (id kVorHeader, x = the cell's
+    // number density, run = the number of neighbour entries) followed by one entry (32 bytes)
+    // per neighbour: the neighbour's site, its id and the header slot of its own run.  A cell's
+    // exit search streams through its run, and the entry it leaves through gives the next
+    // cell's site and run, so a crossing makes one dependent read (the run) instead of two (the
+    // cell record, then its neighbours).  Domain walls have ids -1..-6 (xmin, xmax, ymin, ymax,
+    // zmin, zmax) and run -1.  The table ends with 3 slots of padding (a run is read from its
+    // header in groups of four slots).
     struct alignas(32) VorNb
     {
         double x, y, z;
         int32_t id;
-        int32_t pad;
+        int32_t run;
     };
     static_assert(sizeof(VorNb) == 32, "Voronoi neighbour entry must be 32 bytes");
+    constexpr int32_t kVorHeader = -100;
 
     // Tetrahedron record (256 bytes, two cache lines, indexed by cell): vertex coordinates,
     // density and face links in the first line, the outward unit face normals in the second.
@@ -184,7 +191,7 @@ namespace skg
         const VorCell& C = g.vcell[h];
         const double hx = x - C.x, hy = y - C.y, hz = z - C.z;
         const double dh = hx * hx + hy * hy + hz * hz;
-        for (int q = C.begin, e = C.begin + C.count; q < e; ++q)
+        for (int q = C.begin + 1, e = C.begin + 1 + C.count; q < e; ++q)
         {
             const VorNb E = g.vnb[q];
             if (E.id < 0) continue;
@@ -347,7 +354,9 @@ namespace skg
         double rx, ry, rz, kx, ky, kz;
         int state;
         int i, j, k;  // Cartesian cell indices; octree, Voronoi, tetra: i = current cell
-                      // (tetra: j = entering face, -1 if unknown)
+                      // (tetra: j = entering face, -1 if unknown; Voronoi: j = the header slot
+                      // of the cell's run in the entry table)
+        double sx, sy, sz;  // Voronoi: the current cell's site
         double b0, b1, b2, bn;  // octree: the current cell's wall ahead on each axis, its density
         int f0, f1, f2;         // octree: the first neighbour across each of those walls
         int hint;  // octree, Voronoi, tetra: a cell that probably contains the start point (-1:
@@ -844,6 +853,7 @@ namespace skg
                         state = kOutside;
                         return false;
                     }
+                    vor_enter(g, i);
                 }
                 else
                 {
@@ -914,15 +924,26 @@ namespace skg
             return false;  // the octree steps through advance_octree
         }
 
+        // the cell c becomes the current cell, from its record: its site and its run
+        __device__ __forceinline__ void vor_enter(const GridView& g, int c)
+        {
+            const VorCell& C = g.vcell[c];
+            j = C.begin;
+            sx = C.x;
+            sy = C.y;
+            sz = C.z;
+        }
+
         // exit through the nearest bisector plane or domain wall (VoronoiMeshSnapshot.cpp:1119-1210)
         __device__ __forceinline__ bool next_voronoi(const GridView& g, int& m, double& ds, double& n)
         {
             for (int tries = 0; tries < kMaxNoExit; ++tries)
             {
-                const VorCell& C = g.vcell[i];
-                const double px0 = C.x, py0 = C.y, pz0 = C.z;
+                const double px0 = sx, py0 = sy, pz0 = sz;
                 double sq = DBL_MAX;
                 int mq = -99;
+                double nsx = 0., nsy = 0., nsz = 0.;  // the winning entry's site and run
+                int nrun = -1;
                 // exit distance through one neighbour entry; the smallest positive one wins,
                 // strict '<' (the first of equal distances wins)
                 auto consider = [&](const VorNb E) {
@@ -952,18 +973,29 @@ namespace skg
                     {
                         sq = si;
                         mq = mi;
+                        nsx = E.x;
+                        nsy = E.y;
+                        nsz = E.z;
+                        nrun = E.run;
                     }
                 };
-                // entries four at a time so that their loads are in flight together
-                int q = C.begin;
-                const int e = C.begin + C.count;
+                // the run: its header and first three entries in one group, then the entries
+                // four at a time so that their loads are in flight together
+                const int hb = j;
+                const VorNb H = g.vnb[hb], E1 = g.vnb[hb + 1], E2 = g.vnb[hb + 2], E3 = g.vnb[hb + 3];
+                const double dens = H.x;
+                const int e = hb + 1 + H.run;
+                if (hb + 1 < e) consider(E1);
+                if (hb + 2 < e) consider(E2);
+                if (hb + 3 < e) consider(E3);
+                int q = hb + 4;
                 for (; q + 4 <= e; q += 4)
                 {
-                    const VorNb E0 = g.vnb[q], E1 = g.vnb[q + 1], E2 = g.vnb[q + 2], E3 = g.vnb[q + 3];
-                    consider(E0);
-                    consider(E1);
-                    consider(E2);
-                    consider(E3);
+                    const VorNb F0 = g.vnb[q], F1 = g.vnb[q + 1], F2 = g.vnb[q + 2], F3 = g.vnb[q + 3];
+                    consider(F0);
+                    consider(F1);
+                    consider(F2);
+                    consider(F3);
                 }
                 for (; q < e; ++q) consider(g.vnb[q]);
                 if (mq == -99)
@@ -977,6 +1009,7 @@ namespace skg
                         state = kOutside;
                         return false;
                     }
+                    vor_enter(g, i);
                     continue;
                 }
                 const double step = sq + g.eps;
@@ -985,8 +1018,12 @@ namespace skg
                 rz += kz * step;
                 m = i;
                 ds = sq;
-                n = C.dens;
+                n = dens;
                 i = mq;
+                j = nrun;
+                sx = nsx;
+                sy = nsy;
+                sz = nsz;
                 if (mq < 0) state = kOutside;
                 return true;
             }

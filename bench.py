@@ -418,8 +418,9 @@ def main():
     else:
         wl = workloads.voronoi_clumpy(num_sites=a.sites, threads=threads)
         ncells, nnb = wl.tables["mesh"].info()[:2]
-        # Voronoi crossing: cell record 64 B + one 32 B entry per neighbour of the cell
-        bytes_per_segment = 64.0 + 32.0 * nnb / ncells
+        # Voronoi crossing: the cell's run in the entry table, a 32 B header (density, entry
+        # count) + one 32 B entry per neighbour (site, id, the neighbour's run)
+        bytes_per_segment = 32.0 + 32.0 * nnb / ncells
         wdesc = "Voronoi %d sites, lognormal clumpy medium (P(k)~k^-3.7, sigma_ln 1), mean tauV=5, 3 frames" % ncells
         a.no_guide = True
         if a.cpu_packets == CPU_PACKETS_DEFAULT:

@@ -2220,6 +2220,18 @@ namespace
         HIPCHK(e, hipGetLastError());
         return SKG_OK;
     }
+
+    GuideTilesW guideTilesW(skg_engine* e)
+    {
+        GuideTilesW t;
+        for (int l = 0; l < 4; ++l)
+        {
+            t.W[l] = const_cast<uint16_t*>(e->net.W[l]);
+            t.WT[l] = const_cast<uint16_t*>(e->net.WT[l]);
+            t.b[l] = const_cast<float*>(e->net.b[l]);
+        }
+        return t;
+    }
 }
 
 int skg_guide_params(skg_engine* e, const float* theta, int32_t n)
@@ -2296,16 +2308,22 @@ int skg_guide_gradient(skg_engine* e, const float* records, int64_t n, float* gr
     HIPCHK(e, hipMalloc(&dL, sizeof(double)));
     HIPCHK(e, hipMemcpy(dR, records, n * 16 * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(e, hipMemset(dL, 0, sizeof(double)));
+    const int64_t nb = (n + kGradSamples - 1) / kGradSamples;
+    if (nb > INT32_MAX) return fail(e, SKG_ERR_ARGUMENT, "skg_guide_gradient: too many records");
+    DevBuf part;
+    HIPCHK(e, part.ensure(static_cast<size_t>(nb) * kGuideParams * sizeof(float)));
     float* g = e->gTheta + 3 * kGuideParams;
-    HIPCHK(e, hipMemset(g, 0, kGuideParams * sizeof(float)));
-    hipLaunchKernelGGL(guide_grad_kernel, dim3(static_cast<unsigned>((n + 63) / 64)), dim3(64), 0, e->stream, e->net, dR, 16,
-                       n, 1.0f, g, dL);
+    hipLaunchKernelGGL(guide_grad_part, dim3(static_cast<unsigned>(nb)), dim3(64 * kGradWaves), 0, e->stream, e->net, dR,
+                       16, n, 1.0f, part.as<float>(), dL);
+    HIPCHK(e, hipGetLastError());
+    hipLaunchKernelGGL(guide_reduce_step, dim3((kGuideParams + kReduceParams - 1) / kReduceParams),
+                       dim3(kReduceParams * kReduceSlices), 0, e->stream, part.as<float>(), static_cast<int>(nb), g,
+                       nullptr, nullptr, nullptr, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, guideTilesW(e));
     HIPCHK(e, hipGetLastError());
     HIPCHK(e, hipMemcpyAsync(grad, g, kGuideParams * sizeof(float), hipMemcpyDeviceToHost, e->stream));
     double l = 0.;
     HIPCHK(e, hipMemcpyAsync(&l, dL, sizeof(double), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
-    HIPCHK(e, hipMemset(g, 0, kGuideParams * sizeof(float)));
     if (loss) *loss = l;
     hipFree(dR);
     hipFree(dL);
@@ -2392,32 +2410,32 @@ int skg_guide_train(skg_engine* e, int32_t steps, int32_t batch, float lr, doubl
     // whole collection (records are stored in the order the histories produced them)
     const int64_t nb = e->recN / batch;
     if (16 * nb > INT32_MAX) return fail(e, SKG_ERR_CAPACITY, "skg_guide_train: too many records per batch stride");
-    double* dL = nullptr;
-    HIPCHK(e, hipMalloc(&dL, sizeof(double)));
+    // one partial gradient per workgroup of kGradSamples records, summed by the Adam step
+    const int nwg = batch / kGradSamples;
+    DevBuf part, loss;
+    HIPCHK(e, part.ensure(static_cast<size_t>(nwg) * kGuideParams * sizeof(float)));
+    HIPCHK(e, loss.ensure(sizeof(double)));
+    double* dL = loss.as<double>();
     float* th = e->gTheta;
     float* m1 = e->gTheta + kGuideParams;
     float* m2 = e->gTheta + 2 * kGuideParams;
-    float* g = e->gTheta + 3 * kGuideParams;
-    HIPCHK(e, hipMemset(g, 0, kGuideParams * sizeof(float)));
+    const GuideTilesW tw = guideTilesW(e);
     for (int s = 0; s < steps; ++s)
     {
         const int64_t first = s % nb;
         bool track = s == 0 || s == steps - 1;
         if (track) HIPCHK(e, hipMemsetAsync(dL, 0, sizeof(double), e->stream));
-        hipLaunchKernelGGL(guide_grad_kernel, dim3(batch / 64), dim3(64), 0, e->stream, e->net, e->rec + first * 16,
-                           static_cast<int>(16 * nb), static_cast<int64_t>(batch), cscale, g, track ? dL : nullptr);
+        hipLaunchKernelGGL(guide_grad_part, dim3(nwg), dim3(64 * kGradWaves), 0, e->stream, e->net, e->rec + first * 16,
+                           static_cast<int>(16 * nb), static_cast<int64_t>(batch), cscale, part.as<float>(),
+                           track ? dL : nullptr);
         ++e->adamStep;
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.f - std::pow(b1, static_cast<float>(e->adamStep));
         const float c2 = 1.f - std::pow(b2, static_cast<float>(e->adamStep));
-        hipLaunchKernelGGL(guide_adam_kernel, dim3((kGuideParams + 255) / 256), dim3(256), 0, e->stream, th, m1, m2, g,
-                           kGuideParams, lr, b1, b2, 1e-8f, c1, c2);
-        int st = packGuide(e);
-        if (st)
-        {
-            hipFree(dL);
-            return st;
-        }
+        hipLaunchKernelGGL(guide_reduce_step, dim3((kGuideParams + kReduceParams - 1) / kReduceParams),
+                           dim3(kReduceParams * kReduceSlices), 0, e->stream, part.as<float>(), nwg, nullptr, th, m1,
+                           m2, lr, b1, b2, 1e-8f, c1, c2, tw);
+        HIPCHK(e, hipGetLastError());
         if (track)
         {
             double l = 0.;
@@ -2428,7 +2446,6 @@ int skg_guide_train(skg_engine* e, int32_t steps, int32_t batch, float lr, doubl
         }
     }
     HIPCHK(e, hipStreamSynchronize(e->stream));
-    hipFree(dL);
     return SKG_OK;
 }
 

@@ -145,47 +145,7 @@ namespace skg
     }
 
     // ---------------------------------------------------------------- mixture on one sample
-    struct Lobes
-    {
-        double mu[kGuideLobes][3];
-        double kap[kGuideLobes];
-        double w[kGuideLobes];
-    };
-
     __device__ __forceinline__ double softplus_d(double x) { return x > 20 ? x : log1p(exp(x)); }
-
-    __device__ __forceinline__ void guide_lobes(const float* o, Lobes& L)
-    {
-        double mx = -1e300;
-        for (int j = 0; j < kGuideLobes; ++j) mx = fmax(mx, static_cast<double>(o[5 * j + 4]));
-        double s = 0.;
-        for (int j = 0; j < kGuideLobes; ++j)
-        {
-            double a = o[5 * j], b = o[5 * j + 1], c = o[5 * j + 2];
-            double n = sqrt(a * a + b * b + c * c);
-            n = n > 1e-12 ? n : 1e-12;
-            L.mu[j][0] = a / n;
-            L.mu[j][1] = b / n;
-            L.mu[j][2] = c / n;
-            L.kap[j] = fmin(kGuideKMin + softplus_d(o[5 * j + 3]), kGuideKMax);
-            L.w[j] = exp(static_cast<double>(o[5 * j + 4]) - mx);
-            s += L.w[j];
-        }
-        for (int j = 0; j < kGuideLobes; ++j) L.w[j] /= s;
-    }
-
-    // mixture density per steradian at unit vector k (FP64, for the training loss)
-    __device__ __forceinline__ double guide_pdf(const Lobes& L, double kx, double ky, double kz)
-    {
-        double q = 0.;
-        for (int j = 0; j < kGuideLobes; ++j)
-        {
-            double c = L.mu[j][0] * kx + L.mu[j][1] * ky + L.mu[j][2] * kz;
-            double kap = L.kap[j];
-            q += L.w[j] * kap / (2 * M_PI * (-expm1(-2 * kap))) * exp(kap * (c - 1));
-        }
-        return q;
-    }
 
     // The sampler inside the life cycle works in FP32: lobe parameters, mixture density and
     // direction sampling.  The weight factor uses the density of this same sampler, so the
@@ -486,174 +446,231 @@ namespace skg
             for (int q = 0; q < kGuideOut; ++q) out[s * kGuideOut + q] = o[q];
     }
 
-    // dL/d(raw outputs) of the weighted negative log-likelihood for one sample (FP64 math),
-    // scaled by cw = c_i / sum(c); the expression of the CPU statement
-    __device__ __forceinline__ void guide_dout(const float* o, double kx, double ky, double kz, double cw, float* d)
+    // ---------------------------------------------------------------- training
+    // Workgroup of the gradient kernel: 4 wavefronts, one 16-sample column tile each (64 samples).
+    constexpr int kGradWaves = 4;
+    constexpr int kGradSamples = 16 * kGradWaves;
+    // LDS pitch (in BF16 elements) of the [unit][sample] staging rows: 68 samples = 136 B =
+    // 34 banks, so the 16 rows one 8-byte MFMA operand load touches start on distinct bank pairs
+    constexpr int kGradPitch = kGradSamples + 4;
+
+    // Two lobes (j0, j0 + 1) of dL/d(raw outputs) of the weighted negative log-likelihood
+    // -cw ln q(k) of one sample, FP64 math.  The four lanes that hold one sample (lanes row,
+    // row + 16, row + 32, row + 48) take lobes 0-1, 2-3, 4-5 and 6-7; the softmax sum and the
+    // mixture density are reduced over the four.  o: the sample's 40 outputs; d10: the 10
+    // derivatives of lobes j0, j0 + 1 (outputs 5 j0 .. 5 j0 + 9).  Returns q(k).
+    __device__ __forceinline__ double guide_dout_pair(const float* o, int j0, double kx, double ky, double kz, double cw,
+                                                      float* d10)
     {
-        Lobes L;
-        guide_lobes(o, L);
-        double f[kGuideLobes], q = 0.;
-        double cdot[kGuideLobes];
-        for (int j = 0; j < kGuideLobes; ++j)
+        double mx = -1e300;
+#pragma unroll
+        for (int j = 0; j < kGuideLobes; ++j) mx = fmax(mx, static_cast<double>(o[5 * j + 4]));
+        double e[2], f[2], kap[2], cdot[2], mu[2][3], nrm[2];
+        double es = 0., efs = 0.;
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
         {
-            cdot[j] = L.mu[j][0] * kx + L.mu[j][1] * ky + L.mu[j][2] * kz;
-            double kap = L.kap[j];
-            f[j] = kap / (2 * M_PI * (-expm1(-2 * kap))) * exp(kap * (cdot[j] - 1));
-            q += L.w[j] * f[j];
+            const int j = j0 + h;
+            const double a = o[5 * j], b = o[5 * j + 1], c = o[5 * j + 2];
+            double n = sqrt(a * a + b * b + c * c);
+            n = n > 1e-12 ? n : 1e-12;
+            nrm[h] = n;
+            mu[h][0] = a / n;
+            mu[h][1] = b / n;
+            mu[h][2] = c / n;
+            kap[h] = fmin(kGuideKMin + softplus_d(o[5 * j + 3]), kGuideKMax);
+            cdot[h] = mu[h][0] * kx + mu[h][1] * ky + mu[h][2] * kz;
+            f[h] = kap[h] / (2 * M_PI * (-expm1(-2 * kap[h]))) * exp(kap[h] * (cdot[h] - 1));
+            e[h] = exp(static_cast<double>(o[5 * j + 4]) - mx);
+            es += e[h];
+            efs += e[h] * f[h];
         }
+        // sums over the sample's four lanes
+        es += __shfl_xor(es, 16);
+        es += __shfl_xor(es, 32);
+        efs += __shfl_xor(efs, 16);
+        efs += __shfl_xor(efs, 32);
+        const double q = efs / es;
         const double g = -cw;
-        for (int j = 0; j < kGuideLobes; ++j)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
         {
-            double r = L.w[j] * f[j] / q;
-            double kap = L.kap[j];
-            double sp = kGuideKMin + softplus_d(o[5 * j + 3]);
-            double dlnf = 1 / kap - 2 * exp(-2 * kap) / (-expm1(-2 * kap)) + (cdot[j] - 1);
-            double dk = (sp < kGuideKMax) ? g * r * dlnf : 0.;
-            double sig = 1 / (1 + exp(-static_cast<double>(o[5 * j + 3])));
-            double a = o[5 * j], b = o[5 * j + 1], c = o[5 * j + 2];
-            double nrm = sqrt(a * a + b * b + c * c);
-            nrm = nrm > 1e-12 ? nrm : 1e-12;
-            double s = g * r * kap;
-            double dmx = s * kx, dmy = s * ky, dmz = s * kz;
-            double proj = dmx * L.mu[j][0] + dmy * L.mu[j][1] + dmz * L.mu[j][2];
-            d[5 * j + 0] = static_cast<float>((dmx - proj * L.mu[j][0]) / nrm);
-            d[5 * j + 1] = static_cast<float>((dmy - proj * L.mu[j][1]) / nrm);
-            d[5 * j + 2] = static_cast<float>((dmz - proj * L.mu[j][2]) / nrm);
-            d[5 * j + 3] = static_cast<float>(dk * sig);
-            d[5 * j + 4] = static_cast<float>(g * (r - L.w[j]));
+            const int j = j0 + h;
+            const double w = e[h] / es;
+            const double r = w * f[h] / q;
+            const double sp = kGuideKMin + softplus_d(o[5 * j + 3]);
+            const double dlnf = 1 / kap[h] - 2 * exp(-2 * kap[h]) / (-expm1(-2 * kap[h])) + (cdot[h] - 1);
+            const double dk = (sp < kGuideKMax) ? g * r * dlnf : 0.;
+            const double sig = 1 / (1 + exp(-static_cast<double>(o[5 * j + 3])));
+            const double s = g * r * kap[h];
+            const double dmx = s * kx, dmy = s * ky, dmz = s * kz;
+            const double proj = dmx * mu[h][0] + dmy * mu[h][1] + dmz * mu[h][2];
+            d10[5 * h + 0] = static_cast<float>((dmx - proj * mu[h][0]) / nrm[h]);
+            d10[5 * h + 1] = static_cast<float>((dmy - proj * mu[h][1]) / nrm[h]);
+            d10[5 * h + 2] = static_cast<float>((dmz - proj * mu[h][2]) / nrm[h]);
+            d10[5 * h + 3] = static_cast<float>(dk * sig);
+            d10[5 * h + 4] = static_cast<float>(g * (r - w));
         }
+        return q;
     }
 
-    // One training pass over records [0, n): records hold 9 features, the sampled direction
-    // (3) and the contribution weight, multiplied by cscale (so a batch's weights sum to ~1).  Each
-    // wavefront handles 64 samples; weight gradients are contracted over samples with MFMA
-    // through LDS and added to grad (FP32, parameter order).  lossOut accumulates the loss.
-    __global__ void __launch_bounds__(64) guide_grad_kernel(GuideNet net, const float* rec, int recStride, int64_t n,
-                                                            float cscale, float* grad, double* lossOut)
+    // Gradient of the training loss over records [0, n), one partial gradient per workgroup of
+    // 64 samples.  Records hold 9 features, the sampled direction (3) and the contribution
+    // weight, multiplied by cscale (so a batch's weights sum to ~1).  Each wavefront runs one
+    // 16-sample tile forward and backward with the activations and deltas in registers (the
+    // MFMA result layout of one layer is the B-operand layout of the next); activations and
+    // deltas are also staged in LDS as [unit][sample] rows, from which the four wavefronts
+    // contract the weight gradients over the workgroup's 64 samples on MFMA, a share of the
+    // output tiles each.  The workgroup writes its partial gradient (FP32, parameter order) to
+    // part[blockIdx.x * kGuideParams ...] with plain stores: no atomics, and the partials are
+    // summed in a fixed order by guide_reduce_step.  lossOut (if set) accumulates the loss.
+    __global__ void __launch_bounds__(64 * kGradWaves) guide_grad_part(GuideNet net, const float* rec, int recStride,
+                                                                       int64_t n, float cscale, float* part,
+                                                                       double* lossOut)
     {
-        // per-wave staging: inputs, activations h1..h3 and deltas as [sample][unit] bf16
-        __shared__ uint16_t sX[64 * 16];
-        __shared__ uint16_t sH[3][64 * 64];
-        __shared__ uint16_t sD[2][64 * 64];
-        __shared__ float sO[64 * 48];
-        const int lane = threadIdx.x;
+        constexpr int P = kGradPitch;
+        __shared__ __align__(16) uint16_t sXT[16 * P];
+        __shared__ __align__(16) uint16_t sHT[3][64 * P];
+        __shared__ __align__(16) uint16_t sDT[2][64 * P];
+        __shared__ float sO[kGradSamples * 48];
+        __shared__ double sLoss[kGradWaves];
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const int row = lane & 15, grp = lane >> 4;
-        const int64_t s0 = blockIdx.x * 64LL;
-        const int64_t s = s0 + lane;
+        const int sl = 16 * wave + row;  // this lane's sample (B-operand column) in the workgroup
+        const int64_t s = blockIdx.x * static_cast<int64_t>(kGradSamples) + sl;
         const bool valid = s < n;
-        // inputs
-        for (int q = 0; q < 16; ++q) sX[lane * 16 + q] = (valid && q < kGuideIn) ? f2bf(rec[s * recStride + q]) : 0;
-        __syncthreads();
-        // forward, keeping the activations in LDS
-        for (int t = 0; t < 4; ++t)
+        float* gp = part + static_cast<int64_t>(blockIdx.x) * kGuideParams;
+
+        // inputs: features 4 grp .. 4 grp + 3 of the sample (zero beyond the 9 features)
+        bf16x4 x;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
         {
-            const int smp = t * 16 + row;
-            bf16x4 x = load_bf4(sX + smp * 16 + grp * 4);
-            f32x4 o[3];
-            bf16x4 h1[4], h2[4], h3[4];
-            guide_tile_forward(net, x, o, h1, h2, h3);
+            const int k = 4 * grp + q;
+            const uint16_t v = (valid && k < kGuideIn) ? f2bf(rec[s * recStride + k]) : 0;
+            x[q] = static_cast<short>(v);
+            sXT[k * P + sl] = v;
+        }
+        // forward, activations kept for the backward pass
+        f32x4 o[3];
+        bf16x4 h[3][4];
+        guide_tile_forward(net, x, o, h[0], h[1], h[2]);
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
             for (int u = 0; u < 4; ++u)
-                for (int q = 0; q < 4; ++q)
-                {
-                    const int unit = u * 16 + grp * 4 + q;
-                    sH[0][smp * 64 + unit] = static_cast<uint16_t>(h1[u][q]);
-                    sH[1][smp * 64 + unit] = static_cast<uint16_t>(h2[u][q]);
-                    sH[2][smp * 64 + unit] = static_cast<uint16_t>(h3[u][q]);
-                }
-            for (int ot = 0; ot < 3; ++ot)
-                for (int q = 0; q < 4; ++q) sO[smp * 48 + ot * 16 + grp * 4 + q] = o[ot][q];
-        }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) sHT[l][(u * 16 + 4 * grp + q) * P + sl] = static_cast<uint16_t>(h[l][u][q]);
+#pragma unroll
+        for (int ot = 0; ot < 3; ++ot)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sO[sl * 48 + ot * 16 + 4 * grp + q] = o[ot][q];
         __syncthreads();
-        // loss gradient per sample (this lane's sample)
-        float d[kGuideOut];
-        double lossv = 0.;
-        if (valid)
+
+        // loss derivative: this lane's two lobes of its sample, staged as the delta of layer 3
         {
-            const float* r = rec + s * recStride;
-            double kx = r[9], ky = r[10], kz = r[11], cw = static_cast<double>(r[12]) * cscale;
-            guide_dout(sO + lane * 48, kx, ky, kz, cw, d);
-            Lobes L;
-            guide_lobes(sO + lane * 48, L);
-            lossv = -cw * log(guide_pdf(L, kx, ky, kz));
+            double kx = 0., ky = 0., kz = 1., cw = 0.;
+            if (valid)
+            {
+                const float* r = rec + s * recStride;
+                kx = r[9];
+                ky = r[10];
+                kz = r[11];
+                cw = static_cast<double>(r[12]) * cscale;
+            }
+            float d10[10];
+            // every lane takes part (the lobe sums are shuffles); lanes past n contribute zero
+            const double q = guide_dout_pair(sO + sl * 48, 2 * grp, kx, ky, kz, cw, d10);
+#pragma unroll
+            for (int k = 0; k < 10; ++k) sDT[0][(10 * grp + k) * P + sl] = valid ? f2bf(d10[k]) : 0;
+            // padding outputs 40..47
+            sDT[0][(40 + 2 * grp) * P + sl] = 0;
+            sDT[0][(41 + 2 * grp) * P + sl] = 0;
+            double lossv = (valid && grp == 0) ? -cw * log(q) : 0.;
+            for (int m = 32; m > 0; m >>= 1) lossv += __shfl_xor(lossv, m);
+            if (lane == 0) sLoss[wave] = lossv;
         }
-        else
-            for (int q = 0; q < kGuideOut; ++q) d[q] = 0.f;
         __syncthreads();
-        for (int q = 0; q < 48; ++q) sD[0][lane * 64 + q] = q < kGuideOut ? f2bf(d[q]) : 0;
-        __syncthreads();
-        // backward through the layers: delta of layer l output in sD[cur] ([sample][unit]),
-        // input activations of layer l in sH[l-1] (or sX for l = 0)
-        int cur = 0;
+
+        // backward, layer 3 down to 0; the delta of layer l is in sDT[(3 - l) & 1]
+        bf16x4 dcur[4];
+#pragma unroll
+        for (int kt = 0; kt < 3; ++kt)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) dcur[kt][q] = static_cast<short>(sDT[0][(kt * 16 + 4 * grp + q) * P + sl]);
+        dcur[3] = bf16x4{0, 0, 0, 0};
+#pragma unroll
         for (int l = 3; l >= 0; --l)
         {
+            const uint16_t* D = sDT[(3 - l) & 1];
+            const uint16_t* Hin = l == 0 ? sXT : sHT[l - 1];
             const int outp = guide_out_pad(l), inp = guide_in_pad(l);
             const int ot_n = outp / 16, it_n = inp / 16;
             const int wOff = l == 0 ? kOffW0 : l == 1 ? kOffW1 : l == 2 ? kOffW2 : kOffW3;
             const int bOff = l == 0 ? kOffB0 : l == 1 ? kOffB1 : l == 2 ? kOffB2 : kOffB3;
             const int nout = l == 3 ? kGuideOut : kGuideHidden, nin = l == 0 ? kGuideIn : kGuideHidden;
-            const uint16_t* Hin = l == 0 ? sX : sH[l - 1];
-            const int hstride = l == 0 ? 16 : 64;
-            // dW_l (out x in) = sum over samples of delta[s][o] * Hin[s][i]: A = delta^T
-            // (row o, k = sample), B = Hin (column i, k = sample)
-            for (int ot = 0; ot < ot_n; ++ot)
-                for (int it = 0; it < it_n; ++it)
+            // dW_l (out x in) = sum over samples of delta[s][o] Hin[s][i]: A = delta^T (row o,
+            // k = sample), B = Hin (k = sample, column i); output tiles dealt round the waves
+            for (int t = wave; t < ot_n * it_n; t += kGradWaves)
+            {
+                const int ot = t / it_n, it = t % it_n;
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kb = 0; kb < kGradSamples / 16; ++kb)
+                    acc = mfma16(load_bf4(D + (ot * 16 + row) * P + kb * 16 + 4 * grp),
+                                 load_bf4(Hin + (it * 16 + row) * P + kb * 16 + 4 * grp), acc);
+                // rows o = ot*16 + 4 grp + q, column i = it*16 + row
+                const int i = it * 16 + row;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
                 {
-                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-                    for (int kb = 0; kb < 4; ++kb)
-                    {
-                        bf16x4 a, b;
-                        for (int q = 0; q < 4; ++q)
-                        {
-                            const int smp = kb * 16 + grp * 4 + q;
-                            a[q] = static_cast<short>(sD[cur][smp * 64 + ot * 16 + row]);
-                            b[q] = static_cast<short>(Hin[smp * hstride + it * 16 + row]);
-                        }
-                        acc = mfma16(a, b, acc);
-                    }
-                    // result: rows o = ot*16 + grp*4 + q, column i = it*16 + row
-                    for (int q = 0; q < 4; ++q)
-                    {
-                        const int o = ot * 16 + grp * 4 + q, i = it * 16 + row;
-                        if (o < nout && i < nin && acc[q] != 0.f) atomicAdd(grad + wOff + o * nin + i, acc[q]);
-                    }
+                    const int oo = ot * 16 + 4 * grp + q;
+                    if (oo < nout && i < nin) gp[wOff + oo * nin + i] = acc[q];
                 }
-            // bias gradient: sum over samples
-            if (lane < nout)
+            }
+            // bias gradient: the delta summed over the workgroup's samples
+            if (threadIdx.x < nout)
             {
                 float sb = 0.f;
-                for (int smp = 0; smp < 64; ++smp) sb += bf2f(sD[cur][smp * 64 + lane]);
-                if (sb != 0.f) atomicAdd(grad + bOff + lane, sb);
+#pragma unroll
+                for (int k = 0; k < kGradSamples; k += 4)
+                {
+                    const bf16x4 v = load_bf4(D + threadIdx.x * P + k);
+                    sb += (bf2f(static_cast<uint16_t>(v[0])) + bf2f(static_cast<uint16_t>(v[1]))) +
+                          (bf2f(static_cast<uint16_t>(v[2])) + bf2f(static_cast<uint16_t>(v[3])));
+                }
+                gp[bOff + threadIdx.x] = sb;
             }
             if (l == 0) break;
-            // delta of the previous layer: (W_l^T delta^T) * relu'(h_{l-1}); A = W_l^T tiles
-            // (row = input unit, k = output unit), B = delta^T (column = sample, k = output unit)
-            const int nxt = cur ^ 1;
-            for (int t = 0; t < 4; ++t)
+            // delta of layer l-1 for this wave's tile: (W_l^T delta^T) * relu'(h_{l-1}); A = W_l^T
+            // tiles (row = input unit, k = output unit), B = the delta in registers
+            bf16x4 dn[4];
+            uint16_t* Dn = sDT[(4 - l) & 1];
+#pragma unroll
+            for (int it = 0; it < 4; ++it)
             {
-                const int smp = t * 16 + row;
-                for (int it = 0; it < 4; ++it)
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kt = 0; kt < 4; ++kt)
+                    if (kt < ot_n) acc = mfma16(load_bf4(net.WT[l] + (it * 16 + row) * outp + kt * 16 + 4 * grp), dcur[kt], acc);
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
                 {
-                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-                    for (int kt = 0; kt < ot_n; ++kt)
-                    {
-                        bf16x4 a = load_bf4(net.WT[l] + (it * 16 + row) * outp + kt * 16 + grp * 4);
-                        bf16x4 b = load_bf4(sD[cur] + smp * 64 + kt * 16 + grp * 4);
-                        acc = mfma16(a, b, acc);
-                    }
-                    for (int q = 0; q < 4; ++q)
-                    {
-                        const int unit = it * 16 + grp * 4 + q;
-                        float hv = bf2f(sH[l - 1][smp * 64 + unit]);
-                        sD[nxt][smp * 64 + unit] = hv > 0.f ? f2bf(acc[q]) : 0;
-                    }
+                    const uint16_t v = h[l - 1][it][q] > 0 ? f2bf(acc[q]) : 0;
+                    dn[it][q] = static_cast<short>(v);
+                    Dn[(it * 16 + 4 * grp + q) * P + sl] = v;
                 }
             }
+#pragma unroll
+            for (int it = 0; it < 4; ++it) dcur[it] = dn[it];
             __syncthreads();
-            cur = nxt;
         }
-        // loss, one atomic per wavefront
-        for (int o = 32; o > 0; o >>= 1) lossv += __shfl_xor(lossv, o);
-        if (lane == 0 && lossOut) atomicAdd(lossOut, lossv);
+        if (threadIdx.x == 0 && lossOut)
+        {
+            double lw = 0.;
+            for (int w = 0; w < kGradWaves; ++w) lw += sLoss[w];
+            atomicAdd(lossOut, lw);
+        }
     }
 
     // sum of the training weights (record field 12) over n records, one FP64 atomic per wave
@@ -667,19 +684,74 @@ namespace skg
         if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
     }
 
-    // Adam step on the FP32 parameters, then zero the gradient
-    __global__ void guide_adam_kernel(float* theta, float* m, float* v, float* grad, int n, float lr, float b1, float b2,
-                                      float eps, float c1, float c2)
+    // BF16 tiles and FP32 biases of the network, writable (the storage GuideNet reads)
+    struct GuideTilesW
     {
-        int i = blockIdx.x * blockDim.x + threadIdx.x;
-        if (i >= n) return;
-        float g = grad[i];
-        float mi = b1 * m[i] + (1.f - b1) * g;
-        float vi = b2 * v[i] + (1.f - b2) * g * g;
+        uint16_t* W[4];
+        uint16_t* WT[4];
+        float* b[4];
+    };
+
+    // Launch shape of guide_reduce_step: 32 parameters per workgroup, the partials of each
+    // summed in 8 slices (workgroups b = slice, slice + 8, ...) and the slices added in LDS.
+    constexpr int kReduceParams = 32;
+    constexpr int kReduceSlices = 8;
+
+    // Per parameter: the sum of the nb workgroups' partial gradients, in a fixed order (the
+    // slices' sums in slice order).  Without theta it is written to grad.  With theta, an Adam
+    // step is taken on theta (moments m, v; bias corrections c1, c2) and the parameter's BF16
+    // tile entries (W and W^T) or FP32 bias are rewritten; the tiles' padding stays as
+    // guide_pack_kernel left it (zero).
+    __global__ void __launch_bounds__(kReduceParams * kReduceSlices)
+        guide_reduce_step(const float* part, int nb, float* grad, float* theta, float* m, float* v, float lr, float b1,
+                          float b2, float eps, float c1, float c2, GuideTilesW tw)
+    {
+        __shared__ float red[kReduceSlices][kReduceParams];
+        const int c = threadIdx.x % kReduceParams, r = threadIdx.x / kReduceParams;
+        const int i = blockIdx.x * kReduceParams + c;
+        float g = 0.f;
+        if (i < kGuideParams)
+        {
+            // four independent sums, so that four loads are in flight per thread
+            float a[4] = {0.f, 0.f, 0.f, 0.f};
+            int b = r;
+            for (; b + 3 * kReduceSlices < nb; b += 4 * kReduceSlices)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) a[k] += part[static_cast<int64_t>(b + k * kReduceSlices) * kGuideParams + i];
+            for (; b < nb; b += kReduceSlices) a[0] += part[static_cast<int64_t>(b) * kGuideParams + i];
+            g = (a[0] + a[1]) + (a[2] + a[3]);
+        }
+        red[r][c] = g;
+        __syncthreads();
+        if (r != 0 || i >= kGuideParams) return;
+        g = red[0][c];
+#pragma unroll
+        for (int k = 1; k < kReduceSlices; ++k) g += red[k][c];
+        if (!theta)
+        {
+            grad[i] = g;
+            return;
+        }
+        const float mi = b1 * m[i] + (1.f - b1) * g;
+        const float vi = b2 * v[i] + (1.f - b2) * g * g;
         m[i] = mi;
         v[i] = vi;
-        theta[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
-        grad[i] = 0.f;
+        const float th = theta[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        theta[i] = th;
+        const int wOff[4] = {kOffW0, kOffW1, kOffW2, kOffW3};
+        const int bOff[4] = {kOffB0, kOffB1, kOffB2, kOffB3};
+        int l = 3;
+        while (l > 0 && i < wOff[l]) --l;
+        if (i >= bOff[l])
+        {
+            tw.b[l][i - bOff[l]] = th;
+            return;
+        }
+        const int nin = l == 0 ? kGuideIn : kGuideHidden;
+        const int j = i - wOff[l], o = j / nin, cc = j % nin;
+        const uint16_t w = f2bf(th);
+        tw.W[l][o * guide_in_pad(l) + cc] = w;
+        tw.WT[l][cc * guide_out_pad(l) + o] = w;
     }
 
     // rebuilds the padded BF16 weight tiles (and transposes) and FP32 biases from theta

@@ -273,8 +273,17 @@ def guided_comparison(eng, pkg, a, first):
         theta[o:o + w.size] = w
         o += w.size + fan_out
     eng.guide_params(theta)
-    eng.guide_mode(2, mix=a.mix)
     t0 = time.perf_counter()
+    if a.guide_target == "pixel":
+        # plain packets for the image that weights the deposits (relative error per pixel)
+        eng.guide_mode(0)
+        eng.frame_clear()
+        eng.run_packets(first, int(a.target_packets))
+        first += int(a.target_packets)
+        eng.guide_target(1, a.target_floor)
+    else:
+        eng.guide_target(0)
+    eng.guide_mode(2, mix=a.mix)
     eng.guide_collect(int(a.record_capacity))
     eng.frame_clear()
     eng.run_packets(first, int(a.train_packets))
@@ -284,7 +293,9 @@ def guided_comparison(eng, pkg, a, first):
     l0, l1 = eng.guide_train(a.train_steps, batch=a.train_batch, lr=a.lr)
     t2 = time.perf_counter()
     out.update(train_packets=int(a.train_packets), records=int(nrec), collect_s=t1 - t0, train_s=t2 - t1,
-               train_steps=a.train_steps, loss_first=l0, loss_last=l1, mix=a.mix)
+               train_steps=a.train_steps, loss_first=l0, loss_last=l1, mix=a.mix, target=a.guide_target)
+    if a.guide_target == "pixel":
+        out.update(target_packets=int(a.target_packets), target_floor=a.target_floor)
     print(f"guide: {nrec} records from {a.train_packets:.0f} packets in {t1 - t0:.2f} s, "
           f"{a.train_steps} Adam steps in {t2 - t1:.2f} s, loss {l0:.4f} -> {l1:.4f}", file=sys.stderr, flush=True)
     res, groups = {}, {}
@@ -358,6 +369,12 @@ def main():
     ap.add_argument("--train-batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--mix", type=float, default=0.5)
+    ap.add_argument("--guide-target", choices=["luminosity", "pixel"], default="luminosity",
+                    help="training target of the records: the luminosity a packet deposits after the event, or "
+                         "that luminosity divided per pixel by the pixel's flux (from --target-packets plain packets)")
+    ap.add_argument("--target-packets", type=float, default=2e6)
+    ap.add_argument("--target-floor", type=float, default=0.1,
+                    help="pixel flux floor of the pixel target, relative to the median pixel of its plane")
     ap.add_argument("--fom-batches", type=int, default=256)
     ap.add_argument("--fom-packets", type=int, default=1 << 18)
     ap.add_argument("--fom-groups", type=int, default=8,

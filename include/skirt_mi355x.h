@@ -438,6 +438,14 @@ int skg_guide_mode(skg_engine* e, int32_t mode, double mix, double half_size, do
                    double lambda_scale);
 /* Allocates (or resets) the training-record buffer for mode-2 runs. */
 int skg_guide_collect(skg_engine* e, int64_t capacity);
+/* Training target of mode-2 runs: the weight of a detector deposit in the records'
+ * contributions.  mode 0: the deposited luminosity itself (default).  mode 1: luminosity /
+ * max(F, floor) per frame entry (pixel, wavelength), F being the image accumulated so far (run
+ * plain packets first) and floor = floor_frac x the median of the positive entries of the
+ * entry's (frame, wavelength) plane: every pixel's relative error then counts alike, in
+ * place of the frame total.  mode 2: the caller's weights, n = one per frame entry in the
+ * layout of skg_frame_read.  No reference counterpart (the guide is new). */
+int skg_guide_target(skg_engine* e, int32_t mode, double floor_frac, const double* weights, int64_t n);
 /* Number of records of the last mode-2 run; copies up to cap of them (16 floats each; the
  * weight is the detector luminosity the packet deposited after the event) when out != NULL. */
 int skg_guide_records(skg_engine* e, int64_t* n, float* out, int64_t cap);

@@ -71,7 +71,7 @@ EXPORTED = [
     "skg_octree_upload", "skg_octree_free", "skg_sample_density_cartesian", "skg_sample_density_octree",
     "skg_dust_meanlist", "skg_tune_pipeline", "skg_tune_queue", "skg_continuation_key", "skg_last_run_stats",
     "skg_guide_params", "skg_guide_read_params", "skg_guide_infer", "skg_guide_gradient", "skg_guide_mode",
-    "skg_guide_collect", "skg_guide_records", "skg_guide_train", "skg_guide_sample",
+    "skg_guide_collect", "skg_guide_records", "skg_guide_train", "skg_guide_sample", "skg_guide_target",
     "skg_voronoi_build", "skg_voronoi_info", "skg_voronoi_export", "skg_voronoi_upload",
     "skg_sample_density_voronoi", "skg_voronoi_free", "skg_autotune",
     "skg_tetra_build", "skg_tetra_info", "skg_tetra_export", "skg_tetra_upload", "skg_tetra_free",
@@ -378,6 +378,14 @@ class Engine:
 
     def guide_collect(self, capacity):
         self._chk(lib().skg_guide_collect(self._h, C.c_int64(capacity)))
+
+    def guide_target(self, mode, floor_frac=0.1, weights=None):
+        """Training target of collect runs (skg_guide_target): 0 the deposited luminosity, 1
+        luminosity / max(F, floor) per frame entry from the image accumulated so far, 2 the
+        given weights (one per frame entry, in frame-read order)."""
+        w = None if weights is None else _f64(weights).ravel()
+        self._chk(lib().skg_guide_target(self._h, C.c_int32(mode), C.c_double(floor_frac),
+                                         None if w is None else _p(w, f64p), C.c_int64(0 if w is None else w.size)))
 
     def guide_records(self, copy=0):
         n = C.c_int64()

@@ -497,6 +497,10 @@ struct skg_engine
     int64_t slotCapGuide = 0;
     double* finalLum = nullptr;
     int64_t finalCap = 0;
+    // training target: per frame entry (pixel, wavelength) the weight of a deposit in the
+    // records' contributions (skg_guide_target); none = the deposited luminosity itself
+    double* pixW = nullptr;
+    int64_t pixWSize = 0;
 };
 
 namespace
@@ -849,7 +853,7 @@ void skg_destroy(skg_engine* e)
                     static_cast<void*>(e->gBf), static_cast<void*>(e->gBias), static_cast<void*>(e->rec),
                     static_cast<void*>(e->recHist), static_cast<void*>(e->recA), static_cast<void*>(e->recCount),
                     static_cast<void*>(e->recSum), static_cast<void*>(e->contrib), static_cast<void*>(e->recLast),
-                    static_cast<void*>(e->finalLum), static_cast<void*>(e->inst), static_cast<void*>(e->ilpack),
+                    static_cast<void*>(e->finalLum), static_cast<void*>(e->pixW), static_cast<void*>(e->inst), static_cast<void*>(e->ilpack),
                     static_cast<void*>(e->rf), static_cast<void*>(e->ent), static_cast<void*>(e->entFirst),
                     static_cast<void*>(e->entLpack), static_cast<void*>(e->entX)})
         if (p) hipFree(p);
@@ -2031,6 +2035,9 @@ namespace
             p.contrib = e->contrib;
             p.recLast = e->recLast;
             p.finalLum = e->finalLum;
+            if (e->pixW && e->pixWSize != e->frameSize)
+                return fail(e, SKG_ERR_STATE, "collect run: the guide target's pixel weights do not match the frames");
+            p.pixW = e->pixW;
         }
         const int iters = std::max(2, e->graphIters & ~1);
         while (static_cast<int>(e->traceEv.size()) < 2 * iters)
@@ -2090,12 +2097,12 @@ namespace
         {
             unsigned long long nrec = 0;
             HIPCHK(e, hipMemcpy(&nrec, e->recCount, sizeof(nrec), hipMemcpyDeviceToHost));
-            // records of earlier collect runs were finalized with their own runs' totals
-            const int64_t n0 = e->recN;
+            // each collect run writes its records from index 0 (the record counter is cleared at
+            // its start): all of them take this run's totals
             int64_t n = std::min<int64_t>(static_cast<int64_t>(nrec), e->recCap);
-            if (n > n0)
-                hipLaunchKernelGGL(guide_finalize_records, dim3(static_cast<unsigned>((n - n0 + 255) / 256)), dim3(256), 0,
-                                   e->stream, e->rec + 16 * n0, e->recHist + n0, e->recA + n0, e->finalLum, n - n0);
+            if (n > 0)
+                hipLaunchKernelGGL(guide_finalize_records, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                                   e->stream, e->rec, e->recHist, e->recA, e->finalLum, n);
             HIPCHK(e, hipStreamSynchronize(e->stream));
             e->recN = n;
         }
@@ -2364,6 +2371,69 @@ int skg_guide_collect(skg_engine* e, int64_t capacity)
     if (!e->recSum) HIPCHK(e, hipMalloc(&e->recSum, sizeof(double)));
     HIPCHK(e, hipMemset(e->recCount, 0, sizeof(unsigned long long)));
     e->recN = 0;
+    return SKG_OK;
+}
+
+int skg_guide_target(skg_engine* e, int32_t mode, double floor_frac, const double* weights, int64_t n)
+{
+    if (!e || mode < 0 || mode > 2 || (mode == 1 && !(floor_frac > 0)) || (mode == 2 && (!weights || n < 1)))
+        return fail(e, SKG_ERR_ARGUMENT, "skg_guide_target: bad arguments");
+    std::lock_guard<std::mutex> lk(e->mu);
+    hipSetDevice(e->device);
+    if (mode == 0)
+    {
+        if (e->pixW) hipFree(e->pixW);
+        e->pixW = nullptr;
+        e->pixWSize = 0;
+        return SKG_OK;
+    }
+    int st = ensureFrames(e);
+    if (st) return st;
+    const int64_t size = e->frameSize;
+    if (size < 1) return fail(e, SKG_ERR_STATE, "skg_guide_target: no frames");
+    std::vector<double> w(static_cast<size_t>(size));
+    if (mode == 2)
+    {
+        if (n != size) return fail(e, SKG_ERR_ARGUMENT, "skg_guide_target: weights must have one value per frame entry");
+        for (int64_t i = 0; i < size; ++i)
+        {
+            if (!(weights[i] >= 0) || !std::isfinite(weights[i]))
+                return fail(e, SKG_ERR_ARGUMENT, "skg_guide_target: weights must be finite and non-negative");
+            w[i] = weights[i];
+        }
+    }
+    else
+    {
+        // 1 / max(F, floor) per entry of the image accumulated so far; the floor is floor_frac
+        // times the median of the positive entries of the entry's (frame, wavelength) plane
+        std::vector<double> F(static_cast<size_t>(size));
+        HIPCHK(e, hipStreamSynchronize(e->stream));
+        HIPCHK(e, hipMemcpy(F.data(), e->frameBuf, size * sizeof(double), hipMemcpyDeviceToHost));
+        int64_t off = 0;
+        for (auto& f : e->frames)
+        {
+            const int64_t plane = static_cast<int64_t>(f.nx) * f.ny;
+            for (int ell = 0; ell < std::max(1, e->nlambda); ++ell, off += plane)
+            {
+                std::vector<double> pos;
+                for (int64_t i = 0; i < plane; ++i)
+                    if (F[off + i] > 0) pos.push_back(F[off + i]);
+                if (pos.empty()) return fail(e, SKG_ERR_STATE, "skg_guide_target: a frame plane holds no flux yet");
+                std::nth_element(pos.begin(), pos.begin() + pos.size() / 2, pos.end());
+                const double fl = floor_frac * pos[pos.size() / 2];
+                for (int64_t i = 0; i < plane; ++i) w[off + i] = 1.0 / std::max(F[off + i], fl);
+            }
+        }
+    }
+    if (e->pixWSize != size)
+    {
+        if (e->pixW) hipFree(e->pixW);
+        e->pixW = nullptr;
+        e->pixWSize = 0;
+        HIPCHK(e, hipMalloc(&e->pixW, size * sizeof(double)));
+        e->pixWSize = size;
+    }
+    HIPCHK(e, hipMemcpy(e->pixW, w.data(), size * sizeof(double), hipMemcpyHostToDevice));
     return SKG_OK;
 }
 

@@ -113,6 +113,7 @@ namespace skg
         double* contrib;     // per slot: luminosity its packet has deposited so far
         int64_t* recLast;    // per slot: record awaiting its luminosity mark, or -1
         double* finalLum;    // per history offset: total luminosity deposited by the packet
+        const double* pixW;  // per frame entry: weight of a deposit in contrib, or null (weight 1)
     };
 
     constexpr int kCtlHist = 0;
@@ -786,15 +787,16 @@ namespace skg
             if (key >= 0) ++deposits;
             if (valid)
             {
-                double dep = key >= 0 ? v : 0.;
+                double dep = key >= 0 ? v * (p.pixW ? p.pixW[key] : 1.) : 0.;
                 // further instruments of the same observer (same direction, shared optical depth)
                 for (int f = f0 + 1; f < p.nframes && p.frames[f].same_as_prev; ++f)
                 {
                     const FrameGeom& fg = p.frames[f];
                     int pix = fg.any_pixel(x, y, z);
                     if (pix < 0) continue;
-                    unsafeAtomicAdd(p.frame + fg.offset + pix + static_cast<int64_t>(ell) * fg.nx * fg.ny, v);
-                    dep += v;
+                    const int64_t k = fg.offset + pix + static_cast<int64_t>(ell) * fg.nx * fg.ny;
+                    unsafeAtomicAdd(p.frame + k, v);
+                    dep += v * (p.pixW ? p.pixW[k] : 1.);
                     ++atomics;
                     ++deposits;
                 }

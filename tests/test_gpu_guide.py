@@ -159,6 +159,64 @@ def test_guide_trains_on_collected_records(eng):
     assert l1 < l0 - 0.05, (l0, l1)
 
 
+def test_guide_target_weights_the_record_contributions(eng):
+    """Training targets of collect runs (skg_guide_target), on the same histories: caller
+    weights of 2 everywhere double every record's contribution, weights of zero leave none,
+    and the image target (mode 1) equals caller weights 1 / max(F, 0.1 x the plane's median
+    positive entry) restated here from the image F of a plain run."""
+    pkg = load_pkg()
+    b, dens = disk_density_cartesian(n=32, samples=10, seed=9, tau_v=5.0)
+    eng.grid_cartesian(b, b, b)
+    eng.medium(dens)
+    s = 1.5e-29
+    eng.dust([0.55e-6, 1.0e-6], [s, 0.5 * s], [0.6 * s, 0.25 * s], [0.6, 0.5])
+    eng.frame_remove_all()
+    fr = frame60()
+    eng.frame_add(fr["inclination"], fr["azimuth"], fr["roll"], 20, 20, fr["fovx"], fr["fovy"])
+    size = 2 * 20 * 20
+    npk = 50000
+    eng.source_point((0.0, 0.0, 0.0), [1.0 / npk, 1.0 / npk])
+    eng.options(0.5, 1e4, 0, 0, 5)
+    eng.tune_pipeline(0, 1 << 18, 0, 8)
+    eng.guide_params(G.flatten(G.init_params(7)).astype(np.float32))
+    eng.guide_collect(1_000_000)
+
+    def collect():
+        eng.guide_mode(2)
+        eng.frame_clear()
+        eng.run_packets(0, npk)
+        n, r = eng.guide_records(copy=1_000_000)
+        assert 0 < n < 1_000_000
+        return np.sort(r[:, 12].astype(np.float64))
+
+    print("case guide target", flush=True)
+    eng.guide_target(0)
+    base = collect()
+    assert base.max() > 0
+    eng.guide_target(2, weights=np.full(size, 2.0))
+    np.testing.assert_allclose(collect(), 2 * base, rtol=1e-6, atol=0)
+    eng.guide_target(2, weights=np.zeros(size))
+    assert not collect().any()
+    with pytest.raises(pkg.EngineError):
+        eng.guide_target(2, weights=np.ones(size - 1))
+    # the image target against its restatement
+    eng.guide_mode(0)
+    eng.frame_clear()
+    eng.run_packets(1 << 30, npk)
+    F = eng.frame_read(0)
+    eng.guide_target(1, 0.1)
+    img = collect()
+    w = np.empty_like(F)
+    for ell in range(F.shape[0]):
+        pos = np.sort(F[ell][F[ell] > 0])
+        w[ell] = 1.0 / np.maximum(F[ell], 0.1 * pos[len(pos) // 2])
+    eng.guide_target(2, weights=w)
+    np.testing.assert_allclose(img, collect(), rtol=1e-6, atol=0)
+    assert not np.allclose(img / img.sum(), base / base.sum())
+    eng.guide_target(0)
+    eng.guide_mode(0)
+
+
 def _guided_vs_plain(eng, mwr, engine, mix):
     b, dens = disk_density_cartesian(n=32, samples=10, seed=9, tau_v=5.0)
     eng.grid_cartesian(b, b, b)

@@ -474,6 +474,12 @@ namespace skg
                 q11 = p[11];
                 q12 = p[12];
                 q13 = p[13];
+                // the words only the neighbour-list states use (node / off_end and the walls'
+                // list starts) are marked as needed here: the compiler would otherwise issue
+                // their loads inside those states, after the wait for the first words, and the
+                // step would read the line twice in a row (101.6 -> 99.6 ms per 2^22-packet
+                // launch on the spiral disk at tauV 50 with all of it issued at once)
+                asm volatile("" : "+v"(q7), "+v"(q11), "+v"(q12), "+v"(q13));
             }
             // double k (bytes 8k..8k+7), k a compile-time constant
             __device__ __forceinline__ double d(int k) const

@@ -761,14 +761,15 @@ namespace skg
 
     // Minimum waves per SIMD per grid type.  The Voronoi and tetra kernels hold less
     // traversal state and stage no octree nodes in LDS (35 KB per workgroup): measured on the
-    // clumpy Voronoi workload 1453 / 1416 / 1493 ms per 2^22-packet launch at 2 / 3 / 4 waves,
-    // on the batched tetra workload 6.72 / 5.80 / 5.27 s.  The octree kernel loses with a
+    // clumpy Voronoi workload 1152 / 1086 / 1061 ms per 2^22-packet launch at 2 / 3 / 4 waves
+    // (each at its best transition threshold), on the batched tetra workload 6.72 / 5.80 /
+    // 5.27 s.  The octree kernel loses with a
     // third wave (101.4 vs 142.5 ms per 2^22-packet launch on the spiral disk at tauV 50, the
     // top-node table halved to fit three workgroups per CU: 168 VGPRs spill the traversal
     // state), and the Cartesian one gains nothing (58.2 / 61.3 / 61.1 ms at 2 / 3 / 4 waves).
     template<int GT> constexpr int lifecycle_waves()
     {
-        return GT == kTetra ? 4 : GT == kVoronoi ? 3 : GT == kCartesian ? SKG_LIFECYCLE_WAVES_CART : SKG_LIFECYCLE_WAVES;
+        return GT == kTetra ? 4 : GT == kVoronoi ? 4 : GT == kCartesian ? SKG_LIFECYCLE_WAVES_CART : SKG_LIFECYCLE_WAVES;
     }
 
     // The kernel's parameter block where it lies in the kernarg segment, behind a pointer the

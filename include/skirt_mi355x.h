@@ -248,11 +248,13 @@ int skg_tune(skg_engine* e, int32_t workgroups, int32_t threads, int32_t transit
  * radiation field and the counters are restored afterwards) for each candidate launch
  * configuration -- persistent
  * engine: transition threshold 12/16/20/24/32/48/64 x 4/8 workgroups per CU; pipeline engine: 2^20/2^21/
- * 2^22 packet slots x 8/16 graph iterations x 4/8 trace workgroups per CU -- and keeps the
- * fastest.  engines: bit 0 persistent, bit 1 pipeline (batched instances, particle sources
- * and radiation-field storage: persistent only).
+ * 2^22 packet slots x 8/16 graph iterations x 4/8 trace workgroups per CU -- then times the
+ * three fastest again on 4 x trial_packets histories (the end of a launch weighs less there;
+ * skipped when batched instances hold too few histories) and keeps the fastest of those.
+ * engines: bit 0 persistent, bit 1 pipeline (batched instances, particle sources and
+ * radiation-field storage: persistent only).
  * best6 (may be NULL) receives {mode, threshold, workgroups, pool_slots, trace_workgroups,
- * graph_iterations}; best_rate the trial packets/s.  Not to be called concurrently with other
+ * graph_iterations}; best_rate the packets/s of its last timed run.  Not to be called concurrently with other
  * calls on the same engine. */
 int skg_autotune(skg_engine* e, uint64_t first_history, uint64_t trial_packets, int32_t engines, int64_t* best6,
                  double* best_rate);

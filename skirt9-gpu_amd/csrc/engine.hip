@@ -2600,22 +2600,50 @@ int skg_autotune(skg_engine* e, uint64_t first_history, uint64_t trial_packets, 
         e->graphIters = c.iters;
     };
     uint64_t h = first_history;
-    for (const Cand& c : cands)
-    {
+    // one warm-up run (pool allocation, graph capture), then the timed run of `packets`
+    auto trial = [&](const Cand& c, uint64_t packets, double& rate) {
         applyTuning(c);
-        // one warm-up run (pool allocation, graph capture), then the timed trial
-        if (!e->hInst.empty() && h + trial_packets / 4 + trial_packets > e->instTotal) h = 0;
-        if ((st = skg_run_packets(e, h, trial_packets / 4))) break;
+        if (!e->hInst.empty() && h + trial_packets / 4 + packets > e->instTotal) h = 0;
+        int r = skg_run_packets(e, h, trial_packets / 4);
+        if (r) return r;
         h += trial_packets / 4;
         auto t0 = std::chrono::steady_clock::now();
-        if ((st = skg_run_packets(e, h, trial_packets))) break;
+        if ((r = skg_run_packets(e, h, packets))) return r;
         double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        h += trial_packets;
-        double rate = static_cast<double>(trial_packets) / dt;
+        h += packets;
+        rate = static_cast<double>(packets) / dt;
+        return SKG_OK;
+    };
+    std::vector<std::pair<double, size_t>> ranked;
+    for (size_t ci = 0; ci < cands.size(); ++ci)
+    {
+        double rate = 0.;
+        if ((st = trial(cands[ci], trial_packets, rate))) break;
+        ranked.push_back({rate, ci});
         if (rate > bestRate)
         {
             bestRate = rate;
-            best = c;
+            best = cands[ci];
+        }
+    }
+    // The end of a launch, when the last waves finish while other units idle, weighs more in a
+    // short trial than in a long run: the three fastest candidates are timed again on four
+    // times the trial (where the instances allow it), and the fastest of those is kept.
+    const uint64_t longTrial = 4 * trial_packets;
+    if (!st && ranked.size() > 1 && (e->hInst.empty() || e->instTotal >= longTrial + trial_packets / 4))
+    {
+        std::sort(ranked.begin(), ranked.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+        double bestLong = -1.;
+        for (size_t r = 0; r < ranked.size() && r < 3; ++r)
+        {
+            double rate = 0.;
+            if ((st = trial(cands[ranked[r].second], longTrial, rate))) break;
+            if (rate > bestLong)
+            {
+                bestLong = rate;
+                best = cands[ranked[r].second];
+                bestRate = rate;
+            }
         }
     }
     std::lock_guard<std::mutex> lk(e->mu);

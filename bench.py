@@ -64,13 +64,17 @@ def host_threads():
     return max(1, min(16, n))
 
 
-def cpu_baseline_reference(tau_v, nlambda, lam_range, packets, threads, cartesian=False, voronoi=None):
+def cpu_baseline_reference(tau_v, nlambda, lam_range, packets, threads, cartesian=False, voronoi=None, tetra=None):
     """Times the reference SKIRT build on the same workload (its own grid and densities; for
     a Voronoi workload our sites and cell densities, imported as a VoronoiMeshMedium whose mesh
-    the grid employs); packets/s over the primary emission phase, as logged by the reference."""
+    the grid employs; for the batched tetra workload a few of the instances, each on the
+    Delaunay mesh of our vertices); packets/s over the primary emission phase, as logged by the
+    reference."""
     exe = os.path.join(ROOT, "oracle", "_ref", "bin", "skirt", "ref", "skirt_ref")
     if not os.path.exists(exe):
         return None
+    if tetra is not None:
+        return cpu_baseline_reference_tetra(exe, tetra, threads)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import skigen
     if cartesian:
@@ -100,6 +104,53 @@ def cpu_baseline_reference(tau_v, nlambda, lam_range, packets, threads, cartesia
                        + ("our sites and cell densities imported, mesh built by the reference" if voronoi is not None
                           else "grid and densities built by the reference")
                        + f"), primary emission {t:.2f} s")
+
+
+def cpu_baseline_reference_tetra(exe, tetra, threads):
+    """The reference on a sample of the batched instances: each sampled instance (its tau_V and
+    inclination, config 0's disk and dust otherwise) runs `packets` packets on the Delaunay
+    mesh of our vertices (TetraMeshSpatialGrid policy File; the reference computes its own cell
+    densities); packets/s = all packets / the summed primary emission times."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import skigen
+    total_t, total_n, taus = 0.0, 0, []
+    with tempfile.TemporaryDirectory() as tmp:
+        skigen.write_points(os.path.join(tmp, "points.txt"), tetra["points"])
+        for k, (tau, inc, n) in enumerate(tetra["runs"]):
+            spec = skigen.SkiSpec(num_packets=n, tau_v=tau, grid=("tetra_file", "points.txt"),
+                                  instruments=(("i0", inc, 0.0),))
+            ski = os.path.join(tmp, f"inst{k}.ski")
+            with open(ski, "w") as f:
+                f.write(skigen.render(spec))
+            res = subprocess.run([exe, "run", ski, tmp, str(threads)], capture_output=True, text=True, timeout=900)
+            if res.returncode != 0:
+                return None
+            log = open(os.path.join(tmp, f"inst{k}_log.txt")).read()
+            m = re.search(r"Finished primary emission in ([0-9.]+) s", log)
+            if not m:
+                return None
+            total_t += float(m.group(1))
+            total_n += n
+            taus.append(f"{tau:.3g}")
+    return dict(value=total_n / total_t, unit="packets/s", cores=threads, kind="reference",
+                sample=f"{len(tetra['runs'])} instances (tau_V {', '.join(taus)}: the packet-weighted quartiles of "
+                       f"the batch) x {tetra['runs'][0][2]} packets on the Delaunay mesh of our "
+                       f"{len(tetra['points'])} vertices, built by the reference; primary emission {total_t:.2f} s")
+
+
+def tetra_sample(wl, packets):
+    """Four instances of the batch for the reference baseline: the instances at the packet-
+    weighted quartiles of tau_V (the batch's cost is spread that way), `packets` // 4 packets
+    each: (tau_V, inclination in degrees, packets)."""
+    I = wl.instances
+    order = np.argsort(I["tau_scale"])
+    cum = np.cumsum(I["packets"][order].astype(np.float64))
+    cum /= cum[-1]
+    runs = []
+    for q in (0.125, 0.375, 0.625, 0.875):
+        i = order[min(int(np.searchsorted(cum, q)), len(order) - 1)]
+        runs.append((float(I["tau_scale"][i] * wl.tau_v), float(I["inclination_deg"][i]), max(1, packets // 4)))
+    return runs
 
 
 def cpu_baseline_port(wl, packets, threads, seed):
@@ -431,7 +482,8 @@ def main():
                  "all) on a %d-tetra Delaunay mesh of 2e5 density-sampled points" %
                  (len(wl.instances["packets"]), a.packets, ncells))
         a.no_guide = True
-        a.no_cpu_baseline = True
+        if a.cpu_packets == CPU_PACKETS_DEFAULT:
+            a.cpu_packets = 1e6  # four instances of 2.5e5 packets, about 11 s (their mesh builds excluded)
     else:
         wl = workloads.voronoi_clumpy(num_sites=a.sites, threads=threads)
         ncells, nnb = wl.tables["mesh"].info()[:2]
@@ -514,11 +566,13 @@ def main():
     cpu = None
     if not a.no_cpu_baseline:
         try:
-            vor = None
+            vor = tet = None
             if a.workload == "voronoi":
                 vor = dict(sites=wl.tables["mesh"].export()["sites"], density=wl.density, frames=wl.frames)
+            if a.workload == "tetra-batch":
+                tet = dict(points=wl.tables["mesh"].export()["verts"], runs=tetra_sample(wl, int(a.cpu_packets)))
             cpu = cpu_baseline_reference(a.tau, nlambda, lam_range, a.cpu_packets, threads,
-                                         cartesian=a.workload == "cartesian", voronoi=vor)
+                                         cartesian=a.workload == "cartesian", voronoi=vor, tetra=tet)
         except Exception as ex:  # the reference build may be absent or fail on this host
             print(f"reference CPU baseline unavailable: {ex}", file=sys.stderr)
         if cpu is None:

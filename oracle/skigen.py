@@ -131,6 +131,10 @@ def render(spec: SkiSpec) -> str:
         # ("tetra", num_sites, policy): Delaunay tetrahedralisation of the sampled vertices
         _, nsites, policy = spec.grid
         grid = f'<TetraMeshSpatialGrid {box} policy="{policy}" numSamples="{nsites}" refine="false"/>'
+    elif spec.grid[0] == "tetra_file":
+        # ("tetra_file", filename): Delaunay tetrahedralisation of given vertices (policy File,
+        # TetraMeshSpatialGrid.cpp:303-312), written by write_points
+        grid = f'<TetraMeshSpatialGrid {box} policy="File" filename="{spec.grid[1]}" refine="false"/>'
     else:
         raise ValueError(spec.grid)
     instr = "".join(
@@ -213,6 +217,16 @@ def write_voronoi_medium(path, sites_m, density_m3):
                                           ("number density", "1/m3"))):
             f.write(f"# column {i + 1}: {name} ({unit})\n")
         np.savetxt(f, np.column_stack([np.asarray(sites_m) / pc, density_m3]), fmt="%.17g")
+
+
+def write_points(path, points_m):
+    """A vertex file of TetraMeshSpatialGrid's File policy: positions x, y, z (pc), one per row."""
+    import numpy as np
+    pc = 3.08567758e16  # the reference's parsec (Constants::pc)
+    with open(path, "w") as f:
+        for i, name in enumerate(("position x", "position y", "position z")):
+            f.write(f"# column {i + 1}: {name} (pc)\n")
+        np.savetxt(f, np.asarray(points_m) / pc, fmt="%.17g")
 
 
 def log_wavelengths(lo: float, hi: float, n: int) -> List[float]:

@@ -131,6 +131,10 @@ def render(spec: SkiSpec) -> str:
         # ("tetra", num_sites, policy): Delaunay tetrahedralisation of the sampled vertices
         _, nsites, policy = spec.grid
         grid = f'<TetraMeshSpatialGrid {box} policy="{policy}" numSamples="{nsites}" refine="false"/>'
+    elif spec.grid[0] == "voronoi_file":
+        # ("voronoi_file", filename): the Voronoi tessellation of given sites (policy File,
+        # VoronoiMeshSpatialGrid.cpp:122-126), written by write_points
+        grid = f'<VoronoiMeshSpatialGrid {box} policy="File" filename="{spec.grid[1]}" relaxSites="false"/>'
     elif spec.grid[0] == "tetra_file":
         # ("tetra_file", filename): Delaunay tetrahedralisation of given vertices (policy File,
         # TetraMeshSpatialGrid.cpp:303-312), written by write_points
@@ -220,7 +224,7 @@ def write_voronoi_medium(path, sites_m, density_m3):
 
 
 def write_points(path, points_m):
-    """A vertex file of TetraMeshSpatialGrid's File policy: positions x, y, z (pc), one per row."""
+    """A site / vertex file of the mesh grids' File policy: positions x, y, z (pc), one per row."""
     import numpy as np
     pc = 3.08567758e16  # the reference's parsec (Constants::pc)
     with open(path, "w") as f:

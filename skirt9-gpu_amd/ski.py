@@ -6,7 +6,7 @@ an oligochromatic `MonteCarloSimulation` with one `PointSource` (`BlackBodySED` 
 inside a `SpiralStructureGeometryDecorator`) with a `MeanListDustMix` and an
 `OpticalDepthMaterialNormalization`, a `CartesianSpatialGrid` (`LinMesh` axes) or a
 `PolicyTreeSpatialGrid` with a `DensityTreePolicy`, a `VoronoiMeshSpatialGrid` or a
-`TetraMeshSpatialGrid` with sites / vertices from the dust density -- or one
+`TetraMeshSpatialGrid` with sites / vertices from the dust density or a column file -- or one
 `VoronoiMeshMedium` (sites and number or mass densities imported from a text file, with a
 `MeanListDustMix`) on a `VoronoiMeshSpatialGrid` that employs its mesh -- `PhotonPacketOptions`,
 `FrameInstrument`s, `SEDInstrument`s and `PerspectiveInstrument`s.  Anything else raises `UnsupportedSki`, so a caller
@@ -249,13 +249,21 @@ def load(path: str) -> Simulation:
             raise UnsupportedSki("the grid and the imported medium must have the same domain")
         grid = dict(kind="voronoi_imported")
     elif grid_el.tag in ("VoronoiMeshSpatialGrid", "TetraMeshSpatialGrid"):
-        # vertices / sites sampled from the medium's density (policy DustDensity)
-        if grid_el.get("policy") != "DustDensity":
-            raise UnsupportedSki("only the DustDensity site policy is supported")
+        # vertices / sites sampled from the medium's density (policy DustDensity) or read from a
+        # column file (policy File: VoronoiMeshSnapshot.cpp's file constructor,
+        # TetraMeshSpatialGrid.cpp:303-312)
+        policy = grid_el.get("policy", "DustDensity")
+        if policy not in ("DustDensity", "File"):
+            raise UnsupportedSki("only the DustDensity and File site policies are supported")
         if grid_el.get("relaxSites", "false") != "false" or grid_el.get("refine", "false") != "false":
             raise UnsupportedSki("site relaxation and tetra refinement are not supported")
-        key = "numSites" if grid_el.tag == "VoronoiMeshSpatialGrid" else "numSamples"
-        grid = dict(kind="voronoi" if grid_el.tag == "VoronoiMeshSpatialGrid" else "tetra", n=int(grid_el.get(key)))
+        kind = "voronoi" if grid_el.tag == "VoronoiMeshSpatialGrid" else "tetra"
+        if policy == "File":
+            pts = _position_file(path, grid_el.get("filename"))
+            grid = dict(kind=kind, n=len(pts), points=pts)
+        else:
+            key = "numSites" if kind == "voronoi" else "numSamples"
+            grid = dict(kind=kind, n=int(grid_el.get(key)))
     else:
         raise UnsupportedSki(f"grid {grid_el.tag} is not supported by the ski front end")
     if not instruments:
@@ -348,6 +356,29 @@ def _voronoi_medium(path, el):
         dens = dens / MU_DUST
     box = tuple(quantity(el.get(k)) for k in ("minX", "minY", "minZ", "maxX", "maxY", "maxZ"))
     return dict(sites=sites, density=dens, box=box)
+
+
+def _position_file(path, name):
+    """Mesh vertices or sites from a text column file (policy File): the columns position x, y,
+    z in the units of the '# column i: name (unit)' header lines, or pc (TextInFile's
+    defaults for these columns)."""
+    units = ["pc", "pc", "pc"]
+    rows = []
+    with open(os.path.join(os.path.dirname(os.path.abspath(path)), name)) as f:
+        for line in f:
+            t = line.strip()
+            if not t:
+                continue
+            if t.startswith("#"):
+                low = t[1:].strip().lower()
+                if low.startswith("column") and "(" in t and ":" in t:
+                    i = int(low.split(":")[0].split()[1]) - 1
+                    if 0 <= i < 3:
+                        units[i] = t[t.rindex("(") + 1:t.rindex(")")].strip()
+                continue
+            rows.append([float(v) for v in t.split()[:3]])
+    a = np.array(rows, dtype=np.float64).reshape(-1, 3)
+    return a * np.array([quantity("1 " + u) for u in units])
 
 
 _PARTICLE_COLUMNS = (("position x", "pc"), ("position y", "pc"), ("position z", "pc"), ("size h", "pc"),
@@ -662,8 +693,11 @@ def setup(sim: Simulation, engine, threads=0, npk_norm=None):
     elif sim.grid["kind"] in ("voronoi", "tetra"):
         # TetraMeshSpatialGrid / VoronoiMeshSpatialGrid with the DustDensity policy: points
         # distributed as the medium (TetraMeshSpatialGrid.cpp:23-40), then the mesh
-        pts = W.disk_points(sim.grid["n"], sim.box, sim.scale_length, sim.scale_height, seed=sim.seed + 3,
-                            spiral=sim.spiral_params if sim.spiral else None)
+        if "points" in sim.grid:
+            pts = sim.grid["points"]
+        else:
+            pts = W.disk_points(sim.grid["n"], sim.box, sim.scale_length, sim.scale_height, seed=sim.seed + 3,
+                                spiral=sim.spiral_params if sim.spiral else None)
         if sim.grid["kind"] == "voronoi":
             mesh = pkg.Voronoi.build(pts, sim.box, threads)
             mesh.upload(engine)

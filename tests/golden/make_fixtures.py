@@ -33,6 +33,9 @@ Every fixture is data: seeded inputs plus the reference's outputs for them.
                    tetrahedral grid of 20000 vertices (DustDensity policy), 100x100 frame at
                    60 deg: per-pixel mean and standard error over independent runs, each run
                    with its own random sites and densities.
+  image_mesh_file.npz (--mesh-files) the same on a Voronoi grid and a tetrahedral grid whose
+                   20000 sites / vertices are read from a file (policy File; the disk's
+                   distribution, workloads.disk_points seed 77): every run on the same mesh.
   image_persp.npz  (--persp) the config-0 workload seen by a PerspectiveInstrument (eye
                    outside the domain, 100x100 pixels, viewport 20 pc wide at 30 pc focal
                    length, from (10, -30, 20) pc toward (1, 0, 0) pc, so the image has no mirror
@@ -526,6 +529,48 @@ def fixtures_mesh_images(tmp, runs=8, packets=1e6):
     np.savez_compressed(os.path.join(HERE, "image_mesh.npz"), **out)
 
 
+MESH_FILE_POINTS = dict(n=20000, seed=77)
+
+
+def mesh_file_points():
+    """The 20000 sites / vertices of the File-policy mesh fixtures: the exponential disk's
+    distribution (workloads.disk_points, seed 77) in the +-10 pc box."""
+    import importlib.util
+    name = "skirt9_gpu_amd"
+    if name not in sys.modules:
+        spec = importlib.util.spec_from_file_location(name, os.path.join(ROOT, "skirt9-gpu_amd", "__init__.py"),
+                                                      submodule_search_locations=[os.path.join(ROOT, "skirt9-gpu_amd")])
+        mod = importlib.util.module_from_spec(spec)
+        sys.modules[name] = mod
+        spec.loader.exec_module(mod)
+    from skirt9_gpu_amd import workloads
+    h = 10 * PC
+    return workloads.disk_points(MESH_FILE_POINTS["n"], np.array([-h, -h, -h, h, h, h]), seed=MESH_FILE_POINTS["seed"])
+
+
+def fixtures_mesh_file_images(tmp, runs=8, packets=1e6):
+    """The config-0 disk on a Voronoi grid and on a tetrahedral grid whose 20000 sites /
+    vertices are read from a file (policy File; mesh_file_points), 100x100 frame at 60 deg:
+    per-pixel mean and standard error over independent runs, all on the same mesh."""
+    outdir = os.path.join(tmp, "meshfile")
+    os.makedirs(outdir, exist_ok=True)
+    skigen.write_points(os.path.join(tmp, "pts.txt"), mesh_file_points())
+    out = {"runs": np.array(runs), "packets": np.array(packets)}
+    for kind in ("voronoi", "tetra"):
+        imgs = []
+        for s in range(runs):
+            spec = skigen.config0(num_packets=packets, seed=6100 + s)
+            spec.grid = (kind + "_file", "pts.txt")
+            spec.num_pixels = 100
+            ski = write_ski(tmp, f"{kind}f{s}", spec)
+            run("run", ski, outdir, 8)
+            imgs.append(fits_image(os.path.join(outdir, f"{kind}f{s}_i60_total.fits")))
+        imgs = np.array(imgs)
+        out[kind + "_mean"] = imgs.mean(axis=0).astype(np.float32)
+        out[kind + "_sem"] = (imgs.std(axis=0, ddof=1) / np.sqrt(runs)).astype(np.float32)
+    np.savez_compressed(os.path.join(HERE, "image_mesh_file.npz"), **out)
+
+
 CAMERA = ("cam", (10.0, -30.0, 20.0), (1.0, 0.0, 0.0), (0.0, 0.0, 1.0), 30.0, 20.0, 100)
 
 
@@ -616,6 +661,7 @@ def main():
     ap.add_argument("--seds", action="store_true")
     ap.add_argument("--rf", action="store_true")
     ap.add_argument("--meshes", action="store_true")
+    ap.add_argument("--mesh-files", action="store_true")
     ap.add_argument("--persp", action="store_true")
     ap.add_argument("--particles", action="store_true")
     ap.add_argument("--clumpy", action="store_true")
@@ -641,6 +687,8 @@ def main():
             fixtures_rf(tmp)
         if a.meshes:
             fixtures_mesh_images(tmp)
+        if a.mesh_files:
+            fixtures_mesh_file_images(tmp)
         if a.persp:
             fixtures_persp(tmp)
         if a.particles:

@@ -12,7 +12,7 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import ROOT, golden, load_pkg, pixel_parity
+from helpers import PC, ROOT, golden, load_pkg, pixel_parity
 
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import skigen  # noqa: E402
@@ -112,6 +112,33 @@ def test_config0_mean_intensity_matches_reference(tmp_path):
     assert abs(vt.mean() - ref.sum()) < 4 * np.hypot(vt.std(ddof=1) / np.sqrt(nb), np.sqrt((rsem ** 2).sum()))
 
 
+def _compare_mesh_images(imgs, f, kind):
+    """Our runs' frames (runs x ny x nx) against the reference's per-pixel mean and standard
+    error: total within 4 sigma (pooled run variance), 5x5-binned pooled chi-square per degree
+    of freedom < 1.5 and under 1 % of the bins beyond 4 sigma."""
+    nb = len(imgs)
+    ref, rsem = f[kind + "_mean"][0].astype(float), f[kind + "_sem"][0].astype(float)
+    nref = int(f["runs"])
+    tot = imgs.sum(axis=(1, 2))
+    ref_run_var_tot = (rsem ** 2).sum() * nref  # an underestimate of the run variance of the total (pixels correlate)
+    pooled_tot = ((nref - 1) * ref_run_var_tot + (nb - 1) * tot.var(ddof=1)) / (nref + nb - 2)
+    assert abs(tot.mean() - ref.sum()) < 4 * np.sqrt(pooled_tot * (1 / nb + 1 / nref)), (tot.mean(), ref.sum())
+
+    def binned(a, B=5):
+        n = a.shape[-1] // B
+        return a.reshape(*a.shape[:-2], n, B, n, B).sum(axis=(-1, -3))
+
+    oi = binned(imgs)
+    var_ref_run = binned((rsem ** 2)[None])[0] * nref
+    pooled = ((nref - 1) * var_ref_run + (nb - 1) * oi.var(0, ddof=1)) / (nref + nb - 2)
+    var = pooled * (1 / nb + 1 / nref)
+    m = var > 0
+    z = (oi.mean(0) - binned(ref[None])[0])[m] / np.sqrt(var[m])
+    assert m.sum() > 200
+    assert np.mean(z ** 2) < 1.5, np.mean(z ** 2)
+    assert np.mean(np.abs(z) > 4) < 0.01
+
+
 @pytest.mark.parametrize("kind", ["voronoi", "tetra"])
 def test_config0_on_mesh_grids_matches_reference(tmp_path, kind):
     """The config-0 ski on a Voronoi grid of 20000 sites / a tetrahedral grid of 20000 vertices
@@ -138,28 +165,40 @@ def test_config0_on_mesh_grids_matches_reference(tmp_path, kind):
         out = S.run(sim, str(tmp_path), f"{kind}{b}", engine=eng, log=lambda *a: None)
         imgs.append(out["i60"][0])
     eng.close()
-    f = golden("image_mesh.npz")
-    ref, rsem = f[kind + "_mean"][0].astype(float), f[kind + "_sem"][0].astype(float)
-    imgs = np.array(imgs)
-    nref = int(f["runs"])
-    tot = imgs.sum(axis=(1, 2))
-    ref_run_var_tot = (rsem ** 2).sum() * nref  # an underestimate of the run variance of the total (pixels correlate)
-    pooled_tot = ((nref - 1) * ref_run_var_tot + (nb - 1) * tot.var(ddof=1)) / (nref + nb - 2)
-    assert abs(tot.mean() - ref.sum()) < 4 * np.sqrt(pooled_tot * (1 / nb + 1 / nref)), (tot.mean(), ref.sum())
+    _compare_mesh_images(np.array(imgs), golden("image_mesh.npz"), kind)
 
-    def binned(a, B=5):
-        n = a.shape[-1] // B
-        return a.reshape(*a.shape[:-2], n, B, n, B).sum(axis=(-1, -3))
 
-    oi = binned(imgs)
-    var_ref_run = binned((rsem ** 2)[None])[0] * nref
-    pooled = ((nref - 1) * var_ref_run + (nb - 1) * oi.var(0, ddof=1)) / (nref + nb - 2)
-    var = pooled * (1 / nb + 1 / nref)
-    m = var > 0
-    z = (oi.mean(0) - binned(ref[None])[0])[m] / np.sqrt(var[m])
-    assert m.sum() > 200
-    assert np.mean(z ** 2) < 1.5, np.mean(z ** 2)
-    assert np.mean(np.abs(z) > 4) < 0.01
+@pytest.mark.parametrize("kind", ["voronoi", "tetra"])
+def test_config0_on_mesh_grids_file_policy_matches_reference(tmp_path, kind):
+    """The config-0 ski on a Voronoi / tetrahedral grid whose 20000 sites / vertices are read
+    from a column file (policy File): the points tests/golden/make_fixtures.py wrote for the
+    reference (the disk's distribution, workloads.disk_points with seed 77), so both sides
+    build the same mesh; 8 runs with their own cell densities against the reference's images
+    (tests/golden/image_mesh_file.npz): total within 4 sigma, 5x5-binned pooled chi-square per
+    degree of freedom < 1.5."""
+    pkg = load_pkg()
+    from skirt9_gpu_amd import ski as S
+    from skirt9_gpu_amd import workloads
+    h = 10 * PC
+    pts = workloads.disk_points(20000, np.array([-h, -h, -h, h, h, h]), seed=77)
+    skigen.write_points(os.path.join(str(tmp_path), "pts.txt"), pts)
+    spec = skigen.config0(num_packets=1e6)
+    spec.grid = (kind + "_file", "pts.txt")
+    spec.num_pixels = 100
+    path = os.path.join(str(tmp_path), f"{kind}f.ski")
+    with open(path, "w") as f:
+        f.write(skigen.render(spec))
+    sim = S.load(path)
+    assert sim.grid["kind"] == kind and sim.grid["n"] == 20000
+    eng = pkg.Engine(0)
+    imgs = []
+    for b in range(8):
+        sim.seed = 7100 + b
+        print("case ski mesh file", kind, b, flush=True)
+        out = S.run(sim, str(tmp_path), f"{kind}f{b}", engine=eng, log=lambda *a: None)
+        imgs.append(out["i60"][0])
+    eng.close()
+    _compare_mesh_images(np.array(imgs), golden("image_mesh_file.npz"), kind)
 
 
 def test_run_ski_everything_combined(tmp_path):

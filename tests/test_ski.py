@@ -265,6 +265,34 @@ def test_particle_source_parse(ski, tmp_path):
             ski.load(p)
 
 
+@pytest.mark.parametrize("kind", ["voronoi", "tetra"])
+def test_mesh_grid_file_policy_parse(ski, tmp_path, kind):
+    """VoronoiMeshSpatialGrid / TetraMeshSpatialGrid with policy File: the sites or vertices
+    are read from the column file (position x, y, z in the header's units, pc without a
+    header line) and the grid keeps them in the file's order."""
+    rng = np.random.default_rng(4)
+    pts = (rng.random((300, 3)) * 2 - 1) * 9 * PC
+    skigen.write_points(os.path.join(str(tmp_path), "pts.txt"), pts)
+    spec = skigen.SkiSpec(num_packets=1e4, grid=("tetra_file", "pts.txt"))
+    p = _write(str(tmp_path), "mf", spec)
+    if kind == "voronoi":
+        text = open(p).read().replace("<TetraMeshSpatialGrid", "<VoronoiMeshSpatialGrid").replace(
+            'refine="false"', 'relaxSites="false"')
+        with open(p, "w") as f:
+            f.write(text)
+    sim = ski.load(p)
+    assert sim.grid["kind"] == kind and sim.grid["n"] == len(pts)
+    np.testing.assert_allclose(sim.grid["points"], pts, rtol=1e-15, atol=64.0)
+    # without header lines the columns are in pc; a header may give another length unit
+    with open(os.path.join(str(tmp_path), "pts.txt"), "w") as f:
+        np.savetxt(f, pts / PC, fmt="%.17g")
+    np.testing.assert_allclose(ski.load(p).grid["points"], pts, rtol=1e-15, atol=64.0)
+    with open(os.path.join(str(tmp_path), "pts.txt"), "w") as f:
+        f.write("# column 1: position x (kpc)\n# column 2: position y (kpc)\n# column 3: position z (kpc)\n")
+        np.savetxt(f, pts / (1e3 * PC), fmt="%.17g")
+    np.testing.assert_allclose(ski.load(p).grid["points"], pts, rtol=1e-14, atol=64.0)
+
+
 def test_voronoi_mesh_medium_parse(ski, tmp_path):
     """VoronoiMeshMedium on a VoronoiMeshSpatialGrid with policy ImportedMesh: the site and
     density columns in the units of the '# column' header lines (or the defaults pc and 1/cm3),

@@ -582,11 +582,13 @@ def main():
     # (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel's reading), when they cover this
     # kernel, workload and batch size
     traffic = None
-    prof = os.path.join(ROOT, "profiles", f"{kname}_{a.workload}_tau{a.tau:g}_traffic.json")
+    # named for the workload (and its tau_V for the octree disk, whose tau_V is an option)
+    tag = f"{a.workload}_tau{a.tau:g}" if a.workload == "octree" else a.workload
+    prof = os.path.join(ROOT, "profiles", f"{kname}_{tag}_traffic.json")
     if os.path.exists(prof):
         try:
             tj = json.load(open(prof))
-            if (kname != "lifecycle" or tj.get("packets_per_launch") == a.packets) and tj.get("nlambda", 5) == nlambda:
+            if (kname != "lifecycle" or tj.get("packets_per_launch") == a.packets) and tj.get("nlambda", 5) == len(wl.lam):
                 traffic = tj.get("hbm_bytes_per_launch")
         except Exception:
             traffic = None

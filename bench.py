@@ -38,7 +38,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
-CPU_PACKETS_DEFAULT = 4e5  # packets of the reference CPU sample (octree and Cartesian workloads)
+CPU_PACKETS_DEFAULT = 1.5e6  # packets of the reference CPU sample on the octree disk (about 12 s on 16 cores)
 BYTES_PER_SEGMENT = 64.0  # octree: node bounds 48 B + leaf/cell 4 B + neighbour entry 4 B + density 8 B
 BYTES_PER_ATOMIC = 8.0
 BYTES_PER_REQUEST = 96.0  # per traversal: request 32 B + packet position/direction/bin 52 B + result 12 B
@@ -472,6 +472,8 @@ def main():
         nlambda, lam_range = 1, (0.55, 0.55)
         wdesc = "Cartesian 64^3 exponential disk, face-on tauV=1, 0.55 um, albedo 0.6, g 0.6"
         a.no_guide = True
+        if a.cpu_packets == CPU_PACKETS_DEFAULT:
+            a.cpu_packets = 1.5e7  # about 11 s of the reference on 16 cores
     elif a.workload == "tetra-batch":
         wl = workloads.tetra_batch(packet_scale=a.batch_scale, threads=threads)
         ncells = int(wl.tables["ntets"])
@@ -493,7 +495,7 @@ def main():
         wdesc = "Voronoi %d sites, lognormal clumpy medium (P(k)~k^-3.7, sigma_ln 1), mean tauV=5, 3 frames" % ncells
         a.no_guide = True
         if a.cpu_packets == CPU_PACKETS_DEFAULT:
-            a.cpu_packets = 1e5  # about 20 s of the reference on 16 cores (its mesh build excluded)
+            a.cpu_packets = 2e5  # about 11 s of the reference on 16 cores (its mesh build excluded)
     print(f"setup: {ncells} cells, {wl.setup_s:.1f} s", file=sys.stderr, flush=True)
 
     wl.apply(eng, npk_norm=1.0)

@@ -506,7 +506,8 @@ def main():
     eng.tune_pipeline(mode, a.pool, a.trace_wgs, a.graph_iters)
     tuned = None
     if not a.no_autotune:
-        cfg, rate = eng.autotune(1 << 40, trial_packets=1 << 20)
+        # --mode pipeline keeps the autotuner to that engine; the default tries both
+        cfg, rate = eng.autotune(1 << 40, trial_packets=1 << 20, engines=2 if a.mode == "pipeline" else 3)
         tuned = dict(cfg, trial_packets_per_s=rate)
         mode = cfg["mode"]
         print(f"autotune: {cfg}, {rate:.3e} packets/s in the trial", file=sys.stderr, flush=True)

@@ -2309,16 +2309,16 @@ int skg_guide_gradient(skg_engine* e, const float* records, int64_t n, float* gr
     std::lock_guard<std::mutex> lk(e->mu);
     hipSetDevice(e->device);
     if (!e->gTheta) return fail(e, SKG_ERR_STATE, "skg_guide_gradient: no guide parameters");
-    float* dR = nullptr;
-    double* dL = nullptr;
-    HIPCHK(e, hipMalloc(&dR, n * 16 * sizeof(float)));
-    HIPCHK(e, hipMalloc(&dL, sizeof(double)));
-    HIPCHK(e, hipMemcpy(dR, records, n * 16 * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(e, hipMemset(dL, 0, sizeof(double)));
     const int64_t nb = (n + kGradSamples - 1) / kGradSamples;
     if (nb > INT32_MAX) return fail(e, SKG_ERR_ARGUMENT, "skg_guide_gradient: too many records");
-    DevBuf part;
+    DevBuf recs, lossBuf, part;  // freed on every return
+    HIPCHK(e, recs.ensure(static_cast<size_t>(n) * 16 * sizeof(float)));
+    HIPCHK(e, lossBuf.ensure(sizeof(double)));
     HIPCHK(e, part.ensure(static_cast<size_t>(nb) * kGuideParams * sizeof(float)));
+    float* dR = recs.as<float>();
+    double* dL = lossBuf.as<double>();
+    HIPCHK(e, hipMemcpyAsync(dR, records, n * 16 * sizeof(float), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(e, hipMemsetAsync(dL, 0, sizeof(double), e->stream));
     float* g = e->gTheta + 3 * kGuideParams;
     hipLaunchKernelGGL(guide_grad_part, dim3(static_cast<unsigned>(nb)), dim3(64 * kGradWaves), 0, e->stream, e->net, dR,
                        16, n, 1.0f, part.as<float>(), dL);
@@ -2332,8 +2332,6 @@ int skg_guide_gradient(skg_engine* e, const float* records, int64_t n, float* gr
     HIPCHK(e, hipMemcpyAsync(&l, dL, sizeof(double), hipMemcpyDeviceToHost, e->stream));
     HIPCHK(e, hipStreamSynchronize(e->stream));
     if (loss) *loss = l;
-    hipFree(dR);
-    hipFree(dL);
     return SKG_OK;
 }
 
